@@ -1,0 +1,852 @@
+// api.cpp — the C ABI of include/hpt.h over the gfx950 kernels.
+//
+// Host-side work per call:
+//  set_scene   validate and pack the std140 arrays and meshes (world-space
+//              triangle vertices, shading vertices, inverse-transpose 3x3s);
+//  commit      build the 8-wide BVH, upload everything, size the path state;
+//  render      per frame: camera, (extend, shade, shadow) x bounces,
+//              accumulate, all on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace {
+
+using namespace hpt;
+
+thread_local std::string g_last_error;
+
+struct Fail : std::runtime_error {
+  hpt_status code;
+  Fail(hpt_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Fail(e == hipErrorOutOfMemory ? HPT_ERROR_OUT_OF_MEMORY : HPT_ERROR_DEVICE,
+               std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    if (n == bytes && p) return;
+    release();
+    if (n == 0) return;
+    check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+struct StageTimer {
+  hipEvent_t a, b;
+  int stage;
+};
+
+}  // namespace
+
+struct hpt_renderer {
+  hpt_create_info info{};
+  hipStream_t stream = nullptr;
+  // Scene, host side.
+  std::vector<hpt_camera> cameras;
+  uint32_t camera_count = 0;
+  std::vector<hpt_light> lights;  // HPT_MAX_LIGHTS + 1 slots
+  uint32_t num_lights = 0;
+  std::vector<hpt_material> materials;
+  std::vector<float> tri_pos;       // 9 floats per global triangle
+  std::vector<uint4> tri_idx;
+  std::vector<ShadeVertex> vtx;
+  std::vector<PrimInfo> prims;
+  bool scene_set = false;
+  bool committed = false;
+  // Environment.
+  uint32_t env_type = HPT_ENV_SKY;
+  float ground[3] = {0.5f, 0.5f, 0.5f}, sky[3] = {0.5f, 0.5f, 0.5f};
+  float env_intensity = 1.0f, env_rotation = 0.0f, env_total = 0.0f;
+  uint32_t env_w = 0, env_h = 0;
+  std::vector<float4> env_rgb, env_table;
+  std::vector<uchar4> blue_noise;
+  // BVH.
+  uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
+  // Device.
+  DevBuf d_nodes, d_ltris, d_tri_idx, d_vtx, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table,
+      d_blue_noise;
+  DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
+  DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
+  DevBuf d_accum, d_albedo, d_normal, d_final;
+  uint32_t npaths = 0;
+  uint64_t frame_index = 0;
+  // Profiling.
+  int profiling = 0;
+  std::vector<StageTimer> timers;
+  size_t timers_used = 0;
+  hpt_stage_stats stats[HPT_STAGE_COUNT] = {};
+};
+
+namespace {
+
+hpt_status fail_with(const Fail& f) {
+  g_last_error = f.what();
+  return f.code;
+}
+
+template <typename F>
+hpt_status guarded(F&& fn) {
+  try {
+    fn();
+    g_last_error.clear();
+    return HPT_OK;
+  } catch (const Fail& f) {
+    return fail_with(f);
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return HPT_ERROR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HPT_ERROR_DEVICE;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return HPT_ERROR_DEVICE;
+  }
+}
+
+void need(bool c, const std::string& m, hpt_status code = HPT_ERROR_INVALID_ARGUMENT) {
+  if (!c) throw Fail(code, m);
+}
+
+void set_device(hpt_renderer* r) { check(hipSetDevice(r->info.device), "hipSetDevice"); }
+
+float luminance(float r, float g, float b) { return 0.212671f * r + 0.715160f * g + 0.072169f * b; }
+
+// Vose's alias method in double precision with both work lists filled in
+// increasing index order; returns the luminance total.
+double build_alias(const std::vector<float>& lum, std::vector<float>& prob, std::vector<uint32_t>& alias) {
+  const size_t n = lum.size();
+  prob.assign(n, 1.0f);
+  alias.resize(n);
+  double total = 0.0;
+  for (size_t i = 0; i < n; ++i) total += (double)lum[i];
+  for (size_t i = 0; i < n; ++i) alias[i] = (uint32_t)i;
+  if (!(total > 0.0)) return total;
+  std::vector<double> scaled(n);
+  std::vector<uint32_t> small_, large_;
+  small_.reserve(n);
+  large_.reserve(n);
+  for (size_t i = 0; i < n; ++i) {
+    scaled[i] = (double)lum[i] * (double)n / total;
+    if (scaled[i] < 1.0) small_.push_back((uint32_t)i);
+    else large_.push_back((uint32_t)i);
+  }
+  while (!small_.empty() && !large_.empty()) {
+    const uint32_t s = small_.back();
+    small_.pop_back();
+    const uint32_t l = large_.back();
+    large_.pop_back();
+    prob[s] = (float)scaled[s];
+    alias[s] = l;
+    scaled[l] = (scaled[l] + scaled[s]) - 1.0;
+    if (scaled[l] < 1.0) small_.push_back(l);
+    else large_.push_back(l);
+  }
+  for (uint32_t i : large_) { prob[i] = 1.0f; alias[i] = i; }
+  for (uint32_t i : small_) { prob[i] = 1.0f; alias[i] = i; }
+  return total;
+}
+
+void default_blue_noise(std::vector<uchar4>& bn) {
+  // 1024x1024 stand-in for assets/textures/blue_noise.png (not in the
+  // reference checkout): top bytes of pcg4d(x, y, 0x6A09E667, 0xBB67AE85).
+  bn.resize(1024 * 1024);
+  for (uint32_t y = 0; y < 1024; ++y)
+    for (uint32_t x = 0; x < 1024; ++x) {
+      uint32_t v[4] = {x, y, 0x6A09E667u, 0xBB67AE85u};
+      for (int i = 0; i < 4; ++i) v[i] = v[i] * 1664525u + 1013904223u;
+      v[0] += v[1] * v[3]; v[1] += v[2] * v[0]; v[2] += v[0] * v[1]; v[3] += v[1] * v[2];
+      for (int i = 0; i < 4; ++i) v[i] ^= v[i] >> 16u;
+      v[0] += v[1] * v[3]; v[1] += v[2] * v[0]; v[2] += v[0] * v[1]; v[3] += v[1] * v[2];
+      bn[(size_t)y * 1024 + x] = make_uchar4((unsigned char)(v[0] >> 24), (unsigned char)(v[1] >> 24),
+                                             (unsigned char)(v[2] >> 24), (unsigned char)(v[3] >> 24));
+    }
+}
+
+SceneView scene_view(const hpt_renderer* r) {
+  SceneView sv;
+  sv.nodes = r->d_nodes.as<Node8>();
+  sv.ltris = r->d_ltris.as<LeafTri>();
+  sv.tri_idx = r->d_tri_idx.as<uint4>();
+  sv.vtx = r->d_vtx.as<ShadeVertex>();
+  sv.prims = r->d_prims.as<PrimInfo>();
+  sv.materials = r->d_materials.as<hpt_material>();
+  sv.lights = r->d_lights.as<hpt_light>();
+  sv.cameras = r->d_cameras.as<hpt_camera>();
+  sv.env_rgb = r->d_env_rgb.as<float4>();
+  sv.env_table = r->d_env_table.as<float4>();
+  sv.blue_noise = r->d_blue_noise.as<uchar4>();
+  sv.node_count = r->node_count;
+  sv.tri_count = r->leaf_tri_count;
+  sv.num_lights = r->num_lights;
+  sv.material_count = (uint32_t)r->materials.size();
+  return sv;
+}
+
+PathState path_state(const hpt_renderer* r) {
+  PathState ps;
+  ps.rng = r->d_rng.as<uint4>();
+  ps.org = r->d_org.as<float4>();
+  ps.dir = r->d_dir.as<float4>();
+  ps.thr = r->d_thr.as<float4>();
+  ps.rad = r->d_rad.as<float4>();
+  ps.albedo = r->d_alb.as<float4>();
+  ps.normal = r->d_nrm.as<float4>();
+  ps.med = r->d_med.as<float4>();
+  ps.med2 = r->d_med2.as<float2>();
+  ps.nee_env = r->d_nee_e.as<float4>();
+  ps.nee_light = r->d_nee_l.as<float4>();
+  ps.hit = r->d_hit.as<float4>();
+  return ps;
+}
+
+// Traversal stack entries beyond the LDS part: needed when the tree has
+// more levels than the short stack holds.
+Spill ensure_spill(hpt_renderer* r, uint32_t threads) {
+  Spill sp{nullptr, 0};
+  if (r->bvh_depth <= (uint32_t)kShortStack) return sp;
+  const size_t entries = (size_t)(r->bvh_depth - kShortStack) * threads;
+  if (r->d_spill.bytes < entries * sizeof(uint2)) r->d_spill.alloc(entries * sizeof(uint2));
+  sp.buf = r->d_spill.as<uint2>();
+  sp.stride = threads;
+  return sp;
+}
+
+uint32_t round_threads(uint32_t n) { return ((n + kBlock - 1) / kBlock) * kBlock; }
+
+void stage_begin(hpt_renderer* r, int stage, StageTimer*& t) {
+  t = nullptr;
+  if (!(r->profiling & 1)) return;
+  if (r->timers_used == r->timers.size()) {
+    StageTimer st;
+    check(hipEventCreate(&st.a), "hipEventCreate");
+    check(hipEventCreate(&st.b), "hipEventCreate");
+    r->timers.push_back(st);
+  }
+  t = &r->timers[r->timers_used++];
+  t->stage = stage;
+  check(hipEventRecord(t->a, r->stream), "hipEventRecord");
+}
+
+void stage_end(hpt_renderer* r, StageTimer* t) {
+  if (!t) return;
+  check(hipEventRecord(t->b, r->stream), "hipEventRecord");
+  r->stats[t->stage].launches++;
+}
+
+void collect_timers(hpt_renderer* r) {
+  if (!r->timers_used) return;
+  check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+  for (size_t i = 0; i < r->timers_used; ++i) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, r->timers[i].a, r->timers[i].b), "hipEventElapsedTime");
+    r->stats[r->timers[i].stage].total_ms += ms;
+  }
+  r->timers_used = 0;
+}
+
+FrameParams frame_params(const hpt_renderer* r, uint32_t camera_index) {
+  FrameParams fp{};
+  for (int i = 0; i < 3; ++i) {
+    fp.ground[i] = r->ground[i];
+    fp.sky[i] = r->sky[i];
+  }
+  fp.width = r->info.width;
+  fp.height = r->info.height;
+  fp.res_x = (float)r->info.width;
+  fp.res_y = (float)r->info.height;
+  fp.max_depth = r->info.max_depth;
+  fp.rr_depth = r->info.rr_depth;
+  fp.frame_index = (uint32_t)r->frame_index;
+  fp.camera_index = camera_index;
+  fp.env_type = r->env_type;
+  fp.env_w = r->env_w;
+  fp.env_h = r->env_h;
+  fp.env_total_sum = r->env_total;
+  fp.env_rotation = r->env_rotation;
+  fp.env_intensity = r->env_intensity;
+  fp.exposure_value = r->info.exposure_value;
+  fp.enable_tonemap = r->info.enable_tonemap;
+  fp.enable_aces = r->info.enable_aces;
+  fp.use_simple_aces = r->info.use_simple_aces;
+  fp.num_lights = r->num_lights;
+  fp.tan_half_fov = std::tan(r->cameras[camera_index].yfov * 0.5f);
+  return fp;
+}
+
+void render_frame(hpt_renderer* r, uint32_t camera_index) {
+  const SceneView sv = scene_view(r);
+  const FrameParams fp = frame_params(r, camera_index);
+  const PathState ps = path_state(r);
+  const uint32_t n = r->npaths;
+  uint32_t* counts = r->d_counts.as<uint32_t>();
+  uint32_t* qbuf[2] = {r->d_qa.as<uint32_t>(), r->d_qb.as<uint32_t>()};
+  ShadowQueue sq{r->d_sh_o.as<float4>(), r->d_sh_d.as<float4>(), r->d_sh_c.as<float4>()};
+  const Spill sp = ensure_spill(r, round_threads(2 * n));
+  hipStream_t s = r->stream;
+  StageTimer* t;
+  auto* stats = r->profiling ? r->d_stats.as<unsigned long long>() : nullptr;
+  const bool count = (r->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
+  const StageCounters sc_extend{stats ? stats + 3 * HPT_STAGE_EXTEND : nullptr, count};
+  const StageCounters sc_shadow{stats ? stats + 3 * HPT_STAGE_SHADOW : nullptr, count};
+
+  stage_begin(r, HPT_STAGE_CAMERA, t);
+  launch_camera(s, sv, fp, ps, n);
+  stage_end(r, t);
+  check(hipGetLastError(), "camera launch");
+
+  // counts[0], counts[1]: path queues (ping-pong); counts[2]: shadow queue.
+  const uint32_t* qin = nullptr;
+  const uint32_t* qin_count = nullptr;
+  int out = 0;
+  const bool transparent = (r->info.features & HPT_FEATURE_TRANSPARENT) != 0;
+  const uint32_t fixed_bounces = r->info.max_depth + 1;
+  for (uint32_t b = 0;; ++b) {
+    if (b >= fixed_bounces) {
+      // Pass-through of transparent surfaces does not count toward depth
+      // (pathtracer.rgen.glsl:199): keep going while paths remain.
+      if (!transparent) break;
+      uint32_t live = 0;
+      check(hipMemcpyAsync(&live, qin_count, 4, hipMemcpyDeviceToHost, s), "queue count");
+      check(hipStreamSynchronize(s), "sync");
+      if (live == 0 || b > 4096) break;
+    }
+    uint32_t* qout = qbuf[out];
+    uint32_t* qout_count = counts + out;
+    stage_begin(r, HPT_STAGE_EXTEND, t);
+    launch_extend(s, sv, ps, qin, qin_count, n, qout_count, counts + 2, sp, sc_extend);
+    stage_end(r, t);
+    stage_begin(r, HPT_STAGE_SHADE, t);
+    launch_shade(s, r->info.features, sv, fp, ps, qin, qin_count, n, qout, qout_count, sq, counts + 2, sp);
+    stage_end(r, t);
+    const bool vmis = (r->info.features & HPT_FEATURE_MEDIUM) && (r->info.features & HPT_FEATURE_VOL_MIS);
+    if (!vmis) {
+      stage_begin(r, HPT_STAGE_SHADOW, t);
+      launch_shadow(s, sv, ps, sq, counts + 2, 2 * n, sp, sc_shadow);
+      stage_end(r, t);
+    }
+    check(hipGetLastError(), "bounce launch");
+    qin = qout;
+    qin_count = qout_count;
+    out ^= 1;
+  }
+  Images im{r->d_accum.as<float4>(), r->d_albedo.as<float4>(), r->d_normal.as<float4>(), r->d_final.as<uchar4>()};
+  stage_begin(r, HPT_STAGE_ACCUMULATE, t);
+  launch_accumulate(s, fp, ps, im, n);
+  stage_end(r, t);
+  check(hipGetLastError(), "accumulate launch");
+}
+
+void require_committed(const hpt_renderer* r) {
+  need(r->committed, "hpt_commit has not been called since the last scene change", HPT_ERROR_STATE);
+}
+
+}  // namespace
+
+extern "C" {
+
+hpt_status hpt_create(const hpt_create_info* info, hpt_renderer** out) {
+  return guarded([&] {
+    need(info && out, "null argument");
+    *out = nullptr;
+    need(info->width > 0 && info->height > 0, "width and height must be non-zero");
+    need((uint64_t)info->width * info->height <= (1u << 30), "image too large");
+    need(info->rr_depth <= info->max_depth, "the Russian Roulette depth is greater than the max depth");
+    need((info->features & ~7u) == 0, "unknown feature bits");
+    int ndev = 0;
+    check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    need(info->device >= 0 && info->device < ndev, "no such HIP device", HPT_ERROR_DEVICE);
+    auto* r = new hpt_renderer();
+    r->info = *info;
+    try {
+      set_device(r);
+      check(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking), "hipStreamCreate");
+      default_blue_noise(r->blue_noise);
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    *out = r;
+  });
+}
+
+hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
+  return guarded([&] {
+    need(r && d, "null argument");
+    need(d->camera_count >= 1 && d->camera_count <= HPT_MAX_CAMERAS && d->cameras, "1..8 cameras required");
+    need(d->light_count <= HPT_MAX_LIGHTS, "at most 16 lights");
+    need(d->light_count == 0 || d->lights, "null lights");
+    need(d->material_count == 0 || d->materials, "null materials");
+    need(d->primitive_count == 0 || d->primitives, "null primitives");
+    for (uint32_t i = 0; i < d->camera_count; ++i)
+      need(d->cameras[i].type == HPT_CAMERA_PERSPECTIVE || d->cameras[i].type == HPT_CAMERA_ORTHOGRAPHIC,
+           "unknown camera type");
+    for (uint32_t i = 0; i < d->light_count; ++i)
+      need(d->lights[i].type >= HPT_LIGHT_POINT && d->lights[i].type <= HPT_LIGHT_SPHERE, "unknown light type");
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+      const hpt_material& m = d->materials[i];
+      need(m.type == HPT_MATERIAL_DIFFUSE || m.type == HPT_MATERIAL_DISNEY, "unknown material type");
+      need(m.medium.type <= HPT_MEDIUM_EMISSIVE, "unknown medium type");
+      need(m.base_color_map_index == HPT_INVALID_INDEX && m.normal_map_index == HPT_INVALID_INDEX &&
+               m.metallic_roughness_map_index == HPT_INVALID_INDEX && m.emission_map_index == HPT_INVALID_INDEX,
+           "texture maps are not supported by this build", HPT_ERROR_UNSUPPORTED);
+    }
+    std::vector<float> tri_pos;
+    std::vector<uint4> tri_idx;
+    std::vector<ShadeVertex> vtx;
+    std::vector<PrimInfo> prims;
+    uint64_t ntri = 0, nvtx = 0;
+    for (uint32_t p = 0; p < d->primitive_count; ++p) {
+      const hpt_primitive_desc& pd = d->primitives[p];
+      need(pd.index_count % 3 == 0, "index_count must be a multiple of 3");
+      need(pd.vertex_count == 0 || pd.vertices, "null vertices");
+      need(pd.index_count == 0 || pd.indices, "null indices");
+      need(pd.material_index == HPT_INVALID_INDEX || pd.material_index < d->material_count,
+           "material index out of range");
+      ntri += pd.index_count / 3;
+      nvtx += pd.vertex_count;
+    }
+    need(ntri < (1u << 31) && nvtx < (1u << 31), "too many triangles or vertices");
+    tri_pos.reserve(ntri * 9);
+    tri_idx.reserve(ntri);
+    vtx.reserve(nvtx);
+    uint32_t tri_base = 0;
+    for (uint32_t p = 0; p < d->primitive_count; ++p) {
+      const hpt_primitive_desc& pd = d->primitives[p];
+      const float* m = pd.transform;
+      // transpose(inverse(mat3(transform))) (triangles.rchit.glsl:21): for
+      // columns a, b, c its columns are cross(b,c), cross(c,a), cross(a,b) / det.
+      const f3 a = mk3(m[0], m[1], m[2]), b = mk3(m[4], m[5], m[6]), c = mk3(m[8], m[9], m[10]);
+      const float det = dot(a, cross(b, c));
+      const f3 i0 = cross(b, c) / det, i1 = cross(c, a) / det, i2 = cross(a, b) / det;
+      PrimInfo pi{};
+      const float it[9] = {i0.x, i0.y, i0.z, i1.x, i1.y, i1.z, i2.x, i2.y, i2.z};
+      std::memcpy(pi.it, it, sizeof it);
+      pi.material_index = pd.material_index;
+      pi.local_base = tri_base;
+      prims.push_back(pi);
+      const uint32_t vbase = (uint32_t)vtx.size();
+      std::vector<float> wp((size_t)pd.vertex_count * 3);
+      for (uint32_t k = 0; k < pd.vertex_count; ++k) {
+        const hpt_vertex& v = pd.vertices[k];
+        const float x = v.position[0], y = v.position[1], z = v.position[2];
+        // world = transform * vec4(p, 1), summed left to right.
+        wp[k * 3 + 0] = ((m[0] * x + m[4] * y) + m[8] * z) + m[12];
+        wp[k * 3 + 1] = ((m[1] * x + m[5] * y) + m[9] * z) + m[13];
+        wp[k * 3 + 2] = ((m[2] * x + m[6] * y) + m[10] * z) + m[14];
+        ShadeVertex sv;
+        std::memcpy(sv.n, v.normal, 12);
+        std::memcpy(sv.t, v.tangent, 12);
+        std::memcpy(sv.uv, v.tex_coord, 8);
+        vtx.push_back(sv);
+      }
+      for (uint32_t t = 0; t < pd.index_count / 3; ++t) {
+        uint32_t ix[3];
+        for (int k = 0; k < 3; ++k) {
+          ix[k] = pd.indices[t * 3 + k];
+          need(ix[k] < pd.vertex_count, "vertex index out of range");
+          for (int a2 = 0; a2 < 3; ++a2) tri_pos.push_back(wp[(size_t)ix[k] * 3 + a2]);
+        }
+        tri_idx.push_back(make_uint4(vbase + ix[0], vbase + ix[1], vbase + ix[2], p));
+      }
+      tri_base += pd.index_count / 3;
+    }
+    r->cameras.assign(d->cameras, d->cameras + d->camera_count);
+    r->cameras.resize(HPT_MAX_CAMERAS, hpt_camera{});  // unused slots zero-filled
+    r->lights.assign(HPT_MAX_LIGHTS + 1, hpt_light{});
+    for (uint32_t i = 0; i < d->light_count; ++i) r->lights[i] = d->lights[i];
+    r->num_lights = d->light_count;
+    r->materials.assign(d->materials, d->materials + d->material_count);
+    if (r->materials.empty()) r->materials.push_back(hpt_material{});
+    r->tri_pos.swap(tri_pos);
+    r->tri_idx.swap(tri_idx);
+    r->vtx.swap(vtx);
+    r->prims.swap(prims);
+    r->camera_count = d->camera_count;
+    r->scene_set = true;
+    r->committed = false;
+  });
+}
+
+hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t w, uint32_t h, float rotation) {
+  return guarded([&] {
+    need(r && rgb && w > 0 && h > 0, "invalid environment map");
+    need((uint64_t)w * h < (1ull << 31), "environment map too large");
+    const size_t n = (size_t)w * h;
+    std::vector<float4> texels(n);
+    std::vector<float> lum(n);
+    for (size_t i = 0; i < n; ++i) {
+      const float a = rgb[i * 3], b = rgb[i * 3 + 1], c = rgb[i * 3 + 2];
+      need(std::isfinite(a) && std::isfinite(b) && std::isfinite(c) && a >= 0 && b >= 0 && c >= 0,
+           "environment texels must be finite and non-negative");
+      texels[i] = make_float4(a, b, c, 0.0f);
+      lum[i] = luminance(a, b, c);
+    }
+    std::vector<float> prob;
+    std::vector<uint32_t> alias;
+    const double total = build_alias(lum, prob, alias);
+    std::vector<float4> table(n);
+    for (size_t i = 0; i < n; ++i) {
+      uint32_t al = alias[i];
+      float alf;
+      std::memcpy(&alf, &al, 4);
+      table[i] = make_float4(prob[i], alf, lum[i], lum[al]);
+    }
+    r->env_rgb.swap(texels);
+    r->env_table.swap(table);
+    r->env_total = (float)total;
+    r->env_w = w;
+    r->env_h = h;
+    r->env_rotation = rotation;
+    r->env_type = HPT_ENV_MAP;
+    if (r->committed) {
+      set_device(r);
+      upload(r->d_env_rgb, r->env_rgb, r->stream);
+      upload(r->d_env_table, r->env_table, r->stream);
+    }
+  });
+}
+
+hpt_status hpt_set_ground_color(hpt_renderer* r, const float rgba[4]) {
+  return guarded([&] {
+    need(r && rgba, "null argument");
+    for (int i = 0; i < 3; ++i) r->ground[i] = rgba[i];
+    r->env_type = HPT_ENV_SKY;
+  });
+}
+
+hpt_status hpt_set_sky_color(hpt_renderer* r, const float rgba[4]) {
+  return guarded([&] {
+    need(r && rgba, "null argument");
+    for (int i = 0; i < 3; ++i) r->sky[i] = rgba[i];
+    r->env_type = HPT_ENV_SKY;
+  });
+}
+
+hpt_status hpt_set_env_intensity(hpt_renderer* r, float v) {
+  return guarded([&] {
+    need(r, "null renderer");
+    r->env_intensity = v;
+  });
+}
+
+hpt_status hpt_set_exposure_value(hpt_renderer* r, float ev) {
+  return guarded([&] {
+    need(r, "null renderer");
+    r->info.exposure_value = ev;
+  });
+}
+
+hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t w, uint32_t h) {
+  return guarded([&] {
+    need(r && rgba8, "null argument");
+    need(w == 1024 && h == 1024, "the blue-noise texture must be 1024x1024 (rand4blue wraps at 1024)");
+    r->blue_noise.resize(1024 * 1024);
+    std::memcpy(r->blue_noise.data(), rgba8, 1024 * 1024 * 4);
+    if (r->committed) {
+      set_device(r);
+      upload(r->d_blue_noise, r->blue_noise, r->stream);
+    }
+  });
+}
+
+hpt_status hpt_commit(hpt_renderer* r) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(r->scene_set, "hpt_set_scene has not been called", HPT_ERROR_STATE);
+    set_device(r);
+    BvhBuildResult bvh;
+    std::string err;
+    need(build_bvh8(r->tri_pos.data(), (uint32_t)(r->tri_pos.size() / 9), bvh, err), err);
+    r->node_count = (uint32_t)bvh.nodes.size();
+    r->leaf_tri_count = (uint32_t)bvh.tris.size();
+    r->bvh_depth = bvh.depth;
+    hipStream_t s = r->stream;
+    upload(r->d_nodes, bvh.nodes, s);
+    upload(r->d_ltris, bvh.tris, s);
+    upload(r->d_tri_idx, r->tri_idx, s);
+    upload(r->d_vtx, r->vtx, s);
+    upload(r->d_prims, r->prims, s);
+    upload(r->d_materials, r->materials, s);
+    upload(r->d_lights, r->lights, s);
+    upload(r->d_cameras, r->cameras, s);
+    if (r->env_rgb.empty()) {  // keep valid pointers for the sky case
+      r->env_rgb.assign(1, make_float4(0, 0, 0, 0));
+      r->env_table.assign(1, make_float4(1, 0, 0, 0));
+    }
+    upload(r->d_env_rgb, r->env_rgb, s);
+    upload(r->d_env_table, r->env_table, s);
+    upload(r->d_blue_noise, r->blue_noise, s);
+    const uint32_t n = r->info.width * r->info.height;
+    r->npaths = n;
+    const size_t f4 = (size_t)n * 16;
+    r->d_rng.alloc(f4);
+    r->d_org.alloc(f4);
+    r->d_dir.alloc(f4);
+    r->d_thr.alloc(f4);
+    r->d_rad.alloc(f4);
+    r->d_alb.alloc(f4);
+    r->d_nrm.alloc(f4);
+    r->d_med.alloc(f4);
+    r->d_med2.alloc((size_t)n * 8);
+    r->d_nee_e.alloc(f4);
+    r->d_nee_l.alloc(f4);
+    r->d_hit.alloc(f4);
+    r->d_qa.alloc((size_t)n * 4);
+    r->d_qb.alloc((size_t)n * 4);
+    r->d_sh_o.alloc(2 * f4);
+    r->d_sh_d.alloc(2 * f4);
+    r->d_sh_c.alloc(2 * f4);
+    r->d_counts.alloc(64);
+    check(hipMemsetAsync(r->d_counts.p, 0, 64, s), "hipMemsetAsync");
+    r->d_accum.alloc(f4);
+    r->d_albedo.alloc(f4);
+    r->d_normal.alloc(f4);
+    r->d_final.alloc((size_t)n * 4);
+    check(hipMemsetAsync(r->d_accum.p, 0, f4, s), "hipMemsetAsync");
+    check(hipMemsetAsync(r->d_albedo.p, 0, f4, s), "hipMemsetAsync");
+    check(hipMemsetAsync(r->d_normal.p, 0, f4, s), "hipMemsetAsync");
+    check(hipMemsetAsync(r->d_final.p, 0, (size_t)n * 4, s), "hipMemsetAsync");
+    check(hipStreamSynchronize(s), "commit");
+    r->frame_index = 0;
+    r->committed = true;
+  });
+}
+
+hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
+  return guarded([&] {
+    need(r, "null renderer");
+    require_committed(r);
+    need(camera_index < r->camera_count, "camera index out of range");
+    set_device(r);
+    for (uint32_t f = 0; f < frames; ++f) {
+      if (r->info.max_samples && r->frame_index >= r->info.max_samples) break;
+      render_frame(r, camera_index);
+      r->frame_index++;
+    }
+    if (r->profiling & 1) collect_timers(r);
+  });
+}
+
+hpt_status hpt_reset_accumulation(hpt_renderer* r) {
+  return guarded([&] {
+    need(r, "null renderer");
+    r->frame_index = 0;
+  });
+}
+
+hpt_status hpt_get_frame_index(hpt_renderer* r, uint64_t* fi) {
+  return guarded([&] {
+    need(r && fi, "null argument");
+    *fi = r->frame_index;
+  });
+}
+
+hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t bytes) {
+  return guarded([&] {
+    need(r && dst, "null argument");
+    require_committed(r);
+    set_device(r);
+    const DevBuf* b = nullptr;
+    switch (which) {
+      case HPT_IMAGE_FINAL_RGBA8: b = &r->d_final; break;
+      case HPT_IMAGE_ACCUM_RGBA32F: b = &r->d_accum; break;
+      case HPT_IMAGE_ALBEDO_RGBA32F: b = &r->d_albedo; break;
+      case HPT_IMAGE_NORMAL_RGBA32F: b = &r->d_normal; break;
+      default: need(false, "unknown image");
+    }
+    need(bytes == b->bytes, "destination size does not match the image size");
+    check(hipMemcpyAsync(dst, b->p, bytes, hipMemcpyDeviceToHost, r->stream), "read image");
+    check(hipStreamSynchronize(r->stream), "read image");
+  });
+}
+
+static void trace_api(hpt_renderer* r, const hpt_ray* rays, uint32_t n, int any, std::vector<float4>& out,
+                      uint32_t* nvis = nullptr, uint32_t* ntri = nullptr) {
+  set_device(r);
+  require_committed(r);
+  out.resize(n);
+  if (n == 0) return;
+  DevBuf dr, dout, dcnt;
+  dr.alloc((size_t)n * sizeof(hpt_ray));
+  dout.alloc((size_t)n * sizeof(float4));
+  uint32_t* dv = nullptr;
+  uint32_t* dt = nullptr;
+  if (nvis && ntri) {
+    dcnt.alloc((size_t)n * 8);
+    dv = dcnt.as<uint32_t>();
+    dt = dv + n;
+  }
+  check(hipMemcpyAsync(dr.p, rays, (size_t)n * sizeof(hpt_ray), hipMemcpyHostToDevice, r->stream), "rays");
+  const Spill sp = ensure_spill(r, round_threads(n));
+  launch_trace_rays(r->stream, scene_view(r), dr.as<hpt_ray>(), n, any, dout.as<float4>(), dv, dt, sp);
+  check(hipGetLastError(), "trace launch");
+  check(hipMemcpyAsync(out.data(), dout.p, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, r->stream), "hits");
+  if (dv) {
+    check(hipMemcpyAsync(nvis, dv, (size_t)n * 4, hipMemcpyDeviceToHost, r->stream), "counts");
+    check(hipMemcpyAsync(ntri, dt, (size_t)n * 4, hipMemcpyDeviceToHost, r->stream), "counts");
+  }
+  check(hipStreamSynchronize(r->stream), "trace");
+}
+
+hpt_status hpt_trace_counts(hpt_renderer* r, const hpt_ray* rays, uint32_t n, int any_hit, uint32_t* node_visits,
+                            uint32_t* tri_tests) {
+  return guarded([&] {
+    need(r && (n == 0 || (rays && node_visits && tri_tests)), "null argument");
+    std::vector<float4> out;
+    trace_api(r, rays, n, any_hit ? 1 : 0, out, node_visits, tri_tests);
+  });
+}
+
+hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint32_t n, hpt_hit* hits) {
+  return guarded([&] {
+    need(r && (n == 0 || (rays && hits)), "null argument");
+    std::vector<float4> out;
+    trace_api(r, rays, n, 0, out);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t id;
+      std::memcpy(&id, &out[i].w, 4);
+      hpt_hit& h = hits[i];
+      if (id == MISS_ID) {
+        h = hpt_hit{0.0f, 0.0f, 0.0f, HPT_HIT_NONE, HPT_INVALID_INDEX, HPT_INVALID_INDEX};
+      } else if (id & LIGHT_BIT) {
+        h = hpt_hit{out[i].x, 0.0f, 0.0f, HPT_HIT_LIGHT, id & ~LIGHT_BIT, HPT_INVALID_INDEX};
+      } else {
+        const uint32_t prim = r->tri_idx[id].w;
+        h = hpt_hit{out[i].x, out[i].y, out[i].z, HPT_HIT_TRIANGLE, prim, id - r->prims[prim].local_base};
+      }
+    }
+  });
+}
+
+hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t n, uint32_t* occluded) {
+  return guarded([&] {
+    need(r && (n == 0 || (rays && occluded)), "null argument");
+    std::vector<float4> out;
+    trace_api(r, rays, n, 1, out);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t v;
+      std::memcpy(&v, &out[i].w, 4);
+      occluded[i] = v;
+    }
+  });
+}
+
+hpt_status hpt_wait_idle(hpt_renderer* r) {
+  return guarded([&] {
+    need(r, "null renderer");
+    set_device(r);
+    check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+  });
+}
+
+void hpt_destroy(hpt_renderer* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->info.device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  for (auto& t : r->timers) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+const char* hpt_last_error(void) { return g_last_error.c_str(); }
+
+hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info) {
+  return guarded([&] {
+    need(r && info, "null argument");
+    require_committed(r);
+    info->node_count = r->node_count;
+    info->triangle_count = r->leaf_tri_count;
+    info->max_depth = r->bvh_depth;
+    info->_pad = 0;
+  });
+}
+
+hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes, void* tris, size_t tri_bytes) {
+  return guarded([&] {
+    need(r, "null renderer");
+    require_committed(r);
+    need(node_bytes == (size_t)r->node_count * sizeof(Node8), "node buffer size mismatch");
+    need(tri_bytes == (size_t)r->leaf_tri_count * sizeof(LeafTri), "triangle buffer size mismatch");
+    set_device(r);
+    if (node_bytes) check(hipMemcpy(nodes, r->d_nodes.p, node_bytes, hipMemcpyDeviceToHost), "export nodes");
+    if (tri_bytes) check(hipMemcpy(tris, r->d_ltris.p, tri_bytes, hipMemcpyDeviceToHost), "export tris");
+  });
+}
+
+hpt_status hpt_set_profiling(hpt_renderer* r, int enable) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need((enable & ~3) == 0, "unknown profiling bits");
+    set_device(r);
+    if (r->timers_used) collect_timers(r);
+    r->profiling = enable;
+    for (auto& s : r->stats) s = hpt_stage_stats{};
+    r->timers_used = 0;
+    if (enable) {
+      r->d_stats.alloc(3 * HPT_STAGE_COUNT * sizeof(unsigned long long));
+      check(hipMemsetAsync(r->d_stats.p, 0, r->d_stats.bytes, r->stream), "hipMemsetAsync");
+      check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+    }
+  });
+}
+
+hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out) {
+  return guarded([&] {
+    need(r && out, "null argument");
+    need(stage >= 0 && stage < HPT_STAGE_COUNT, "unknown stage");
+    set_device(r);
+    if (r->timers_used) collect_timers(r);
+    *out = r->stats[stage];
+    if (r->profiling && r->d_stats.p) {
+      unsigned long long v[3 * HPT_STAGE_COUNT];
+      check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+      check(hipMemcpy(v, r->d_stats.p, sizeof v, hipMemcpyDeviceToHost), "stats");
+      out->rays = v[3 * stage];
+      out->node_visits = v[3 * stage + 1];
+      out->tri_tests = v[3 * stage + 2];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// dev_traverse.h — ray queries on gfx950: the watertight ray/triangle test
+// and the walk of the 8-wide compressed BVH, plus the analytic light shapes
+// that the reference puts in its acceleration structure as procedural AABBs
+// (lights.rint.glsl:16-37).
+//
+// Query semantics (the reference leaves them to the Vulkan driver):
+//  * triangles are double-sided (gl_RayFlagsOpaqueEXT, no culling);
+//  * closest hit = minimum of (t, global triangle id) over t in [tmin, tmax];
+//    an analytic light replaces it only when strictly closer;
+//  * any hit = any triangle or analytic light on [tmin, tmax].
+#pragma once
+
+#include "common.h"
+
+namespace hpt {
+
+// ----------------------------------------------------------------- triangle
+struct RayPre {
+  f3 org, dir;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ RayPre prepare_ray(f3 org, f3 dir) {
+#pragma clang fp contract(off)
+  RayPre r;
+  r.org = org;
+  r.dir = dir;
+  const float ax = fabsf(dir.x), ay = fabsf(dir.y), az = fabsf(dir.z);
+  int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  const float dz = comp(dir, kz);
+  if (dz < 0.0f) {
+    int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = comp(dir, kx) / dz;
+  r.Sy = comp(dir, ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+
+// Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1)
+// 2013; double-sided, with the paper's double-precision recomputation when an
+// edge function is exactly 0.  t = T/det, (u, v) = weights of B and C.
+__device__ __forceinline__ bool tri_test(const RayPre& r, f3 A0, f3 B0, f3 C0, float& t, float& u,
+                                         float& v) {
+#pragma clang fp contract(off)
+  const f3 A = A0 - r.org, B = B0 - r.org, C = C0 - r.org;
+  const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  const float Ax = comp(A, r.kx) - r.Sx * Akz;
+  const float Ay = comp(A, r.ky) - r.Sy * Akz;
+  const float Bx = comp(B, r.kx) - r.Sx * Bkz;
+  const float By = comp(B, r.ky) - r.Sy * Bkz;
+  const float Cx = comp(C, r.kx) - r.Sx * Ckz;
+  const float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz;
+  const float Bz = r.Sz * Bkz;
+  const float Cz = r.Sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  u = V / det;
+  v = W / det;
+  return true;
+}
+
+// ------------------------------------------------------------ analytic lights
+// common/intersection.glsl:6-33
+__device__ __forceinline__ float sphere_isect(float rad, f3 pos, f3 o, f3 d) {
+  const f3 op = pos - o;
+  const float b = dot(op, d);
+  float det = b * b - dot(op, op) + rad * rad;
+  if (det < 0.0f) return INF;
+  det = sqrtf(det);
+  const float t1 = b - det;
+  if (t1 > EPS) return t1;
+  const float t2 = b + det;
+  if (t2 > EPS) return t2;
+  return INF;
+}
+
+// inc/light/quad.glsl:42-60 with common/intersection.glsl:43-72.
+__device__ __forceinline__ float quad_light_isect(const hpt_light& L, f3 o, f3 d, float& pdf) {
+  const f3 lu = ld3(L.u), lv = ld3(L.v), lp = ld3(L.position);
+  const f3 normal = normalize(cross(lv, lu));
+  const float cos_theta = dot(-d, normal);
+  if (cos_theta < 0.0f) return INF;
+  const float pw = dot(normal, lp);
+  const f3 u = lu / dot(lu, lu);
+  const f3 v = lv / dot(lv, lv);
+  const float dt = dot(d, normal);
+  const float t = (pw - dot(normal, o)) / dt;
+  if (t > EPS) {
+    const f3 p = o + d * t;
+    const f3 vi = p - lp;
+    const float a1 = dot(u, vi);
+    if (a1 >= 0.0f && a1 <= 1.0f) {
+      const float a2 = dot(v, vi);
+      if (a2 >= 0.0f && a2 <= 1.0f) {
+        pdf = (t * t) / (L.area * cos_theta);
+        return t;
+      }
+    }
+  }
+  return INF;
+}
+
+// inc/light/sphere.glsl:50-66.
+__device__ __forceinline__ float sphere_light_isect(const hpt_light& L, f3 o, f3 d, float& pdf) {
+  const f3 lp = ld3(L.position);
+  const float dist = length(lp - o);
+  if (dist < L.radius) return INF;
+  const float t = sphere_isect(L.radius, lp, o, d);
+  if (t < 0.0f) return INF;
+  const f3 hit_pt = o + t * d;
+  const float cos_theta = dot(-d, normalize(hit_pt - lp));
+  pdf = (t * t) / (L.area * cos_theta * 0.5f);
+  return t;
+}
+
+__device__ __forceinline__ float light_isect(const hpt_light& L, f3 o, f3 d, float& pdf) {
+  pdf = 1.0f;
+  if (L.type == HPT_LIGHT_QUAD) return quad_light_isect(L, o, d, pdf);
+  if (L.type == HPT_LIGHT_SPHERE) return sphere_light_isect(L, o, d, pdf);
+  return INF;
+}
+
+// ------------------------------------------------------------------ BVH walk
+struct TraceResult {
+  float t, u, v;
+  uint32_t id;  // global triangle id, LIGHT_BIT|light, or MISS_ID
+};
+
+// Per-lane traversal stack: `lds` points at this lane's slot 0 and entries
+// are `lds_stride` uint2 apart; entries past kShortStack go to `spill`
+// (stride `spill_stride`).  Needed depth <= tree levels.
+
+struct LaneStack {
+  uint2* lds;
+  int lds_stride;
+  uint2* spill;
+  int spill_stride;
+  int sp;
+  __device__ __forceinline__ void push(uint2 e) {
+    if (sp < kShortStack) lds[sp * lds_stride] = e;
+    else spill[(sp - kShortStack) * spill_stride] = e;
+    sp++;
+  }
+  __device__ __forceinline__ uint2 pop() {
+    sp--;
+    return sp < kShortStack ? lds[sp * lds_stride] : spill[(sp - kShortStack) * spill_stride];
+  }
+};
+
+__device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+// Walks the 8-wide tree.  Node group = (base, hits << 8 | imask); hit bit b
+// stands for slot b ^ oct.  ANY: stop at the first triangle on [tmin, tmax].
+// NODES/TRIS count visits when non-null (parity and roofline accounting).
+template <bool ANY>
+__device__ __forceinline__ bool trace_triangles(const SceneView& sv, const RayPre& r, float tmin,
+                                                float tmax, TraceResult& res, LaneStack& st,
+                                                uint32_t* nodes_visited, uint32_t* tris_tested) {
+  if (sv.node_count == 0) return false;
+  const f3 o = r.org;
+  f3 inv;
+  {
+    float dx = fabsf(r.dir.x) < 1e-30f ? copysignf(1e-30f, r.dir.x) : r.dir.x;
+    float dy = fabsf(r.dir.y) < 1e-30f ? copysignf(1e-30f, r.dir.y) : r.dir.y;
+    float dz = fabsf(r.dir.z) < 1e-30f ? copysignf(1e-30f, r.dir.z) : r.dir.z;
+    inv = mk3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+  }
+  const uint32_t oct = (inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 4u : 0u);
+  float best_t = tmax;
+  uint32_t best_id = MISS_ID;
+  float bu = 0.0f, bv = 0.0f;
+  bool found = false;
+  uint32_t nv = 0, nt = 0;
+  st.sp = 0;
+  uint2 grp = make_uint2(0u, ((1u << oct) << 8) | 1u);
+  for (;;) {
+    uint32_t hits = grp.y >> 8;
+    const uint32_t b = __builtin_ctz(hits);
+    hits &= hits - 1u;
+    const uint32_t slot = b ^ oct;
+    const uint32_t imask = grp.y & 0xFFu;
+    const uint32_t ni = grp.x + __builtin_popcount(imask & ((1u << slot) - 1u));
+    if (hits) st.push(make_uint2(grp.x, (hits << 8) | imask));
+
+    const uint4* np = reinterpret_cast<const uint4*>(sv.nodes + ni);
+    const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+    ++nv;
+    const f3 p = mk3(__uint_as_float(n0.x), __uint_as_float(n0.y), __uint_as_float(n0.z));
+    const float sx = __uint_as_float((n0.w & 0xFFu) << 23);
+    const float sy = __uint_as_float(((n0.w >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((n0.w >> 16) & 0xFFu) << 23);
+    const uint32_t nimask = n0.w >> 24;
+    // Bytes per child: qlo_x = n2.x|n2.y, qlo_y = n2.z|n2.w, qlo_z = n3.x|n3.y,
+    // qhi_x = n3.z|n3.w, qhi_y = n4.x|n4.y, qhi_z = n4.z|n4.w, meta = n1.z|n1.w.
+    uint32_t node_hits = 0, tri_hits = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = c & 3;
+      const bool hi4 = c >= 4;
+      const uint32_t meta = (hi4 ? n1.w : n1.z) >> (8 * k) & 0xFFu;
+      const float qlx = byte_f(hi4 ? n2.y : n2.x, k), qhx = byte_f(hi4 ? n3.w : n3.z, k);
+      const float qly = byte_f(hi4 ? n2.w : n2.z, k), qhy = byte_f(hi4 ? n4.y : n4.x, k);
+      const float qlz = byte_f(hi4 ? n3.y : n3.x, k), qhz = byte_f(hi4 ? n4.w : n4.z, k);
+      float tn, tf;
+      {
+#pragma clang fp contract(off)
+        const float lox = fmaf(qlx, sx, p.x), hix = fmaf(qhx, sx, p.x);
+        const float loy = fmaf(qly, sy, p.y), hiy = fmaf(qhy, sy, p.y);
+        const float loz = fmaf(qlz, sz, p.z), hiz = fmaf(qhz, sz, p.z);
+        float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+        float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+        float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+        if (oct & 1u) { float q = t0x; t0x = t1x; t1x = q; }
+        if (oct & 2u) { float q = t0y; t0y = t1y; t1y = q; }
+        if (oct & 4u) { float q = t0z; t0z = t1z; t1z = q; }
+        tn = fmaxf(fmaxf(tmin, t0x), fmaxf(t0y, t0z));
+        tf = fminf(fminf(best_t, t1x), fminf(t1y, t1z));
+        tf = tf * BOX_MARGIN;
+      }
+      const bool hit = meta != 0u && tn <= tf;
+      if (hit) {
+        if (nimask & (1u << c)) node_hits |= 1u << (c ^ oct);
+        else tri_hits |= (meta >> 5) << (meta & 31u);
+      }
+    }
+    grp = make_uint2(n1.x, (node_hits << 8) | nimask);
+    const uint32_t tri_base = n1.y;
+    while (tri_hits) {
+      const uint32_t tb = __builtin_ctz(tri_hits);
+      tri_hits &= tri_hits - 1u;
+      const float4* tp = reinterpret_cast<const float4*>(sv.ltris + tri_base + tb);
+      const float4 a = tp[0], bq = tp[1], cq = tp[2];
+      ++nt;
+      float t, u, v;
+      if (!tri_test(r, mk3(a.x, a.y, a.z), mk3(bq.x, bq.y, bq.z), mk3(cq.x, cq.y, cq.z), t, u, v)) continue;
+      if (!(t >= tmin && t <= best_t)) continue;
+      const uint32_t id = __float_as_uint(a.w);
+      if (ANY) {
+        found = true;
+        best_t = t;
+        best_id = id;
+        bu = u;
+        bv = v;
+        break;
+      }
+      if (found && !(t < best_t || id < best_id)) continue;
+      found = true;
+      best_t = t;
+      best_id = id;
+      bu = u;
+      bv = v;
+    }
+    if (ANY && found) break;
+    if ((grp.y >> 8) == 0u) {
+      if (st.sp == 0) break;
+      grp = st.pop();
+    }
+  }
+  if (nodes_visited) *nodes_visited = nv;
+  if (tris_tested) *tris_tested = nt;
+  if (found) {
+    res.t = best_t;
+    res.u = bu;
+    res.v = bv;
+    res.id = best_id;
+  }
+  return found;
+}
+
+// Closest hit over triangles and analytic lights.
+__device__ __forceinline__ TraceResult trace_closest(const SceneView& sv, f3 o, f3 d, float tmin,
+                                                     float tmax, LaneStack& st,
+                                                     uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
+  TraceResult res{0.0f, 0.0f, 0.0f, MISS_ID};
+  const RayPre r = prepare_ray(o, d);
+  bool found = trace_triangles<false>(sv, r, tmin, tmax, res, st, nv, nt);
+  for (uint32_t i = 0; i < sv.num_lights; ++i) {
+    const hpt_light& L = sv.lights[i];
+    if (L.type != HPT_LIGHT_QUAD && L.type != HPT_LIGHT_SPHERE) continue;
+    float pdf;
+    const float t = light_isect(L, o, d, pdf);
+    if (!(t >= tmin && t <= tmax)) continue;
+    if (found && !(t < res.t)) continue;
+    found = true;
+    res = TraceResult{t, 0.0f, 0.0f, LIGHT_BIT | i};
+  }
+  return res;
+}
+
+__device__ __forceinline__ bool trace_any(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
+                                          LaneStack& st, uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
+  TraceResult res;
+  const RayPre r = prepare_ray(o, d);
+  if (trace_triangles<true>(sv, r, tmin, tmax, res, st, nv, nt)) return true;
+  for (uint32_t i = 0; i < sv.num_lights; ++i) {
+    const hpt_light& L = sv.lights[i];
+    if (L.type != HPT_LIGHT_QUAD && L.type != HPT_LIGHT_SPHERE) continue;
+    float pdf;
+    const float t = light_isect(L, o, d, pdf);
+    if (t >= tmin && t <= tmax) return true;
+  }
+  return false;
+}
+
+}  // namespace hpt

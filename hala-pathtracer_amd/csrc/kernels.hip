@@ -1,0 +1,622 @@
+// kernels.hip — wavefront path tracing kernels for gfx950.
+//
+// One frame = camera -> (extend -> shade -> shadow) per bounce -> accumulate.
+// The bounce loop of pathtracer.rgen.glsl:55-257 is split at its traceRayEXT
+// calls: `extend` answers the closest-hit query for every live path, `shade`
+// runs one loop iteration (surface/medium interaction, NEE sampling, BSDF
+// sampling, Russian roulette) and appends the surviving paths and the shadow
+// rays to compacted queues (wave ballot + one atomic per wave), `shadow`
+// answers the any-hit queries and adds the unoccluded NEE terms.
+#include <hip/hip_runtime.h>
+
+#include "dev_shading.h"
+#include "dev_traverse.h"
+#include "kernels.h"
+
+namespace hpt {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends `val` for every lane with want == true; one atomic per wave.
+__device__ __forceinline__ void wave_append(bool want, uint32_t* count, uint32_t* out, uint32_t val) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (want) {
+    const uint64_t below = m & ((1ull << lane_id()) - 1ull);
+    out[base + (uint32_t)__popcll(below)] = val;
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_reserve(bool want, uint32_t* count) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// ------------------------------------------------------------------ camera
+// pathtracer.rgen.glsl:23-54: seed, camera callable, payload initialisation.
+__global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp, PathState ps, uint32_t npaths) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npaths) return;
+  const uint32_t x = p % fp.width, y = p / fp.width;
+  f3 o, d;
+  camera_ray(sv, fp, x, y, o, d);
+  ps.rng[p] = make_uint4(x, y, fp.frame_index, x + y);
+  ps.org[p] = make_float4(o.x, o.y, o.z, 1.0f);
+  ps.dir[p] = make_float4(d.x, d.y, d.z, __uint_as_float(0u));
+  ps.thr[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+  ps.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.albedo[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.normal[p] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+  ps.med[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.med2[p] = make_float2(1.0f, __uint_as_float(HPT_MEDIUM_NONE));
+  ps.nee_env[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.nee_light[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Adds per-lane node/triangle counts to the stage counters: a wave sum, then
+// one atomic per wave.
+__device__ __forceinline__ void add_visits(unsigned long long* stats, uint32_t nv, uint32_t nt) {
+  unsigned long long a = nv, b = nt;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(stats + 1, a);
+    atomicAdd(stats + 2, b);
+  }
+}
+
+// ------------------------------------------------------------------ extend
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_extend(SceneView sv, PathState ps, const uint32_t* q, const uint32_t* qcount,
+                                                   uint32_t n_max, uint32_t* zero0, uint32_t* zero1, Spill sp,
+                                                   unsigned long long* stats) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = q ? *qcount : n_max;
+  if (k == 0) {
+    if (zero0) *zero0 = 0;
+    if (zero1) *zero1 = 0;
+    if (stats) atomicAdd(stats, (unsigned long long)n);
+  }
+  uint32_t nv = 0, nt = 0;
+  if (k < n) {
+    const uint32_t p = q ? q[k] : k;
+    const float4 o4 = ps.org[p], d4 = ps.dir[p];
+    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st,
+                                        COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
+    ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+  }
+  if (COUNT) add_visits(stats, nv, nt);
+}
+
+// ------------------------------------------------------------------ shading
+struct HitState {  // the State fields triangles.rchit.glsl:66-75 writes
+  f3 pos, normal, ffnormal, tangent, bitangent;
+  uint32_t material;
+};
+
+// triangles.rchit.glsl:13-76.
+__device__ __forceinline__ HitState triangle_state(const SceneView& sv, uint32_t id, float t, float u, float v,
+                                                   f3 org, f3 dir) {
+  HitState h;
+  const uint4 ti = sv.tri_idx[id];
+  const PrimInfo& pi = sv.prims[ti.w];
+  const ShadeVertex a = sv.vtx[ti.x], b = sv.vtx[ti.y], c = sv.vtx[ti.z];
+  const float bx = 1.0f - u - v, by = u, bz = v;
+  const f3 c0 = mk3(pi.it[0], pi.it[1], pi.it[2]), c1 = mk3(pi.it[3], pi.it[4], pi.it[5]),
+           c2 = mk3(pi.it[6], pi.it[7], pi.it[8]);
+  f3 n = ld3(a.n) * bx + ld3(b.n) * by + ld3(c.n) * bz;
+  n = normalize(c0 * n.x + c1 * n.y + c2 * n.z);
+  f3 tg = ld3(a.t) * bx + ld3(b.t) * by + ld3(c.t) * bz;
+  tg = normalize(c0 * tg.x + c1 * tg.y + c2 * tg.z);
+  h.normal = n;
+  h.tangent = tg;
+  h.bitangent = cross(n, tg);
+  h.pos = org + dir * t;
+  h.ffnormal = dot(n, dir) <= 0.0f ? n : -n;
+  h.material = pi.material_index;
+  return h;
+}
+
+// inc/lighting.glsl:7-61 (MEDIUM + VOL_MIS): closest-hit steps through
+// transparent or refractive surfaces, Beer-Lambert on exit faces.
+template <bool TRANSP>
+__device__ f3 eval_transmittance(const SceneView& sv, const FrameParams& fp, uint4& rng, f3 o, f3 d, LaneStack& st) {
+  f3 tr = mk3(1.0f);
+  for (uint32_t depth = 0; depth < fp.max_depth; depth++) {
+    const TraceResult r = trace_closest(sv, o, d, 0.0f, WORLD_SIZE, st);
+    if (r.id == MISS_ID || (r.id & LIGHT_BIT)) break;
+    const HitState h = triangle_state(sv, r.id, r.t, r.u, r.v, o, d);
+    if (h.material == INVALID) return mk3(0.0f);
+    const Mat m = load_material(sv.materials + h.material);
+    const bool is_transparent = TRANSP ? (m.opacity == 0.0f || rnd(rng) > m.opacity) : false;
+    const bool is_refractive = (1.0f - m.metallic) * m.specular_transmission > 0.0f;
+    if (!(is_transparent || is_refractive)) return mk3(0.0f);
+    if (dot(d, h.normal) > 0.0f && m.medium_type != HPT_MEDIUM_NONE) {
+      const f3 color = m.medium_type == HPT_MEDIUM_ABSORB ? mk3(1.0f) - m.medium_color : mk3(1.0f);
+      const f3 e = -color * m.medium_density * r.t;
+      tr *= mk3(expf(e.x), expf(e.y), expf(e.z));
+    }
+    o = h.pos + d * EPS;
+  }
+  return tr;
+}
+
+struct ShadowOut {
+  bool valid;
+  f3 o, d, c;
+  float tmax;
+};
+
+// inc/lighting.glsl:69-244.  Without VOL_MIS the shadow queries are deferred:
+// each unoccluded-candidate term is returned with its ray.  With VOL_MIS the
+// transmittance walk runs inline (it draws from the path RNG between the
+// environment and light samples) and the sum is returned directly.
+template <bool MED, bool VMIS, bool TRANSP>
+__device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4& rng, const Mat& mat,
+                              const Frame& fr, f3 hit_pos, f3 ffn, f3 V, float aniso, bool is_surface, bool anb,
+                              f3 weight, ShadowOut& s_env, ShadowOut& s_light, LaneStack& st) {
+  f3 Ld = mk3(0.0f);
+  const f3 scatter_pos = hit_pos + ffn * EPS;
+  s_env.valid = false;
+  s_light.valid = false;
+  if (fp.env_type != HPT_ENV_SKY) {
+    f3 Li;
+    float light_pdf;
+    const f3 light_dir = sample_env_map(rng, sv, fp, Li, light_pdf);
+    if (light_pdf > 0.0f) {
+      if (VMIS) {
+        const f3 trm = eval_transmittance<TRANSP>(sv, fp, rng, scatter_pos, light_dir, st);
+        if (luminance(trm) > 0.0f) {
+          Li *= trm;
+          float pdf = 0.0f;
+          f3 f;
+          if (is_surface) {
+            f = bxdf_eval(anb, fr, mat, V, ffn, light_dir, pdf);
+          } else {
+            pdf = phase_hg(dot(V, light_dir), aniso);
+            f = mk3(pdf);
+          }
+          if (pdf > 0.0f) {
+            const float mis = power_heuristic(light_pdf, pdf);
+            if (mis > 0.0f) Ld += mis * Li * f / light_pdf;
+          }
+        }
+      } else {
+        float pdf;
+        const f3 f = bxdf_eval(anb, fr, mat, V, ffn, light_dir, pdf);
+        if (pdf > 0.0f) {
+          const float mis = power_heuristic(light_pdf, pdf);
+          if (mis > 0.0f) {
+            s_env.valid = true;
+            s_env.o = scatter_pos;
+            s_env.d = light_dir;
+            s_env.tmax = WORLD_SIZE;
+            s_env.c = weight * (mis * Li * f / light_pdf);
+          }
+        }
+      }
+    }
+  }
+  if (fp.num_lights > 0) {
+    // rand() may return 1.0 and pick index num_lights: the zero-filled slot
+    // after the last light, as in the reference's lights buffer.
+    const int li = (int)(rnd(rng) * (float)fp.num_lights);
+    const hpt_light& L = sv.lights[li];
+    LightSample ls;
+    light_sample(rng, fp.num_lights, hit_pos, L, ls);
+    const f3 light_dir = ls.direction;
+    const float light_pdf = ls.pdf;
+    f3 Li = ls.emission;
+    if (light_pdf > 0.0f && dot(light_dir, ls.normal) < 0.0f) {
+      if (VMIS) {
+        const f3 trm = eval_transmittance<TRANSP>(sv, fp, rng, scatter_pos, light_dir, st);
+        if (luminance(trm) > 0.0f) {
+          Li *= trm;
+          float pdf = 0.0f;
+          f3 f;
+          if (is_surface) {
+            f = bxdf_eval(anb, fr, mat, V, ffn, light_dir, pdf);
+          } else {
+            pdf = phase_hg(dot(V, light_dir), aniso);
+            f = mk3(pdf);
+          }
+          if (pdf > 0.0f) {
+            float mis = 1.0f;
+            if (L.area > 0.0f) mis = power_heuristic(light_pdf, pdf);
+            if (mis > 0.0f) Ld += mis * Li * f / light_pdf;
+          }
+        }
+      } else {
+        float pdf;
+        const f3 f = bxdf_eval(anb, fr, mat, V, ffn, light_dir, pdf);
+        if (pdf > 0.0f) {
+          float mis = 1.0f;
+          if (L.area > 0.0f) mis = power_heuristic(light_pdf, pdf);
+          if (mis > 0.0f) {
+            s_light.valid = true;
+            s_light.o = scatter_pos;
+            s_light.d = light_dir;
+            s_light.tmax = ls.dist - 2.0f * EPS;
+            s_light.c = weight * (mis * Li * f / light_pdf);
+          }
+        }
+      }
+    }
+  }
+  return Ld;
+}
+
+// One iteration of the bounce loop, pathtracer.rgen.glsl:70-256.
+template <bool MED, bool VMIS, bool TRANSP>
+__global__ void __launch_bounds__(kBlock)
+    k_shade(SceneView sv, FrameParams fp, PathState ps, const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max,
+            uint32_t* qout, uint32_t* qout_count, ShadowQueue sq, uint32_t* sq_count, Spill sp) {
+  __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = qin ? *qin_count : n_max;
+  const bool active = k < n;
+  bool cont = false;
+  ShadowOut s_env, s_light;
+  s_env.valid = false;
+  s_light.valid = false;
+  uint32_t p = 0;
+  if (active) {
+    p = qin ? qin[k] : k;
+    LaneStack st{lds_stack + (VMIS ? threadIdx.x : 0), kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p], h4 = ps.hit[p];
+    f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
+    float pdf = o4.w;
+    uint32_t depth = __float_as_uint(d4.w);
+    uint32_t flags = __float_as_uint(t4.w);
+    bool anb = (flags & PATH_ANB) != 0;
+    bool in_medium = (flags & PATH_IN_MEDIUM) != 0;
+    bool surface_scatter = (flags & PATH_SURFACE_SCATTER) != 0;
+    uint4 rng = ps.rng[p];
+    const float4 r4 = ps.rad[p];
+    f3 radiance = mk3(r4.x, r4.y, r4.z);
+    f3 med_color = mk3(0.0f);
+    float med_density = 0.0f, med_aniso = 1.0f;
+    uint32_t med_type = HPT_MEDIUM_NONE;
+    if (MED) {
+      const float4 m4 = ps.med[p];
+      const float2 m2 = ps.med2[p];
+      med_color = mk3(m4.x, m4.y, m4.z);
+      med_density = m4.w;
+      med_aniso = m2.x;
+      med_type = __float_as_uint(m2.y);
+    }
+    const uint32_t id = __float_as_uint(h4.w);
+    const float t_hit = h4.x;
+    constexpr bool kScatterFlag = MED && !VMIS;
+
+    if (id == MISS_ID) {
+      const f3 env = env_eval(sv, fp, depth, pdf, dir, kScatterFlag && !surface_scatter);
+      radiance += T * env;
+      if (depth == 0) {
+        ps.albedo[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        ps.normal[p] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
+      }
+    } else if (id & LIGHT_BIT) {
+      const hpt_light& L = sv.lights[id & ~LIGHT_BIT];
+      float light_pdf = 1.0f;
+      light_isect(L, org, dir, light_pdf);
+      float mis = 1.0f;
+      if (depth > 0) mis = power_heuristic(pdf, light_pdf);
+      if (kScatterFlag && !surface_scatter) mis = 1.0f;
+      radiance += mis * ld3(L.intensity) * T;
+    } else {
+      HitState h = triangle_state(sv, id, t_hit, h4.y, h4.z, org, dir);
+      if (h.material == INVALID) {
+        radiance = mk3(0.7f, 0.0f, 0.7f);
+        ps.nee_env[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.nee_light[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      } else {
+        const Mat m = load_material(sv.materials + h.material);
+        Frame fr;
+        fr.tangent = h.tangent;
+        fr.bitangent = h.bitangent;
+        fr.eta = dot(dir, h.normal) < 0.0f ? (1.0f / m.ior) : m.ior;
+        fr.depth = depth;
+        if (depth == 0) {
+          ps.albedo[p] = make_float4(m.base_color.x, m.base_color.y, m.base_color.z, 0.0f);
+          ps.normal[p] = make_float4(h.normal.x, h.normal.y, h.normal.z, 0.0f);
+        }
+        radiance += T * m.emission;
+        bool alive = true;
+        bool medium_sampled = false;
+        if (MED) {
+          if (!VMIS) surface_scatter = false;
+          if (in_medium) {
+            if (med_type == HPT_MEDIUM_ABSORB) {
+              const f3 e = -(1.0f - med_color) * t_hit * med_density;
+              T *= mk3(expf(e.x), expf(e.y), expf(e.z));
+            } else if (med_type == HPT_MEDIUM_EMISSIVE) {
+              radiance += med_color * t_hit * med_density * T;
+            } else {
+              const float scatter_dist = fminf(-logf(rnd(rng)) / med_density, t_hit);
+              medium_sampled = scatter_dist < t_hit;
+              if (medium_sampled) {
+                T *= med_color;
+                org = org + dir * scatter_dist;
+                h.pos = org;
+                const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, rng, m, fr, h.pos, h.ffnormal, -dir,
+                                                                  med_aniso, false, anb, T, s_env, s_light, st);
+                radiance += Ld * T;
+                const float r1 = rnd(rng);
+                const float r2 = rnd(rng);
+                const f3 sdir = sample_hg(-dir, med_aniso, r1, r2);
+                pdf = phase_hg(dot(-dir, sdir), med_aniso);
+                dir = sdir;
+              }
+            }
+          }
+        }
+        if (!medium_sampled) {
+          bool pass = false;
+          if (TRANSP) {
+            if (m.opacity == 0.0f || rnd(rng) > m.opacity) {
+              depth--;  // uint: wraps at depth 0 and the max_depth test ends the path
+              pass = true;
+            }
+          }
+          if (!pass) {
+            if (kScatterFlag) surface_scatter = true;
+            const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, rng, m, fr, h.pos, h.ffnormal, -dir, med_aniso,
+                                                              true, anb, T, s_env, s_light, st);
+            radiance += T * Ld;
+            f3 L;
+            float spdf;
+            uint32_t sflags = 0;
+            const f3 f = bxdf_sample(rng, anb, fr, m, -dir, h.ffnormal, L, spdf, sflags);
+            dir = L;
+            pdf = spdf;
+            if (pdf > 0.0f) T *= f / pdf;
+            else alive = false;
+          }
+          if (alive) {
+            org = h.pos + dir * EPS;
+            if (MED) {
+              if (dot(dir, h.normal) < 0.0f && m.medium_type != HPT_MEDIUM_NONE) {
+                in_medium = true;
+                med_color = m.medium_color;
+                med_density = m.medium_density;
+                med_aniso = m.medium_anisotropy;
+                med_type = m.medium_type;
+              } else {
+                in_medium = false;
+                med_type = HPT_MEDIUM_NONE;
+              }
+            }
+          }
+        }
+        if (alive && depth >= fp.max_depth) alive = false;
+        if (alive && depth >= fp.rr_depth && fp.rr_depth > 0) {
+          const float q = fminf(fmaxf(T.x, fmaxf(T.y, T.z)) + 0.001f, 0.95f);
+          if (rnd(rng) > q) alive = false;
+          else T /= q;
+        }
+        if (alive) {
+          cont = true;
+          depth++;
+          const uint32_t nf = (anb ? PATH_ANB : 0u) | (in_medium ? PATH_IN_MEDIUM : 0u) |
+                              (surface_scatter ? PATH_SURFACE_SCATTER : 0u);
+          ps.org[p] = make_float4(org.x, org.y, org.z, pdf);
+          ps.dir[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(depth));
+          ps.thr[p] = make_float4(T.x, T.y, T.z, __uint_as_float(nf));
+          if (MED) {
+            ps.med[p] = make_float4(med_color.x, med_color.y, med_color.z, med_density);
+            ps.med2[p] = make_float2(med_aniso, __uint_as_float(med_type));
+          }
+        }
+        ps.rng[p] = rng;
+      }
+    }
+    ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+  }
+  // Compact the survivors and the shadow rays.
+  wave_append(cont, qout_count, qout, p);
+  if (!VMIS) {
+    const bool w0 = s_env.valid, w1 = s_light.valid;
+    const uint32_t i0 = wave_reserve(w0, sq_count);
+    const uint32_t i1 = wave_reserve(w1, sq_count);
+    if (w0) {
+      sq.o[i0] = make_float4(s_env.o.x, s_env.o.y, s_env.o.z, s_env.tmax);
+      sq.d[i0] = make_float4(s_env.d.x, s_env.d.y, s_env.d.z, __uint_as_float(p << 1));
+      sq.c[i0] = make_float4(s_env.c.x, s_env.c.y, s_env.c.z, 0.0f);
+    }
+    if (w1) {
+      sq.o[i1] = make_float4(s_light.o.x, s_light.o.y, s_light.o.z, s_light.tmax);
+      sq.d[i1] = make_float4(s_light.d.x, s_light.d.y, s_light.d.z, __uint_as_float((p << 1) | 1u));
+      sq.c[i1] = make_float4(s_light.c.x, s_light.c.y, s_light.c.z, 0.0f);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ shadow
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, const uint32_t* sq_count,
+                                                   Spill sp, unsigned long long* stats) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = *sq_count;
+  if (k == 0 && stats) atomicAdd(stats, (unsigned long long)n);
+  uint32_t nv = 0, nt = 0;
+  if (k < n) {
+    const float4 o4 = sq.o[k], d4 = sq.d[k];
+    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    const bool occluded = trace_any(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, o4.w, st,
+                                    COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
+    if (!occluded) {
+      const uint32_t tag = __float_as_uint(d4.w);
+      const uint32_t p = tag >> 1;
+      const float4 c = sq.c[k];
+      float4* dst = (tag & 1u) ? ps.nee_light : ps.nee_env;
+      float4 a = dst[p];
+      a.x += c.x;
+      a.y += c.y;
+      a.z += c.z;
+      dst[p] = a;
+    }
+  }
+  if (COUNT) add_visits(stats, nv, nt);
+}
+
+// -------------------------------------------------------------- accumulate
+__device__ __forceinline__ float srgb(float x) {  // common/common.glsl:26-32
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ unsigned char unorm8(float x) {
+  const float c = fminf(fmaxf(x, 0.0f), 1.0f);
+  return (unsigned char)floorf(c * 255.0f + 0.5f);
+}
+__device__ __forceinline__ f3 row_mul(f3 v, f3 r0, f3 r1, f3 r2) { return mk3(dot(v, r0), dot(v, r1), dot(v, r2)); }
+
+// pathtracer.rgen.glsl:259-304 with common/tonemap.glsl.
+__global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState ps, Images im, uint32_t npaths) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= npaths) return;
+  const float4 r = ps.rad[p], e = ps.nee_env[p], l = ps.nee_light[p];
+  const f3 L = mk3(r.x, r.y, r.z) + mk3(e.x, e.y, e.z) + mk3(l.x, l.y, l.z);
+  const float4 a4 = ps.albedo[p], n4 = ps.normal[p];
+  const float w = 1.0f / ((float)fp.frame_index + 1.0f);
+  auto acc = [&](float4* img, f3 v) {
+    f3 nv = v;
+    if (fp.frame_index != 0) {
+      const float4 o = img[p];
+      nv = mix3(mk3(o.x, o.y, o.z), v, w);
+    }
+    img[p] = make_float4(nv.x, nv.y, nv.z, 1.0f);
+    return nv;
+  };
+  f3 c = acc(im.accum, L);
+  acc(im.albedo, mk3(a4.x, a4.y, a4.z));
+  acc(im.normal, mk3(n4.x, n4.y, n4.z));
+  c = c * powf(2.0f, fp.exposure_value);
+  if (fp.enable_tonemap) {
+    if (fp.enable_aces) {
+      if (fp.use_simple_aces) {
+        const float A = 2.51f, B = 0.03f, Y = 2.43f, D = 0.59f, E = 0.14f;
+        const f3 num = c * (A * c + B), den = c * (Y * c + D) + E;
+        c = num / den;
+        c = mk3(fminf(fmaxf(c.x, 0.f), 1.f), fminf(fmaxf(c.y, 0.f), 1.f), fminf(fmaxf(c.z, 0.f), 1.f));
+      } else {
+        c = row_mul(c, mk3(0.59719f, 0.35458f, 0.04823f), mk3(0.07600f, 0.90834f, 0.01566f),
+                    mk3(0.02840f, 0.13383f, 0.83777f));
+        const f3 aa = c * (c + 0.0245786f) - 0.000090537f;
+        const f3 bb = c * (0.983729f * c + 0.4329510f) + 0.238081f;
+        c = aa / bb;
+        c = row_mul(c, mk3(1.60475f, -0.53108f, -0.07367f), mk3(-0.10208f, 1.10813f, -0.00605f),
+                    mk3(-0.00327f, -0.07276f, 1.07602f));
+        c = mk3(fminf(fmaxf(c.x, 0.f), 1.f), fminf(fmaxf(c.y, 0.f), 1.f), fminf(fmaxf(c.z, 0.f), 1.f));
+      }
+    } else {
+      c = c * 1.0f / (1.0f + luminance(c) / 1.5f);
+    }
+  }
+  im.final_rgba8[p] = make_uchar4(unorm8(srgb(c.x)), unorm8(srgb(c.y)), unorm8(srgb(c.z)), 255);
+}
+
+// --------------------------------------------------------------- ray queries
+template <bool ANY>
+__global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sv, const hpt_ray* rays, uint32_t n, float4* out,
+                                                       uint32_t* nvis, uint32_t* ntri, Spill sp) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const hpt_ray r = rays[k];
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+  uint32_t a = 0, b = 0;
+  const f3 o = ld3(r.origin), d = ld3(r.direction);
+  if (ANY) {
+    const bool hit = trace_any(sv, o, d, r.tmin, r.tmax, st, &a, &b);
+    out[k] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(hit ? 1u : 0u));
+  } else {
+    const TraceResult t = trace_closest(sv, o, d, r.tmin, r.tmax, st, &a, &b);
+    out[k] = make_float4(t.t, t.u, t.v, __uint_as_float(t.id));
+  }
+  if (nvis) nvis[k] = a;
+  if (ntri) ntri[k] = b;
+}
+
+// ------------------------------------------------------------------ launchers
+static inline uint32_t grid_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps, uint32_t npaths) {
+  hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths)), dim3(kBlock), 0, s, sv, fp, ps, npaths);
+}
+
+void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const uint32_t* q, const uint32_t* qcount,
+                   uint32_t n_max, uint32_t* zero0, uint32_t* zero1, Spill sp, StageCounters sc) {
+  if (sc.count && sc.stats)
+    hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, q, qcount, n_max, zero0,
+                       zero1, sp, sc.stats);
+  else
+    hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, q, qcount, n_max, zero0,
+                       zero1, sp, sc.stats);
+}
+
+template <bool MED, bool VMIS, bool TRANSP>
+static void shade_variant(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                          const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max, uint32_t* qout,
+                          uint32_t* qout_count, const ShadowQueue& sq, uint32_t* sq_count, Spill sp) {
+  hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP>), dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, fp, ps, qin,
+                     qin_count, n_max, qout, qout_count, sq, sq_count, sp);
+}
+
+void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                  const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max, uint32_t* qout, uint32_t* qout_count,
+                  const ShadowQueue& sq, uint32_t* sq_count, Spill sp) {
+  const bool med = features & HPT_FEATURE_MEDIUM;
+  const bool vmis = med && (features & HPT_FEATURE_VOL_MIS);
+  const bool tr = features & HPT_FEATURE_TRANSPARENT;
+#define HPT_SHADE(M, V, T) shade_variant<M, V, T>(s, sv, fp, ps, qin, qin_count, n_max, qout, qout_count, sq, sq_count, sp)
+  if (!med) {
+    if (tr) HPT_SHADE(false, false, true);
+    else HPT_SHADE(false, false, false);
+  } else if (!vmis) {
+    if (tr) HPT_SHADE(true, false, true);
+    else HPT_SHADE(true, false, false);
+  } else {
+    if (tr) HPT_SHADE(true, true, true);
+    else HPT_SHADE(true, true, false);
+  }
+#undef HPT_SHADE
+}
+
+void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq,
+                   const uint32_t* sq_count, uint32_t n_max, Spill sp, StageCounters sc) {
+  if (sc.count && sc.stats)
+    hipLaunchKernelGGL(k_shadow<true>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, sq, sq_count, sp, sc.stats);
+  else
+    hipLaunchKernelGGL(k_shadow<false>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, sq, sq_count, sp, sc.stats);
+}
+
+void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im, uint32_t npaths) {
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npaths)), dim3(kBlock), 0, s, fp, ps, im, npaths);
+}
+
+void launch_trace_rays(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, int any_hit, float4* out,
+                       uint32_t* node_visits, uint32_t* tri_tests, Spill sp) {
+  if (any_hit)
+    hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, sv, rays, n, out, node_visits,
+                       tri_tests, sp);
+  else
+    hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, sv, rays, n, out, node_visits,
+                       tri_tests, sp);
+}
+
+}  // namespace hpt

@@ -1,0 +1,433 @@
+"""Python binding of the MI355X path tracer's C ABI (include/hpt.h).
+
+`HalaRenderer` mirrors the renderer interface the reference host drives
+(`hala_renderer::rt_renderer::HalaRenderer`, called from
+src/bin/hala-pathtracer.rs:128-285): the same method names, argument meaning
+and validation errors.  Scenes are plain numpy arrays in the std140 layouts
+of shaders/src/hala-pathtracer/inc/types.glsl.
+
+The library is the HIP build under hala-pathtracer_amd/lib/libhpt.so; there is
+no CPU fallback -- loading fails loudly when it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhpt.so")
+
+# ----------------------------------------------------------------- constants
+MAX_CAMERAS = 8
+MAX_LIGHTS = 16
+INVALID_INDEX = 0xFFFFFFFF
+WORLD_SIZE = 10000.0
+
+CAMERA_PERSPECTIVE, CAMERA_ORTHOGRAPHIC = 0, 1
+LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_QUAD, LIGHT_SPHERE = range(5)
+MEDIUM_NONE, MEDIUM_ABSORB, MEDIUM_SCATTER, MEDIUM_EMISSIVE = range(4)
+MATERIAL_DIFFUSE, MATERIAL_DISNEY = 0, 1
+FEATURE_MEDIUM, FEATURE_VOL_MIS, FEATURE_TRANSPARENT = 1, 2, 4
+IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
+HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
+STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE = range(5)
+PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
+
+STATUS_NAMES = {
+    0: "HPT_OK",
+    1: "HPT_ERROR_INVALID_ARGUMENT",
+    2: "HPT_ERROR_OUT_OF_MEMORY",
+    3: "HPT_ERROR_DEVICE",
+    4: "HPT_ERROR_STATE",
+    5: "HPT_ERROR_UNSUPPORTED",
+}
+
+# ------------------------------------------------------- std140 numpy dtypes
+CAMERA_DTYPE = np.dtype({
+    "names": ["position", "right", "up", "forward", "yfov", "focal_distance_or_xmag", "aperture_or_ymag", "type"],
+    "formats": [("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "<f4", "<f4", "<f4", "<u4"],
+    "offsets": [0, 16, 32, 48, 60, 64, 68, 72],
+    "itemsize": 80,
+})
+LIGHT_DTYPE = np.dtype({
+    "names": ["intensity", "position", "u", "v", "radius", "area", "type"],
+    "formats": [("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "<f4", "<f4", "<i4"],
+    "offsets": [0, 16, 32, 48, 60, 64, 68],
+    "itemsize": 80,
+})
+MATERIAL_DTYPE = np.dtype({
+    "names": ["medium_color", "medium_density", "medium_anisotropy", "medium_type",
+              "base_color", "opacity", "emission", "anisotropic", "metallic", "roughness", "subsurface",
+              "specular_tint", "sheen", "sheen_tint", "clearcoat", "clearcoat_roughness", "clearcoat_tint",
+              "specular_transmission", "ior", "ax", "ay", "base_color_map_index", "normal_map_index",
+              "metallic_roughness_map_index", "emission_map_index", "type"],
+    "formats": [("<f4", 3), "<f4", "<f4", "<u4", ("<f4", 3), "<f4", ("<f4", 3), "<f4", "<f4", "<f4", "<f4",
+                "<f4", "<f4", "<f4", "<f4", "<f4", ("<f4", 3), "<f4", "<f4", "<f4", "<f4", "<u4", "<u4",
+                "<u4", "<u4", "<u4"],
+    "offsets": [0, 12, 16, 20, 32, 44, 48, 60, 64, 68, 72, 76, 80, 84, 88, 92, 96, 108, 112, 116, 120, 124,
+                128, 132, 136, 140],
+    "itemsize": 144,
+})
+RAY_DTYPE = np.dtype({"names": ["origin", "tmin", "direction", "tmax"],
+                      "formats": [("<f4", 3), "<f4", ("<f4", 3), "<f4"], "offsets": [0, 12, 16, 28],
+                      "itemsize": 32})
+HIT_DTYPE = np.dtype({"names": ["t", "u", "v", "kind", "instance", "primitive"],
+                      "formats": ["<f4", "<f4", "<f4", "<u4", "<u4", "<u4"], "itemsize": 24})
+VERTEX_FLOATS = 11  # position, normal, tangent, tex_coord (types.glsl:86-98)
+
+
+def new_materials(n: int) -> np.ndarray:
+    """Materials with the reference's neutral defaults and no texture maps."""
+    m = np.zeros(n, MATERIAL_DTYPE)
+    m["opacity"] = 1.0
+    m["ior"] = 1.5
+    m["medium_anisotropy"] = 0.0
+    for k in ("base_color_map_index", "normal_map_index", "metallic_roughness_map_index", "emission_map_index"):
+        m[k] = INVALID_INDEX
+    m["clearcoat_tint"] = 1.0
+    return m
+
+
+# ------------------------------------------------------------- ctypes ABI
+class _Vertex(C.Structure):
+    _fields_ = [("data", C.c_float * 11)]
+
+
+class _PrimitiveDesc(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("material_index", C.c_uint32), ("vertex_count", C.c_uint32),
+                ("index_count", C.c_uint32), ("_pad", C.c_uint32), ("vertices", C.c_void_p),
+                ("indices", C.c_void_p)]
+
+
+class _SceneDesc(C.Structure):
+    _fields_ = [("cameras", C.c_void_p), ("camera_count", C.c_uint32), ("light_count", C.c_uint32),
+                ("lights", C.c_void_p), ("materials", C.c_void_p), ("material_count", C.c_uint32),
+                ("primitive_count", C.c_uint32), ("primitives", C.c_void_p)]
+
+
+class _CreateInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("enable_tonemap", C.c_uint32), ("enable_aces", C.c_uint32),
+                ("use_simple_aces", C.c_uint32), ("features", C.c_uint32), ("max_samples", C.c_uint64),
+                ("exposure_value", C.c_float), ("device", C.c_int32)]
+
+
+class _BvhInfo(C.Structure):
+    _fields_ = [("node_count", C.c_uint32), ("triangle_count", C.c_uint32), ("max_depth", C.c_uint32),
+                ("_pad", C.c_uint32)]
+
+
+class _StageStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("rays", C.c_uint64),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
+
+
+# Every exported entry point with its argument types (the declarations of
+# include/hpt.h).
+SIGNATURES = {
+    "hpt_create": [C.POINTER(_CreateInfo), C.POINTER(C.c_void_p)],
+    "hpt_set_scene": [C.c_void_p, C.POINTER(_SceneDesc)],
+    "hpt_set_env_map": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float],
+    "hpt_set_ground_color": [C.c_void_p, C.POINTER(C.c_float)],
+    "hpt_set_sky_color": [C.c_void_p, C.POINTER(C.c_float)],
+    "hpt_set_env_intensity": [C.c_void_p, C.c_float],
+    "hpt_set_exposure_value": [C.c_void_p, C.c_float],
+    "hpt_set_blue_noise": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32],
+    "hpt_commit": [C.c_void_p],
+    "hpt_render": [C.c_void_p, C.c_uint32, C.c_uint32],
+    "hpt_reset_accumulation": [C.c_void_p],
+    "hpt_get_frame_index": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "hpt_read_image": [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t],
+    "hpt_trace_closest": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "hpt_trace_any": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "hpt_wait_idle": [C.c_void_p],
+    "hpt_destroy": [C.c_void_p],
+    "hpt_last_error": [],
+    "hpt_get_bvh_info": [C.c_void_p, C.POINTER(_BvhInfo)],
+    "hpt_export_bvh": [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t],
+    "hpt_trace_counts": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p],
+    "hpt_set_profiling": [C.c_void_p, C.c_int],
+    "hpt_get_stage_stats": [C.c_void_p, C.c_int, C.POINTER(_StageStats)],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libhpt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        if name == "hpt_destroy":
+            fn.restype = None
+        elif name == "hpt_last_error":
+            fn.restype = C.c_char_p
+        else:
+            fn.restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+class HptError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise HptError(status, _lib.hpt_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data) if a.size else C.c_void_p(0)
+
+
+# ------------------------------------------------------------------ scenes
+@dataclass
+class Primitive:
+    """One mesh: vertices (N, 11) float32, indices (M, 3) uint32, a 4x4
+    world transform (row-major numpy; passed to the ABI column-major)."""
+    vertices: np.ndarray
+    indices: np.ndarray
+    material_index: int
+    transform: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32))
+
+
+@dataclass
+class Scene:
+    cameras: np.ndarray          # CAMERA_DTYPE
+    lights: np.ndarray           # LIGHT_DTYPE
+    materials: np.ndarray        # MATERIAL_DTYPE
+    primitives: list             # [Primitive]
+    env_rgb: np.ndarray | None = None   # (H, W, 3) float32, row 0 = +Z
+    env_rotation: float = 0.0
+    ground_color: tuple = (0.5, 0.5, 0.5)
+    sky_color: tuple = (0.5, 0.5, 0.5)
+    env_intensity: float = 1.0
+
+    @property
+    def triangle_count(self) -> int:
+        return int(sum(len(p.indices) for p in self.primitives))
+
+    def to_desc(self):
+        """(hpt_scene_desc, keep-alive list) for the C ABI."""
+        keep = []
+        cams = np.ascontiguousarray(self.cameras, CAMERA_DTYPE)
+        lights = np.ascontiguousarray(self.lights, LIGHT_DTYPE)
+        mats = np.ascontiguousarray(self.materials, MATERIAL_DTYPE)
+        keep += [cams, lights, mats]
+        prims = (_PrimitiveDesc * max(1, len(self.primitives)))()
+        for i, p in enumerate(self.primitives):
+            v = np.ascontiguousarray(p.vertices, np.float32).reshape(-1, VERTEX_FLOATS)
+            ix = np.ascontiguousarray(p.indices, np.uint32).reshape(-1)
+            t = np.ascontiguousarray(np.asarray(p.transform, np.float32).T).reshape(-1)  # column-major
+            keep += [v, ix, t]
+            prims[i].transform[:] = t.tolist()
+            prims[i].material_index = int(p.material_index) & 0xFFFFFFFF
+            prims[i].vertex_count = v.shape[0]
+            prims[i].index_count = ix.shape[0]
+            prims[i].vertices = v.ctypes.data
+            prims[i].indices = ix.ctypes.data
+        keep.append(prims)
+        d = _SceneDesc(cams.ctypes.data, len(cams), len(lights), lights.ctypes.data if len(lights) else None,
+                       mats.ctypes.data if len(mats) else None, len(mats), len(self.primitives),
+                       C.cast(prims, C.c_void_p) if self.primitives else None)
+        return d, keep
+
+    def world_triangles(self) -> np.ndarray:
+        """(T, 3, 3) float32 world-space vertices in global triangle order,
+        transformed like the library does (column-major, summed left to right)."""
+        out = []
+        for p in self.primitives:
+            v = np.asarray(p.vertices, np.float32).reshape(-1, VERTEX_FLOATS)[:, :3]
+            m = np.asarray(p.transform, np.float32)
+            x, y, z = v[:, 0], v[:, 1], v[:, 2]
+            w = np.stack([((m[r, 0] * x + m[r, 1] * y) + m[r, 2] * z) + m[r, 3] for r in range(3)], axis=1)
+            out.append(w[np.asarray(p.indices, np.int64).reshape(-1, 3)])
+        return np.concatenate(out).astype(np.float32) if out else np.zeros((0, 3, 3), np.float32)
+
+
+def make_rays(origins, directions, tmin=0.0, tmax=WORLD_SIZE) -> np.ndarray:
+    o = np.asarray(origins, np.float32).reshape(-1, 3)
+    d = np.asarray(directions, np.float32).reshape(-1, 3)
+    r = np.zeros(len(o), RAY_DTYPE)
+    r["origin"] = o
+    r["direction"] = d
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+# ---------------------------------------------------------------- renderer
+class HalaRenderer:
+    """The renderer the reference host creates at hala-pathtracer.rs:128-138.
+
+    The Vulkan-only calls (push_*_shader_with_file, the imgui context) have no
+    counterpart; the shader permutation is the `features` bit set
+    (hala-pathtracer.rs:101-114 chooses it from the scene).
+    """
+
+    def __init__(self, width: int, height: int, max_depth: int, rr_depth: int, enable_tonemap: bool = True,
+                 enable_aces: bool = True, use_simple_aces: bool = False, max_samples: int = 0,
+                 features: int = 0, exposure_value: float = 0.0, device: int = 0):
+        lib = load_library()
+        info = _CreateInfo(width, height, max_depth, rr_depth, int(enable_tonemap), int(enable_aces),
+                           int(use_simple_aces), features, max_samples, exposure_value, device)
+        h = C.c_void_p()
+        _check(lib.hpt_create(C.byref(info), C.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.width, self.height = width, height
+
+    # -- set-up (hala-pathtracer.rs:195-213)
+    def set_scene(self, scene: Scene) -> None:
+        d, keep = scene.to_desc()
+        _check(self._lib.hpt_set_scene(self._h, C.byref(d)))
+        del keep
+        if scene.env_rgb is not None:
+            self.set_envmap(scene.env_rgb, scene.env_rotation)
+        else:
+            self.set_ground_color(scene.ground_color)
+            self.set_sky_color(scene.sky_color)
+        self.set_env_intensity(scene.env_intensity)
+
+    def set_envmap(self, rgb: np.ndarray, rotation: float = 0.0) -> None:
+        a = np.ascontiguousarray(rgb, np.float32)
+        h, w = a.shape[:2]
+        _check(self._lib.hpt_set_env_map(self._h, _ptr(a), w, h, rotation))
+
+    def set_ground_color(self, rgb) -> None:
+        c = (C.c_float * 4)(*[float(x) for x in rgb][:3], 1.0)
+        _check(self._lib.hpt_set_ground_color(self._h, c))
+
+    def set_sky_color(self, rgb) -> None:
+        c = (C.c_float * 4)(*[float(x) for x in rgb][:3], 1.0)
+        _check(self._lib.hpt_set_sky_color(self._h, c))
+
+    def set_env_intensity(self, v: float) -> None:
+        _check(self._lib.hpt_set_env_intensity(self._h, v))
+
+    def set_exposure_value(self, ev: float) -> None:
+        _check(self._lib.hpt_set_exposure_value(self._h, ev))
+
+    def load_blue_noise_texture(self, rgba8: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba8, np.uint8)
+        _check(self._lib.hpt_set_blue_noise(self._h, _ptr(a), a.shape[1], a.shape[0]))
+
+    def commit(self) -> None:
+        _check(self._lib.hpt_commit(self._h))
+
+    # -- per frame (hala-pathtracer.rs:264-285)
+    def render(self, frames: int = 1, camera_index: int = 0) -> None:
+        _check(self._lib.hpt_render(self._h, frames, camera_index))
+
+    def reset_accumulation(self) -> None:
+        _check(self._lib.hpt_reset_accumulation(self._h))
+
+    @property
+    def frame_index(self) -> int:
+        v = C.c_uint64()
+        _check(self._lib.hpt_get_frame_index(self._h, C.byref(v)))
+        return v.value
+
+    def read_image(self, which: int) -> np.ndarray:
+        if which == IMAGE_FINAL_RGBA8:
+            out = np.empty((self.height, self.width, 4), np.uint8)
+        else:
+            out = np.empty((self.height, self.width, 4), np.float32)
+        _check(self._lib.hpt_read_image(self._h, which, _ptr(out), out.nbytes))
+        return out
+
+    def save_images(self, prefix: str) -> None:
+        """HalaRenderer::save_images (hala-pathtracer.rs:251): colour, albedo
+        and normal as PFM, the names denoise.py:11,18-19 looks for."""
+        for which, suffix in ((IMAGE_ACCUM, "color"), (IMAGE_ALBEDO, "albedo"), (IMAGE_NORMAL, "normal")):
+            write_pfm(f"{prefix}_{suffix}.pfm", self.read_image(which)[..., :3])
+
+    def wait_idle(self) -> None:
+        _check(self._lib.hpt_wait_idle(self._h))
+
+    # -- ray queries and introspection
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        _check(self._lib.hpt_trace_closest(self._h, _ptr(rays), len(rays), _ptr(hits)))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint32)
+        _check(self._lib.hpt_trace_any(self._h, _ptr(rays), len(rays), _ptr(occ)))
+        return occ
+
+    def trace_counts(self, rays: np.ndarray, any_hit: bool = False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        nv = np.zeros(len(rays), np.uint32)
+        nt = np.zeros(len(rays), np.uint32)
+        _check(self._lib.hpt_trace_counts(self._h, _ptr(rays), len(rays), int(any_hit), _ptr(nv), _ptr(nt)))
+        return nv, nt
+
+    def bvh_info(self) -> dict:
+        info = _BvhInfo()
+        _check(self._lib.hpt_get_bvh_info(self._h, C.byref(info)))
+        return {"node_count": info.node_count, "triangle_count": info.triangle_count, "max_depth": info.max_depth}
+
+    def export_bvh(self):
+        info = self.bvh_info()
+        nodes = np.zeros(info["node_count"] * 80, np.uint8)
+        tris = np.zeros(info["triangle_count"] * 48, np.uint8)
+        _check(self._lib.hpt_export_bvh(self._h, _ptr(nodes), nodes.nbytes, _ptr(tris), tris.nbytes))
+        return nodes, tris
+
+    def set_profiling(self, flags: int) -> None:
+        _check(self._lib.hpt_set_profiling(self._h, flags))
+
+    def stage_stats(self, stage: int) -> dict:
+        s = _StageStats()
+        _check(self._lib.hpt_get_stage_stats(self._h, stage, C.byref(s)))
+        return {"launches": s.launches, "total_ms": s.total_ms, "rays": s.rays, "node_visits": s.node_visits,
+                "tri_tests": s.tri_tests}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.hpt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def write_pfm(path: str, rgb: np.ndarray) -> None:
+    """Little-endian colour PFM, bottom row first."""
+    a = np.ascontiguousarray(np.flipud(np.asarray(rgb, np.float32)))
+    with open(path, "wb") as f:
+        f.write(b"PF\n%d %d\n-1.0\n" % (a.shape[1], a.shape[0]))
+        f.write(a.astype("<f4").tobytes())
+
+
+def read_pfm(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(), "<f4" if scale < 0 else ">f4").reshape(h, w, 3)
+    return np.flipud(data).astype(np.float32)

@@ -1,0 +1,274 @@
+"""Seeded synthetic scenes for the path tracer's workloads and tests.
+
+Every generator is deterministic for a given seed.  Scenes are Z-up, the
+convention of the reference's environment lookup
+(shaders/src/hala-pathtracer/inc/env_map.glsl:22-26).
+
+Material parameter conversions the reference performs in its (unvendored)
+renderer are fixed here as follows:
+  * Disney ax, ay: aspect = sqrt(1 - 0.9*anisotropic), ax = max(0.001, r^2/aspect),
+    ay = max(0.001, r^2*aspect) (the 0.001 clamp cited at inc/bxdf/disney.glsl:420);
+  * diffuse roughness -> Oren-Nayar A, B in ax, ay with sigma = roughness*pi/2
+    (the formula kept as a comment at inc/bxdf/oren_nayar.glsl:37-43);
+  * "specular 0.5" -> ior 1.5 (F0 = 0.04 through disney.glsl:24-25).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import hpt
+
+# ------------------------------------------------------------------ helpers
+
+
+def look_at_camera(position, target, up=(0.0, 0.0, 1.0), yfov_deg=39.0, aperture=0.0, focal_distance=1.0):
+    cam = np.zeros(1, hpt.CAMERA_DTYPE)
+    p = np.asarray(position, np.float64)
+    f = np.asarray(target, np.float64) - p
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up, np.float64))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    cam["position"] = p
+    cam["forward"] = f
+    cam["right"] = r
+    cam["up"] = u
+    cam["yfov"] = math.radians(yfov_deg)
+    cam["focal_distance_or_xmag"] = focal_distance
+    cam["aperture_or_ymag"] = aperture
+    cam["type"] = hpt.CAMERA_PERSPECTIVE
+    return cam
+
+
+def diffuse_material(m, color, roughness=0.0, emission=(0, 0, 0)):
+    m["type"] = hpt.MATERIAL_DIFFUSE
+    m["base_color"] = color
+    m["roughness"] = roughness
+    m["emission"] = emission
+    sigma2 = (roughness * math.pi / 2) ** 2
+    m["ax"] = 1.0 - sigma2 / (2.0 * (sigma2 + 0.33))
+    m["ay"] = 0.45 * sigma2 / (sigma2 + 0.09)
+
+
+def disney_material(m, color, metallic=0.0, roughness=0.5, anisotropic=0.0, sheen=0.0, sheen_tint=0.5,
+                    clearcoat=0.0, clearcoat_roughness=0.1, specular_tint=0.0, subsurface=0.0,
+                    specular_transmission=0.0, ior=1.5):
+    m["type"] = hpt.MATERIAL_DISNEY
+    m["base_color"] = color
+    m["metallic"] = metallic
+    m["roughness"] = roughness
+    m["anisotropic"] = anisotropic
+    m["sheen"] = sheen
+    m["sheen_tint"] = sheen_tint
+    m["clearcoat"] = clearcoat
+    m["clearcoat_roughness"] = clearcoat_roughness
+    m["specular_tint"] = specular_tint
+    m["subsurface"] = subsurface
+    m["specular_transmission"] = specular_transmission
+    m["ior"] = ior
+    aspect = math.sqrt(1.0 - 0.9 * anisotropic)
+    m["ax"] = max(0.001, roughness * roughness / aspect)
+    m["ay"] = max(0.001, roughness * roughness * aspect)
+
+
+def quad_mesh(p0, e1, e2):
+    """Two triangles over the parallelogram p0 + s*e1 + t*e2; normal e1 x e2."""
+    p0, e1, e2 = (np.asarray(x, np.float64) for x in (p0, e1, e2))
+    n = np.cross(e1, e2)
+    n /= np.linalg.norm(n)
+    t = e1 / np.linalg.norm(e1)
+    v = np.zeros((4, hpt.VERTEX_FLOATS), np.float32)
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    for i, (s, q) in enumerate(corners):
+        v[i, 0:3] = p0 + s * e1 + q * e2
+        v[i, 3:6] = n
+        v[i, 6:9] = t
+        v[i, 9:11] = (s, q)
+    return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def cube_mesh():
+    """[-1,1]^3 with outward face normals: 24 vertices, 12 triangles."""
+    vs, ids = [], []
+    faces = [((-1, -1, -1), (0, 2, 0), (2, 0, 0)), ((-1, -1, 1), (2, 0, 0), (0, 2, 0)),
+             ((-1, -1, -1), (2, 0, 0), (0, 0, 2)), ((-1, 1, -1), (0, 0, 2), (2, 0, 0)),
+             ((-1, -1, -1), (0, 0, 2), (0, 2, 0)), ((1, -1, -1), (0, 2, 0), (0, 0, 2))]
+    for p0, e1, e2 in faces:
+        v, ix = quad_mesh(p0, e1, e2)
+        ids.append(ix + len(vs) * 4)
+        vs.append(v)
+    return np.concatenate(vs), np.concatenate(ids)
+
+
+def trs(translation, rotation_xyzw, scale):
+    x, y, z, w = rotation_xyzw
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    m = np.eye(4)
+    m[:3, :3] = r @ np.diag(scale)
+    m[:3, 3] = translation
+    return m.astype(np.float32)
+
+
+def uv_sphere_mesh(center, radius, segments=64, rings=32):
+    """UV sphere with fan poles: segments*(rings-2)*2 + 2*segments triangles."""
+    cx, cy, cz = center
+    verts = []
+    for j in range(rings + 1):
+        th = math.pi * j / rings
+        for i in range(segments + 1):
+            ph = 2 * math.pi * i / segments
+            n = (math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th))
+            t = (-math.sin(ph), math.cos(ph), 0.0)
+            verts.append((cx + radius * n[0], cy + radius * n[1], cz + radius * n[2], *n, *t,
+                          i / segments, j / rings))
+    v = np.array(verts, np.float32)
+    ids = []
+    row = segments + 1
+    for j in range(rings):
+        for i in range(segments):
+            a, b = j * row + i, j * row + i + 1
+            c, d = (j + 1) * row + i, (j + 1) * row + i + 1
+            if j == 0:
+                ids.append((a, c, d))
+            elif j == rings - 1:
+                ids.append((a, c, b))
+            else:
+                ids.append((a, c, d))
+                ids.append((a, d, b))
+    return v, np.array(ids, np.uint32)
+
+
+# ---------------------------------------------------------------- scenes
+
+def cornell_box(width=512, height=512):
+    """Procedural Cornell box: 5 wall quads, short and tall box (34 triangles),
+    Lambert white 0.73 / red (0.65,0.05,0.05) / green (0.12,0.45,0.15),
+    0.25x0.25 quad light of radiance (17,12,4) just below the ceiling, pinhole
+    camera with a 39 degree vertical field of view, black sky."""
+    mats = hpt.new_materials(3)
+    diffuse_material(mats[0], (0.73, 0.73, 0.73))
+    diffuse_material(mats[1], (0.65, 0.05, 0.05))
+    diffuse_material(mats[2], (0.12, 0.45, 0.15))
+    prims = []
+    walls = [((-1, -1, 0), (2, 0, 0), (0, 2, 0), 0),   # floor, normal +z
+             ((-1, -1, 2), (0, 2, 0), (2, 0, 0), 0),   # ceiling, normal -z
+             ((-1, 1, 0), (0, 0, 2), (2, 0, 0), 0),    # back, normal -y
+             ((-1, -1, 0), (0, 2, 0), (0, 0, 2), 1),   # left (red), normal +x
+             ((1, -1, 0), (0, 0, 2), (0, 2, 0), 2)]    # right (green), normal -x
+    for p0, e1, e2, mi in walls:
+        v, ix = quad_mesh(p0, e1, e2)
+        prims.append(hpt.Primitive(v, ix, mi))
+    cv, ci = cube_mesh()
+    # Node transforms of ShortBox / TallBox in the reference's assets/cornell-box.gltf.
+    short = trs((0.3286310136318207, -0.37459200620651245, 0.30000001192092896),
+                (0.8019842505455017, 0.5973451733589172, 0.0, 5.680963965914998e-08),
+                (0.29740551114082336, 0.30219709873199463, 0.30000001192092896))
+    tall = trs((-0.335438996553421, 0.2914150059223175, 0.6000000238418579),
+               (0.986013650894165, 0.166665181517601, 0.0, 7.563915005448507e-08),
+               (0.3036448657512665, 0.29886960983276367, 0.6000000238418579))
+    prims.append(hpt.Primitive(cv, ci, 0, short))
+    prims.append(hpt.Primitive(cv, ci, 0, tall))
+    lights = np.zeros(1, hpt.LIGHT_DTYPE)
+    lights["type"] = hpt.LIGHT_QUAD
+    lights["position"] = (-0.125, -0.125, 1.999)
+    lights["u"] = (0.25, 0.0, 0.0)
+    lights["v"] = (0.0, 0.25, 0.0)
+    lights["area"] = 0.0625
+    lights["intensity"] = (17.0, 12.0, 4.0)
+    cam = look_at_camera((0.0, -3.85, 1.0), (0.0, 0.0, 1.0), yfov_deg=39.0)
+    return hpt.Scene(cam, lights, mats, prims, ground_color=(0, 0, 0), sky_color=(0, 0, 0))
+
+
+def sky_env_map(width=2048, height=1024, seed=0, sun_elevation_deg=35.0, sun_azimuth=0.3,
+                sun_radiance=1.0e4, sun_diameter_deg=0.5):
+    """Lat-long sky: gradient from horizon white to zenith blue, a dim ground
+    below the horizon, lognormal(0, 0.5) texel noise and a sun disc."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    cz = np.cos(theta)[:, None]
+    t = np.clip(cz, 0.0, 1.0)
+    zenith = np.array([0.25, 0.45, 1.0], np.float32)
+    horizon = np.array([1.0, 1.0, 1.0], np.float32)
+    ground = np.array([0.25, 0.22, 0.18], np.float32)
+    sky = horizon[None, None, :] * (1 - t[..., None]) + zenith[None, None, :] * t[..., None]
+    img = np.where(cz[..., None] > 0, sky, ground[None, None, :]) * np.ones((height, width, 1), np.float32)
+    img = img * rng.lognormal(0.0, 0.5, size=(height, width, 1)).astype(np.float32)
+    # Sun: texels whose centre direction lies within half the disc diameter.
+    sth = np.sin(theta)[:, None]
+    phi = (u[None, :]) * 2 * np.pi
+    d = np.stack([-sth * np.cos(phi), sth * np.sin(phi), np.broadcast_to(cz, (height, width))], axis=-1)
+    el, az = math.radians(sun_elevation_deg), sun_azimuth * 2 * math.pi
+    s = np.array([-math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)])
+    ang = np.degrees(np.arccos(np.clip(d @ s, -1, 1)))
+    img[ang <= sun_diameter_deg / 2] = sun_radiance
+    return img.astype(np.float32)
+
+
+def disney_grid(seed=0, width=1920, height=1080, env_width=2048, env_height=1024, segments=64, rings=32):
+    """7x7 Disney UV spheres (64x32 segments) on a Lambert ground plane:
+    metallic = col/6, roughness = 0.05 + 0.95*row/6, ior 1.5; sheen,
+    clearcoat, specular tint and anisotropy U(0,1) from the seed; sky
+    environment map with a sun, sampled by the alias table."""
+    rng = np.random.default_rng(seed)
+    n = 7
+    mats = hpt.new_materials(n * n + 1)
+    prims = []
+    radius, spacing = 0.4, 1.0
+    for row in range(n):
+        for col in range(n):
+            k = row * n + col
+            disney_material(mats[k], tuple(rng.uniform(0.2, 0.9, 3)), metallic=col / 6.0,
+                            roughness=0.05 + 0.95 * row / 6.0, anisotropic=rng.uniform(),
+                            sheen=rng.uniform(), clearcoat=rng.uniform(), specular_tint=rng.uniform())
+            c = ((col - 3) * spacing, (row - 3) * spacing, radius)
+            v, ix = uv_sphere_mesh(c, radius, segments, rings)
+            prims.append(hpt.Primitive(v, ix, k))
+    diffuse_material(mats[n * n], (0.5, 0.5, 0.5))
+    v, ix = quad_mesh((-20, -20, 0), (40, 0, 0), (0, 40, 0))
+    prims.append(hpt.Primitive(v, ix, n * n))
+    cam = look_at_camera((0.0, -7.5, 5.0), (0.0, 0.0, 0.3), yfov_deg=45.0)
+    env = sky_env_map(env_width, env_height, seed)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, prims, env_rgb=env)
+
+
+def triangle_soup(count, seed=0):
+    """`count` triangles: v0 uniform in the unit cube, v1 and v2 at
+    v0 + lognormal(-4, 0.5) * random unit vector; Lambert 0.5."""
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform(0, 1, (count, 3))
+    def offs():
+        d = rng.normal(size=(count, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        return d * rng.lognormal(-4.0, 0.5, (count, 1))
+    v1, v2 = v0 + offs(), v0 + offs()
+    p = np.stack([v0, v1, v2], axis=1).reshape(-1, 3)
+    e1, e2 = v1 - v0, v2 - v0
+    nrm = np.cross(e1, e2)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    tan = e1 / np.maximum(np.linalg.norm(e1, axis=1, keepdims=True), 1e-30)
+    v = np.zeros((count * 3, hpt.VERTEX_FLOATS), np.float32)
+    v[:, 0:3] = p
+    v[:, 3:6] = np.repeat(nrm, 3, axis=0)
+    v[:, 6:9] = np.repeat(tan, 3, axis=0)
+    ix = np.arange(count * 3, dtype=np.uint32).reshape(-1, 3)
+    mats = hpt.new_materials(1)
+    diffuse_material(mats[0], (0.5, 0.5, 0.5))
+    cam = look_at_camera((0.5, -1.5, 0.5), (0.5, 0.5, 0.5), yfov_deg=50.0)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
+
+
+def furnace_sphere(albedo=0.5, emission=1.0, segments=48, rings=24):
+    """Camera inside a closed Lambert sphere that also emits: every path
+    gathers emission * sum_k albedo^k exactly (no lights, black sky)."""
+    mats = hpt.new_materials(1)
+    diffuse_material(mats[0], (albedo, albedo, albedo), emission=(emission, emission, emission))
+    v, ix = uv_sphere_mesh((0, 0, 0), 1.0, segments, rings)
+    cam = look_at_camera((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), yfov_deg=60.0)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)],
+                     ground_color=(0, 0, 0), sky_color=(0, 0, 0))

@@ -1,0 +1,409 @@
+/*
+ * hpt.h — C ABI of the MI355X (gfx950) path tracer.
+ *
+ * This library replaces the GPU half of the renderer that the reference host
+ * drives (`hala_renderer::rt_renderer::HalaRenderer`, called from
+ * src/bin/hala-pathtracer.rs:128-285) together with the GLSL programs under
+ * shaders/src/hala-pathtracer/.  The host keeps its scene loader, its YAML
+ * config and its camera/light/material structs; it hands those structs to
+ * this library in their std140 layout (shaders/src/hala-pathtracer/inc/types.glsl)
+ * and calls the entry points below instead of the Vulkan renderer.
+ *
+ * Conventions
+ *  - Every entry point returns hpt_status (0 = HPT_OK).  On failure the
+ *    calling thread's message is available from hpt_last_error().
+ *    C++ exceptions never cross this boundary.
+ *  - Inputs are borrowed for the duration of the call only and are copied to
+ *    device memory.  The handle owns every device allocation it makes.
+ *  - One handle is externally synchronised (the reference calls its renderer
+ *    through `&mut self` from one thread).  Distinct handles are independent.
+ *  - Work is queued on the handle's own HIP stream; hpt_read_image and
+ *    hpt_wait_idle synchronise with it.
+ */
+#ifndef HPT_H
+#define HPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPT_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+typedef int32_t hpt_status;
+enum {
+  HPT_OK = 0,
+  HPT_ERROR_INVALID_ARGUMENT = 1, /* bad pointer, size or enum value       */
+  HPT_ERROR_OUT_OF_MEMORY = 2,    /* device or host allocation failed      */
+  HPT_ERROR_DEVICE = 3,           /* a HIP call failed                     */
+  HPT_ERROR_STATE = 4,            /* call made in the wrong order          */
+  HPT_ERROR_UNSUPPORTED = 5       /* feature not available in this build   */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Constants shared with the GLSL ABI (inc/constants.glsl:1-44)              */
+/* ------------------------------------------------------------------------ */
+#define HPT_MAX_CAMERAS 8
+#define HPT_MAX_LIGHTS 16
+#define HPT_INVALID_INDEX 0xFFFFFFFFu
+#define HPT_WORLD_SIZE 10000.0f
+
+enum { HPT_CAMERA_PERSPECTIVE = 0, HPT_CAMERA_ORTHOGRAPHIC = 1 };
+enum {
+  HPT_LIGHT_POINT = 0,
+  HPT_LIGHT_DIRECTIONAL = 1,
+  HPT_LIGHT_SPOT = 2,
+  HPT_LIGHT_QUAD = 3,
+  HPT_LIGHT_SPHERE = 4
+};
+enum { HPT_ENV_SKY = 0, HPT_ENV_MAP = 1 };
+enum { HPT_MEDIUM_NONE = 0, HPT_MEDIUM_ABSORB = 1, HPT_MEDIUM_SCATTER = 2, HPT_MEDIUM_EMISSIVE = 3 };
+enum { HPT_MATERIAL_DIFFUSE = 0, HPT_MATERIAL_DISNEY = 1 };
+
+/* Shader permutation bits (shaders/src/make_shaders.yaml:7-22).  The
+ * reference host selects MEDIUM and VOL_MIS together when the scene has a
+ * medium and TRANSPARENT when it has a transparent material
+ * (src/bin/hala-pathtracer.rs:101-114).  Every one of the six combinations
+ * the YAML lists is accepted here.  ROUGHNESS_REGULARIZE_EVAL is always on
+ * (make_shaders.yaml:4). */
+enum {
+  HPT_FEATURE_MEDIUM = 1u << 0,
+  HPT_FEATURE_VOL_MIS = 1u << 1,
+  HPT_FEATURE_TRANSPARENT = 1u << 2
+};
+
+/* ------------------------------------------------------------------------ */
+/* std140 structs (inc/types.glsl).  Byte offsets are asserted at the end.   */
+/* ------------------------------------------------------------------------ */
+
+/* Camera, inc/types.glsl:2-11 (80-byte std140 array stride). */
+typedef struct hpt_camera {
+  float position[3];
+  float _pad0;
+  float right[3];
+  float _pad1;
+  float up[3];
+  float _pad2;
+  float forward[3];
+  float yfov;                    /* radians */
+  float focal_distance_or_xmag;  /* perspective: focal distance; ortho: xmag */
+  float aperture_or_ymag;        /* perspective: aperture; ortho: ymag */
+  uint32_t type;                 /* HPT_CAMERA_* */
+  float _pad3;
+} hpt_camera;
+
+/* Light, inc/types.glsl:14-39 (80-byte std140 array stride). */
+typedef struct hpt_light {
+  float intensity[3];
+  float _pad0;
+  float position[3];
+  float _pad1;
+  float u[3];
+  float _pad2;
+  float v[3];
+  float radius;
+  float area;
+  int32_t type; /* HPT_LIGHT_* */
+  float _pad3[2];
+} hpt_light;
+
+/* Medium, inc/types.glsl:42-49. */
+typedef struct hpt_medium {
+  float color[3];
+  float density;
+  float anisotropy;
+  uint32_t type; /* HPT_MEDIUM_* */
+  float _padding0;
+  float _padding1;
+} hpt_medium;
+
+/* Material, inc/types.glsl:52-83 (144 bytes). */
+typedef struct hpt_material {
+  hpt_medium medium;
+  float base_color[3];
+  float opacity;
+  float emission[3];
+  float anisotropic;
+  float metallic;
+  float roughness;
+  float subsurface;
+  float specular_tint;
+  float sheen;
+  float sheen_tint;
+  float clearcoat;
+  float clearcoat_roughness;
+  float clearcoat_tint[3];
+  float specular_transmission;
+  float ior;
+  float ax;
+  float ay;
+  uint32_t base_color_map_index;
+  uint32_t normal_map_index;
+  uint32_t metallic_roughness_map_index;
+  uint32_t emission_map_index;
+  uint32_t type; /* HPT_MATERIAL_* */
+} hpt_material;
+
+/* Vertex, inc/types.glsl:86-98 (std430, 44 bytes, 4-byte aligned). */
+typedef struct hpt_vertex {
+  float position[3];
+  float normal[3];
+  float tangent[3];
+  float tex_coord[2];
+} hpt_vertex;
+
+/* One mesh primitive: the host-side view of Primitive (inc/types.glsl:109-114)
+ * plus the vertex/index buffers that the GLSL reaches through
+ * buffer_reference.  `transform` is a column-major mat4 (GLSL / glam order):
+ * world = transform * vec4(position, 1). */
+typedef struct hpt_primitive_desc {
+  float transform[16];
+  uint32_t material_index;
+  uint32_t vertex_count;
+  uint32_t index_count; /* multiple of 3 */
+  uint32_t _pad;
+  const hpt_vertex* vertices;
+  const uint32_t* indices;
+} hpt_primitive_desc;
+
+typedef struct hpt_scene_desc {
+  const hpt_camera* cameras;
+  uint32_t camera_count; /* 1..HPT_MAX_CAMERAS */
+  uint32_t light_count;  /* 0..HPT_MAX_LIGHTS */
+  const hpt_light* lights;
+  const hpt_material* materials;
+  uint32_t material_count;
+  uint32_t primitive_count;
+  const hpt_primitive_desc* primitives;
+} hpt_scene_desc;
+
+/* Arguments of HalaRenderer::new (src/bin/hala-pathtracer.rs:128-138) minus
+ * the window, plus the feature bits that choose the shader permutation
+ * (hala-pathtracer.rs:101-114) and the exposure set at :211. */
+typedef struct hpt_create_info {
+  uint32_t width;
+  uint32_t height;
+  uint32_t max_depth;
+  uint32_t rr_depth;
+  uint32_t enable_tonemap;
+  uint32_t enable_aces;
+  uint32_t use_simple_aces;
+  uint32_t features;     /* HPT_FEATURE_* */
+  uint64_t max_samples;  /* 0 = unlimited */
+  float exposure_value;
+  int32_t device;        /* HIP device ordinal */
+} hpt_create_info;
+
+/* Images the renderer keeps resident (inc/global.glsl:2-4). */
+typedef enum hpt_image {
+  HPT_IMAGE_FINAL_RGBA8 = 0,  /* width*height*4 bytes, tonemapped sRGB */
+  HPT_IMAGE_ACCUM_RGBA32F = 1, /* width*height*16 bytes, running mean radiance */
+  HPT_IMAGE_ALBEDO_RGBA32F = 2,
+  HPT_IMAGE_NORMAL_RGBA32F = 3
+} hpt_image;
+
+/* Ray for the trace entry points: t is searched on [tmin, tmax] inclusive. */
+typedef struct hpt_ray {
+  float origin[3];
+  float tmin;
+  float direction[3];
+  float tmax;
+} hpt_ray;
+
+enum { HPT_HIT_NONE = 0, HPT_HIT_TRIANGLE = 1, HPT_HIT_LIGHT = 2 };
+
+/* Closest hit.  For triangles `instance` is the primitive index (the GLSL's
+ * gl_InstanceCustomIndexEXT, triangles.rchit.glsl:17) and `primitive` the
+ * triangle index inside it (gl_PrimitiveID, :24-26); u,v are the barycentric
+ * weights of the triangle's second and third vertex (hit attribute `attribs`,
+ * :58).  For analytic lights `instance` is the light index (lights.rint.glsl:21)
+ * and `primitive` is HPT_INVALID_INDEX. */
+typedef struct hpt_hit {
+  float t;
+  float u;
+  float v;
+  uint32_t kind; /* HPT_HIT_* */
+  uint32_t instance;
+  uint32_t primitive;
+} hpt_hit;
+
+typedef struct hpt_renderer hpt_renderer;
+
+/* ------------------------------------------------------------------------ */
+/* Entry points                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* HalaRenderer::new (hala-pathtracer.rs:128-138). */
+HPT_API hpt_status hpt_create(const hpt_create_info* info, hpt_renderer** out);
+
+/* HalaRenderer::set_scene (hala-pathtracer.rs:198).  Copies cameras, lights,
+ * materials and meshes; the BVH is built by hpt_commit. */
+HPT_API hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* scene);
+
+/* HalaRenderer::set_envmap (hala-pathtracer.rs:203).  `rgb` is width*height
+ * linear RGB float triples, row 0 = theta 0 (+Z).  `rotation` is the u offset
+ * of inc/env_map.glsl:18 / env_map_eval.rcall.glsl:18 in turns.  Builds the
+ * luminance alias table that replaces the reference's marginal/conditional
+ * distribution textures (inc/env_map.glsl:6-37).  Selects ENV_TYPE_MAP. */
+HPT_API hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t width,
+                                   uint32_t height, float rotation);
+
+/* HalaRenderer::set_ground_color / set_sky_color (hala-pathtracer.rs:205-206).
+ * Either selects ENV_TYPE_SKY (env_sky_eval.rcall.glsl:12-15). */
+HPT_API hpt_status hpt_set_ground_color(hpt_renderer* r, const float rgba[4]);
+HPT_API hpt_status hpt_set_sky_color(hpt_renderer* r, const float rgba[4]);
+
+/* HalaRenderer::set_env_intensity (hala-pathtracer.rs:208). */
+HPT_API hpt_status hpt_set_env_intensity(hpt_renderer* r, float intensity);
+
+/* HalaRenderer::set_exposure_value (hala-pathtracer.rs:211). */
+HPT_API hpt_status hpt_set_exposure_value(hpt_renderer* r, float ev);
+
+/* HalaRenderer::load_blue_noise_texture (hala-pathtracer.rs:195): RGBA8 texels
+ * read by rand4blue (common/wltcRS.glsl:46-53).  Without this call the
+ * renderer uses its built-in 1024x1024 procedural substitute. */
+HPT_API hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t width,
+                                      uint32_t height);
+
+/* HalaRenderer::commit (hala-pathtracer.rs:213): builds the BVH, uploads
+ * everything and resets accumulation. */
+HPT_API hpt_status hpt_commit(hpt_renderer* r);
+
+/* The per-frame update()+render() pair (hala-pathtracer.rs:264-285) without
+ * presentation: renders `frames` one-sample frames with the camera
+ * `camera_index`, each folded into the running mean of
+ * pathtracer.rgen.glsl:259-304.  Asynchronous on the handle's stream.
+ * Frames beyond max_samples (when non-zero) are skipped. */
+HPT_API hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index);
+
+/* Restart accumulation at frame 0 (what the renderer does when the camera or
+ * scene changes). */
+HPT_API hpt_status hpt_reset_accumulation(hpt_renderer* r);
+
+/* Number of frames accumulated so far (the UBO frame_index). */
+HPT_API hpt_status hpt_get_frame_index(hpt_renderer* r, uint64_t* frame_index);
+
+/* Copies one resident image into `dst` (the data behind
+ * HalaRenderer::save_images, hala-pathtracer.rs:251).  Synchronises. */
+HPT_API hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t bytes);
+
+/* Closest-hit query over the committed scene: the semantics of traceRayEXT
+ * with gl_RayFlagsOpaqueEXT (pathtracer.rgen.glsl:56-68): triangles are
+ * double-sided, analytic quad/sphere lights are hit through
+ * lights.rint.glsl:16-37.  Test and parity entry point; synchronous. */
+HPT_API hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint32_t count,
+                                     hpt_hit* hits);
+
+/* Any-hit (shadow) query: traceRayEXT with TerminateOnFirstHit|Opaque|
+ * SkipClosestHitShader (inc/lighting.glsl:119-130,205-216).  occluded[i] = 1
+ * when anything, triangles or analytic lights, lies on [tmin, tmax]. */
+HPT_API hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t count,
+                                 uint32_t* occluded);
+
+/* HalaRenderer::wait_idle (hala-pathtracer.rs:228). */
+HPT_API hpt_status hpt_wait_idle(hpt_renderer* r);
+
+/* Drops the renderer and all its device memory. */
+HPT_API void hpt_destroy(hpt_renderer* r);
+
+/* Message of the last failed call on this thread ("" if none). */
+HPT_API const char* hpt_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Introspection (no counterpart in the reference: the Vulkan driver keeps  */
+/* its acceleration structure opaque).  Used by parity tests and the        */
+/* roofline accounting, which need the exact node and triangle arrays.      */
+/* ------------------------------------------------------------------------ */
+
+/* Sizes of the committed BVH: 8-wide compressed nodes (80 bytes each) and
+ * leaf triangles (48 bytes each, in leaf order). */
+typedef struct hpt_bvh_info {
+  uint32_t node_count;
+  uint32_t triangle_count;
+  uint32_t max_depth;   /* levels of the 8-wide tree */
+  uint32_t _pad;
+} hpt_bvh_info;
+
+HPT_API hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info);
+
+/* Copies the node array (node_count*80 bytes) and leaf triangle array
+ * (triangle_count*48 bytes) to host memory. */
+HPT_API hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes,
+                                  void* triangles, size_t triangle_bytes);
+
+/* Per-ray BVH node visits and triangle tests of the closest-hit (any_hit = 0)
+ * or shadow (any_hit = 1) walk: the unit counts behind the roofline's
+ * algorithmic bytes, 80 B per node + 48 B per triangle + 48 B per ray. */
+HPT_API hpt_status hpt_trace_counts(hpt_renderer* r, const hpt_ray* rays, uint32_t count, int any_hit,
+                                    uint32_t* node_visits, uint32_t* tri_tests);
+
+/* Kernel timing with HIP events recorded on the handle's stream around each
+ * launch.  `stage` is one of the HPT_STAGE_* values.  Any non-zero value
+ * resets the counters and records per-launch timings and ray counts;
+ * HPT_PROFILE_COUNT_VISITS additionally makes the extend and shadow kernels
+ * count BVH node visits and triangle tests (a slower kernel variant, for
+ * roofline accounting only). */
+enum { HPT_PROFILE_TIMING = 1, HPT_PROFILE_COUNT_VISITS = 2 };
+enum {
+  HPT_STAGE_CAMERA = 0,
+  HPT_STAGE_EXTEND = 1,  /* closest-hit traversal of the path queue */
+  HPT_STAGE_SHADE = 2,   /* surface/medium interaction, NEE, BSDF sampling */
+  HPT_STAGE_SHADOW = 3,  /* any-hit traversal of the shadow queue */
+  HPT_STAGE_ACCUMULATE = 4,
+  HPT_STAGE_COUNT = 5
+};
+typedef struct hpt_stage_stats {
+  uint64_t launches;
+  double total_ms;
+  uint64_t rays;        /* rays the stage consumed (device-side counts) */
+  uint64_t node_visits; /* with HPT_PROFILE_COUNT_VISITS */
+  uint64_t tri_tests;   /* with HPT_PROFILE_COUNT_VISITS */
+} hpt_stage_stats;
+
+HPT_API hpt_status hpt_set_profiling(hpt_renderer* r, int enable);
+HPT_API hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Layout checks against the std140 offsets of inc/types.glsl.              */
+/* ------------------------------------------------------------------------ */
+#ifdef __cplusplus
+#define HPT_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define HPT_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+HPT_STATIC_ASSERT(sizeof(hpt_camera) == 80, "Camera std140 stride");
+HPT_STATIC_ASSERT(offsetof(hpt_camera, right) == 16, "Camera.right");
+HPT_STATIC_ASSERT(offsetof(hpt_camera, forward) == 48, "Camera.forward");
+HPT_STATIC_ASSERT(offsetof(hpt_camera, yfov) == 60, "Camera.yfov");
+HPT_STATIC_ASSERT(offsetof(hpt_camera, type) == 72, "Camera.type");
+HPT_STATIC_ASSERT(sizeof(hpt_light) == 80, "Light std140 stride");
+HPT_STATIC_ASSERT(offsetof(hpt_light, position) == 16, "Light.position");
+HPT_STATIC_ASSERT(offsetof(hpt_light, u) == 32, "Light.u");
+HPT_STATIC_ASSERT(offsetof(hpt_light, v) == 48, "Light.v");
+HPT_STATIC_ASSERT(offsetof(hpt_light, radius) == 60, "Light.radius");
+HPT_STATIC_ASSERT(offsetof(hpt_light, area) == 64, "Light.area");
+HPT_STATIC_ASSERT(offsetof(hpt_light, type) == 68, "Light.type");
+HPT_STATIC_ASSERT(sizeof(hpt_medium) == 32, "Medium size");
+HPT_STATIC_ASSERT(sizeof(hpt_material) == 144, "Material size");
+HPT_STATIC_ASSERT(offsetof(hpt_material, base_color) == 32, "Material.base_color");
+HPT_STATIC_ASSERT(offsetof(hpt_material, emission) == 48, "Material.emission");
+HPT_STATIC_ASSERT(offsetof(hpt_material, metallic) == 64, "Material.metallic");
+HPT_STATIC_ASSERT(offsetof(hpt_material, clearcoat_tint) == 96, "Material.clearcoat_tint");
+HPT_STATIC_ASSERT(offsetof(hpt_material, ior) == 112, "Material.ior");
+HPT_STATIC_ASSERT(offsetof(hpt_material, ax) == 116, "Material.ax");
+HPT_STATIC_ASSERT(offsetof(hpt_material, base_color_map_index) == 124, "Material.maps");
+HPT_STATIC_ASSERT(offsetof(hpt_material, type) == 140, "Material.type");
+HPT_STATIC_ASSERT(sizeof(hpt_vertex) == 44, "Vertex std430 size");
+HPT_STATIC_ASSERT(sizeof(hpt_ray) == 32, "ray size");
+HPT_STATIC_ASSERT(sizeof(hpt_hit) == 24, "hit size");
+
+#endif /* HPT_H */

@@ -1,0 +1,146 @@
+"""GPU ray-query parity: the HIP BVH walk against the oracle's brute force.
+
+Bar: closest-hit kind / primitive / triangle ids bit-exact, t, u, v
+bit-exact (both sides evaluate the same watertight test in the same IEEE
+order), any-hit flags identical, and the per-ray node/triangle visit counts
+identical to the oracle's restatement of the 8-wide walk.
+"""
+import numpy as np
+import pytest
+
+import hpt
+import hpt_scenes
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _renderer(scene, w=64, h=64, **kw):
+    print(f"[gpu] scene with {scene.triangle_count} triangles, {w}x{h}", flush=True)
+    r = hpt.HalaRenderer(w, h, max_depth=kw.pop("max_depth", 8), rr_depth=kw.pop("rr_depth", 3), **kw)
+    r.set_scene(scene)
+    r.commit()
+    return r
+
+
+def _same_hits(a, b):
+    return (a["kind"] == b["kind"]) & (a["instance"] == b["instance"]) & (a["primitive"] == b["primitive"])
+
+
+def _random_rays(n, seed, lo=-0.5, hi=1.5, target_lo=0.0, target_hi=1.0):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3))
+    t = rng.uniform(target_lo, target_hi, (n, 3))
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return hpt.make_rays(o, d)
+
+
+def test_cornell_primary_hits_bitexact():
+    scene = hpt_scenes.cornell_box()
+    w = h = 128
+    r = _renderer(scene, w, h)
+    o = pyoracle.OracleScene(scene)
+    st = pyoracle.settings(w, h, 8, 3)
+    ys, xs = np.mgrid[0:h, 0:w]
+    for frame in (0, 1, 7):
+        rays = o.camera_rays(st, frame, xs.ravel(), ys.ravel())
+        gpu = r.trace_closest(rays)
+        ref = o.trace_closest(rays, brute=True)
+        assert _same_hits(gpu, ref).all()
+        assert np.array_equal(gpu["t"], ref["t"])
+        assert np.array_equal(gpu["u"], ref["u"]) and np.array_equal(gpu["v"], ref["v"])
+        assert (ref["kind"] == hpt.HIT_TRIANGLE).sum() > 0.9 * len(rays)
+        assert (ref["kind"] == hpt.HIT_LIGHT).sum() > 0
+    r.close()
+
+
+@pytest.mark.parametrize("ntri", [1, 7, 1000, 100_000])
+def test_soup_closest_bitexact(ntri):
+    scene = hpt_scenes.triangle_soup(ntri, seed=ntri)
+    r = _renderer(scene)
+    o = pyoracle.OracleScene(scene)
+    rays = _random_rays(20000, seed=1)
+    gpu = r.trace_closest(rays)
+    ref = o.trace_closest(rays, brute=ntri <= 1000)
+    assert _same_hits(gpu, ref).all(), int((~_same_hits(gpu, ref)).sum())
+    assert np.array_equal(gpu["t"], ref["t"])
+    if ntri >= 1000:
+        assert (ref["kind"] == hpt.HIT_TRIANGLE).sum() > 1000
+    r.close()
+
+
+def test_soup_any_hit_matches():
+    scene = hpt_scenes.triangle_soup(50_000, seed=3)
+    r = _renderer(scene)
+    o = pyoracle.OracleScene(scene)
+    rays = _random_rays(20000, seed=2)
+    rng = np.random.default_rng(5)
+    rays["tmax"] = rng.uniform(0.01, 2.0, len(rays)).astype(np.float32)
+    assert np.array_equal(r.trace_any(rays), o.trace_any(rays))
+    r.close()
+
+
+def test_visit_counts_match_oracle_walk():
+    scene = hpt_scenes.triangle_soup(30_000, seed=4)
+    r = _renderer(scene)
+    o = pyoracle.OracleScene(scene)
+    nodes, tris = r.export_bvh()
+    rays = _random_rays(5000, seed=6)
+    for any_hit in (False, True):
+        gv, gt = r.trace_counts(rays, any_hit=any_hit)
+        hits, ov, ot = o.trace_bvh8(nodes, tris, rays, any_hit=any_hit)
+        assert np.array_equal(gv, ov)
+        assert np.array_equal(gt, ot)
+        assert gv.mean() > 1.0
+    # the oracle's walk of the exported tree finds the brute-force closest hits
+    hits, _, _ = o.trace_bvh8(nodes, tris, rays)
+    assert _same_hits(hits, o.trace_closest(rays)).all()
+    r.close()
+
+
+def test_edge_and_vertex_rays_are_watertight():
+    # Rays aimed exactly at shared edges and vertices of a closed cube from
+    # inside must never escape.
+    scene = hpt_scenes.furnace_sphere(segments=16, rings=8)
+    r = _renderer(scene)
+    tri = scene.world_triangles()
+    targets = np.concatenate([tri.reshape(-1, 3), 0.5 * (tri[:, 0] + tri[:, 1]), 0.5 * (tri[:, 1] + tri[:, 2])])
+    o = np.zeros_like(targets) + np.array([0.01, -0.02, 0.03], np.float32)
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hits = r.trace_closest(hpt.make_rays(o, d))
+    assert (hits["kind"] == hpt.HIT_TRIANGLE).all()
+    r.close()
+
+
+def test_hit_distance_vs_fp64():
+    scene = hpt_scenes.triangle_soup(2000, seed=9)
+    r = _renderer(scene)
+    o = pyoracle.OracleScene(scene)
+    rays = _random_rays(4000, seed=10)
+    gpu = r.trace_closest(rays)
+    t64, tri64, gap = o.trace_fp64(rays)
+    hit = (gpu["kind"] == hpt.HIT_TRIANGLE) & np.isfinite(t64) & (gap > 1e-4)
+    assert hit.sum() > 200
+    rel = np.abs(gpu["t"][hit] - t64[hit]) / t64[hit]
+    assert rel.max() < 1e-5
+    r.close()
+
+
+def test_lights_in_closest_and_any():
+    scene = hpt_scenes.cornell_box()
+    r = _renderer(scene)
+    o = pyoracle.OracleScene(scene)
+    rng = np.random.default_rng(12)
+    org = rng.uniform([-0.9, -0.9, 0.1], [0.9, 0.9, 1.9], (4000, 3))
+    tgt = rng.uniform([-0.125, -0.125, 1.999], [0.125, 0.125, 1.999], (4000, 3))
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = hpt.make_rays(org, d)
+    gpu, ref = r.trace_closest(rays), o.trace_closest(rays, brute=True)
+    assert _same_hits(gpu, ref).all()
+    assert np.array_equal(gpu["t"], ref["t"])
+    assert (ref["kind"] == hpt.HIT_LIGHT).sum() > 100
+    assert np.array_equal(r.trace_any(rays), o.trace_any(rays, brute=True))
+    r.close()
