@@ -263,9 +263,22 @@ def triangle_soup(count, seed=0):
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
 
 
+def furnace_box(albedo=0.5, emission=1.0):
+    """Camera inside a closed, flat-shaded Lambert cube that also emits:
+    every path gathers emission * sum_k albedo^k exactly (no lights, black
+    sky).  Flat faces keep the shading frame orthonormal, so no cosine sample
+    falls below the horizon."""
+    mats = hpt.new_materials(1)
+    diffuse_material(mats[0], (albedo, albedo, albedo), emission=(emission, emission, emission))
+    v, ix = cube_mesh()
+    cam = look_at_camera((0.1, -0.2, 0.05), (0.3, 1.0, 0.2), yfov_deg=70.0)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)],
+                     ground_color=(0, 0, 0), sky_color=(0, 0, 0))
+
+
 def furnace_sphere(albedo=0.5, emission=1.0, segments=48, rings=24):
-    """Camera inside a closed Lambert sphere that also emits: every path
-    gathers emission * sum_k albedo^k exactly (no lights, black sky)."""
+    """Camera inside a closed Lambert UV sphere that also emits (smooth
+    normals: the interpolated shading frame is not exactly orthonormal)."""
     mats = hpt.new_materials(1)
     diffuse_material(mats[0], (albedo, albedo, albedo), emission=(emission, emission, emission))
     v, ix = uv_sphere_mesh((0, 0, 0), 1.0, segments, rings)

@@ -1304,6 +1304,9 @@ void or_sample_ggx_vndf(const float wi[3], float ax, float ay, float r1, float r
 void or_cosine_sample_hemisphere(float r1, float r2, float out[3]) {
   S3(out, cosine_sample_hemisphere(r1, r2));
 }
+void or_sample_gtr1(float roughness, float r1, float r2, float out[3]) {
+  S3(out, sample_gtr1(roughness, r1, r2));
+}
 void or_tonemap_aces_fitted(const float in[3], float out[3]) { S3(out, aces_fitted(V3(in))); }
 void or_tonemap_aces(const float in[3], float out[3]) { S3(out, aces(V3(in))); }
 void or_tonemap_reinhard(const float in[3], float limit, float out[3]) { S3(out, tonemap(V3(in), limit)); }

@@ -112,6 +112,7 @@ void or_sample_hg(const float V[3], float g, float r1, float r2, float out[3]);
 float or_phase_hg(float cos_theta, float g);
 void or_sample_ggx_vndf(const float wi[3], float ax, float ay, float r1, float r2, float out[3]);
 void or_cosine_sample_hemisphere(float r1, float r2, float out[3]);
+void or_sample_gtr1(float roughness, float r1, float r2, float out[3]);
 void or_tonemap_aces_fitted(const float in[3], float out[3]);
 void or_tonemap_aces(const float in[3], float out[3]);
 void or_tonemap_reinhard(const float in[3], float limit, float out[3]);

@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
             "or_sample_hg": ([vp, f32, f32, f32, vp], None), "or_phase_hg": ([f32, f32], f32),
             "or_sample_ggx_vndf": ([vp, f32, f32, f32, f32, vp], None),
             "or_cosine_sample_hemisphere": ([f32, f32, vp], None),
+            "or_sample_gtr1": ([f32, f32, f32, vp], None),
             "or_tonemap_aces_fitted": ([vp, vp], None), "or_tonemap_aces": ([vp, vp], None),
             "or_tonemap_reinhard": ([vp, f32, vp], None), "or_linear_2_srgb": ([f32], f32),
         }
@@ -82,8 +83,11 @@ def P(a: np.ndarray):
     return C.c_void_p(a.ctypes.data) if a.size else None
 
 
-def f3(x) -> np.ndarray:
-    return np.ascontiguousarray(np.asarray(x, np.float32).reshape(3))
+def f3(x):
+    """A ctypes float[3]: passed directly as an argument it stays alive for
+    the duration of the call (a numpy temporary behind a raw address would
+    not)."""
+    return (C.c_float * 3)(*[float(v) for v in np.asarray(x, np.float64).reshape(3)])
 
 
 class OracleScene:
@@ -101,7 +105,7 @@ class OracleScene:
             if L.or_scene_set_env_map(self._p, P(a), a.shape[1], a.shape[0], scene.env_rotation) != 0:
                 raise RuntimeError("or_scene_set_env_map failed")
         else:
-            L.or_scene_set_env_sky(self._p, P(f3(scene.ground_color)), P(f3(scene.sky_color)))
+            L.or_scene_set_env_sky(self._p, f3((scene.ground_color)), f3((scene.sky_color)))
         L.or_scene_set_env_intensity(self._p, scene.env_intensity)
 
     def close(self):
@@ -199,11 +203,11 @@ class OracleScene:
 
     def env_eval(self, direction, depth=0, bsdf_pdf=1.0):
         out = np.zeros(3, np.float32)
-        lib().or_env_eval(self._p, P(f3(direction)), depth, bsdf_pdf, P(out))
+        lib().or_env_eval(self._p, f3((direction)), depth, bsdf_pdf, P(out))
         return out
 
     def env_pdf(self, direction) -> float:
-        return lib().or_env_pdf_dir(self._p, P(f3(direction)))
+        return lib().or_env_pdf_dir(self._p, f3((direction)))
 
     def blue_noise_texel(self, tx, ty):
         out = np.zeros(4, np.uint8)
@@ -250,7 +254,7 @@ def bxdf_eval(mat, fr: Frame, V, L):
     m = np.ascontiguousarray(np.asarray(mat, hpt.MATERIAL_DTYPE).reshape(1))
     f = np.zeros(3, np.float32)
     pdf = C.c_float()
-    lib().or_bxdf_eval(P(m), C.byref(fr), P(f3(V)), P(f3(L)), P(f), C.byref(pdf))
+    lib().or_bxdf_eval(P(m), C.byref(fr), f3((V)), f3((L)), P(f), C.byref(pdf))
     return f, pdf.value
 
 
@@ -261,7 +265,7 @@ def bxdf_sample(mat, fr: Frame, V, rng4):
     f = np.zeros(3, np.float32)
     pdf = C.c_float()
     flags = C.c_uint32()
-    lib().or_bxdf_sample(P(m), C.byref(fr), P(f3(V)), P(rng), P(L), P(f), C.byref(pdf), C.byref(flags))
+    lib().or_bxdf_sample(P(m), C.byref(fr), f3((V)), P(rng), P(L), P(f), C.byref(pdf), C.byref(flags))
     return L, f, pdf.value, flags.value, rng
 
 
@@ -270,14 +274,14 @@ def light_sample(light, num_lights, p, rng4):
     rng = np.ascontiguousarray(np.asarray(rng4, np.uint32).reshape(4).copy())
     d, n, e = (np.zeros(3, np.float32) for _ in range(3))
     dist, pdf = C.c_float(), C.c_float()
-    lib().or_light_sample(P(l), num_lights, P(f3(p)), P(rng), P(d), P(n), P(e), C.byref(dist), C.byref(pdf))
+    lib().or_light_sample(P(l), num_lights, f3((p)), P(rng), P(d), P(n), P(e), C.byref(dist), C.byref(pdf))
     return d, n, e, dist.value, pdf.value, rng
 
 
 def light_intersect(light, org, d):
     l = np.ascontiguousarray(np.asarray(light, hpt.LIGHT_DTYPE).reshape(1))
     pdf = C.c_float()
-    t = lib().or_light_intersect(P(l), P(f3(org)), P(f3(d)), C.byref(pdf))
+    t = lib().or_light_intersect(P(l), f3((org)), f3((d)), C.byref(pdf))
     return t, pdf.value
 
 
@@ -291,7 +295,7 @@ def build_alias(lum):
 
 def sample_hg(V, g, r1, r2):
     out = np.zeros(3, np.float32)
-    lib().or_sample_hg(P(f3(V)), g, r1, r2, P(out))
+    lib().or_sample_hg(f3((V)), g, r1, r2, P(out))
     return out
 
 
@@ -301,7 +305,7 @@ def phase_hg(c, g):
 
 def sample_ggx_vndf(wi, ax, ay, r1, r2):
     out = np.zeros(3, np.float32)
-    lib().or_sample_ggx_vndf(P(f3(wi)), ax, ay, r1, r2, P(out))
+    lib().or_sample_ggx_vndf(f3((wi)), ax, ay, r1, r2, P(out))
     return out
 
 
@@ -311,14 +315,20 @@ def cosine_sample_hemisphere(r1, r2):
     return out
 
 
+def sample_gtr1(roughness, r1, r2):
+    out = np.zeros(3, np.float32)
+    lib().or_sample_gtr1(roughness, r1, r2, P(out))
+    return out
+
+
 def tonemap(kind, rgb, limit=1.5):
     out = np.zeros(3, np.float32)
     if kind == "aces_fitted":
-        lib().or_tonemap_aces_fitted(P(f3(rgb)), P(out))
+        lib().or_tonemap_aces_fitted(f3((rgb)), P(out))
     elif kind == "aces":
-        lib().or_tonemap_aces(P(f3(rgb)), P(out))
+        lib().or_tonemap_aces(f3((rgb)), P(out))
     else:
-        lib().or_tonemap_reinhard(P(f3(rgb)), limit, P(out))
+        lib().or_tonemap_reinhard(f3((rgb)), limit, P(out))
     return out
 
 
@@ -328,5 +338,5 @@ def linear_2_srgb(x):
 
 def final_color(st: Settings, rgb):
     out = np.zeros(3, np.float32)
-    lib().or_final_color(C.byref(st), P(f3(rgb)), P(out))
+    lib().or_final_color(C.byref(st), f3((rgb)), P(out))
     return out

@@ -1,0 +1,83 @@
+"""The C-ABI library loads and exports every entry point include/hpt.h
+declares, and the Python/numpy mirrors of its structs have the std140 sizes
+and offsets of shaders/src/hala-pathtracer/inc/types.glsl.  No compute calls:
+these run without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hpt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hpt.h")
+
+
+def declared_entry_points():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"HPT_API\s+[\w\s\*]+?\b(hpt_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_entry_points()
+    for required in ("hpt_create", "hpt_set_scene", "hpt_set_env_map", "hpt_set_ground_color", "hpt_set_sky_color",
+                     "hpt_set_env_intensity", "hpt_set_exposure_value", "hpt_set_blue_noise", "hpt_commit",
+                     "hpt_render", "hpt_read_image", "hpt_trace_closest", "hpt_trace_any", "hpt_wait_idle",
+                     "hpt_destroy", "hpt_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = hpt.load_library()
+    for name in declared_entry_points():
+        assert hasattr(lib, name), name
+    assert set(declared_entry_points()) == set(hpt.SIGNATURES)
+
+
+def test_last_error_is_empty_string_initially():
+    lib = hpt.load_library()
+    assert isinstance(lib.hpt_last_error(), bytes)
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    lib = hpt.load_library()
+    assert lib.hpt_create(None, None) == 1
+    assert b"null" in lib.hpt_last_error()
+    assert lib.hpt_commit(None) == 1
+    lib.hpt_destroy(None)
+
+
+def test_std140_layouts():
+    assert hpt.CAMERA_DTYPE.itemsize == 80
+    assert hpt.LIGHT_DTYPE.itemsize == 80
+    assert hpt.MATERIAL_DTYPE.itemsize == 144
+    assert hpt.MATERIAL_DTYPE.fields["type"][1] == 140
+    assert hpt.MATERIAL_DTYPE.fields["ior"][1] == 112
+    assert hpt.MATERIAL_DTYPE.fields["clearcoat_tint"][1] == 96
+    assert hpt.LIGHT_DTYPE.fields["area"][1] == 64
+    assert hpt.CAMERA_DTYPE.fields["type"][1] == 72
+    assert C.sizeof(hpt._PrimitiveDesc) == 96
+    assert C.sizeof(hpt._CreateInfo) == 48
+    assert hpt.RAY_DTYPE.itemsize == 32 and hpt.HIT_DTYPE.itemsize == 24
+
+
+def test_scene_desc_round_trip_fields():
+    import hpt_scenes
+
+    sc = hpt_scenes.cornell_box()
+    d, keep = sc.to_desc()
+    assert d.camera_count == 1 and d.light_count == 1 and d.material_count == 3
+    assert d.primitive_count == 7
+    assert sc.triangle_count == 34
+    tri = sc.world_triangles()
+    assert tri.shape == (34, 3, 3)
+    assert np.isclose(tri[:, :, 2].max(), 2.0) and np.isclose(tri[:, :, 2].min(), 0.0, atol=1e-6)
+
+
+def test_pfm_round_trip(tmp_path):
+    img = np.random.default_rng(0).random((5, 7, 3)).astype(np.float32)
+    p = tmp_path / "x.pfm"
+    hpt.write_pfm(str(p), img)
+    assert np.array_equal(hpt.read_pfm(str(p)), img)

@@ -81,7 +81,7 @@ def test_cornell_converged_3sigma():
 @pytest.mark.parametrize("max_depth", [0, 1, 5])
 def test_white_furnace_closed_form(max_depth):
     rho, le = 0.5, 1.0
-    scene = hpt_scenes.furnace_sphere(albedo=rho, emission=le)
+    scene = hpt_scenes.furnace_box(albedo=rho, emission=le)
     w = h = 48
     r = _renderer(scene, w, h, max_depth=max_depth, rr_depth=0, enable_tonemap=False)
     r.render(4)
@@ -93,7 +93,7 @@ def test_white_furnace_closed_form(max_depth):
 
 def test_white_furnace_russian_roulette_unbiased():
     rho, le, depth = 0.7, 1.0, 8
-    scene = hpt_scenes.furnace_sphere(albedo=rho, emission=le)
+    scene = hpt_scenes.furnace_box(albedo=rho, emission=le)
     w = h = 64
     frames = 32
     r = _renderer(scene, w, h, max_depth=depth, rr_depth=2, enable_tonemap=False)
