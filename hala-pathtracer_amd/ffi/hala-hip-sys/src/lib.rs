@@ -1,0 +1,317 @@
+//! Rust binding of the MI355X path tracer (include/hpt.h).
+//!
+//! `HalaRenderer` keeps the method names the reference host calls on
+//! `hala_renderer::rt_renderer::HalaRenderer` (src/bin/hala-pathtracer.rs:128-285);
+//! the Vulkan-only calls (`push_*_shader_with_file`, the imgui context) have no
+//! counterpart, and the shader permutation becomes the `features` bits.
+//! Errors map a non-zero status to `anyhow!(hpt_last_error())`.
+//!
+//! Written against the C header; no Rust toolchain was available where it was
+//! written, so it has not been compiled.
+#![allow(non_camel_case_types)]
+
+use std::ffi::CStr;
+use std::os::raw::{c_char, c_int, c_void};
+
+pub type hpt_status = i32;
+
+pub const HPT_FEATURE_MEDIUM: u32 = 1;
+pub const HPT_FEATURE_VOL_MIS: u32 = 2;
+pub const HPT_FEATURE_TRANSPARENT: u32 = 4;
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_camera {
+    pub position: [f32; 3],
+    pub _pad0: f32,
+    pub right: [f32; 3],
+    pub _pad1: f32,
+    pub up: [f32; 3],
+    pub _pad2: f32,
+    pub forward: [f32; 3],
+    pub yfov: f32,
+    pub focal_distance_or_xmag: f32,
+    pub aperture_or_ymag: f32,
+    pub camera_type: u32,
+    pub _pad3: f32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_light {
+    pub intensity: [f32; 3],
+    pub _pad0: f32,
+    pub position: [f32; 3],
+    pub _pad1: f32,
+    pub u: [f32; 3],
+    pub _pad2: f32,
+    pub v: [f32; 3],
+    pub radius: f32,
+    pub area: f32,
+    pub light_type: i32,
+    pub _pad3: [f32; 2],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_medium {
+    pub color: [f32; 3],
+    pub density: f32,
+    pub anisotropy: f32,
+    pub medium_type: u32,
+    pub _padding0: f32,
+    pub _padding1: f32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_material {
+    pub medium: hpt_medium,
+    pub base_color: [f32; 3],
+    pub opacity: f32,
+    pub emission: [f32; 3],
+    pub anisotropic: f32,
+    pub metallic: f32,
+    pub roughness: f32,
+    pub subsurface: f32,
+    pub specular_tint: f32,
+    pub sheen: f32,
+    pub sheen_tint: f32,
+    pub clearcoat: f32,
+    pub clearcoat_roughness: f32,
+    pub clearcoat_tint: [f32; 3],
+    pub specular_transmission: f32,
+    pub ior: f32,
+    pub ax: f32,
+    pub ay: f32,
+    pub base_color_map_index: u32,
+    pub normal_map_index: u32,
+    pub metallic_roughness_map_index: u32,
+    pub emission_map_index: u32,
+    pub material_type: u32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_vertex {
+    pub position: [f32; 3],
+    pub normal: [f32; 3],
+    pub tangent: [f32; 3],
+    pub tex_coord: [f32; 2],
+}
+
+#[repr(C)]
+pub struct hpt_primitive_desc {
+    pub transform: [f32; 16],
+    pub material_index: u32,
+    pub vertex_count: u32,
+    pub index_count: u32,
+    pub _pad: u32,
+    pub vertices: *const hpt_vertex,
+    pub indices: *const u32,
+}
+
+#[repr(C)]
+pub struct hpt_scene_desc {
+    pub cameras: *const hpt_camera,
+    pub camera_count: u32,
+    pub light_count: u32,
+    pub lights: *const hpt_light,
+    pub materials: *const hpt_material,
+    pub material_count: u32,
+    pub primitive_count: u32,
+    pub primitives: *const hpt_primitive_desc,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct hpt_create_info {
+    pub width: u32,
+    pub height: u32,
+    pub max_depth: u32,
+    pub rr_depth: u32,
+    pub enable_tonemap: u32,
+    pub enable_aces: u32,
+    pub use_simple_aces: u32,
+    pub features: u32,
+    pub max_samples: u64,
+    pub exposure_value: f32,
+    pub device: i32,
+}
+
+#[repr(C)]
+pub struct hpt_renderer {
+    _private: [u8; 0],
+}
+
+pub const HPT_IMAGE_FINAL_RGBA8: c_int = 0;
+pub const HPT_IMAGE_ACCUM_RGBA32F: c_int = 1;
+pub const HPT_IMAGE_ALBEDO_RGBA32F: c_int = 2;
+pub const HPT_IMAGE_NORMAL_RGBA32F: c_int = 3;
+
+extern "C" {
+    pub fn hpt_create(info: *const hpt_create_info, out: *mut *mut hpt_renderer) -> hpt_status;
+    pub fn hpt_set_scene(r: *mut hpt_renderer, scene: *const hpt_scene_desc) -> hpt_status;
+    pub fn hpt_set_env_map(r: *mut hpt_renderer, rgb: *const f32, width: u32, height: u32, rotation: f32) -> hpt_status;
+    pub fn hpt_set_ground_color(r: *mut hpt_renderer, rgba: *const f32) -> hpt_status;
+    pub fn hpt_set_sky_color(r: *mut hpt_renderer, rgba: *const f32) -> hpt_status;
+    pub fn hpt_set_env_intensity(r: *mut hpt_renderer, intensity: f32) -> hpt_status;
+    pub fn hpt_set_exposure_value(r: *mut hpt_renderer, ev: f32) -> hpt_status;
+    pub fn hpt_set_blue_noise(r: *mut hpt_renderer, rgba8: *const u8, width: u32, height: u32) -> hpt_status;
+    pub fn hpt_commit(r: *mut hpt_renderer) -> hpt_status;
+    pub fn hpt_render(r: *mut hpt_renderer, frames: u32, camera_index: u32) -> hpt_status;
+    pub fn hpt_reset_accumulation(r: *mut hpt_renderer) -> hpt_status;
+    pub fn hpt_get_frame_index(r: *mut hpt_renderer, frame_index: *mut u64) -> hpt_status;
+    pub fn hpt_read_image(r: *mut hpt_renderer, which: c_int, dst: *mut c_void, bytes: usize) -> hpt_status;
+    pub fn hpt_wait_idle(r: *mut hpt_renderer) -> hpt_status;
+    pub fn hpt_destroy(r: *mut hpt_renderer);
+    pub fn hpt_last_error() -> *const c_char;
+}
+
+fn check(status: hpt_status) -> anyhow::Result<()> {
+    if status == 0 {
+        return Ok(());
+    }
+    let msg = unsafe { CStr::from_ptr(hpt_last_error()) }.to_string_lossy().into_owned();
+    Err(anyhow::anyhow!("hpt status {}: {}", status, msg))
+}
+
+/// Owned mesh data of one primitive (what `HalaScene` holds per primitive).
+pub struct Mesh {
+    pub transform: [f32; 16],
+    pub material_index: u32,
+    pub vertices: Vec<hpt_vertex>,
+    pub indices: Vec<u32>,
+}
+
+/// The scene arrays the renderer uploads in `set_scene`.
+pub struct SceneArrays {
+    pub cameras: Vec<hpt_camera>,
+    pub lights: Vec<hpt_light>,
+    pub materials: Vec<hpt_material>,
+    pub meshes: Vec<Mesh>,
+}
+
+pub struct HalaRenderer {
+    raw: *mut hpt_renderer,
+    width: u32,
+    height: u32,
+}
+
+impl HalaRenderer {
+    /// hala-pathtracer.rs:128-138, without the window; `features` is the
+    /// permutation the host picks at :101-114.
+    #[allow(clippy::too_many_arguments)]
+    pub fn new(width: u32, height: u32, max_depth: u32, rr_depth: u32, enable_tonemap: bool, enable_aces: bool,
+               use_simple_aces: bool, max_samples: u64, features: u32) -> anyhow::Result<Self> {
+        let info = hpt_create_info {
+            width,
+            height,
+            max_depth,
+            rr_depth,
+            enable_tonemap: enable_tonemap as u32,
+            enable_aces: enable_aces as u32,
+            use_simple_aces: use_simple_aces as u32,
+            features,
+            max_samples,
+            exposure_value: 0.0,
+            device: 0,
+        };
+        let mut raw = std::ptr::null_mut();
+        check(unsafe { hpt_create(&info, &mut raw) })?;
+        Ok(Self { raw, width, height })
+    }
+
+    /// hala-pathtracer.rs:198.
+    pub fn set_scene(&mut self, scene: &SceneArrays) -> anyhow::Result<()> {
+        let prims: Vec<hpt_primitive_desc> = scene
+            .meshes
+            .iter()
+            .map(|m| hpt_primitive_desc {
+                transform: m.transform,
+                material_index: m.material_index,
+                vertex_count: m.vertices.len() as u32,
+                index_count: m.indices.len() as u32,
+                _pad: 0,
+                vertices: m.vertices.as_ptr(),
+                indices: m.indices.as_ptr(),
+            })
+            .collect();
+        let desc = hpt_scene_desc {
+            cameras: scene.cameras.as_ptr(),
+            camera_count: scene.cameras.len() as u32,
+            light_count: scene.lights.len() as u32,
+            lights: scene.lights.as_ptr(),
+            materials: scene.materials.as_ptr(),
+            material_count: scene.materials.len() as u32,
+            primitive_count: prims.len() as u32,
+            primitives: prims.as_ptr(),
+        };
+        check(unsafe { hpt_set_scene(self.raw, &desc) })
+    }
+
+    /// hala-pathtracer.rs:203 after the HDR file is decoded to linear RGB.
+    pub fn set_envmap(&mut self, rgb: &[f32], width: u32, height: u32, rotation: f32) -> anyhow::Result<()> {
+        anyhow::ensure!(rgb.len() == (width * height * 3) as usize, "envmap size mismatch");
+        check(unsafe { hpt_set_env_map(self.raw, rgb.as_ptr(), width, height, rotation) })
+    }
+
+    pub fn set_ground_color(&mut self, c: [f32; 4]) -> anyhow::Result<()> {
+        check(unsafe { hpt_set_ground_color(self.raw, c.as_ptr()) })
+    }
+
+    pub fn set_sky_color(&mut self, c: [f32; 4]) -> anyhow::Result<()> {
+        check(unsafe { hpt_set_sky_color(self.raw, c.as_ptr()) })
+    }
+
+    pub fn set_env_intensity(&mut self, v: f32) -> anyhow::Result<()> {
+        check(unsafe { hpt_set_env_intensity(self.raw, v) })
+    }
+
+    pub fn set_exposure_value(&mut self, ev: f32) -> anyhow::Result<()> {
+        check(unsafe { hpt_set_exposure_value(self.raw, ev) })
+    }
+
+    /// hala-pathtracer.rs:195 after the PNG is decoded to RGBA8.
+    pub fn load_blue_noise_texture(&mut self, rgba8: &[u8], width: u32, height: u32) -> anyhow::Result<()> {
+        check(unsafe { hpt_set_blue_noise(self.raw, rgba8.as_ptr(), width, height) })
+    }
+
+    /// hala-pathtracer.rs:213.
+    pub fn commit(&mut self) -> anyhow::Result<()> {
+        check(unsafe { hpt_commit(self.raw) })
+    }
+
+    /// The update()+render() pair of hala-pathtracer.rs:264-285: one frame.
+    pub fn render(&mut self, camera_index: u32) -> anyhow::Result<()> {
+        check(unsafe { hpt_render(self.raw, 1, camera_index) })
+    }
+
+    pub fn reset_accumulation(&mut self) -> anyhow::Result<()> {
+        check(unsafe { hpt_reset_accumulation(self.raw) })
+    }
+
+    /// Linear RGBA32F planes behind save_images (hala-pathtracer.rs:251).
+    pub fn read_rgba32f(&self, which: c_int) -> anyhow::Result<Vec<f32>> {
+        let mut v = vec![0f32; (self.width * self.height * 4) as usize];
+        check(unsafe { hpt_read_image(self.raw, which, v.as_mut_ptr() as *mut c_void, v.len() * 4) })?;
+        Ok(v)
+    }
+
+    pub fn read_final_rgba8(&self) -> anyhow::Result<Vec<u8>> {
+        let mut v = vec![0u8; (self.width * self.height * 4) as usize];
+        check(unsafe { hpt_read_image(self.raw, HPT_IMAGE_FINAL_RGBA8, v.as_mut_ptr() as *mut c_void, v.len()) })?;
+        Ok(v)
+    }
+
+    /// hala-pathtracer.rs:228.
+    pub fn wait_idle(&self) -> anyhow::Result<()> {
+        check(unsafe { hpt_wait_idle(self.raw) })
+    }
+}
+
+impl Drop for HalaRenderer {
+    fn drop(&mut self) {
+        unsafe { hpt_destroy(self.raw) }
+    }
+}
