@@ -61,6 +61,21 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, budget_s=15.0):
                       f"{dt:.1f} s)"}
 
 
+def copy_calibration(mib=1024, reps=4):
+    """Plain device-to-device copies of a known size, for scaling rocprof's
+    FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_dispatch": n * 4, "copy_bytes_written_per_dispatch": n * 4,
+                      "dispatches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +84,12 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run known-size copies (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of frames is "
               f"described in DESIGN.md and not measured here.", file=sys.stderr)

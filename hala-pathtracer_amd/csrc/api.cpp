@@ -669,6 +669,21 @@ hpt_status hpt_reset_accumulation(hpt_renderer* r) {
   return guarded([&] {
     need(r, "null renderer");
     r->frame_index = 0;
+    if (r->committed) {
+      set_device(r);
+      check(hipMemsetAsync(r->d_accum.p, 0, r->d_accum.bytes, r->stream), "hipMemsetAsync");
+      check(hipMemsetAsync(r->d_albedo.p, 0, r->d_albedo.bytes, r->stream), "hipMemsetAsync");
+      check(hipMemsetAsync(r->d_normal.p, 0, r->d_normal.bytes, r->stream), "hipMemsetAsync");
+      check(hipMemsetAsync(r->d_final.p, 0, r->d_final.bytes, r->stream), "hipMemsetAsync");
+    }
+  });
+}
+
+hpt_status hpt_set_frame_index(hpt_renderer* r, uint64_t frame_index) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(frame_index <= 0xFFFFFFFFull, "frame index must fit the 32-bit UBO field");
+    r->frame_index = frame_index;
   });
 }
 

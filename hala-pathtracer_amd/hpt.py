@@ -139,6 +139,7 @@ SIGNATURES = {
     "hpt_commit": [C.c_void_p],
     "hpt_render": [C.c_void_p, C.c_uint32, C.c_uint32],
     "hpt_reset_accumulation": [C.c_void_p],
+    "hpt_set_frame_index": [C.c_void_p, C.c_uint64],
     "hpt_get_frame_index": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_read_image": [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t],
     "hpt_trace_closest": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
@@ -339,6 +340,18 @@ class HalaRenderer:
         v = C.c_uint64()
         _check(self._lib.hpt_get_frame_index(self._h, C.byref(v)))
         return v.value
+
+    @frame_index.setter
+    def frame_index(self, f: int) -> None:
+        _check(self._lib.hpt_set_frame_index(self._h, f))
+
+    def render_single_frame(self, frame: int) -> np.ndarray:
+        """Radiance of frame `frame` alone: clears the accumulators, renders
+        that frame and undoes the running-mean weight 1/(frame+1)."""
+        self.reset_accumulation()
+        self.frame_index = frame
+        self.render(1)
+        return self.read_image(IMAGE_ACCUM)[..., :3] * np.float32(frame + 1)
 
     def read_image(self, which: int) -> np.ndarray:
         if which == IMAGE_FINAL_RGBA8:

@@ -237,6 +237,153 @@ def disney_grid(seed=0, width=1920, height=1080, env_width=2048, env_height=1024
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, prims, env_rgb=env)
 
 
+def _hash3(ix, iy, iz, seed):
+    """Integer lattice hash -> [0, 1) (one pcg-style round, uint32)."""
+    m = np.uint32(0xFFFFFFFF)
+    v = (ix.astype(np.uint32) * np.uint32(73856093)) ^ (iy.astype(np.uint32) * np.uint32(19349663)) ^ \
+        (iz.astype(np.uint32) * np.uint32(83492791)) ^ np.uint32(seed * 2654435761 & 0xFFFFFFFF)
+    v = (v * np.uint32(1664525) + np.uint32(1013904223)) & m
+    v ^= v >> np.uint32(16)
+    v = (v * np.uint32(2246822519)) & m
+    v ^= v >> np.uint32(13)
+    return (v >> np.uint32(8)).astype(np.float64) / float(1 << 24)
+
+
+def value_noise(p, seed=0):
+    """Trilinear value noise with smoothstep weights, in [0, 1)."""
+    i = np.floor(p)
+    f = p - i
+    u = f * f * (3 - 2 * f)
+    i = i.astype(np.int64)
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (u[:, 0] if dx else 1 - u[:, 0]) * (u[:, 1] if dy else 1 - u[:, 1]) * (u[:, 2] if dz else 1 - u[:, 2])
+                out = out + w * _hash3(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed)
+    return out
+
+
+def fbm(p, octaves=6, lacunarity=2.0, gain=0.5, seed=0):
+    """Fractal sum of value noise in [-1, 1) scaled by sum of amplitudes."""
+    amp, freq, total, norm = 1.0, 1.0, 0.0, 0.0
+    for o in range(octaves):
+        total = total + amp * (2 * value_noise(p * freq, seed + o) - 1)
+        norm += amp
+        amp *= gain
+        freq *= lacunarity
+    return total / norm
+
+
+def icosphere(level):
+    """Unit icosphere: 20 * 4^level triangles, vertices deduplicated per level."""
+    t = (1 + 5 ** 0.5) / 2
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e[:, 0] * len(v) + e[:, 1], return_inverse=True)
+        a, b = uniq // len(v), uniq % len(v)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = inv.reshape(3, -1) + len(v)
+        v = np.concatenate([v, mid])
+        m01, m12, m20 = m[0], m[1], m[2]
+        f = np.concatenate([np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+                            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def smooth_mesh(pos, faces):
+    """Vertices (N, 11) with area-weighted smooth normals and tangents
+    orthogonal to them, uv from spherical angles."""
+    e1 = pos[faces[:, 1]] - pos[faces[:, 0]]
+    e2 = pos[faces[:, 2]] - pos[faces[:, 0]]
+    fn = np.cross(e1, e2)
+    n = np.zeros_like(pos)
+    for k in range(3):
+        np.add.at(n, faces[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    ref = np.where(np.abs(n[:, 2:3]) < 0.9, np.array([[0, 0, 1.0]]), np.array([[1.0, 0, 0]]))
+    t = np.cross(ref, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    v = np.zeros((len(pos), hpt.VERTEX_FLOATS), np.float32)
+    v[:, 0:3] = pos
+    v[:, 3:6] = n
+    v[:, 6:9] = t
+    v[:, 9] = (np.arctan2(pos[:, 1], pos[:, 0]) / (2 * np.pi)) % 1.0
+    v[:, 10] = np.arccos(np.clip(n[:, 2], -1, 1)) / np.pi
+    return v, faces.astype(np.uint32)
+
+
+def displaced_icosphere_lights_medium(level=8, seed=0, amplitude=0.1, width=1920, height=1080):
+    """Large mesh + mixed lights + medium: an icosphere (20*4^level
+    triangles; level 8 = 1,310,720) displaced by fBm (6 octaves, lacunarity
+    2, gain 0.5, amplitude 0.1) on a ground quad; one point, spot (inner 20,
+    outer 30 degrees), directional, rect and sphere (r = 0.2) light; a
+    homogeneous scattering sphere (sigma_s 0.5, sigma_a 0.05, HG g 0.3) behind
+    an opacity-0 boundary; thin-lens camera (f = 50 mm on a 24 mm sensor,
+    aperture 0.05) focused on the mesh.  Needs MEDIUM|VOL_MIS|TRANSPARENT."""
+    mats = hpt.new_materials(4)
+    disney_material(mats[0], (0.8, 0.55, 0.35), metallic=0.2, roughness=0.45, clearcoat=0.3, sheen=0.2)
+    diffuse_material(mats[1], (0.6, 0.6, 0.6), roughness=0.3)
+    # medium boundary: invisible surface carrying the scattering medium
+    diffuse_material(mats[2], (1.0, 1.0, 1.0))
+    mats[2]["opacity"] = 0.0
+    mats[2]["medium_type"] = hpt.MEDIUM_SCATTER
+    mats[2]["medium_density"] = 0.55
+    mats[2]["medium_color"] = 0.5 / 0.55
+    mats[2]["medium_anisotropy"] = 0.3
+    disney_material(mats[3], (0.9, 0.9, 0.9), roughness=0.2)
+    v, f = icosphere(level)
+    d = 1.0 + amplitude * fbm(v * 2.0, seed=seed)
+    pos = v * d[:, None] * 0.8 + np.array([0.0, 0.0, 0.9])
+    sv, sf = smooth_mesh(pos, f)
+    prims = [hpt.Primitive(sv, sf, 0)]
+    q, qi = quad_mesh((-6, -6, 0), (12, 0, 0), (0, 12, 0))
+    prims.append(hpt.Primitive(q, qi, 1))
+    mv, mf = uv_sphere_mesh((1.6, 0.4, 0.6), 0.6, 32, 16)
+    prims.append(hpt.Primitive(mv, mf, 2))
+    lights = np.zeros(5, hpt.LIGHT_DTYPE)
+    lights[0]["type"] = hpt.LIGHT_POINT
+    lights[0]["position"] = (-1.5, -1.5, 2.5)
+    lights[0]["intensity"] = (6.0, 5.0, 4.0)
+    lights[1]["type"] = hpt.LIGHT_SPOT
+    lights[1]["position"] = (1.5, -2.5, 3.0)
+    axis = np.array([-1.5, 2.5, -2.2])
+    lights[1]["u"] = axis / np.linalg.norm(axis)
+    lights[1]["v"] = (math.cos(math.radians(20)), math.cos(math.radians(30)), 0.0)
+    lights[1]["intensity"] = (20.0, 20.0, 24.0)
+    lights[2]["type"] = hpt.LIGHT_DIRECTIONAL
+    dd = np.array([0.3, 0.5, -0.8])
+    lights[2]["u"] = dd / np.linalg.norm(dd)
+    lights[2]["v"] = (0.0, 0.0, 0.0)
+    lights[2]["intensity"] = (1.0, 0.95, 0.9)
+    lights[3]["type"] = hpt.LIGHT_QUAD
+    lights[3]["position"] = (-0.5, 1.0, 3.0)
+    lights[3]["u"] = (1.0, 0.0, 0.0)
+    lights[3]["v"] = (0.0, 0.5, 0.0)
+    lights[3]["area"] = 0.5
+    lights[3]["intensity"] = (8.0, 8.0, 8.0)
+    lights[4]["type"] = hpt.LIGHT_SPHERE
+    lights[4]["position"] = (-1.6, 0.8, 1.6)
+    lights[4]["radius"] = 0.2
+    lights[4]["area"] = 4 * math.pi * 0.04
+    lights[4]["intensity"] = (10.0, 6.0, 3.0)
+    eye, target = np.array([0.0, -4.5, 1.6]), np.array([0.0, 0.0, 0.9])
+    yfov = 2 * math.degrees(math.atan(12.0 / 50.0))
+    cam = look_at_camera(eye, target, yfov_deg=yfov, aperture=0.05, focal_distance=float(np.linalg.norm(target - eye)))
+    return hpt.Scene(cam, lights, mats, prims, ground_color=(0.05, 0.05, 0.06), sky_color=(0.1, 0.12, 0.16))
+
+
+MEDIUM_SCENE_FEATURES = hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS | hpt.FEATURE_TRANSPARENT
+
+
 def triangle_soup(count, seed=0):
     """`count` triangles: v0 uniform in the unit cube, v1 and v2 at
     v0 + lognormal(-4, 0.5) * random unit vector; Lambert 0.5."""
@@ -263,16 +410,20 @@ def triangle_soup(count, seed=0):
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
 
 
-def furnace_box(albedo=0.5, emission=1.0):
+def furnace_box(albedo=0.5, emission=1.0, half_size=10.0):
     """Camera inside a closed, flat-shaded Lambert cube that also emits:
     every path gathers emission * sum_k albedo^k exactly (no lights, black
     sky).  Flat faces keep the shading frame orthonormal, so no cosine sample
-    falls below the horizon."""
+    falls below the horizon.  The cube is large against the reference's
+    absolute ray offset (EPS = 3e-4): a bounce from within EPS of an edge can
+    place the next origin outside a small cube and let the path escape."""
     mats = hpt.new_materials(1)
     diffuse_material(mats[0], (albedo, albedo, albedo), emission=(emission, emission, emission))
     v, ix = cube_mesh()
-    cam = look_at_camera((0.1, -0.2, 0.05), (0.3, 1.0, 0.2), yfov_deg=70.0)
-    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)],
+    s = np.diag([half_size, half_size, half_size, 1.0]).astype(np.float32)
+    cam = look_at_camera((0.1 * half_size, -0.2 * half_size, 0.05 * half_size),
+                         (0.3 * half_size, half_size, 0.2 * half_size), yfov_deg=70.0)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0, s)],
                      ground_color=(0, 0, 0), sky_color=(0, 0, 0))
 
 

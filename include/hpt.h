@@ -282,8 +282,13 @@ HPT_API hpt_status hpt_commit(hpt_renderer* r);
 HPT_API hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index);
 
 /* Restart accumulation at frame 0 (what the renderer does when the camera or
- * scene changes). */
+ * scene changes); the accumulated images are cleared to zero. */
 HPT_API hpt_status hpt_reset_accumulation(hpt_renderer* r);
+
+/* Sets the frame counter (the UBO frame_index) the next frame renders with.
+ * The RNG is a pure function of (pixel, frame) (common/wltcRS.glsl:9-19), so
+ * a job resumes from its accumulated images plus this counter. */
+HPT_API hpt_status hpt_set_frame_index(hpt_renderer* r, uint64_t frame_index);
 
 /* Number of frames accumulated so far (the UBO frame_index). */
 HPT_API hpt_status hpt_get_frame_index(hpt_renderer* r, uint64_t* frame_index);
