@@ -170,31 +170,42 @@ struct LaneStack {
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
-// Walks the 8-wide tree.  Node group = (base, hits << 8 | imask); hit bit b
-// stands for slot b ^ oct.  ANY: stop at the first triangle on [tmin, tmax].
-// NODES/TRIS count visits when non-null (parity and roofline accounting).
+// Resumable walk of the 8-wide tree for one ray.  Node group =
+// (base, hits << 8 | imask); hit bit b stands for slot b ^ oct.  step()
+// visits one node and tests its triangles; ANY stops at the first triangle
+// on [tmin, tmax].  Every kernel drives the same step(): one ray per lane,
+// or persistent lanes that fetch a new ray when theirs ends.
 template <bool ANY>
-__device__ __forceinline__ bool trace_triangles(const SceneView& sv, const RayPre& r, float tmin,
-                                                float tmax, TraceResult& res, LaneStack& st,
-                                                uint32_t* nodes_visited, uint32_t* tris_tested) {
-  if (sv.node_count == 0) return false;
-  const f3 o = r.org;
+struct Walker {
+  RayPre r;
   f3 inv;
-  {
-    float dx = fabsf(r.dir.x) < 1e-30f ? copysignf(1e-30f, r.dir.x) : r.dir.x;
-    float dy = fabsf(r.dir.y) < 1e-30f ? copysignf(1e-30f, r.dir.y) : r.dir.y;
-    float dz = fabsf(r.dir.z) < 1e-30f ? copysignf(1e-30f, r.dir.z) : r.dir.z;
+  uint32_t oct;
+  float tmin, best_t;
+  uint32_t best_id;
+  float bu, bv;
+  bool found;
+  uint2 grp;
+  uint32_t nv, nt;
+
+  __device__ __forceinline__ void begin(const RayPre& ray, float t0, float t1, LaneStack& st) {
+    r = ray;
+    const float dx = fabsf(r.dir.x) < 1e-30f ? copysignf(1e-30f, r.dir.x) : r.dir.x;
+    const float dy = fabsf(r.dir.y) < 1e-30f ? copysignf(1e-30f, r.dir.y) : r.dir.y;
+    const float dz = fabsf(r.dir.z) < 1e-30f ? copysignf(1e-30f, r.dir.z) : r.dir.z;
     inv = mk3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    oct = (inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 4u : 0u);
+    tmin = t0;
+    best_t = t1;
+    best_id = MISS_ID;
+    bu = bv = 0.0f;
+    found = false;
+    nv = nt = 0;
+    st.sp = 0;
+    grp = make_uint2(0u, ((1u << oct) << 8) | 1u);
   }
-  const uint32_t oct = (inv.x < 0.0f ? 1u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 4u : 0u);
-  float best_t = tmax;
-  uint32_t best_id = MISS_ID;
-  float bu = 0.0f, bv = 0.0f;
-  bool found = false;
-  uint32_t nv = 0, nt = 0;
-  st.sp = 0;
-  uint2 grp = make_uint2(0u, ((1u << oct) << 8) | 1u);
-  for (;;) {
+
+  // One node visit; returns true when the walk is over.
+  __device__ __forceinline__ bool step(const SceneView& sv, LaneStack& st) {
     uint32_t hits = grp.y >> 8;
     const uint32_t b = __builtin_ctz(hits);
     hits &= hits - 1u;
@@ -206,6 +217,7 @@ __device__ __forceinline__ bool trace_triangles(const SceneView& sv, const RayPr
     const uint4* np = reinterpret_cast<const uint4*>(sv.nodes + ni);
     const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
     ++nv;
+    const f3 o = r.org;
     const f3 p = mk3(__uint_as_float(n0.x), __uint_as_float(n0.y), __uint_as_float(n0.z));
     const float sx = __uint_as_float((n0.w & 0xFFu) << 23);
     const float sy = __uint_as_float(((n0.w >> 8) & 0xFFu) << 23);
@@ -256,45 +268,46 @@ __device__ __forceinline__ bool trace_triangles(const SceneView& sv, const RayPr
       if (!tri_test(r, mk3(a.x, a.y, a.z), mk3(bq.x, bq.y, bq.z), mk3(cq.x, cq.y, cq.z), t, u, v)) continue;
       if (!(t >= tmin && t <= best_t)) continue;
       const uint32_t id = __float_as_uint(a.w);
-      if (ANY) {
-        found = true;
-        best_t = t;
-        best_id = id;
-        bu = u;
-        bv = v;
-        break;
-      }
-      if (found && !(t < best_t || id < best_id)) continue;
+      if (!ANY && found && !(t < best_t || id < best_id)) continue;
       found = true;
       best_t = t;
       best_id = id;
       bu = u;
       bv = v;
+      if (ANY) return true;
     }
-    if (ANY && found) break;
     if ((grp.y >> 8) == 0u) {
-      if (st.sp == 0) break;
+      if (st.sp == 0) return true;
       grp = st.pop();
     }
+    return false;
   }
-  if (nodes_visited) *nodes_visited = nv;
-  if (tris_tested) *tris_tested = nt;
-  if (found) {
-    res.t = best_t;
-    res.u = bu;
-    res.v = bv;
-    res.id = best_id;
+};
+
+template <bool ANY>
+__device__ __forceinline__ bool trace_triangles(const SceneView& sv, const RayPre& r, float tmin,
+                                                float tmax, TraceResult& res, LaneStack& st,
+                                                uint32_t* nodes_visited, uint32_t* tris_tested) {
+  if (sv.node_count == 0) return false;
+  Walker<ANY> w;
+  w.begin(r, tmin, tmax, st);
+  while (!w.step(sv, st)) {
   }
-  return found;
+  if (nodes_visited) *nodes_visited = w.nv;
+  if (tris_tested) *tris_tested = w.nt;
+  if (w.found) {
+    res.t = w.best_t;
+    res.u = w.bu;
+    res.v = w.bv;
+    res.id = w.best_id;
+  }
+  return w.found;
 }
 
-// Closest hit over triangles and analytic lights.
-__device__ __forceinline__ TraceResult trace_closest(const SceneView& sv, f3 o, f3 d, float tmin,
-                                                     float tmax, LaneStack& st,
-                                                     uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
-  TraceResult res{0.0f, 0.0f, 0.0f, MISS_ID};
-  const RayPre r = prepare_ray(o, d);
-  bool found = trace_triangles<false>(sv, r, tmin, tmax, res, st, nv, nt);
+// Analytic quad/sphere lights after the triangle walk: a light replaces the
+// triangle hit only when strictly closer; among lights the lowest index wins.
+__device__ __forceinline__ void closest_lights(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, bool found,
+                                               TraceResult& res) {
   for (uint32_t i = 0; i < sv.num_lights; ++i) {
     const hpt_light& L = sv.lights[i];
     if (L.type != HPT_LIGHT_QUAD && L.type != HPT_LIGHT_SPHERE) continue;
@@ -305,14 +318,9 @@ __device__ __forceinline__ TraceResult trace_closest(const SceneView& sv, f3 o, 
     found = true;
     res = TraceResult{t, 0.0f, 0.0f, LIGHT_BIT | i};
   }
-  return res;
 }
 
-__device__ __forceinline__ bool trace_any(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
-                                          LaneStack& st, uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
-  TraceResult res;
-  const RayPre r = prepare_ray(o, d);
-  if (trace_triangles<true>(sv, r, tmin, tmax, res, st, nv, nt)) return true;
+__device__ __forceinline__ bool any_light(const SceneView& sv, f3 o, f3 d, float tmin, float tmax) {
   for (uint32_t i = 0; i < sv.num_lights; ++i) {
     const hpt_light& L = sv.lights[i];
     if (L.type != HPT_LIGHT_QUAD && L.type != HPT_LIGHT_SPHERE) continue;
@@ -321,6 +329,25 @@ __device__ __forceinline__ bool trace_any(const SceneView& sv, f3 o, f3 d, float
     if (t >= tmin && t <= tmax) return true;
   }
   return false;
+}
+
+// Closest hit over triangles and analytic lights.
+__device__ __forceinline__ TraceResult trace_closest(const SceneView& sv, f3 o, f3 d, float tmin,
+                                                     float tmax, LaneStack& st,
+                                                     uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
+  TraceResult res{0.0f, 0.0f, 0.0f, MISS_ID};
+  const RayPre r = prepare_ray(o, d);
+  const bool found = trace_triangles<false>(sv, r, tmin, tmax, res, st, nv, nt);
+  closest_lights(sv, o, d, tmin, tmax, found, res);
+  return res;
+}
+
+__device__ __forceinline__ bool trace_any(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
+                                          LaneStack& st, uint32_t* nv = nullptr, uint32_t* nt = nullptr) {
+  TraceResult res;
+  const RayPre r = prepare_ray(o, d);
+  if (trace_triangles<true>(sv, r, tmin, tmax, res, st, nv, nt)) return true;
+  return any_light(sv, o, d, tmin, tmax);
 }
 
 }  // namespace hpt
