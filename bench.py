@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -61,6 +62,20 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, budget_s=15.0):
                       f"{dt:.1f} s)"}
 
 
+def profiled_traffic(frames_in_flight):
+    """HBM bytes per k_extend launch from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the reading of a
+    1 GiB copy), when it was taken at the same batch size; else None."""
+    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_1080p.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch k_extend \(frames_in_flight (\d+)\) = (\d+)", line)
+        if m and int(m.group(1)) == frames_in_flight:
+            return int(m.group(2))
+    return None
+
+
 def copy_calibration(mib=1024, reps=4):
     """Plain device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -84,6 +99,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--frames-in-flight", type=int, default=0,
+                    help="frames traced together per batch (0: sized by the library)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run known-size copies (rocprof counter calibration)")
     args = ap.parse_args()
@@ -101,9 +118,12 @@ def main():
     log(f"{scene.triangle_count} triangles; creating renderer")
     r = hpt.HalaRenderer(W, H, max_depth=DEPTH, rr_depth=RR)
     r.set_scene(scene)
+    r.frames_in_flight = args.frames_in_flight
     t0 = time.perf_counter()
     r.commit()
-    log(f"commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}")
+    log(f"commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
+        f"{r.frames_in_flight} frames in flight")
+    fif = r.frames_in_flight
 
     # Visit counts per ray from a counted warm-up (separate kernel variant).
     r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
@@ -155,9 +175,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "disney_grid_env_1080p", "width": W, "height": H, "spp": args.steps,
                    "max_depth": DEPTH, "rr_depth": RR, "triangles": scene.triangle_count,
-                   "env_map": [2048, 1024], "mrays_per_s": round(rays_total / wall / 1e6, 2)},
+                   "env_map": [2048, 1024], "frames_in_flight": fif,
+                   "mrays_per_s": round(rays_total / wall / 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(fif),
                      "kernel": "k_extend", "avg_launch_ms": round(ext_ms, 4),
                      "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
         "cpu_baseline": base,

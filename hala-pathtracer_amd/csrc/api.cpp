@@ -8,6 +8,7 @@
 //              accumulate, all on the handle's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -105,7 +106,9 @@ struct hpt_renderer {
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
   DevBuf d_accum, d_albedo, d_normal, d_final;
-  uint32_t npaths = 0;
+  uint32_t npix = 0;          // width * height
+  uint32_t batch_cap = 0;     // frames the path buffers hold
+  uint32_t frames_in_flight = 0;  // requested batch size, 0 = sized from kPathBudget
   uint64_t frame_index = 0;
   // Profiling.
   int profiling = 0;
@@ -295,6 +298,8 @@ FrameParams frame_params(const hpt_renderer* r, uint32_t camera_index) {
   fp.max_depth = r->info.max_depth;
   fp.rr_depth = r->info.rr_depth;
   fp.frame_index = (uint32_t)r->frame_index;
+  fp.batch_frames = 1;
+  fp.pixels = r->npix;
   fp.camera_index = camera_index;
   fp.env_type = r->env_type;
   fp.env_w = r->env_w;
@@ -311,67 +316,109 @@ FrameParams frame_params(const hpt_renderer* r, uint32_t camera_index) {
   return fp;
 }
 
-void render_frame(hpt_renderer* r, uint32_t camera_index) {
+// Traces `frames` consecutive frames (starting at r->frame_index) as one
+// wavefront of frames * width * height paths.
+void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
   const SceneView sv = scene_view(r);
-  const FrameParams fp = frame_params(r, camera_index);
+  FrameParams fp = frame_params(r, camera_index);
+  fp.batch_frames = frames;
   const PathState ps = path_state(r);
-  const uint32_t n = r->npaths;
+  const uint32_t n = frames * r->npix;
+  const SegGeom g = seg_geom(n);
   uint32_t* counts = r->d_counts.as<uint32_t>();
-  uint32_t* qbuf[2] = {r->d_qa.as<uint32_t>(), r->d_qb.as<uint32_t>()};
-  ShadowQueue sq{r->d_sh_o.as<float4>(), r->d_sh_d.as<float4>(), r->d_sh_c.as<float4>()};
-  const Spill sp = ensure_spill(r, round_threads(2 * n));
+  // Segment counters: path queues A and B (ping-pong), then the shadow queue.
+  const Queue qbuf[2] = {{r->d_qa.as<uint32_t>(), counts, g.seg_cap},
+                         {r->d_qb.as<uint32_t>(), counts + kSeg * kSegStride, g.seg_cap}};
+  const ShadowQueue sq{r->d_sh_o.as<float4>(), r->d_sh_d.as<float4>(), r->d_sh_c.as<float4>(),
+                       counts + 2 * kSeg * kSegStride, 2 * g.seg_cap};
+  const Spill sp = ensure_spill(r, 2 * kSeg * g.seg_cap);  // threads of the widest launch (shadow)
   hipStream_t s = r->stream;
   StageTimer* t;
   auto* stats = r->profiling ? r->d_stats.as<unsigned long long>() : nullptr;
   const bool count = (r->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
-  const StageCounters sc_extend{stats ? stats + 3 * HPT_STAGE_EXTEND : nullptr, count};
-  const StageCounters sc_shadow{stats ? stats + 3 * HPT_STAGE_SHADOW : nullptr, count};
+  const StageCounters sc_extend{stats ? stats + kStatsPerStage * HPT_STAGE_EXTEND : nullptr, count};
+  const StageCounters sc_shadow{stats ? stats + kStatsPerStage * HPT_STAGE_SHADOW : nullptr, count};
 
   stage_begin(r, HPT_STAGE_CAMERA, t);
   launch_camera(s, sv, fp, ps, n);
   stage_end(r, t);
   check(hipGetLastError(), "camera launch");
 
-  // counts[0], counts[1]: path queues (ping-pong); counts[2]: shadow queue.
-  const uint32_t* qin = nullptr;
-  const uint32_t* qin_count = nullptr;
+  Queue qin{nullptr, nullptr, g.seg_cap};  // the first bounce walks the camera paths in order
   int out = 0;
   const bool transparent = (r->info.features & HPT_FEATURE_TRANSPARENT) != 0;
   const uint32_t fixed_bounces = r->info.max_depth + 1;
+  std::vector<uint32_t> seg_counts;
   for (uint32_t b = 0;; ++b) {
     if (b >= fixed_bounces) {
       // Pass-through of transparent surfaces does not count toward depth
       // (pathtracer.rgen.glsl:199): keep going while paths remain.
       if (!transparent) break;
-      uint32_t live = 0;
-      check(hipMemcpyAsync(&live, qin_count, 4, hipMemcpyDeviceToHost, s), "queue count");
+      seg_counts.resize(kSeg * kSegStride);
+      check(hipMemcpyAsync(seg_counts.data(), qin.counts, seg_counts.size() * 4, hipMemcpyDeviceToHost, s),
+            "queue count");
       check(hipStreamSynchronize(s), "sync");
+      uint64_t live = 0;
+      for (uint32_t i = 0; i < kSeg; ++i) live += seg_counts[i * kSegStride];
       if (live == 0 || b > 4096) break;
     }
-    uint32_t* qout = qbuf[out];
-    uint32_t* qout_count = counts + out;
+    const Queue& qout = qbuf[out];
     stage_begin(r, HPT_STAGE_EXTEND, t);
-    launch_extend(s, sv, ps, qin, qin_count, n, qout_count, counts + 2, sp, sc_extend);
+    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend);
     stage_end(r, t);
     stage_begin(r, HPT_STAGE_SHADE, t);
-    launch_shade(s, r->info.features, sv, fp, ps, qin, qin_count, n, qout, qout_count, sq, counts + 2, sp);
+    launch_shade(s, r->info.features, sv, fp, ps, qin, n, qout, sq, sp);
     stage_end(r, t);
     const bool vmis = (r->info.features & HPT_FEATURE_MEDIUM) && (r->info.features & HPT_FEATURE_VOL_MIS);
     if (!vmis) {
       stage_begin(r, HPT_STAGE_SHADOW, t);
-      launch_shadow(s, sv, ps, sq, counts + 2, 2 * n, sp, sc_shadow);
+      launch_shadow(s, sv, ps, sq, n, sp, sc_shadow);
       stage_end(r, t);
     }
     check(hipGetLastError(), "bounce launch");
     qin = qout;
-    qin_count = qout_count;
     out ^= 1;
   }
   Images im{r->d_accum.as<float4>(), r->d_albedo.as<float4>(), r->d_normal.as<float4>(), r->d_final.as<uchar4>()};
   stage_begin(r, HPT_STAGE_ACCUMULATE, t);
-  launch_accumulate(s, fp, ps, im, n);
+  launch_accumulate(s, fp, ps, im, r->npix);
   stage_end(r, t);
   check(hipGetLastError(), "accumulate launch");
+}
+
+// Device memory a batch may take for path state and queues; 1080p fits 8
+// frames in about 5 GB.
+constexpr size_t kPathBudget = size_t(6) << 30;
+constexpr uint32_t kMaxBatchFrames = 64;
+// Bytes per path: 11 float4 path fields, the float2 medium field, two queue
+// slots and two shadow-queue entries of 3 float4.
+constexpr size_t kPathBytes = 11 * 16 + 8 + 2 * 4 + 2 * 48;  // (+ segment slack)
+
+uint32_t batch_frames_for(const hpt_renderer* r) {
+  if (r->frames_in_flight) return r->frames_in_flight;
+  const size_t per_frame = (size_t)r->npix * kPathBytes;
+  size_t k = kPathBudget / (per_frame ? per_frame : 1);
+  if (k > 8) k = 8;
+  if (k < 1) k = 1;
+  while (k > 1 && (uint64_t)k * r->npix * 2 > 0xFFFFFFFFull) --k;  // queue indices are 32-bit
+  return (uint32_t)k;
+}
+
+// Sizes the path state and the queues for `frames` frames per batch.
+void alloc_paths(hpt_renderer* r, uint32_t frames) {
+  const size_t n = (size_t)frames * r->npix;
+  const size_t f4 = n * 16;
+  for (DevBuf* b : {&r->d_rng, &r->d_org, &r->d_dir, &r->d_thr, &r->d_rad, &r->d_alb, &r->d_nrm, &r->d_med,
+                    &r->d_nee_e, &r->d_nee_l, &r->d_hit})
+    b->alloc(f4);
+  r->d_med2.alloc(n * 8);
+  const size_t qn = (size_t)kSeg * seg_geom((uint32_t)n).seg_cap;  // segment capacity, >= n
+  r->d_qa.alloc(qn * 4);
+  r->d_qb.alloc(qn * 4);
+  r->d_sh_o.alloc(2 * qn * 16);
+  r->d_sh_d.alloc(2 * qn * 16);
+  r->d_sh_c.alloc(2 * qn * 16);
+  r->batch_cap = frames;
 }
 
 void require_committed(const hpt_renderer* r) {
@@ -615,27 +662,11 @@ hpt_status hpt_commit(hpt_renderer* r) {
     upload(r->d_env_table, r->env_table, s);
     upload(r->d_blue_noise, r->blue_noise, s);
     const uint32_t n = r->info.width * r->info.height;
-    r->npaths = n;
+    r->npix = n;
+    alloc_paths(r, batch_frames_for(r));
     const size_t f4 = (size_t)n * 16;
-    r->d_rng.alloc(f4);
-    r->d_org.alloc(f4);
-    r->d_dir.alloc(f4);
-    r->d_thr.alloc(f4);
-    r->d_rad.alloc(f4);
-    r->d_alb.alloc(f4);
-    r->d_nrm.alloc(f4);
-    r->d_med.alloc(f4);
-    r->d_med2.alloc((size_t)n * 8);
-    r->d_nee_e.alloc(f4);
-    r->d_nee_l.alloc(f4);
-    r->d_hit.alloc(f4);
-    r->d_qa.alloc((size_t)n * 4);
-    r->d_qb.alloc((size_t)n * 4);
-    r->d_sh_o.alloc(2 * f4);
-    r->d_sh_d.alloc(2 * f4);
-    r->d_sh_c.alloc(2 * f4);
-    r->d_counts.alloc(64);
-    check(hipMemsetAsync(r->d_counts.p, 0, 64, s), "hipMemsetAsync");
+    r->d_counts.alloc(3 * kSeg * kSegStride * sizeof(uint32_t));
+    check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
     r->d_accum.alloc(f4);
     r->d_albedo.alloc(f4);
     r->d_normal.alloc(f4);
@@ -656,10 +687,20 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
     require_committed(r);
     need(camera_index < r->camera_count, "camera index out of range");
     set_device(r);
-    for (uint32_t f = 0; f < frames; ++f) {
-      if (r->info.max_samples && r->frame_index >= r->info.max_samples) break;
-      render_frame(r, camera_index);
-      r->frame_index++;
+    const uint32_t cap = batch_frames_for(r);
+    if (cap != r->batch_cap) alloc_paths(r, cap);
+    // Queue counters start from zero on every call, whatever an earlier
+    // failed launch left behind.
+    check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, r->stream), "hipMemsetAsync");
+    uint64_t left = frames;
+    if (r->info.max_samples)
+      left = r->frame_index >= r->info.max_samples ? 0 : std::min<uint64_t>(left, r->info.max_samples - r->frame_index);
+    left = std::min<uint64_t>(left, 0xFFFFFFFFull - r->frame_index);  // frame indices stay 32-bit
+    while (left) {
+      const uint32_t k = (uint32_t)std::min<uint64_t>(left, cap);
+      render_batch(r, camera_index, k);
+      r->frame_index += k;
+      left -= k;
     }
     if (r->profiling & 1) collect_timers(r);
   });
@@ -829,6 +870,23 @@ hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes, void*
   });
 }
 
+hpt_status hpt_set_frames_in_flight(hpt_renderer* r, uint32_t frames) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(frames <= kMaxBatchFrames, "at most 64 frames in flight");
+    const uint64_t px = (uint64_t)r->info.width * r->info.height;
+    need((uint64_t)frames * px <= (1ull << 31), "frames * width * height must not exceed 2^31");
+    r->frames_in_flight = frames;
+  });
+}
+
+hpt_status hpt_get_frames_in_flight(hpt_renderer* r, uint32_t* frames) {
+  return guarded([&] {
+    need(r && frames, "null argument");
+    *frames = r->npix ? batch_frames_for(r) : r->frames_in_flight;
+  });
+}
+
 hpt_status hpt_set_profiling(hpt_renderer* r, int enable) {
   return guarded([&] {
     need(r, "null renderer");
@@ -839,7 +897,7 @@ hpt_status hpt_set_profiling(hpt_renderer* r, int enable) {
     for (auto& s : r->stats) s = hpt_stage_stats{};
     r->timers_used = 0;
     if (enable) {
-      r->d_stats.alloc(3 * HPT_STAGE_COUNT * sizeof(unsigned long long));
+      r->d_stats.alloc((size_t)kStatsPerStage * HPT_STAGE_COUNT * sizeof(unsigned long long));
       check(hipMemsetAsync(r->d_stats.p, 0, r->d_stats.bytes, r->stream), "hipMemsetAsync");
       check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
     }
@@ -854,12 +912,17 @@ hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out)
     if (r->timers_used) collect_timers(r);
     *out = r->stats[stage];
     if (r->profiling && r->d_stats.p) {
-      unsigned long long v[3 * HPT_STAGE_COUNT];
+      std::vector<unsigned long long> v(kStatsPerStage);
       check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
-      check(hipMemcpy(v, r->d_stats.p, sizeof v, hipMemcpyDeviceToHost), "stats");
-      out->rays = v[3 * stage];
-      out->node_visits = v[3 * stage + 1];
-      out->tri_tests = v[3 * stage + 2];
+      check(hipMemcpy(v.data(), r->d_stats.as<unsigned long long>() + (size_t)kStatsPerStage * stage,
+                      v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost),
+            "stats");
+      uint64_t sum[3] = {0, 0, 0};
+      for (uint32_t j = 0; j < 3; ++j)
+        for (uint32_t sg = 0; sg < kSeg; ++sg) sum[j] += v[(j * kSeg + sg) * kStatStride];
+      out->rays = sum[0];
+      out->node_visits = sum[1];
+      out->tri_tests = sum[2];
     }
   });
 }

@@ -151,7 +151,9 @@ struct FrameParams {
   float res_x, res_y;
   uint32_t width, height;
   uint32_t max_depth, rr_depth;
-  uint32_t frame_index;
+  uint32_t frame_index;   // frame of the first path slice of the batch
+  uint32_t batch_frames;  // frames traced together: path p is pixel p % pixels of frame frame_index + p / pixels
+  uint32_t pixels;        // width * height
   uint32_t camera_index;
   uint32_t env_type;
   uint32_t env_w, env_h;

@@ -683,10 +683,9 @@ __device__ __forceinline__ f3 env_eval(const SceneView& sv, const FrameParams& f
 // ----------------------------------------------------------------- cameras
 // perspective_ray.rcall.glsl:12-64 / orthographic_ray.rcall.glsl:12-45 with
 // rand4blue (common/wltcRS.glsl:46-53) over the 1024x1024 RGBA8 texture.
-__device__ __forceinline__ void camera_ray(const SceneView& sv, const FrameParams& fp, uint32_t x, uint32_t y,
-                                           f3& org, f3& dir) {
+__device__ __forceinline__ void camera_ray(const SceneView& sv, const FrameParams& fp, uint32_t frame, uint32_t x,
+                                           uint32_t y, f3& org, f3& dir) {
 #pragma clang fp contract(off)
-  const uint32_t frame = fp.frame_index;
   uint4 s1 = make_uint4(frame, frame * 15843u, frame * 31u + 4566u, frame * 2345u + 58585u);
   pcg4d(s1);
   const int tx = ((int)x + (int)(s1.x % 0x0fffffffu)) % 1024;

@@ -25,10 +25,44 @@ constexpr uint32_t PATH_ANB = 1u;             // any_non_specular_bounce
 constexpr uint32_t PATH_IN_MEDIUM = 2u;       // is_in_medium
 constexpr uint32_t PATH_SURFACE_SCATTER = 4u; // is_surface_scatter (MEDIUM without VOL_MIS)
 
+constexpr int kBlock = 256;
+
+// Compacted queues are split into kSeg segments, each with its own append
+// counter kSegStride words (256 B) from the next.  Block b of a launch reads
+// segment b % kSeg and appends to the same segment, so appends spread over
+// kSeg counters -- a single counter serialises one atomic per wave at about
+// 11 ns on MI355X -- and, with blocks dealt round-robin to the 8 XCDs, a
+// path keeps to one XCD from bounce to bounce.
+constexpr uint32_t kSeg = 64;
+constexpr uint32_t kSegStride = 64;
+
+// Segment geometry of a launch over at most n_max paths: the first bounce
+// runs grid_for(n_max) blocks over the identity queue; later launches run
+// kSeg * chunks blocks, block b taking entries (b / kSeg) * kBlock + tid of
+// segment b % kSeg.  Segments hold chunks * kBlock paths (twice that for
+// shadow rays), which bounds what one segment's producer blocks can append.
+struct SegGeom {
+  uint32_t chunks;   // blocks per segment
+  uint32_t seg_cap;  // path entries per segment = chunks * kBlock
+};
+inline SegGeom seg_geom(uint32_t n_max) {
+  const uint32_t blocks = (n_max + kBlock - 1) / kBlock;
+  const uint32_t chunks = (blocks + kSeg - 1) / kSeg;
+  return SegGeom{chunks, chunks * (uint32_t)kBlock};
+}
+
+struct Queue {
+  uint32_t* items;   // path ids, segment s at items + s * seg_cap; null = identity over n_max
+  uint32_t* counts;  // segment s count at counts[s * kSegStride]
+  uint32_t seg_cap;
+};
+
 struct ShadowQueue {
   float4* o;  // origin, .w = tmax
   float4* d;  // direction, .w = (path << 1 | slot) bits
   float4* c;  // throughput-weighted contribution
+  uint32_t* counts;  // segment s holds entries [s * seg_cap, s * seg_cap + counts[s * kSegStride])
+  uint32_t seg_cap;  // 2 * the path segment capacity
 };
 
 struct Spill {
@@ -44,27 +78,30 @@ struct Images {
   uchar4* final_rgba8;
 };
 
-constexpr int kBlock = 256;
-
 void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                    uint32_t npaths);
-// Device counters of one stage: stats[stage*3 + {0,1,2}] = rays, node
-// visits, triangle tests (64-bit).  Rays are counted whenever stats is
-// non-null; visits only by the counting kernel variant (count == true).
+// Device counters of one stage, 64-bit, spread like the queue counters:
+// rays at stats[seg * kStatStride], node visits at stats[(kSeg + seg) *
+// kStatStride], triangle tests at stats[(2 * kSeg + seg) * kStatStride].
+// Rays are counted whenever stats is non-null; visits only by the counting
+// kernel variant (count == true).
+constexpr uint32_t kStatStride = 32;
+constexpr uint32_t kStatsPerStage = 3 * kSeg * kStatStride;
 struct StageCounters {
   unsigned long long* stats;
   bool count;
 };
 
-// q == nullptr: the queue is the identity over n_max paths.
-void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const uint32_t* q,
-                   const uint32_t* qcount, uint32_t n_max, uint32_t* zero0, uint32_t* zero1, Spill sp,
-                   StageCounters sc);
+// qin.items == nullptr: the first bounce, the identity queue over n_max
+// paths.  Extend zeroes the counters of qnext and sq, which the shade launch
+// after it fills.
+void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc);
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp,
-                  const PathState& ps, const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max,
-                  uint32_t* qout, uint32_t* qout_count, const ShadowQueue& sq, uint32_t* sq_count, Spill sp);
-void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq,
-                   const uint32_t* sq_count, uint32_t n_max, Spill sp, StageCounters sc);
+                  const PathState& ps, const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq,
+                  Spill sp);
+void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
+                   Spill sp, StageCounters sc);
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im,
                        uint32_t npaths);
 

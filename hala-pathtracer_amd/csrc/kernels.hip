@@ -41,15 +41,43 @@ __device__ __forceinline__ uint32_t wave_reserve(bool want, uint32_t* count) {
   return base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
 }
 
+// Appends `val` to this block's segment of q.
+__device__ __forceinline__ void seg_append(bool want, const Queue& q, uint32_t val) {
+  const uint32_t s = blockIdx.x % kSeg;
+  wave_append(want, q.counts + s * kSegStride, q.items + s * q.seg_cap, val);
+}
+
+// Shadow-queue position for lanes with want == true, in this block's segment.
+__device__ __forceinline__ uint32_t seg_reserve(bool want, const ShadowQueue& sq) {
+  const uint32_t s = blockIdx.x % kSeg;
+  return s * sq.seg_cap + wave_reserve(want, sq.counts + s * kSegStride);
+}
+
+// This thread's entry in a launch over a segmented queue: entry j of
+// segment seg, live while j < n (the segment's count).
+struct SegSlot {
+  uint32_t seg, j, n;
+};
+__device__ __forceinline__ SegSlot seg_slot(const uint32_t* counts) {
+  SegSlot t;
+  t.seg = blockIdx.x % kSeg;
+  t.j = (blockIdx.x / kSeg) * kBlock + threadIdx.x;
+  t.n = counts[t.seg * kSegStride];
+  return t;
+}
+
 // ------------------------------------------------------------------ camera
 // pathtracer.rgen.glsl:23-54: seed, camera callable, payload initialisation.
+// A batch holds batch_frames consecutive frames, one path per pixel each.
 __global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp, PathState ps, uint32_t npaths) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= npaths) return;
-  const uint32_t x = p % fp.width, y = p / fp.width;
+  const uint32_t j = p / fp.pixels, pix = p - j * fp.pixels;
+  const uint32_t x = pix % fp.width, y = pix / fp.width;
+  const uint32_t frame = fp.frame_index + j;
   f3 o, d;
-  camera_ray(sv, fp, x, y, o, d);
-  ps.rng[p] = make_uint4(x, y, fp.frame_index, x + y);
+  camera_ray(sv, fp, frame, x, y, o, d);
+  ps.rng[p] = make_uint4(x, y, frame, x + y);
   ps.org[p] = make_float4(o.x, o.y, o.z, 1.0f);
   ps.dir[p] = make_float4(d.x, d.y, d.z, __uint_as_float(0u));
   ps.thr[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
@@ -63,7 +91,7 @@ __global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp,
 }
 
 // Adds per-lane node/triangle counts to the stage counters: a wave sum, then
-// one atomic per wave.
+// one atomic per wave on this block's counter segment.
 __device__ __forceinline__ void add_visits(unsigned long long* stats, uint32_t nv, uint32_t nt) {
   unsigned long long a = nv, b = nt;
   for (int off = 32; off > 0; off >>= 1) {
@@ -71,29 +99,44 @@ __device__ __forceinline__ void add_visits(unsigned long long* stats, uint32_t n
     b += __shfl_xor(b, off);
   }
   if (lane_id() == 0) {
-    atomicAdd(stats + 1, a);
-    atomicAdd(stats + 2, b);
+    const uint32_t s = blockIdx.x % kSeg;
+    atomicAdd(stats + (kSeg + s) * kStatStride, a);
+    atomicAdd(stats + (2 * kSeg + s) * kStatStride, b);
   }
+}
+
+// Path id of this thread for a launch over qin (identity when qin.items is
+// null) and the ray count for the stage counters.
+__device__ __forceinline__ bool queue_entry(const Queue& qin, uint32_t n_max, unsigned long long* stats,
+                                            uint32_t& p) {
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (!qin.items) {
+    if (stats && gid == 0) atomicAdd(stats, (unsigned long long)n_max);
+    p = gid;
+    return gid < n_max;
+  }
+  const SegSlot t = seg_slot(qin.counts);
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
+  p = t.j < t.n ? qin.items[t.seg * qin.seg_cap + t.j] : 0u;
+  return t.j < t.n;
 }
 
 // ------------------------------------------------------------------ extend
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_extend(SceneView sv, PathState ps, const uint32_t* q, const uint32_t* qcount,
-                                                   uint32_t n_max, uint32_t* zero0, uint32_t* zero1, Spill sp,
-                                                   unsigned long long* stats) {
+__global__ void __launch_bounds__(kBlock) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
+                                                   ShadowQueue sq, Spill sp, unsigned long long* stats) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = q ? *qcount : n_max;
-  if (k == 0) {
-    if (zero0) *zero0 = 0;
-    if (zero1) *zero1 = 0;
-    if (stats) atomicAdd(stats, (unsigned long long)n);
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (gid < kSeg) {  // the shade launch after this one appends to these
+    qnext.counts[gid * kSegStride] = 0;
+    sq.counts[gid * kSegStride] = 0;
   }
+  uint32_t p;
+  const bool active = queue_entry(qin, n_max, stats, p);
   uint32_t nv = 0, nt = 0;
-  if (k < n) {
-    const uint32_t p = q ? q[k] : k;
+  if (active) {
     const float4 o4 = ps.org[p], d4 = ps.dir[p];
-    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
     const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st,
                                         COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
     ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
@@ -259,23 +302,25 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4&
   return Ld;
 }
 
-// One iteration of the bounce loop, pathtracer.rgen.glsl:70-256.
+// One iteration of the bounce loop, pathtracer.rgen.glsl:70-256.  The
+// occupancy target trades register spills for latency hiding.
+#ifndef HPT_SHADE_WAVES_PER_EU
+#define HPT_SHADE_WAVES_PER_EU 3
+#endif
 template <bool MED, bool VMIS, bool TRANSP>
-__global__ void __launch_bounds__(kBlock)
-    k_shade(SceneView sv, FrameParams fp, PathState ps, const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max,
-            uint32_t* qout, uint32_t* qout_count, ShadowQueue sq, uint32_t* sq_count, Spill sp) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_SHADE_WAVES_PER_EU)))
+    k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
+            Spill sp) {
   __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = qin ? *qin_count : n_max;
-  const bool active = k < n;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t p;
+  const bool active = queue_entry(qin, n_max, nullptr, p);
   bool cont = false;
   ShadowOut s_env, s_light;
   s_env.valid = false;
   s_light.valid = false;
-  uint32_t p = 0;
   if (active) {
-    p = qin ? qin[k] : k;
-    LaneStack st{lds_stack + (VMIS ? threadIdx.x : 0), kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    LaneStack st{lds_stack + (VMIS ? threadIdx.x : 0), kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
     const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p], h4 = ps.hit[p];
     f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
     float pdf = o4.w;
@@ -427,11 +472,11 @@ __global__ void __launch_bounds__(kBlock)
     ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
   }
   // Compact the survivors and the shadow rays.
-  wave_append(cont, qout_count, qout, p);
+  seg_append(cont, qout, p);
   if (!VMIS) {
     const bool w0 = s_env.valid, w1 = s_light.valid;
-    const uint32_t i0 = wave_reserve(w0, sq_count);
-    const uint32_t i1 = wave_reserve(w1, sq_count);
+    const uint32_t i0 = seg_reserve(w0, sq);
+    const uint32_t i1 = seg_reserve(w1, sq);
     if (w0) {
       sq.o[i0] = make_float4(s_env.o.x, s_env.o.y, s_env.o.z, s_env.tmax);
       sq.d[i0] = make_float4(s_env.d.x, s_env.d.y, s_env.d.z, __uint_as_float(p << 1));
@@ -447,16 +492,17 @@ __global__ void __launch_bounds__(kBlock)
 
 // ------------------------------------------------------------------ shadow
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, const uint32_t* sq_count,
-                                                   Spill sp, unsigned long long* stats) {
+__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, Spill sp,
+                                                   unsigned long long* stats) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = *sq_count;
-  if (k == 0 && stats) atomicAdd(stats, (unsigned long long)n);
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const SegSlot t = seg_slot(sq.counts);
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
   uint32_t nv = 0, nt = 0;
-  if (k < n) {
+  if (t.j < t.n) {
+    const uint32_t k = t.seg * sq.seg_cap + t.j;
     const float4 o4 = sq.o[k], d4 = sq.d[k];
-    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
     const bool occluded = trace_any(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, o4.w, st,
                                     COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
     if (!occluded) {
@@ -485,27 +531,42 @@ __device__ __forceinline__ unsigned char unorm8(float x) {
 }
 __device__ __forceinline__ f3 row_mul(f3 v, f3 r0, f3 r1, f3 r2) { return mk3(dot(v, r0), dot(v, r1), dot(v, r2)); }
 
-// pathtracer.rgen.glsl:259-304 with common/tonemap.glsl.
-__global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState ps, Images im, uint32_t npaths) {
+// pathtracer.rgen.glsl:259-304 with common/tonemap.glsl.  The batch's frames
+// are folded into the running means in frame order, one mix per frame with
+// the weight 1 / (frame + 1) of that frame, so the images equal those of
+// frame-by-frame rendering; the final image is the one after the last frame.
+__global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState ps, Images im, uint32_t npix) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= npaths) return;
-  const float4 r = ps.rad[p], e = ps.nee_env[p], l = ps.nee_light[p];
-  const f3 L = mk3(r.x, r.y, r.z) + mk3(e.x, e.y, e.z) + mk3(l.x, l.y, l.z);
-  const float4 a4 = ps.albedo[p], n4 = ps.normal[p];
-  const float w = 1.0f / ((float)fp.frame_index + 1.0f);
-  auto acc = [&](float4* img, f3 v) {
-    f3 nv = v;
-    if (fp.frame_index != 0) {
-      const float4 o = img[p];
-      nv = mix3(mk3(o.x, o.y, o.z), v, w);
+  if (p >= npix) return;
+  f3 acc_c = mk3(0.0f), acc_a = mk3(0.0f), acc_n = mk3(0.0f);
+  if (fp.frame_index != 0) {
+    const float4 c4 = im.accum[p], a4 = im.albedo[p], n4 = im.normal[p];
+    acc_c = mk3(c4.x, c4.y, c4.z);
+    acc_a = mk3(a4.x, a4.y, a4.z);
+    acc_n = mk3(n4.x, n4.y, n4.z);
+  }
+  for (uint32_t j = 0; j < fp.batch_frames; ++j) {
+    const uint32_t q = j * npix + p;
+    const uint32_t frame = fp.frame_index + j;
+    const float4 r = ps.rad[q], e = ps.nee_env[q], l = ps.nee_light[q];
+    const f3 L = mk3(r.x, r.y, r.z) + mk3(e.x, e.y, e.z) + mk3(l.x, l.y, l.z);
+    const float4 a4 = ps.albedo[q], n4 = ps.normal[q];
+    const f3 A = mk3(a4.x, a4.y, a4.z), N = mk3(n4.x, n4.y, n4.z);
+    if (frame == 0) {
+      acc_c = L;
+      acc_a = A;
+      acc_n = N;
+    } else {
+      const float w = 1.0f / ((float)frame + 1.0f);
+      acc_c = mix3(acc_c, L, w);
+      acc_a = mix3(acc_a, A, w);
+      acc_n = mix3(acc_n, N, w);
     }
-    img[p] = make_float4(nv.x, nv.y, nv.z, 1.0f);
-    return nv;
-  };
-  f3 c = acc(im.accum, L);
-  acc(im.albedo, mk3(a4.x, a4.y, a4.z));
-  acc(im.normal, mk3(n4.x, n4.y, n4.z));
-  c = c * powf(2.0f, fp.exposure_value);
+  }
+  im.accum[p] = make_float4(acc_c.x, acc_c.y, acc_c.z, 1.0f);
+  im.albedo[p] = make_float4(acc_a.x, acc_a.y, acc_a.z, 1.0f);
+  im.normal[p] = make_float4(acc_n.x, acc_n.y, acc_n.z, 1.0f);
+  f3 c = acc_c * powf(2.0f, fp.exposure_value);
   if (fp.enable_tonemap) {
     if (fp.enable_aces) {
       if (fp.use_simple_aces) {
@@ -559,31 +620,34 @@ void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, co
   hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths)), dim3(kBlock), 0, s, sv, fp, ps, npaths);
 }
 
-void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const uint32_t* q, const uint32_t* qcount,
-                   uint32_t n_max, uint32_t* zero0, uint32_t* zero1, Spill sp, StageCounters sc) {
+// Blocks of a launch over qin: the identity queue's grid, or kSeg blocks per
+// chunk of the segment capacity.
+static inline uint32_t queue_grid(const Queue& qin, uint32_t n_max) {
+  return qin.items ? kSeg * seg_geom(n_max).chunks : grid_for(n_max);
+}
+
+void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc) {
+  const dim3 grid(queue_grid(qin, n_max));
   if (sc.count && sc.stats)
-    hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, q, qcount, n_max, zero0,
-                       zero1, sp, sc.stats);
+    hipLaunchKernelGGL(k_extend<true>, grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, q, qcount, n_max, zero0,
-                       zero1, sp, sc.stats);
+    hipLaunchKernelGGL(k_extend<false>, grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats);
 }
 
 template <bool MED, bool VMIS, bool TRANSP>
 static void shade_variant(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                          const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max, uint32_t* qout,
-                          uint32_t* qout_count, const ShadowQueue& sq, uint32_t* sq_count, Spill sp) {
-  hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP>), dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, fp, ps, qin,
-                     qin_count, n_max, qout, qout_count, sq, sq_count, sp);
+                          const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp) {
+  hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp, ps,
+                     qin, n_max, qout, sq, sp);
 }
 
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                  const uint32_t* qin, const uint32_t* qin_count, uint32_t n_max, uint32_t* qout, uint32_t* qout_count,
-                  const ShadowQueue& sq, uint32_t* sq_count, Spill sp) {
+                  const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp) {
   const bool med = features & HPT_FEATURE_MEDIUM;
   const bool vmis = med && (features & HPT_FEATURE_VOL_MIS);
   const bool tr = features & HPT_FEATURE_TRANSPARENT;
-#define HPT_SHADE(M, V, T) shade_variant<M, V, T>(s, sv, fp, ps, qin, qin_count, n_max, qout, qout_count, sq, sq_count, sp)
+#define HPT_SHADE(M, V, T) shade_variant<M, V, T>(s, sv, fp, ps, qin, n_max, qout, sq, sp)
   if (!med) {
     if (tr) HPT_SHADE(false, false, true);
     else HPT_SHADE(false, false, false);
@@ -597,12 +661,13 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
 #undef HPT_SHADE
 }
 
-void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq,
-                   const uint32_t* sq_count, uint32_t n_max, Spill sp, StageCounters sc) {
+void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
+                   Spill sp, StageCounters sc) {
+  const dim3 grid(kSeg * 2 * seg_geom(n_max).chunks);
   if (sc.count && sc.stats)
-    hipLaunchKernelGGL(k_shadow<true>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, sq, sq_count, sp, sc.stats);
+    hipLaunchKernelGGL(k_shadow<true>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
   else
-    hipLaunchKernelGGL(k_shadow<false>, dim3(grid_for(n_max)), dim3(kBlock), 0, s, sv, ps, sq, sq_count, sp, sc.stats);
+    hipLaunchKernelGGL(k_shadow<false>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
 }
 
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im, uint32_t npaths) {

@@ -152,6 +152,8 @@ SIGNATURES = {
     "hpt_trace_counts": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p],
     "hpt_set_profiling": [C.c_void_p, C.c_int],
     "hpt_get_stage_stats": [C.c_void_p, C.c_int, C.POINTER(_StageStats)],
+    "hpt_set_frames_in_flight": [C.c_void_p, C.c_uint32],
+    "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
 }
 
 _lib = None
@@ -401,6 +403,18 @@ class HalaRenderer:
         tris = np.zeros(info["triangle_count"] * 48, np.uint8)
         _check(self._lib.hpt_export_bvh(self._h, _ptr(nodes), nodes.nbytes, _ptr(tris), tris.nbytes))
         return nodes, tris
+
+    @property
+    def frames_in_flight(self) -> int:
+        """Frames traced together per hpt_render batch (the images do not
+        depend on it)."""
+        v = C.c_uint32()
+        _check(self._lib.hpt_get_frames_in_flight(self._h, C.byref(v)))
+        return v.value
+
+    @frames_in_flight.setter
+    def frames_in_flight(self, k: int) -> None:
+        _check(self._lib.hpt_set_frames_in_flight(self._h, k))
 
     def set_profiling(self, flags: int) -> None:
         _check(self._lib.hpt_set_profiling(self._h, flags))

@@ -189,3 +189,38 @@ def test_max_samples_caps_accumulation():
     r.render(5)
     assert r.frame_index == 3
     r.close()
+
+
+@pytest.mark.parametrize("features", [0, hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS | hpt.FEATURE_TRANSPARENT,
+                                      hpt.FEATURE_MEDIUM | hpt.FEATURE_TRANSPARENT])
+def test_batched_frames_equal_frame_by_frame(features):
+    # A batch of k frames is one wavefront of k * w * h paths; every path
+    # depends only on its own state and the accumulation folds the frames in
+    # order, so the images are bit-identical for every batch size.
+    scene = hpt_scenes.displaced_icosphere_lights_medium(level=2) if features else hpt_scenes.cornell_box()
+    w, h = 40, 24
+    images = []
+    for k in (1, 3, 0):
+        r = _renderer(scene, w, h, max_depth=6, rr_depth=2, features=features)
+        r.frames_in_flight = k
+        r.render(7)
+        r.render(2)  # continues at frame 7
+        assert r.frame_index == 9
+        images.append([r.read_image(i) for i in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL,
+                                                 hpt.IMAGE_FINAL_RGBA8)])
+        if k == 0:
+            assert r.frames_in_flight == 8
+        r.close()
+    for other in images[1:]:
+        for a, b in zip(images[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_frames_in_flight_limits():
+    r = _renderer(hpt_scenes.cornell_box(), 16, 16)
+    with pytest.raises(hpt.HptError):
+        r.frames_in_flight = 65
+    r.frames_in_flight = 64
+    r.render(70)
+    assert r.frame_index == 70
+    r.close()
