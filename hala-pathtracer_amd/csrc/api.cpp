@@ -86,8 +86,8 @@ struct hpt_renderer {
   uint32_t num_lights = 0;
   std::vector<hpt_material> materials;
   std::vector<float> tri_pos;       // 9 floats per global triangle
-  std::vector<uint4> tri_idx;
-  std::vector<ShadeVertex> vtx;
+  std::vector<TriShade> tri_shade;
+  std::vector<uint32_t> tri_prim;  // primitive of each global triangle
   std::vector<PrimInfo> prims;
   bool scene_set = false;
   bool committed = false;
@@ -101,7 +101,7 @@ struct hpt_renderer {
   // BVH.
   uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
   // Device.
-  DevBuf d_nodes, d_ltris, d_tri_idx, d_vtx, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table,
+  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table, d_env_texels,
       d_blue_noise;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
@@ -207,14 +207,13 @@ SceneView scene_view(const hpt_renderer* r) {
   SceneView sv;
   sv.nodes = r->d_nodes.as<Node8>();
   sv.ltris = r->d_ltris.as<LeafTri>();
-  sv.tri_idx = r->d_tri_idx.as<uint4>();
-  sv.vtx = r->d_vtx.as<ShadeVertex>();
+  sv.tri_shade = r->d_tri_shade.as<TriShade>();
   sv.prims = r->d_prims.as<PrimInfo>();
   sv.materials = r->d_materials.as<hpt_material>();
   sv.lights = r->d_lights.as<hpt_light>();
   sv.cameras = r->d_cameras.as<hpt_camera>();
   sv.env_rgb = r->d_env_rgb.as<float4>();
-  sv.env_table = r->d_env_table.as<float4>();
+  sv.env_texels = r->d_env_texels.as<EnvTexel>();
   sv.blue_noise = r->d_blue_noise.as<uchar4>();
   sv.node_count = r->node_count;
   sv.tri_count = r->leaf_tri_count;
@@ -386,10 +385,11 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
   check(hipGetLastError(), "accumulate launch");
 }
 
-// Device memory a batch may take for path state and queues; 1080p fits 8
-// frames in about 5 GB.
-constexpr size_t kPathBudget = size_t(6) << 30;
+// Device memory a batch may take for path state and queues; 1080p fits 32
+// frames in about 19 GB of the 288 GB.
+constexpr size_t kPathBudget = size_t(24) << 30;
 constexpr uint32_t kMaxBatchFrames = 64;
+constexpr uint32_t kAutoBatchFrames = 32;
 // Bytes per path: 11 float4 path fields, the float2 medium field, two queue
 // slots and two shadow-queue entries of 3 float4.
 constexpr size_t kPathBytes = 11 * 16 + 8 + 2 * 4 + 2 * 48;  // (+ segment slack)
@@ -398,7 +398,7 @@ uint32_t batch_frames_for(const hpt_renderer* r) {
   if (r->frames_in_flight) return r->frames_in_flight;
   const size_t per_frame = (size_t)r->npix * kPathBytes;
   size_t k = kPathBudget / (per_frame ? per_frame : 1);
-  if (k > 8) k = 8;
+  if (k > kAutoBatchFrames) k = kAutoBatchFrames;
   if (k < 1) k = 1;
   while (k > 1 && (uint64_t)k * r->npix * 2 > 0xFFFFFFFFull) --k;  // queue indices are 32-bit
   return (uint32_t)k;
@@ -419,6 +419,21 @@ void alloc_paths(hpt_renderer* r, uint32_t frames) {
   r->d_sh_d.alloc(2 * qn * 16);
   r->d_sh_c.alloc(2 * qn * 16);
   r->batch_cap = frames;
+}
+
+// Uploads the environment texture and builds its EnvTexel records (the sky
+// keeps a 1x1 black map so the pointers stay valid).
+void upload_env(hpt_renderer* r, hipStream_t s) {
+  if (r->env_rgb.empty()) {
+    r->env_rgb.assign(1, make_float4(0, 0, 0, 0));
+    r->env_table.assign(1, make_float4(1, 0, 0, 0));
+  }
+  const uint32_t w = r->env_type == HPT_ENV_MAP ? r->env_w : 1, h = r->env_type == HPT_ENV_MAP ? r->env_h : 1;
+  upload(r->d_env_rgb, r->env_rgb, s);
+  upload(r->d_env_table, r->env_table, s);
+  r->d_env_texels.alloc((size_t)w * h * sizeof(EnvTexel));
+  launch_env_texels(s, r->d_env_rgb.as<float4>(), r->d_env_table.as<float4>(), w, h, r->d_env_texels.as<EnvTexel>());
+  check(hipGetLastError(), "environment records");
 }
 
 void require_committed(const hpt_renderer* r) {
@@ -476,8 +491,8 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
            "texture maps are not supported by this build", HPT_ERROR_UNSUPPORTED);
     }
     std::vector<float> tri_pos;
-    std::vector<uint4> tri_idx;
-    std::vector<ShadeVertex> vtx;
+    std::vector<TriShade> tri_shade;
+    std::vector<uint32_t> tri_prim;
     std::vector<PrimInfo> prims;
     uint64_t ntri = 0, nvtx = 0;
     for (uint32_t p = 0; p < d->primitive_count; ++p) {
@@ -492,8 +507,8 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
     }
     need(ntri < (1u << 31) && nvtx < (1u << 31), "too many triangles or vertices");
     tri_pos.reserve(ntri * 9);
-    tri_idx.reserve(ntri);
-    vtx.reserve(nvtx);
+    tri_shade.reserve(ntri);
+    tri_prim.reserve(ntri);
     uint32_t tri_base = 0;
     for (uint32_t p = 0; p < d->primitive_count; ++p) {
       const hpt_primitive_desc& pd = d->primitives[p];
@@ -509,7 +524,6 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
       pi.material_index = pd.material_index;
       pi.local_base = tri_base;
       prims.push_back(pi);
-      const uint32_t vbase = (uint32_t)vtx.size();
       std::vector<float> wp((size_t)pd.vertex_count * 3);
       for (uint32_t k = 0; k < pd.vertex_count; ++k) {
         const hpt_vertex& v = pd.vertices[k];
@@ -518,11 +532,6 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
         wp[k * 3 + 0] = ((m[0] * x + m[4] * y) + m[8] * z) + m[12];
         wp[k * 3 + 1] = ((m[1] * x + m[5] * y) + m[9] * z) + m[13];
         wp[k * 3 + 2] = ((m[2] * x + m[6] * y) + m[10] * z) + m[14];
-        ShadeVertex sv;
-        std::memcpy(sv.n, v.normal, 12);
-        std::memcpy(sv.t, v.tangent, 12);
-        std::memcpy(sv.uv, v.tex_coord, 8);
-        vtx.push_back(sv);
       }
       for (uint32_t t = 0; t < pd.index_count / 3; ++t) {
         uint32_t ix[3];
@@ -531,7 +540,16 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
           need(ix[k] < pd.vertex_count, "vertex index out of range");
           for (int a2 = 0; a2 < 3; ++a2) tri_pos.push_back(wp[(size_t)ix[k] * 3 + a2]);
         }
-        tri_idx.push_back(make_uint4(vbase + ix[0], vbase + ix[1], vbase + ix[2], p));
+        TriShade ts{};
+        for (int k = 0; k < 3; ++k) {
+          const hpt_vertex& v = pd.vertices[ix[k]];
+          std::memcpy(ts.n[k], v.normal, 12);
+          std::memcpy(ts.t[k], v.tangent, 12);
+          std::memcpy(ts.uv[k], v.tex_coord, 8);
+        }
+        ts.prim = p;
+        tri_shade.push_back(ts);
+        tri_prim.push_back(p);
       }
       tri_base += pd.index_count / 3;
     }
@@ -543,8 +561,8 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
     r->materials.assign(d->materials, d->materials + d->material_count);
     if (r->materials.empty()) r->materials.push_back(hpt_material{});
     r->tri_pos.swap(tri_pos);
-    r->tri_idx.swap(tri_idx);
-    r->vtx.swap(vtx);
+    r->tri_shade.swap(tri_shade);
+    r->tri_prim.swap(tri_prim);
     r->prims.swap(prims);
     r->camera_count = d->camera_count;
     r->scene_set = true;
@@ -585,8 +603,7 @@ hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t w, uint32
     r->env_type = HPT_ENV_MAP;
     if (r->committed) {
       set_device(r);
-      upload(r->d_env_rgb, r->env_rgb, r->stream);
-      upload(r->d_env_table, r->env_table, r->stream);
+      upload_env(r, r->stream);
     }
   });
 }
@@ -648,18 +665,12 @@ hpt_status hpt_commit(hpt_renderer* r) {
     hipStream_t s = r->stream;
     upload(r->d_nodes, bvh.nodes, s);
     upload(r->d_ltris, bvh.tris, s);
-    upload(r->d_tri_idx, r->tri_idx, s);
-    upload(r->d_vtx, r->vtx, s);
+    upload(r->d_tri_shade, r->tri_shade, s);
     upload(r->d_prims, r->prims, s);
     upload(r->d_materials, r->materials, s);
     upload(r->d_lights, r->lights, s);
     upload(r->d_cameras, r->cameras, s);
-    if (r->env_rgb.empty()) {  // keep valid pointers for the sky case
-      r->env_rgb.assign(1, make_float4(0, 0, 0, 0));
-      r->env_table.assign(1, make_float4(1, 0, 0, 0));
-    }
-    upload(r->d_env_rgb, r->env_rgb, s);
-    upload(r->d_env_table, r->env_table, s);
+    upload_env(r, s);
     upload(r->d_blue_noise, r->blue_noise, s);
     const uint32_t n = r->info.width * r->info.height;
     r->npix = n;
@@ -805,7 +816,7 @@ hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint32_t n, h
       } else if (id & LIGHT_BIT) {
         h = hpt_hit{out[i].x, 0.0f, 0.0f, HPT_HIT_LIGHT, id & ~LIGHT_BIT, HPT_INVALID_INDEX};
       } else {
-        const uint32_t prim = r->tri_idx[id].w;
+        const uint32_t prim = r->tri_prim[id];
         h = hpt_hit{out[i].x, out[i].y, out[i].z, HPT_HIT_TRIANGLE, prim, id - r->prims[prim].local_base};
       }
     }

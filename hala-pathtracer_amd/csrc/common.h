@@ -3,8 +3,7 @@
 // HBM layout of one committed scene (all arrays 16-byte aligned):
 //   nodes    Node8[node_count]      80 B  8-wide compressed BVH nodes
 //   ltris    LeafTri[tri_count]     48 B  world-space vertices in leaf order
-//   tri_idx  uint4[tri_count]       16 B  global vertex indices + primitive, by global triangle id
-//   vtx      ShadeVertex[nverts]    32 B  normal, tangent, uv for shading
+//   tri_shade TriShade[tri_count]  128 B  vertex normals/tangents/uvs + primitive, by global triangle id
 //   prims    PrimInfo[nprims]       48 B  inverse-transpose 3x3 + material
 //   materials hpt_material[nmat]   144 B  the std140 struct as given
 //   lights   hpt_light[17]          80 B  std140; slot num_lights.. zero-filled
@@ -66,9 +65,14 @@ constexpr uint32_t RAY_FLAGS_TRANSMISSION = 0x10;
 // box test conservative against the rounding of (lo - o) * inv.
 constexpr float BOX_MARGIN = 1.0f + 1.0f / 262144.0f;
 
-// Traversal stack entries each lane keeps in LDS; deeper trees spill the rest
-// to a global buffer.
-constexpr int kShortStack = 12;
+// Traversal stack entries each lane keeps in LDS: 8 entries x 8 B x 256 lanes
+// = 16 KB per block, small enough for 7 resident waves per SIMD (12 entries
+// capped the traversal kernels at 6 and measured 6% slower).  Deeper trees
+// spill the rest to a global buffer.
+#ifndef HPT_SHORT_STACK
+#define HPT_SHORT_STACK 8
+#endif
+constexpr int kShortStack = HPT_SHORT_STACK;
 
 // Hit record id encoding: triangle -> global triangle id (< 2^31);
 // light -> LIGHT_BIT | light index; miss -> MISS_ID.
@@ -108,12 +112,18 @@ struct LeafTri {
 };
 static_assert(sizeof(LeafTri) == 48, "LeafTri is 48 bytes");
 
-struct ShadeVertex {  // 32 B
-  float n[3];
-  float t[3];
-  float uv[2];
+// Shading record of one triangle (128 B, one cache line): its vertices'
+// normals and tangents copied from the vertex buffer, its primitive and its
+// texture coordinates.  A hit reads one line here instead of an index
+// triple and three scattered vertices.
+struct TriShade {
+  float n[3][3];   // vertex normals (object space)
+  float t[3][3];   // vertex tangents (object space)
+  uint32_t prim;
+  float uv[3][2];
+  uint32_t pad[7];
 };
-static_assert(sizeof(ShadeVertex) == 32, "ShadeVertex is 32 bytes");
+static_assert(sizeof(TriShade) == 128, "TriShade is 128 bytes");
 
 struct PrimInfo {  // 48 B
   float it[9];  // transpose(inverse(mat3(transform))), column-major
@@ -123,20 +133,31 @@ struct PrimInfo {  // 48 B
 };
 static_assert(sizeof(PrimInfo) == 48, "PrimInfo is 48 bytes");
 
+// One environment texel as the shading kernels read it (128 B, one cache
+// line): its alias-table entry and the RGB of the 3x3 texels around it, so a
+// bilinear look-up inside the texel and the alias draw touch one line
+// instead of the table and two texture rows.
+struct EnvTexel {
+  float prob;       // acceptance probability of the texel
+  uint32_t alias;   // texel drawn otherwise
+  float lum;        // luminance of the texel
+  float lum_alias;  // luminance of the alias
+  float rgb[3][3][3];  // [row j-1..j+1, clamped][column i-1..i+1, wrapped][rgb]
+  float pad;
+};
+static_assert(sizeof(EnvTexel) == 128, "EnvTexel is 128 bytes");
+
 // Device view of a committed scene, passed to kernels by value.
 struct SceneView {
   const Node8* nodes;
   const LeafTri* ltris;
-  const uint4* tri_idx;
-  const ShadeVertex* vtx;
+  const TriShade* tri_shade;
   const PrimInfo* prims;
   const hpt_material* materials;
   const hpt_light* lights;
   const hpt_camera* cameras;
-  const float4* env_rgb;     // width*height texels, .w unused
-  // Per texel: (acceptance probability, alias index as bits, luminance of the
-  // texel, luminance of its alias) -- one 16-B load per environment draw.
-  const float4* env_table;
+  const float4* env_rgb;       // width*height texels, .w unused
+  const EnvTexel* env_texels;  // width*height records built from env_rgb and the alias table
   const uchar4* blue_noise;  // 1024x1024 RGBA8
   uint32_t node_count;
   uint32_t tri_count;

@@ -616,6 +616,33 @@ __device__ __forceinline__ f3 env_texture(const SceneView& sv, uint32_t W, uint3
   return mix3(top, bot, fy);
 }
 
+// env_texture through the record of texel (ti, tj): the same four texels
+// and the same arithmetic when the bilinear footprint lies in the record's
+// 3x3 neighbourhood (always, for points inside or on the edge of the texel),
+// the texture itself otherwise.
+__device__ __forceinline__ f3 env_texture_at(const SceneView& sv, uint32_t W, uint32_t H, float u, float v,
+                                             uint32_t ti, uint32_t tj) {
+  const float x = u * (float)W - 0.5f;
+  const float y = v * (float)H - 0.5f;
+  const float fx0 = floorf(x), fy0 = floorf(y);
+  const float fx = x - fx0, fy = y - fy0;
+  const int w = (int)W;
+  int i0 = (int)fx0 % w;
+  if (i0 < 0) i0 += w;
+  const int j = (int)fy0;
+  const int di = i0 - (int)ti;
+  const int dx = di == 0 ? 0 : ((di == -1 || di == w - 1) ? -1 : 2);
+  const int dy = j - (int)tj;
+  if (dx > 0 || dy < -1 || dy > 0) return env_texture(sv, W, H, u, v);
+  const float* t = sv.env_texels[tj * W + ti].rgb[0][0];
+  const float* r0 = t + (dy + 1) * 9 + (dx + 1) * 3;  // row j0, column i0
+  const float* r1 = r0 + 9;                           // row j1
+  const f3 a = ld3(r0), b = ld3(r0 + 3), c = ld3(r1), d = ld3(r1 + 3);
+  const f3 top = mix3(a, b, fx);
+  const f3 bot = mix3(c, d, fx);
+  return mix3(top, bot, fy);
+}
+
 __device__ __forceinline__ uint32_t env_texel(uint32_t W, uint32_t H, float u, float v) {
   const float fu = u - floorf(u);
   int i = (int)(fu * (float)W);
@@ -633,7 +660,7 @@ __device__ __forceinline__ f3 sample_env_map(uint4& rng, const SceneView& sv, co
   pcg4d(rng);
   const uint32_t W = fp.env_w, H = fp.env_h;
   uint32_t idx = (uint32_t)(((uint64_t)rng.x * (uint64_t)(W * H)) >> 32);
-  const float4 e = sv.env_table[idx];
+  const float4 e = *reinterpret_cast<const float4*>(sv.env_texels + idx);
   float lum = e.z;
   if (!(u2f(rng.y) < e.x)) {
     idx = __float_as_uint(e.y);
@@ -642,7 +669,7 @@ __device__ __forceinline__ f3 sample_env_map(uint4& rng, const SceneView& sv, co
   const uint32_t ti = idx % W, tj = idx / W;
   const float u = ((float)ti + u2f(rng.z)) / (float)W;
   const float v = ((float)tj + u2f(rng.w)) / (float)H;
-  color = env_texture(sv, W, H, u, v) * fp.env_intensity;
+  color = env_texture_at(sv, W, H, u, v, ti, tj) * fp.env_intensity;
   pdf = lum / fp.env_total_sum;
   const float phi = (u + fp.env_rotation) * TWO_PI;
   const float theta = v * PI;
@@ -665,19 +692,21 @@ __device__ __forceinline__ f3 env_eval(const SceneView& sv, const FrameParams& f
   const float theta = acosf(dir.z);
   const float u = -(PI + atan2f(dir.y, dir.x)) * INV_TWO_PI - fp.env_rotation;
   const float v = theta * INV_PI;
+  const uint32_t texel = env_texel(fp.env_w, fp.env_h, u, v);
   float mis_weight = 1.0f;
   if (depth > 0) {
     // env_pdf, inc/env_map.glsl:43-55.
     float p = 0.0f;
     const float st = sinf(theta);
     if (st > EPS) {
-      const float pp = sv.env_table[env_texel(fp.env_w, fp.env_h, u, v)].z / fp.env_total_sum;
+      const float pp = sv.env_texels[texel].lum / fp.env_total_sum;
       if (pp > EPS) p = (pp * (float)fp.env_w * (float)fp.env_h) / (TWO_PI * PI * st);
     }
     mis_weight = power_heuristic(bsdf_pdf, p);
   }
   if (mis_off) mis_weight = 1.0f;
-  return mis_weight * env_texture(sv, fp.env_w, fp.env_h, u, v) * fp.env_intensity;
+  const f3 L = env_texture_at(sv, fp.env_w, fp.env_h, u, v, texel % fp.env_w, texel / fp.env_w);
+  return mis_weight * L * fp.env_intensity;
 }
 
 // ----------------------------------------------------------------- cameras

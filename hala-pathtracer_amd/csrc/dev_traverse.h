@@ -223,38 +223,38 @@ struct Walker {
     const float sy = __uint_as_float(((n0.w >> 8) & 0xFFu) << 23);
     const float sz = __uint_as_float(((n0.w >> 16) & 0xFFu) << 23);
     const uint32_t nimask = n0.w >> 24;
-    // Bytes per child: qlo_x = n2.x|n2.y, qlo_y = n2.z|n2.w, qlo_z = n3.x|n3.y,
-    // qhi_x = n3.z|n3.w, qhi_y = n4.x|n4.y, qhi_z = n4.z|n4.w, meta = n1.z|n1.w.
+    // Bytes per child (children 0-3 | 4-7): qlo_x = n2.x|n2.y, qlo_y =
+    // n2.z|n2.w, qlo_z = n3.x|n3.y, qhi_x = n3.z|n3.w, qhi_y = n4.x|n4.y,
+    // qhi_z = n4.z|n4.w, meta = n1.z|n1.w.  The near plane of an axis is
+    // qhi when the ray runs negative along it: choosing the plane words once
+    // per node gives the same distances as testing lo/hi and swapping.
+    const bool fx = oct & 1u, fy = oct & 2u, fz = oct & 4u;
+    const uint32_t nx0 = fx ? n3.z : n2.x, nx1 = fx ? n3.w : n2.y, ex0 = fx ? n2.x : n3.z, ex1 = fx ? n2.y : n3.w;
+    const uint32_t ny0 = fy ? n4.x : n2.z, ny1 = fy ? n4.y : n2.w, ey0 = fy ? n2.z : n4.x, ey1 = fy ? n2.w : n4.y;
+    const uint32_t nz0 = fz ? n4.z : n3.x, nz1 = fz ? n4.w : n3.y, ez0 = fz ? n3.x : n4.z, ez1 = fz ? n3.y : n4.w;
     uint32_t node_hits = 0, tri_hits = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int k = c & 3;
       const bool hi4 = c >= 4;
       const uint32_t meta = (hi4 ? n1.w : n1.z) >> (8 * k) & 0xFFu;
-      const float qlx = byte_f(hi4 ? n2.y : n2.x, k), qhx = byte_f(hi4 ? n3.w : n3.z, k);
-      const float qly = byte_f(hi4 ? n2.w : n2.z, k), qhy = byte_f(hi4 ? n4.y : n4.x, k);
-      const float qlz = byte_f(hi4 ? n3.y : n3.x, k), qhz = byte_f(hi4 ? n4.w : n4.z, k);
+      const float qnx = byte_f(hi4 ? nx1 : nx0, k), qfx = byte_f(hi4 ? ex1 : ex0, k);
+      const float qny = byte_f(hi4 ? ny1 : ny0, k), qfy = byte_f(hi4 ? ey1 : ey0, k);
+      const float qnz = byte_f(hi4 ? nz1 : nz0, k), qfz = byte_f(hi4 ? ez1 : ez0, k);
       float tn, tf;
       {
 #pragma clang fp contract(off)
-        const float lox = fmaf(qlx, sx, p.x), hix = fmaf(qhx, sx, p.x);
-        const float loy = fmaf(qly, sy, p.y), hiy = fmaf(qhy, sy, p.y);
-        const float loz = fmaf(qlz, sz, p.z), hiz = fmaf(qhz, sz, p.z);
-        float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
-        float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
-        float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
-        if (oct & 1u) { float q = t0x; t0x = t1x; t1x = q; }
-        if (oct & 2u) { float q = t0y; t0y = t1y; t1y = q; }
-        if (oct & 4u) { float q = t0z; t0z = t1z; t1z = q; }
-        tn = fmaxf(fmaxf(tmin, t0x), fmaxf(t0y, t0z));
-        tf = fminf(fminf(best_t, t1x), fminf(t1y, t1z));
+        const float tnx = (fmaf(qnx, sx, p.x) - o.x) * inv.x, tfx = (fmaf(qfx, sx, p.x) - o.x) * inv.x;
+        const float tny = (fmaf(qny, sy, p.y) - o.y) * inv.y, tfy = (fmaf(qfy, sy, p.y) - o.y) * inv.y;
+        const float tnz = (fmaf(qnz, sz, p.z) - o.z) * inv.z, tfz = (fmaf(qfz, sz, p.z) - o.z) * inv.z;
+        tn = fmaxf(fmaxf(tmin, tnx), fmaxf(tny, tnz));
+        tf = fminf(fminf(best_t, tfx), fminf(tfy, tfz));
         tf = tf * BOX_MARGIN;
       }
-      const bool hit = meta != 0u && tn <= tf;
-      if (hit) {
-        if (nimask & (1u << c)) node_hits |= 1u << (c ^ oct);
-        else tri_hits |= (meta >> 5) << (meta & 31u);
-      }
+      // Inner slots carry meta 0x1F, whose triangle mask (meta >> 5) is 0.
+      const uint32_t hit = (meta != 0u && tn <= tf) ? ~0u : 0u;
+      node_hits |= hit & (((nimask >> c) & 1u) << (c ^ oct));
+      tri_hits |= hit & ((meta >> 5) << (meta & 31u));
     }
     grp = make_uint2(n1.x, (node_hits << 8) | nimask);
     const uint32_t tri_base = n1.y;

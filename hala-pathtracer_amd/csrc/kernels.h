@@ -34,6 +34,11 @@ constexpr int kBlock = 256;
 // 11 ns on MI355X -- and, with blocks dealt round-robin to the 8 XCDs, a
 // path keeps to one XCD from bounce to bounce.
 constexpr uint32_t kSeg = 64;
+// Queue entries one lane of the extend and shadow kernels traces.
+#ifndef HPT_RAYS_PER_LANE
+#define HPT_RAYS_PER_LANE 1
+#endif
+constexpr uint32_t kRaysPerLane = HPT_RAYS_PER_LANE;
 constexpr uint32_t kSegStride = 64;
 
 // Segment geometry of a launch over at most n_max paths: the first bounce
@@ -104,6 +109,10 @@ void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, cons
                    Spill sp, StageCounters sc);
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im,
                        uint32_t npaths);
+
+// Builds the EnvTexel records of a w x h environment map.
+void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, uint32_t w, uint32_t h,
+                       EnvTexel* out);
 
 // Ray-query entry points: out = (t, u, v, id bits) per ray; counters optional.
 void launch_trace_rays(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, int any_hit,

@@ -122,8 +122,48 @@ __device__ __forceinline__ bool queue_entry(const Queue& qin, uint32_t n_max, un
 }
 
 // ------------------------------------------------------------------ extend
+// Occupancy target of the traversal kernels (waves per SIMD): the walks are
+// bound by memory latency, and 7 waves measured fastest (6 without the
+// target; 8 forces register spills).
+#ifndef HPT_WALK_WAVES_PER_EU
+#define HPT_WALK_WAVES_PER_EU 7
+#endif
+// Extend and shadow lanes trace up to kRaysPerLane queue entries back to
+// back (entries j, j + 64, ... of the wave's range), so a wave steps until
+// its lanes' summed walks end instead of idling behind its longest ray:
+// per-ray node visits range from 1 to tens.  A lane's next ray (origin,
+// direction and, for shadow rays, everything its end needs) is loaded while
+// the current one walks, so starting the next walk waits on no memory.
+// Entry range of this lane for a launch over a queue: the identity queue's
+// grid covers [0, n_max); a segmented queue's block b covers chunk b / kSeg
+// of segment b % kSeg.
+struct LaneRange {
+  uint32_t j;     // first entry (segment-local)
+  uint32_t n;     // entries in the segment (or n_max)
+  uint32_t base;  // offset of the segment in the entry arrays
+};
+__device__ __forceinline__ LaneRange lane_range(const uint32_t* counts, uint32_t seg_cap, bool identity,
+                                                uint32_t n_max, unsigned long long* stats) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  constexpr uint32_t kWaveSpan = 64u * kRaysPerLane, kBlockSpan = kBlock * kRaysPerLane;
+  LaneRange r;
+  if (identity) {
+    if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n_max);
+    r.j = blockIdx.x * kBlockSpan + wave * kWaveSpan + lane;
+    r.n = n_max;
+    r.base = 0;
+    return r;
+  }
+  const uint32_t seg = blockIdx.x % kSeg;
+  r.n = counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)r.n);
+  r.j = (blockIdx.x / kSeg) * kBlockSpan + wave * kWaveSpan + lane;
+  r.base = seg * seg_cap;
+  return r;
+}
+
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
                                                    ShadowQueue sq, Spill sp, unsigned long long* stats) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
@@ -131,17 +171,69 @@ __global__ void __launch_bounds__(kBlock) k_extend(SceneView sv, PathState ps, Q
     qnext.counts[gid * kSegStride] = 0;
     sq.counts[gid * kSegStride] = 0;
   }
-  uint32_t p;
-  const bool active = queue_entry(qin, n_max, stats, p);
-  uint32_t nv = 0, nt = 0;
+  const LaneRange lr = lane_range(qin.counts, qin.seg_cap, qin.items == nullptr, n_max, stats);
+  const uint32_t* items = qin.items ? qin.items + lr.base : nullptr;
+  auto entry = [&](uint32_t j) { return items ? items[j] : j; };
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  uint32_t cv = 0, ct = 0;
+  auto store = [&](uint32_t p, const TraceResult& r) {
+    ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+  };
+  if (sv.node_count == 0) {  // lights only
+    for (uint32_t i = 0, j = lr.j; i < kRaysPerLane && j < lr.n; ++i, j += 64) {
+      const uint32_t p = entry(j);
+      const float4 o4 = ps.org[p], d4 = ps.dir[p];
+      store(p, trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st));
+    }
+    return;
+  }
+  // Pipeline: current ray (path p), next ray (path pn, origin/direction
+  // loaded), the one after (path pnn, id loaded).
+  uint32_t i = 0, j = lr.j;
+  bool active = j < lr.n;
+  uint32_t p = active ? entry(j) : 0u;
+  bool has_n = active && kRaysPerLane > 1 && j + 64 < lr.n;
+  uint32_t pn = has_n ? entry(j + 64) : 0u;
+  bool has_nn = has_n && kRaysPerLane > 2 && j + 128 < lr.n;
+  uint32_t pnn = has_nn ? entry(j + 128) : 0u;
+  Walker<false> w;
   if (active) {
     const float4 o4 = ps.org[p], d4 = ps.dir[p];
-    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
-    const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st,
-                                        COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
-    ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+    w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, WORLD_SIZE, st);
   }
-  if (COUNT) add_visits(stats, nv, nt);
+  float4 on4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dn4 = on4;
+  if (has_n) {
+    on4 = ps.org[pn];
+    dn4 = ps.dir[pn];
+  }
+  while (__ballot(active) != 0) {
+    if (active && w.step(sv, st)) {
+      TraceResult r{0.0f, 0.0f, 0.0f, MISS_ID};
+      if (w.found) r = TraceResult{w.best_t, w.bu, w.bv, w.best_id};
+      closest_lights(sv, w.r.org, w.r.dir, 0.0f, WORLD_SIZE, w.found, r);
+      store(p, r);
+      if (COUNT) {
+        cv += w.nv;
+        ct += w.nt;
+      }
+      active = has_n;
+      if (active) {
+        ++i;
+        j += 64;
+        p = pn;
+        w.begin(prepare_ray(mk3(on4.x, on4.y, on4.z), mk3(dn4.x, dn4.y, dn4.z)), 0.0f, WORLD_SIZE, st);
+        has_n = has_nn;
+        pn = pnn;
+        if (has_n) {
+          on4 = ps.org[pn];
+          dn4 = ps.dir[pn];
+        }
+        has_nn = has_n && i + 2 < kRaysPerLane && j + 128 < lr.n;
+        if (has_nn) pnn = entry(j + 128);
+      }
+    }
+  }
+  if (COUNT) add_visits(stats, cv, ct);
 }
 
 // ------------------------------------------------------------------ shading
@@ -154,15 +246,17 @@ struct HitState {  // the State fields triangles.rchit.glsl:66-75 writes
 __device__ __forceinline__ HitState triangle_state(const SceneView& sv, uint32_t id, float t, float u, float v,
                                                    f3 org, f3 dir) {
   HitState h;
-  const uint4 ti = sv.tri_idx[id];
-  const PrimInfo& pi = sv.prims[ti.w];
-  const ShadeVertex a = sv.vtx[ti.x], b = sv.vtx[ti.y], c = sv.vtx[ti.z];
+  const float4* rec = reinterpret_cast<const float4*>(sv.tri_shade + id);
+  const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3], q4 = rec[4];
+  const PrimInfo& pi = sv.prims[__float_as_uint(q4.z)];
+  const f3 na = mk3(q0.x, q0.y, q0.z), nb = mk3(q0.w, q1.x, q1.y), nc = mk3(q1.z, q1.w, q2.x);
+  const f3 ta = mk3(q2.y, q2.z, q2.w), tb = mk3(q3.x, q3.y, q3.z), tc = mk3(q3.w, q4.x, q4.y);
   const float bx = 1.0f - u - v, by = u, bz = v;
   const f3 c0 = mk3(pi.it[0], pi.it[1], pi.it[2]), c1 = mk3(pi.it[3], pi.it[4], pi.it[5]),
            c2 = mk3(pi.it[6], pi.it[7], pi.it[8]);
-  f3 n = ld3(a.n) * bx + ld3(b.n) * by + ld3(c.n) * bz;
+  f3 n = na * bx + nb * by + nc * bz;
   n = normalize(c0 * n.x + c1 * n.y + c2 * n.z);
-  f3 tg = ld3(a.t) * bx + ld3(b.t) * by + ld3(c.t) * bz;
+  f3 tg = ta * bx + tb * by + tc * bz;
   tg = normalize(c0 * tg.x + c1 * tg.y + c2 * tg.z);
   h.normal = n;
   h.tangent = tg;
@@ -491,33 +585,89 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
 }
 
 // ------------------------------------------------------------------ shadow
+// Any-hit per shadow ray; an unoccluded ray adds its contribution to the
+// path's environment (slot 0) or light (slot 1) NEE sum.  Each path has at
+// most one ray per slot per bounce, so the sum can be read when the ray
+// starts and written when it ends.
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, Spill sp,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, Spill sp,
                                                    unsigned long long* stats) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  const SegSlot t = seg_slot(sq.counts);
-  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
-  uint32_t nv = 0, nt = 0;
-  if (t.j < t.n) {
-    const uint32_t k = t.seg * sq.seg_cap + t.j;
+  const LaneRange lr = lane_range(sq.counts, sq.seg_cap, false, 0, stats);
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  uint32_t cv = 0, ct = 0;
+  // Current ray: tmax, tag, contribution, the NEE sum it adds to.
+  float tmax = 0.0f;
+  uint32_t tag = 0;
+  float4 c4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), acc = c4;
+  auto sum_of = [&](uint32_t tg) -> float4* { return ((tg & 1u) ? ps.nee_light : ps.nee_env) + (tg >> 1); };
+  auto end = [&](bool occluded) {
+    if (occluded) return;
+    acc.x += c4.x;
+    acc.y += c4.y;
+    acc.z += c4.z;
+    *sum_of(tag) = acc;
+  };
+  if (sv.node_count == 0) {  // lights only
+    for (uint32_t i = 0, j = lr.j; i < kRaysPerLane && j < lr.n; ++i, j += 64) {
+      const uint32_t k = lr.base + j;
+      const float4 o4 = sq.o[k], d4 = sq.d[k];
+      tmax = o4.w;
+      tag = __float_as_uint(d4.w);
+      c4 = sq.c[k];
+      acc = *sum_of(tag);
+      end(any_light(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, tmax));
+    }
+    return;
+  }
+  uint32_t i = 0, j = lr.j;
+  bool active = j < lr.n;
+  Walker<true> w;
+  if (active) {
+    const uint32_t k = lr.base + j;
     const float4 o4 = sq.o[k], d4 = sq.d[k];
-    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
-    const bool occluded = trace_any(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, o4.w, st,
-                                    COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
-    if (!occluded) {
-      const uint32_t tag = __float_as_uint(d4.w);
-      const uint32_t p = tag >> 1;
-      const float4 c = sq.c[k];
-      float4* dst = (tag & 1u) ? ps.nee_light : ps.nee_env;
-      float4 a = dst[p];
-      a.x += c.x;
-      a.y += c.y;
-      a.z += c.z;
-      dst[p] = a;
+    c4 = sq.c[k];
+    tmax = o4.w;
+    tag = __float_as_uint(d4.w);
+    acc = *sum_of(tag);
+    w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, tmax, st);
+  }
+  bool has_n = active && kRaysPerLane > 1 && j + 64 < lr.n;
+  float4 on4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dn4 = on4, cn4 = on4;
+  if (has_n) {
+    const uint32_t k = lr.base + j + 64;
+    on4 = sq.o[k];
+    dn4 = sq.d[k];
+    cn4 = sq.c[k];
+  }
+  while (__ballot(active) != 0) {
+    if (active && w.step(sv, st)) {
+      end(w.found || any_light(sv, w.r.org, w.r.dir, 0.0f, tmax));
+      if (COUNT) {
+        cv += w.nv;
+        ct += w.nt;
+      }
+      active = has_n;
+      if (active) {
+        ++i;
+        j += 64;
+        tmax = on4.w;
+        tag = __float_as_uint(dn4.w);
+        c4 = cn4;
+        acc = *sum_of(tag);
+        w.begin(prepare_ray(mk3(on4.x, on4.y, on4.z), mk3(dn4.x, dn4.y, dn4.z)), 0.0f, tmax, st);
+        has_n = i + 1 < kRaysPerLane && j + 64 < lr.n;
+        if (has_n) {
+          const uint32_t k = lr.base + j + 64;
+          on4 = sq.o[k];
+          dn4 = sq.d[k];
+          cn4 = sq.c[k];
+        }
+      }
     }
   }
-  if (COUNT) add_visits(stats, nv, nt);
+  if (COUNT) add_visits(stats, cv, ct);
 }
 
 // -------------------------------------------------------------- accumulate
@@ -591,6 +741,33 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState
   im.final_rgba8[p] = make_uchar4(unorm8(srgb(c.x)), unorm8(srgb(c.y)), unorm8(srgb(c.z)), 255);
 }
 
+// ------------------------------------------------------ environment records
+// One EnvTexel per texel from the texture and the alias table (float4 per
+// texel: probability, alias bits, luminance, luminance of the alias).
+__global__ void __launch_bounds__(kBlock) k_env_texels(const float4* rgb, const float4* table, uint32_t W, uint32_t H,
+                                                       EnvTexel* out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= W * H) return;
+  const uint32_t ti = i % W, tj = i / W;
+  EnvTexel& e = out[i];
+  const float4 t = table[i];
+  e.prob = t.x;
+  e.alias = __float_as_uint(t.y);
+  e.lum = t.z;
+  e.lum_alias = t.w;
+  for (int dy = 0; dy < 3; ++dy) {
+    const int j = min(max((int)tj + dy - 1, 0), (int)H - 1);
+    for (int dx = 0; dx < 3; ++dx) {
+      const int c = (int)((ti + W + (uint32_t)dx - 1u) % W);
+      const float4 v = rgb[(size_t)j * W + c];
+      e.rgb[dy][dx][0] = v.x;
+      e.rgb[dy][dx][1] = v.y;
+      e.rgb[dy][dx][2] = v.z;
+    }
+  }
+  e.pad = 0.0f;
+}
+
 // --------------------------------------------------------------- ray queries
 template <bool ANY>
 __global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sv, const hpt_ray* rays, uint32_t n, float4* out,
@@ -626,9 +803,15 @@ static inline uint32_t queue_grid(const Queue& qin, uint32_t n_max) {
   return qin.items ? kSeg * seg_geom(n_max).chunks : grid_for(n_max);
 }
 
+// Blocks of a traversal launch (kRaysPerLane entries per lane).
+static inline uint32_t walk_blocks(uint32_t entries) {
+  constexpr uint32_t span = kBlock * kRaysPerLane;
+  return (entries + span - 1) / span;
+}
+
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc) {
-  const dim3 grid(queue_grid(qin, n_max));
+  const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap) : walk_blocks(n_max));
   if (sc.count && sc.stats)
     hipLaunchKernelGGL(k_extend<true>, grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats);
   else
@@ -663,11 +846,15 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
 
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc) {
-  const dim3 grid(kSeg * 2 * seg_geom(n_max).chunks);
+  const dim3 grid(kSeg * walk_blocks(sq.seg_cap));
   if (sc.count && sc.stats)
     hipLaunchKernelGGL(k_shadow<true>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
   else
     hipLaunchKernelGGL(k_shadow<false>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
+}
+
+void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, uint32_t w, uint32_t h, EnvTexel* out) {
+  hipLaunchKernelGGL(k_env_texels, dim3(grid_for(w * h)), dim3(kBlock), 0, s, rgb, table, w, h, out);
 }
 
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im, uint32_t npaths) {

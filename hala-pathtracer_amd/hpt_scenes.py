@@ -410,6 +410,27 @@ def triangle_soup(count, seed=0):
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
 
 
+def nested_triangles(count=200, ratio=0.7):
+    """`count` right triangles in the x = s plane with legs s along y and z,
+    s = ratio^i: each is ratio times the previous and nearer the origin, so
+    the binned SAH peels a few off per level and the tree is deep (21 levels
+    at the defaults), deeper than the traversal's LDS short stack."""
+    s = ratio ** np.arange(count, dtype=np.float64)
+    p = np.zeros((count, 3, 3))
+    p[:, :, 0] = s[:, None]
+    p[:, 1, 1] = s
+    p[:, 2, 2] = s
+    v = np.zeros((count * 3, hpt.VERTEX_FLOATS), np.float32)
+    v[:, 0:3] = p.reshape(-1, 3)
+    v[:, 3] = 1.0   # normal +x
+    v[:, 7] = 1.0   # tangent +y
+    ix = np.arange(count * 3, dtype=np.uint32).reshape(-1, 3)
+    mats = hpt.new_materials(1)
+    diffuse_material(mats[0], (0.6, 0.6, 0.6))
+    cam = look_at_camera((1.5, 0.4, 0.4), (0.0, 0.05, 0.05), yfov_deg=40.0)
+    return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
+
+
 def furnace_box(albedo=0.5, emission=1.0, half_size=10.0):
     """Camera inside a closed, flat-shaded Lambert cube that also emits:
     every path gathers emission * sum_k albedo^k exactly (no lights, black

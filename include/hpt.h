@@ -379,7 +379,7 @@ HPT_API hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_sta
  * images fold the frames in order with the reference's per-frame weights,
  * so every k gives the images of frame-by-frame rendering.  Batching
  * amortises the short, latency-bound tail bounces over k frames.  frames = 0
- * sizes the batch from a 6 GiB path-state budget (at most 8 frames); 1..64
+ * sizes the batch from a 24 GiB path-state budget (at most 32 frames); 1..64
  * forces it (HPT_ERROR_INVALID_ARGUMENT above 64 or when k * width * height
  * exceeds 2^31).  Takes effect at the next hpt_render. */
 HPT_API hpt_status hpt_set_frames_in_flight(hpt_renderer* r, uint32_t frames);

@@ -209,7 +209,7 @@ def test_batched_frames_equal_frame_by_frame(features):
         images.append([r.read_image(i) for i in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL,
                                                  hpt.IMAGE_FINAL_RGBA8)])
         if k == 0:
-            assert r.frames_in_flight == 8
+            assert r.frames_in_flight == 32
         r.close()
     for other in images[1:]:
         for a, b in zip(images[0], other):
@@ -223,4 +223,21 @@ def test_frames_in_flight_limits():
     r.frames_in_flight = 64
     r.render(70)
     assert r.frame_index == 70
+    r.close()
+
+
+def test_deep_tree_render_per_sample():
+    # The wavefront extend and shadow kernels on a 21-level tree (global
+    # stack spill), frame 0 against the oracle path by path.
+    scene = hpt_scenes.nested_triangles(200, 0.7)
+    w = h = 48
+    r = _renderer(scene, w, h, max_depth=4, rr_depth=2)
+    o = pyoracle.OracleScene(scene)
+    st = pyoracle.settings(w, h, 4, 2)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    ref, _, _ = o.trace_paths(st, 0)
+    agree = _per_sample_agreement(gpu, ref)
+    print(f"[gpu] deep tree per-sample agreement {agree:.4f}", flush=True)
+    assert agree > 0.97
     r.close()

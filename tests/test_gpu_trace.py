@@ -144,3 +144,36 @@ def test_lights_in_closest_and_any():
     assert (ref["kind"] == hpt.HIT_LIGHT).sum() > 100
     assert np.array_equal(r.trace_any(rays), o.trace_any(rays, brute=True))
     r.close()
+
+
+def test_deep_tree_spill_bitexact():
+    # Nested triangles shrinking by 0.7 give a 21-level tree: walks push more
+    # entries than the LDS short stack holds and use the global spill.
+    scene = hpt_scenes.nested_triangles(200, 0.7)
+    r = _renderer(scene)
+    assert r.bvh_info()["max_depth"] > 12
+    o = pyoracle.OracleScene(scene)
+    rng = np.random.default_rng(9)
+    n = 20000
+    tri = scene.world_triangles().astype(np.float64)
+    pick = rng.integers(0, len(tri), n)
+    b = rng.dirichlet((1, 1, 1), n)
+    target = np.einsum("nk,nkd->nd", b, tri[pick])
+    org = rng.uniform(-0.5, 1.5, (n, 3))
+    d = target - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = hpt.make_rays(org, d)
+    gpu = r.trace_closest(rays)
+    ref = o.trace_closest(rays, brute=True)
+    assert _same_hits(gpu, ref).all()
+    assert np.array_equal(gpu["t"], ref["t"])
+    assert (ref["kind"] == hpt.HIT_TRIANGLE).sum() > 1000
+    rays["tmax"] = rng.uniform(0.01, 2.0, n).astype(np.float32)
+    assert np.array_equal(r.trace_any(rays), o.trace_any(rays, brute=True))
+    nodes, tris = r.export_bvh()
+    for any_hit in (False, True):
+        gv, gt = r.trace_counts(rays, any_hit=any_hit)
+        _, ov, ot = o.trace_bvh8(nodes, tris, rays, any_hit=any_hit)
+        assert np.array_equal(gv, ov) and np.array_equal(gt, ot)
+        assert gv.max() > 12
+    r.close()
