@@ -162,6 +162,9 @@ extern "C" {
     pub fn hpt_render(r: *mut hpt_renderer, frames: u32, camera_index: u32) -> hpt_status;
     pub fn hpt_reset_accumulation(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_get_frame_index(r: *mut hpt_renderer, frame_index: *mut u64) -> hpt_status;
+    pub fn hpt_set_frame_index(r: *mut hpt_renderer, frame_index: u64) -> hpt_status;
+    pub fn hpt_set_frames_in_flight(r: *mut hpt_renderer, frames: u32) -> hpt_status;
+    pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
     pub fn hpt_read_image(r: *mut hpt_renderer, which: c_int, dst: *mut c_void, bytes: usize) -> hpt_status;
     pub fn hpt_wait_idle(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_destroy(r: *mut hpt_renderer);
