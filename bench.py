@@ -1,10 +1,14 @@
 """Throughput benchmark of the path-tracing hot path on one MI355X.
 
-Workload: the Disney material grid with an importance-sampled sky map -- 7x7
-UV spheres (64x32 segments, 194,434 triangles with the ground) at 1920x1080,
-max depth 10, Russian roulette from depth 3.  A step is one frame: one camera
-path per pixel through the full bounce loop plus accumulation; the default
---steps 256 is the configuration's 256 samples per pixel.
+Default workload: the Disney material grid with an importance-sampled sky
+map -- 7x7 UV spheres (64x32 segments, 194,434 triangles with the ground) at
+1920x1080, max depth 10, Russian roulette from depth 3.  A step is one frame:
+one camera path per pixel through the full bounce loop plus accumulation; the
+default --steps is the configuration's 256 samples per pixel.  --workload
+selects the Cornell box (512x512, 64 spp, depth 8) or the 1.3 M-triangle
+displaced icosphere with five light types and a scattering medium (1080p,
+128 spp, depth 12); --relmse adds the plain tracer's time to relMSE <= 1e-3
+against a 4096-spp reference.
 
 Reports Msamples/s (pixels x frames / wall seconds, inputs resident in HBM),
 the roofline of the dominant kernel (BVH traversal of the extension rays,
@@ -37,7 +41,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, width, height, max_depth, rr_depth, budget_s=12.0):
+def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s=12.0):
     """Oracle throughput on the host: centred pixel rows of frame 0, grown
     until the sample takes a quarter of the budget; once the whole image
     fits, whole frames 0, 1, ... until the budget is spent.  All host threads
@@ -46,7 +50,7 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, budget_s=12.0):
     import pyoracle
 
     o = pyoracle.OracleScene(scene)
-    st = pyoracle.settings(width, height, max_depth, rr_depth)
+    st = pyoracle.settings(width, height, max_depth, rr_depth, features)
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     rows = 4
     while True:
@@ -72,11 +76,12 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, budget_s=12.0):
             "sample": f"{what} ({paths} paths, {dt:.1f} s)"}
 
 
-def profiled_traffic(frames_in_flight):
-    """HBM bytes per k_extend launch from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the reading of a
-    1 GiB copy), when it was taken at the same batch size; else None."""
-    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_1080p.md")
+def profiled_traffic(workload, frames_in_flight):
+    """HBM bytes per k_extend launch from the committed rocprofv3 summary of
+    the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    reading of a 1 GiB copy), when it was taken at the same batch size; else
+    None."""
+    path = os.path.join(ROOT, "profiles", f"k_extend_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -101,16 +106,77 @@ def copy_calibration(mib=1024, reps=4):
                       "dispatches": reps}), flush=True)
 
 
+# The workloads (scene, image, depth, Russian-roulette start, shader permutation,
+# samples per pixel): the Disney grid is the headline; the Cornell box and the
+# large-mesh scene with five light types and a medium are run with --workload.
+WORKLOADS = {
+    "disney_grid_env_1080p": dict(scene=lambda: hpt_scenes.disney_grid(seed=0), width=1920, height=1080,
+                                  depth=10, rr=3, features=0, spp=256),
+    "cornell_512": dict(scene=hpt_scenes.cornell_box, width=512, height=512, depth=8, rr=3, features=0, spp=64),
+    "icosphere_lights_medium_1080p": dict(scene=lambda: hpt_scenes.displaced_icosphere_lights_medium(level=8),
+                                          width=1920, height=1080, depth=12, rr=3,
+                                          features=hpt_scenes.MEDIUM_SCENE_FEATURES, spp=128),
+}
+
+
+def relmse(img, ref):
+    """mean over pixels and RGB of (I - R)^2 / (R^2 + 0.01)."""
+    img = img[..., :3].astype(np.float64)
+    ref = ref[..., :3].astype(np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def time_to_relmse(make_renderer, ref_spp, threshold, chunk, max_spp):
+    """Renders a ref_spp reference from frames [65536, 65536 + ref_spp) (so it
+    shares no samples with the estimate), then the plain progressive estimate
+    from frame 0 in chunks of `chunk` frames; the render time until the
+    estimate's relMSE first meets `threshold` (relMSE evaluation excluded)."""
+    r = make_renderer()
+    base = 1 << 16
+    r.frame_index = base
+    t0 = time.perf_counter()
+    r.render(ref_spp)
+    r.wait_idle()
+    t_ref = time.perf_counter() - t0
+    ref = r.read_image(hpt.IMAGE_ACCUM)[..., :3].astype(np.float64) * ((base + ref_spp) / ref_spp)
+    r.close()
+    log(f"reference: {ref_spp} spp in {t_ref:.2f} s")
+    r = make_renderer()
+    curve, t_render, spp, hit = [], 0.0, 0, None
+    while spp < max_spp:
+        r.wait_idle()
+        t0 = time.perf_counter()
+        r.render(chunk)
+        r.wait_idle()
+        t_render += time.perf_counter() - t0
+        spp += chunk
+        e = relmse(r.read_image(hpt.IMAGE_ACCUM), ref)
+        curve.append([spp, round(t_render, 4), e])
+        if hit is None and e <= threshold:
+            hit = (spp, t_render)
+            break
+    r.close()
+    log(f"relMSE after {spp} spp: {curve[-1][2]:.3e}" + (f"; threshold met at {hit[0]} spp, {hit[1]:.3f} s" if hit else ""))
+    return {"threshold": threshold, "ref_spp": ref_spp, "ref_seconds": round(t_ref, 3),
+            "time_to_threshold_s": round(hit[1], 4) if hit else None, "spp_at_threshold": hit[0] if hit else None,
+            "curve": curve[:: max(1, len(curve) // 16)] + ([curve[-1]] if len(curve) % 16 else [])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="disney_grid_env_1080p")
+    ap.add_argument("--steps", type=int, default=None, help="frames (default: the workload's spp)")
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--width", type=int, default=None)
+    ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced together per batch (0: sized by the library)")
+    ap.add_argument("--relmse", action="store_true",
+                    help="also measure the plain tracer's time to relMSE <= --threshold against a reference")
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--threshold", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run known-size copies (rocprof counter calibration)")
     args = ap.parse_args()
@@ -122,16 +188,25 @@ def main():
               f"described in DESIGN.md and not measured here.", file=sys.stderr)
         sys.exit(2)
 
-    W, H, DEPTH, RR = args.width, args.height, 10, 3
-    log("building scene")
-    scene = hpt_scenes.disney_grid(seed=0, width=W, height=H)
+    wl = WORKLOADS[args.workload]
+    W = args.width or wl["width"]
+    H = args.height or wl["height"]
+    DEPTH, RR, FEATURES = wl["depth"], wl["rr"], wl["features"]
+    steps = args.steps or wl["spp"]
+    log(f"building scene {args.workload}")
+    scene = wl["scene"]()
     log(f"{scene.triangle_count} triangles; creating renderer")
-    r = hpt.HalaRenderer(W, H, max_depth=DEPTH, rr_depth=RR)
-    r.set_scene(scene)
-    r.frames_in_flight = args.frames_in_flight
+
+    def make_renderer():
+        rr = hpt.HalaRenderer(W, H, max_depth=DEPTH, rr_depth=RR, features=FEATURES)
+        rr.set_scene(scene)
+        rr.frames_in_flight = args.frames_in_flight
+        rr.commit()
+        return rr
+
     t0 = time.perf_counter()
-    r.commit()
-    log(f"commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
+    r = make_renderer()
+    log(f"create + commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
         f"{r.frames_in_flight} frames in flight")
     fif = r.frames_in_flight
 
@@ -139,7 +214,6 @@ def main():
     r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
     r.render(max(1, args.warmup))
     ext = r.stage_stats(hpt.STAGE_EXTEND)
-    shd = r.stage_stats(hpt.STAGE_SHADOW)
     nodes_per_ray = ext["node_visits"] / max(1, ext["rays"])
     tris_per_ray = ext["tri_tests"] / max(1, ext["rays"])
     log(f"warm-up: extend {ext['rays']} rays, {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray")
@@ -149,7 +223,7 @@ def main():
     r.set_profiling(hpt.PROFILE_TIMING)
     r.wait_idle()
     t0 = time.perf_counter()
-    r.render(args.steps)
+    r.render(steps)
     r.wait_idle()
     wall = time.perf_counter() - t0
     stats = {s: r.stage_stats(s) for s in range(5)}
@@ -157,7 +231,7 @@ def main():
     shadow = stats[hpt.STAGE_SHADOW]
     img = r.read_image(hpt.IMAGE_ACCUM)
     assert np.isfinite(img).all()
-    samples = W * H * args.steps
+    samples = W * H * steps
     msps = samples / wall / 1e6
     ext_ms = ext["total_ms"] / max(1, ext["launches"])
     bytes_per_launch = (ext["rays"] / max(1, ext["launches"])) * (
@@ -167,32 +241,41 @@ def main():
     log("stage totals: " + ", ".join(f"{n} {stats[s]['total_ms']:.1f} ms/{stats[s]['launches']}"
                                        for s, n in enumerate(["camera", "extend", "shade", "shadow", "accum"])))
     log(f"wall {wall:.3f} s, {msps:.1f} Msamples/s, {rays_total / wall / 1e6:.1f} Mrays/s")
-
-    base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR)
     r.close()
+
+    conv = None
+    if args.relmse:
+        conv = time_to_relmse(make_renderer, args.ref_spp, args.threshold, max(1, fif), args.ref_spp // 2)
+    base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES)
+    config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
+              "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,
+              "lights": int(len(scene.lights)), "frames_in_flight": fif,
+              "mrays_per_s": round(rays_total / wall / 1e6, 2)}
+    if scene.env_rgb is not None:
+        config["env_map"] = [int(scene.env_rgb.shape[1]), int(scene.env_rgb.shape[0])]
     out = {
         "metric": "Msamples/s",
         "value": round(msps, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(wall / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "disney_grid_env_1080p", "width": W, "height": H, "spp": args.steps,
-                   "max_depth": DEPTH, "rr_depth": RR, "triangles": scene.triangle_count,
-                   "env_map": [2048, 1024], "frames_in_flight": fif,
-                   "mrays_per_s": round(rays_total / wall / 1e6, 2)},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(fif),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(args.workload, fif),
                      "kernel": "k_extend", "avg_launch_ms": round(ext_ms, 4),
                      "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
         "cpu_baseline": base,
     }
+    if conv is not None:
+        out["relmse"] = conv
     print(json.dumps(out), flush=True)
 
 

@@ -689,7 +689,9 @@ __device__ __forceinline__ f3 env_eval(const SceneView& sv, const FrameParams& f
     const float a = fmaxf(dir.z, 0.0f);
     return mix3(ld3(fp.ground), ld3(fp.sky), a) * fp.env_intensity;
   }
-  const float theta = acosf(dir.z);
+  // acos is undefined above 1 in GLSL; a normalised direction can carry
+  // z = 1 + 2^-23, so z is clamped to [-1, 1] (as the oracle does).
+  const float theta = acosf(fminf(fmaxf(dir.z, -1.0f), 1.0f));
   const float u = -(PI + atan2f(dir.y, dir.x)) * INV_TWO_PI - fp.env_rotation;
   const float v = theta * INV_PI;
   const uint32_t texel = env_texel(fp.env_w, fp.env_h, u, v);

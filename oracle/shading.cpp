@@ -813,8 +813,10 @@ vec3 env_eval(const Globals& g, const EnvMap& env, uint32_t flags, uint32_t dept
     const float a = gmax(dir.z, 0.0f);
     return mix3(g.ground_color, g.sky_color, a) * g.env_intensity;
   }
-  // env_map_eval.rcall.glsl:14-35
-  const float theta = std::acos(dir.z);
+  // env_map_eval.rcall.glsl:14-35.  acos is undefined above 1 in GLSL; a
+  // normalised direction can carry z = 1 + 2^-23, so z is clamped to [-1, 1]
+  // (otherwise the NaN would poison the running mean of the pixel for good).
+  const float theta = std::acos(gmin(gmax(dir.z, -1.0f), 1.0f));
   const vec2 uv = vec2(-(PI + std::atan2(dir.y, dir.x)) * INV_TWO_PI - g.env_rotation, theta * INV_PI);
   float mis_weight = 1.0f;
   if (depth > 0 && (flags & RAY_FLAGS_SPECULAR) == 0) {

@@ -217,3 +217,17 @@ def test_env_mis_weight_at_depth():
     ep = o.env_pdf(d)
     wgt = pdf * pdf / (pdf * pdf + ep * ep) if ep > 0 else 1.0
     assert np.allclose(e1, e0 * wgt, rtol=1e-4)
+
+
+def test_env_eval_is_finite_past_the_pole():
+    # normalize() can return z = 1 + 2^-23, where acos is undefined in GLSL
+    # (a path of the 1080p grid scene met it at frame 66581); the argument is
+    # clamped, so the look-up equals the one at z = 1 instead of a NaN that
+    # would poison the pixel's running mean.
+    sc, env = _env_scene()
+    o = pyoracle.OracleScene(sc)
+    z = float(np.nextafter(np.float32(1.0), np.float32(2.0)))
+    for depth in (0, 1):
+        e = o.env_eval([1.6e-4, 1.6e-4, z], depth=depth, bsdf_pdf=6.5)
+        assert np.isfinite(e).all()
+        assert np.array_equal(e, o.env_eval([1.6e-4, 1.6e-4, 1.0], depth=depth, bsdf_pdf=6.5))
