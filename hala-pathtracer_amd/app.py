@@ -1,0 +1,268 @@
+"""Headless counterpart of the reference application (src/bin/hala-pathtracer.rs
+with src/config/*.rs): read the YAML config, load the glTF scene, pick the
+shader permutation from the scene, set the environment, render and save the
+images -- through the C ABI.  The window, imgui and the interactive frame loop
+are not part of the hot path; `--frames` replaces the loop.
+
+    python hala-pathtracer_amd/app.py -c conf/config.yaml --frames 64
+
+writes ./out/<scene stem>_color.pfm, _albedo.pfm, _normal.pfm (the names
+denoise.py looks for) and _final.png.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+from dataclasses import dataclass, field
+
+import numpy as np
+import yaml
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+import gltf_scene  # noqa: E402
+import hpt  # noqa: E402
+
+
+class ConfigError(ValueError):
+    pass
+
+
+# src/config/window.rs, tracer.rs, mod.rs: fields, serde defaults, validation.
+@dataclass
+class WindowConfig:
+    width: int = 0
+    height: int = 0
+
+
+@dataclass
+class TonemapConfig:
+    enable: bool = False
+    enable_aces: bool = False
+    use_simple_aces: bool = False
+
+
+@dataclass
+class TracerConfig:
+    max_depth: int = 1
+    rr_depth: int = 0
+    exposure_value: float = 0.0
+    tonemap: TonemapConfig = field(default_factory=TonemapConfig)
+    max_samples: int = 0
+
+
+@dataclass
+class AppConfig:
+    window: WindowConfig
+    tracer: TracerConfig
+    scene_file: str
+    hdri_file: str = ""
+    hdri_rotation: float = 0.0
+    ground_color: tuple = (0.5, 0.5, 0.5)
+    sky_color: tuple = (0.5, 0.5, 0.5)
+    env_intensity: float = 1.0
+
+
+def _get(d, key, default, kind, required=False):
+    if d is None or key not in d or d[key] is None:
+        if required:
+            raise ConfigError(f"missing field `{key}`")
+        return default
+    try:
+        return kind(d[key])
+    except (TypeError, ValueError) as e:
+        raise ConfigError(f"invalid field `{key}`: {d[key]!r}") from e
+
+
+def parse_app_config(text: str) -> AppConfig:
+    """serde_yaml::from_str::<AppConfig> (src/config/mod.rs:58-64): window,
+    tracer, scene_file, hdri_file and hdri_rotation are required; the rest
+    take their serde defaults.  An empty YAML scalar (`hdri_file:`) is ""."""
+    d = yaml.safe_load(text)
+    if not isinstance(d, dict):
+        raise ConfigError("the config is not a mapping")
+    for k in ("window", "tracer", "scene_file", "hdri_file", "hdri_rotation"):
+        if k not in d:
+            raise ConfigError(f"missing field `{k}`")
+    w = d["window"] or {}
+    window = WindowConfig(_get(w, "width", 0, int, True), _get(w, "height", 0, int, True))
+    t = d["tracer"] or {}
+    tm = t.get("tonemap") or {}
+    tracer = TracerConfig(_get(t, "max_depth", 0, int), _get(t, "rr_depth", 0, int),
+                          _get(t, "exposure_value", 0.0, float),
+                          TonemapConfig(_get(tm, "enable", False, bool), _get(tm, "enable_aces", False, bool),
+                                        _get(tm, "use_simple_aces", False, bool)),
+                          _get(t, "max_samples", 0, int))
+
+    def color(key):
+        v = d.get(key)
+        if v is None:
+            return (0.5, 0.5, 0.5)
+        if not isinstance(v, (list, tuple)) or len(v) != 3:
+            raise ConfigError(f"invalid field `{key}`: {v!r}")
+        return tuple(float(x) for x in v)
+
+    return AppConfig(window, tracer, str(d["scene_file"] or ""), str(d["hdri_file"] or ""),
+                     float(d["hdri_rotation"] or 0.0), color("ground_color"), color("sky_color"),
+                     _get(d, "env_intensity", 1.0, float))
+
+
+def validate_app_config(config: AppConfig) -> None:
+    """validate_app_config (src/config/mod.rs:43-53) with the window and tracer
+    checks of window.rs and tracer.rs."""
+    if config.window.width == 0:
+        raise ConfigError("The width is 0.")
+    if config.window.height == 0:
+        raise ConfigError("The height is 0.")
+    if config.tracer.rr_depth > config.tracer.max_depth:
+        raise ConfigError("The Russian Roulette depth is greater than the max depth.")
+    if not os.path.exists(config.scene_file):
+        raise ConfigError(f'The scene file "{config.scene_file}" is not found.')
+
+
+def load_app_config(path: str) -> AppConfig:
+    """load_app_config (src/config/mod.rs:58-64)."""
+    try:
+        with open(path) as f:
+            text = f.read()
+    except OSError as e:
+        raise ConfigError(f"Failed to read the config file: {path}") from e
+    return parse_app_config(text)
+
+
+# ------------------------------------------------------------- environment
+def read_hdr(path: str) -> np.ndarray:
+    """Radiance RGBE (.hdr) picture as (H, W, 3) float32, row 0 at the top:
+    flat or new-style run-length scanlines, -Y H +X W orientation."""
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    header = []
+    while True:
+        end = data.index(b"\n", pos)
+        line = data[pos:end]
+        pos = end + 1
+        if not line:
+            break
+        header.append(line)
+    if not header or not header[0].startswith(b"#?"):
+        raise ValueError(f"{path}: not a Radiance picture")
+    if not any(h.strip() == b"FORMAT=32-bit_rle_rgbe" for h in header if h.startswith(b"FORMAT")) and \
+            any(h.startswith(b"FORMAT") for h in header):
+        raise ValueError(f"{path}: only 32-bit_rle_rgbe is supported")
+    end = data.index(b"\n", pos)
+    res = data[pos:end].split()
+    pos = end + 1
+    if len(res) != 4 or res[0] != b"-Y" or res[2] != b"+X":
+        raise ValueError(f"{path}: unsupported orientation {res!r}")
+    h, w = int(res[1]), int(res[3])
+    out = np.zeros((h, w, 4), np.uint8)
+    buf = np.frombuffer(data, np.uint8)
+    for y in range(h):
+        if 8 <= w < 32768 and buf[pos] == 2 and buf[pos + 1] == 2 and (buf[pos + 2] << 8 | buf[pos + 3]) == w:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < w:
+                    n = int(buf[pos])
+                    pos += 1
+                    if n > 128:
+                        n -= 128
+                        out[y, x:x + n, c] = buf[pos]
+                        pos += 1
+                    else:
+                        out[y, x:x + n, c] = buf[pos:pos + n]
+                        pos += n
+                    x += n
+        else:
+            out[y] = buf[pos:pos + 4 * w].reshape(w, 4)
+            pos += 4 * w
+    e = out[..., 3].astype(np.int32)
+    scale = np.where(e > 0, np.ldexp(1.0, e - 136), 0.0)
+    return (out[..., :3].astype(np.float64) * scale[..., None]).astype(np.float32)
+
+
+def read_env(path: str) -> np.ndarray:
+    ext = os.path.splitext(path)[1].lower()
+    if ext == ".hdr":
+        return read_hdr(path)
+    if ext == ".pfm":
+        return hpt.read_pfm(path)
+    raise ValueError(f"{path}: environment maps are read from .hdr or .pfm (no EXR decoder in this build)")
+
+
+# ----------------------------------------------------------------- running
+def scene_features(scene: hpt.Scene) -> int:
+    """The permutation hala-pathtracer.rs:101-114 selects: MEDIUM and VOL_MIS
+    when a material has a medium, TRANSPARENT when a material's opacity is
+    below 1 (the unvendored has_medium / has_transparent, restated)."""
+    mats = scene.materials
+    used = {p.material_index for p in scene.primitives if p.material_index != hpt.INVALID_INDEX}
+    used = [i for i in used if i < len(mats)]
+    f = 0
+    if any(mats[i]["medium_type"] != hpt.MEDIUM_NONE for i in used):
+        f |= hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS
+    if any(mats[i]["opacity"] < 1.0 for i in used):
+        f |= hpt.FEATURE_TRANSPARENT
+    return f
+
+
+def create_renderer(config: AppConfig, device: int = 0):
+    """before_run (hala-pathtracer.rs:95-227) without the window: load the
+    scene, choose the permutation, create the renderer, set the scene and
+    the environment, commit.  Returns (renderer, loaded scene)."""
+    loaded = gltf_scene.load_gltf(config.scene_file)
+    scene = loaded.scene
+    t = config.tracer
+    r = hpt.HalaRenderer(config.window.width, config.window.height, t.max_depth, t.rr_depth,
+                         enable_tonemap=t.tonemap.enable, enable_aces=t.tonemap.enable_aces,
+                         use_simple_aces=t.tonemap.use_simple_aces, max_samples=t.max_samples,
+                         features=scene_features(scene), device=device)
+    r.set_scene(scene)  # with the glTF images as its texture table
+    if config.hdri_file:
+        r.set_envmap(read_env(config.hdri_file), config.hdri_rotation)
+    else:
+        r.set_ground_color(config.ground_color)
+        r.set_sky_color([c * 4.0 for c in config.sky_color])  # hala-pathtracer.rs:206
+    r.set_env_intensity(config.env_intensity)
+    r.set_exposure_value(t.exposure_value)
+    r.commit()
+    return r, loaded
+
+
+def save_final_png(path: str, rgba8: np.ndarray) -> None:
+    from PIL import Image
+
+    Image.fromarray(np.ascontiguousarray(rgba8[..., :3])).save(path)
+
+
+def run(config_path: str, frames: int, out_dir: str = "./out", device: int = 0) -> str:
+    config = load_app_config(config_path)
+    validate_app_config(config)
+    os.makedirs(out_dir, exist_ok=True)
+    r, _ = create_renderer(config, device)
+    r.render(frames)
+    r.wait_idle()
+    stem = os.path.splitext(os.path.basename(config.scene_file))[0]
+    prefix = os.path.join(out_dir, stem)
+    r.save_images(prefix)  # hala-pathtracer.rs:245-251
+    save_final_png(prefix + "_final.png", r.read_image(hpt.IMAGE_FINAL_RGBA8))
+    r.close()
+    return prefix
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="pathtracer", description="The PathTracer renderer (headless).")
+    ap.add_argument("-c", "--config", required=True, help="The file path of the config file.")
+    ap.add_argument("--frames", type=int, default=64, help="1-spp frames to accumulate")
+    ap.add_argument("--out", default="./out")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    prefix = run(a.config, a.frames, a.out, a.device)
+    print(f"saved {prefix}_color.pfm, _albedo.pfm, _normal.pfm, _final.png")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -98,10 +98,13 @@ struct hpt_renderer {
   uint32_t env_w = 0, env_h = 0;
   std::vector<float4> env_rgb, env_table;
   std::vector<uchar4> blue_noise;
+  // Material textures: one RGBA8 pool, per-texture offsets and sizes.
+  std::vector<uchar4> tex_texels;
+  std::vector<TexInfo> tex_info;
   // BVH.
   uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
   // Device.
-  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table, d_env_texels,
+  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table, d_env_texels, d_tex_texels, d_tex_info, d_srgb_lut,
       d_blue_noise;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
@@ -187,6 +190,12 @@ double build_alias(const std::vector<float>& lum, std::vector<float>& prob, std:
   return total;
 }
 
+// 8-bit sRGB code -> linear, the IEC 61966-2-1 decode in double rounded to float.
+float srgb_to_linear(int code) {
+  const double c = code / 255.0;
+  return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+
 void default_blue_noise(std::vector<uchar4>& bn) {
   // 1024x1024 stand-in for assets/textures/blue_noise.png (not in the
   // reference checkout): top bytes of pcg4d(x, y, 0x6A09E667, 0xBB67AE85).
@@ -215,6 +224,9 @@ SceneView scene_view(const hpt_renderer* r) {
   sv.env_rgb = r->d_env_rgb.as<float4>();
   sv.env_texels = r->d_env_texels.as<EnvTexel>();
   sv.blue_noise = r->d_blue_noise.as<uchar4>();
+  sv.tex_texels = r->d_tex_texels.as<uchar4>();
+  sv.tex_info = r->d_tex_info.as<TexInfo>();
+  sv.srgb_lut = r->d_srgb_lut.as<float>();
   sv.node_count = r->node_count;
   sv.tri_count = r->leaf_tri_count;
   sv.num_lights = r->num_lights;
@@ -486,9 +498,6 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
       const hpt_material& m = d->materials[i];
       need(m.type == HPT_MATERIAL_DIFFUSE || m.type == HPT_MATERIAL_DISNEY, "unknown material type");
       need(m.medium.type <= HPT_MEDIUM_EMISSIVE, "unknown medium type");
-      need(m.base_color_map_index == HPT_INVALID_INDEX && m.normal_map_index == HPT_INVALID_INDEX &&
-               m.metallic_roughness_map_index == HPT_INVALID_INDEX && m.emission_map_index == HPT_INVALID_INDEX,
-           "texture maps are not supported by this build", HPT_ERROR_UNSUPPORTED);
     }
     std::vector<float> tri_pos;
     std::vector<TriShade> tri_shade;
@@ -651,6 +660,34 @@ hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t w,
   });
 }
 
+hpt_status hpt_set_textures(hpt_renderer* r, const hpt_texture_desc* textures, uint32_t count) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(count == 0 || textures, "null textures");
+    std::vector<uchar4> texels;
+    std::vector<TexInfo> info;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      const hpt_texture_desc& t = textures[i];
+      need(t.width > 0 && t.height > 0 && t.rgba8, "texture " + std::to_string(i) + " is empty");
+      total += (uint64_t)t.width * t.height;
+      need(total < (1ull << 32), "textures too large");
+    }
+    texels.resize(total);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      const hpt_texture_desc& t = textures[i];
+      const size_t n = (size_t)t.width * t.height;
+      std::memcpy(texels.data() + off, t.rgba8, n * 4);
+      info.push_back(TexInfo{(uint32_t)off, t.width, t.height, 0});
+      off += n;
+    }
+    r->tex_texels.swap(texels);
+    r->tex_info.swap(info);
+    r->committed = false;
+  });
+}
+
 hpt_status hpt_commit(hpt_renderer* r) {
   return guarded([&] {
     need(r, "null renderer");
@@ -671,6 +708,20 @@ hpt_status hpt_commit(hpt_renderer* r) {
     upload(r->d_lights, r->lights, s);
     upload(r->d_cameras, r->cameras, s);
     upload_env(r, s);
+    const uint32_t ntex = (uint32_t)r->tex_info.size();
+    for (const hpt_material& m : r->materials)
+      for (uint32_t t : {m.base_color_map_index, m.normal_map_index, m.metallic_roughness_map_index,
+                         m.emission_map_index})
+        need(t == HPT_INVALID_INDEX || t < ntex, "material texture index out of range of hpt_set_textures");
+    // Uploads are asynchronous: every host array stays alive until the
+    // stream synchronisation at the end of this call.
+    const std::vector<uchar4> no_texels(1);
+    const std::vector<TexInfo> no_info(1);
+    upload(r->d_tex_texels, r->tex_texels.empty() ? no_texels : r->tex_texels, s);
+    upload(r->d_tex_info, r->tex_info.empty() ? no_info : r->tex_info, s);
+    std::vector<float> lut(256);
+    for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear(i);
+    upload(r->d_srgb_lut, lut, s);
     upload(r->d_blue_noise, r->blue_noise, s);
     const uint32_t n = r->info.width * r->info.height;
     r->npix = n;

@@ -147,6 +147,13 @@ struct EnvTexel {
 };
 static_assert(sizeof(EnvTexel) == 128, "EnvTexel is 128 bytes");
 
+// One material texture in the texel pool (RGBA8, row 0 at v = 0).
+struct TexInfo {
+  uint32_t offset;  // first texel in the pool
+  uint32_t width, height;
+  uint32_t pad;
+};
+
 // Device view of a committed scene, passed to kernels by value.
 struct SceneView {
   const Node8* nodes;
@@ -159,6 +166,9 @@ struct SceneView {
   const float4* env_rgb;       // width*height texels, .w unused
   const EnvTexel* env_texels;  // width*height records built from env_rgb and the alias table
   const uchar4* blue_noise;  // 1024x1024 RGBA8
+  const uchar4* tex_texels;  // all material textures, RGBA8
+  const TexInfo* tex_info;
+  const float* srgb_lut;     // 256 entries: 8-bit sRGB code -> linear
   uint32_t node_count;
   uint32_t tri_count;
   uint32_t num_lights;

@@ -164,6 +164,7 @@ struct Mat {
   float clearcoat, clearcoat_roughness, specular_transmission, ior, ax, ay;
   float medium_density, medium_anisotropy;
   uint32_t medium_type, type;
+  uint32_t base_color_map, normal_map, metallic_roughness_map, emission_map;
 };
 
 __device__ __forceinline__ Mat load_material(const hpt_material* mp) {
@@ -192,7 +193,41 @@ __device__ __forceinline__ Mat load_material(const hpt_material* mp) {
   m.ax = m7.y;
   m.ay = m7.z;
   m.type = __float_as_uint(m8.w);
+  m.base_color_map = __float_as_uint(m7.w);
+  m.normal_map = __float_as_uint(m8.x);
+  m.metallic_roughness_map = __float_as_uint(m8.y);
+  m.emission_map = __float_as_uint(m8.z);
   return m;
+}
+
+// textureLod(g_textures[t], uv, 0) (pathtracer.rgen.glsl:122-146): bilinear,
+// REPEAT in u and v, texel centres at (i + 0.5) / W.  Colour maps (base
+// colour, emission) decode each texel from sRGB before filtering, as an sRGB
+// image format does; data maps (metallic-roughness, normal) read c / 255.
+__device__ __forceinline__ float4 tex_sample(const SceneView& sv, uint32_t t, float u, float v, bool srgb) {
+  const TexInfo ti = sv.tex_info[t];
+  const int w = (int)ti.width, h = (int)ti.height;
+  const float x = u * (float)w - 0.5f;
+  const float y = v * (float)h - 0.5f;
+  const float fx0 = floorf(x), fy0 = floorf(y);
+  const float fx = x - fx0, fy = y - fy0;
+  int i0 = (int)fx0 % w;
+  if (i0 < 0) i0 += w;
+  int j0 = (int)fy0 % h;
+  if (j0 < 0) j0 += h;
+  const int i1 = i0 + 1 == w ? 0 : i0 + 1;
+  const int j1 = j0 + 1 == h ? 0 : j0 + 1;
+  const uchar4* px = sv.tex_texels + ti.offset;
+  auto texel = [&](int i, int j) {
+    const uchar4 c = px[(size_t)j * w + i];
+    if (srgb) return make_float4(sv.srgb_lut[c.x], sv.srgb_lut[c.y], sv.srgb_lut[c.z], (float)c.w / 255.0f);
+    return make_float4((float)c.x / 255.0f, (float)c.y / 255.0f, (float)c.z / 255.0f, (float)c.w / 255.0f);
+  };
+  const float4 a = texel(i0, j0), b = texel(i1, j0), c = texel(i0, j1), d = texel(i1, j1);
+  auto mix4 = [](float4 p, float4 q, float s) {
+    return make_float4(mixf(p.x, q.x, s), mixf(p.y, q.y, s), mixf(p.z, q.z, s), mixf(p.w, q.w, s));
+  };
+  return mix4(mix4(a, b, fx), mix4(c, d, fx), fy);
 }
 
 // Shading frame of one interaction (the State fields the BxDFs read).

@@ -267,6 +267,48 @@ __device__ __forceinline__ HitState triangle_state(const SceneView& sv, uint32_t
   return h;
 }
 
+// Texture coordinate of a triangle hit (triangles.rchit.glsl, tex_coord):
+// the vertex uvs of the triangle's shading record at the barycentrics.
+__device__ __forceinline__ void hit_uv(const SceneView& sv, uint32_t id, float u, float v, float& tu, float& tv) {
+  const float4* rec = reinterpret_cast<const float4*>(sv.tri_shade + id);
+  const float4 q4 = rec[4], q5 = rec[5], q6 = rec[6];
+  const float bx = 1.0f - u - v, by = u, bz = v;
+  tu = q4.w * bx + q5.y * by + q5.w * bz;
+  tv = q5.x * bx + q5.z * by + q6.x * bz;
+}
+
+// The material maps of pathtracer.rgen.glsl:122-146: base colour,
+// metallic (blue) and roughness (green), emission, and the normal map
+// (z scaled by 5 before the [0,1] -> [-1,1] remap; the bitangent follows the
+// new normal, the tangent is kept).
+__device__ __forceinline__ void apply_textures(const SceneView& sv, uint32_t id, float u, float v, f3 dir, Mat& m,
+                                               HitState& h) {
+  float tu, tv;
+  hit_uv(sv, id, u, v, tu, tv);
+  if (m.base_color_map != INVALID) {
+    const float4 c = tex_sample(sv, m.base_color_map, tu, tv, true);
+    m.base_color = mk3(c.x, c.y, c.z);
+  }
+  if (m.metallic_roughness_map != INVALID) {
+    const float4 mr = tex_sample(sv, m.metallic_roughness_map, tu, tv, false);
+    m.metallic = mr.z;
+    m.roughness = mr.y;
+  }
+  if (m.emission_map != INVALID) {
+    const float4 c = tex_sample(sv, m.emission_map, tu, tv, true);
+    m.emission = mk3(c.x, c.y, c.z);
+  }
+  if (m.normal_map != INVALID) {
+    const float4 c = tex_sample(sv, m.normal_map, tu, tv, false);
+    f3 nts = mk3(c.x, c.y, c.z * 5.0f);
+    nts = normalize(nts * 2.0f - 1.0f);
+    const f3 original = h.normal;
+    h.normal = normalize(h.tangent * nts.x + h.bitangent * nts.y + h.normal * nts.z);
+    h.ffnormal = dot(original, dir) <= 0.0f ? h.normal : -h.normal;
+    h.bitangent = cross(h.normal, h.tangent);
+  }
+}
+
 // inc/lighting.glsl:7-61 (MEDIUM + VOL_MIS): closest-hit steps through
 // transparent or refractive surfaces, Beer-Lambert on exit faces.
 template <bool TRANSP>
@@ -463,11 +505,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
         ps.nee_env[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         ps.nee_light[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       } else {
-        const Mat m = load_material(sv.materials + h.material);
+        Mat m = load_material(sv.materials + h.material);
         Frame fr;
+        fr.eta = dot(dir, h.normal) < 0.0f ? (1.0f / m.ior) : m.ior;  // before the normal map
+        if ((m.base_color_map & m.metallic_roughness_map & m.emission_map & m.normal_map) != INVALID)
+          apply_textures(sv, id, h4.y, h4.z, dir, m, h);
         fr.tangent = h.tangent;
         fr.bitangent = h.bitangent;
-        fr.eta = dot(dir, h.normal) < 0.0f ? (1.0f / m.ior) : m.ior;
         fr.depth = depth;
         if (depth == 0) {
           ps.albedo[p] = make_float4(m.base_color.x, m.base_color.y, m.base_color.z, 0.0f);

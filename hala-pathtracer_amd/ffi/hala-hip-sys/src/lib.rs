@@ -112,6 +112,13 @@ pub struct hpt_primitive_desc {
 }
 
 #[repr(C)]
+pub struct hpt_texture_desc {
+    pub width: u32,
+    pub height: u32,
+    pub rgba8: *const u8,
+}
+
+#[repr(C)]
 pub struct hpt_scene_desc {
     pub cameras: *const hpt_camera,
     pub camera_count: u32,
@@ -158,6 +165,7 @@ extern "C" {
     pub fn hpt_set_env_intensity(r: *mut hpt_renderer, intensity: f32) -> hpt_status;
     pub fn hpt_set_exposure_value(r: *mut hpt_renderer, ev: f32) -> hpt_status;
     pub fn hpt_set_blue_noise(r: *mut hpt_renderer, rgba8: *const u8, width: u32, height: u32) -> hpt_status;
+    pub fn hpt_set_textures(r: *mut hpt_renderer, textures: *const hpt_texture_desc, count: u32) -> hpt_status;
     pub fn hpt_commit(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_render(r: *mut hpt_renderer, frames: u32, camera_index: u32) -> hpt_status;
     pub fn hpt_reset_accumulation(r: *mut hpt_renderer) -> hpt_status;
@@ -281,6 +289,18 @@ impl HalaRenderer {
     }
 
     /// hala-pathtracer.rs:213.
+    /// The g_textures[] table of the scene's materials: (width, height, RGBA8 texels).
+    pub fn set_textures(&mut self, textures: &[(u32, u32, &[u8])]) -> anyhow::Result<()> {
+        for (w, h, t) in textures {
+            anyhow::ensure!(t.len() == (*w as usize) * (*h as usize) * 4, "texture size mismatch");
+        }
+        let descs: Vec<hpt_texture_desc> = textures
+            .iter()
+            .map(|(w, h, t)| hpt_texture_desc { width: *w, height: *h, rgba8: t.as_ptr() })
+            .collect();
+        check(unsafe { hpt_set_textures(self.raw, descs.as_ptr(), descs.len() as u32) })
+    }
+
     pub fn commit(&mut self) -> anyhow::Result<()> {
         check(unsafe { hpt_commit(self.raw) })
     }

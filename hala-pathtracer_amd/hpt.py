@@ -102,6 +102,22 @@ class _PrimitiveDesc(C.Structure):
                 ("indices", C.c_void_p)]
 
 
+class _TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba8", C.c_void_p)]
+
+
+def texture_descs(textures):
+    """(hpt_texture_desc array, keep-alive list) for RGBA8 (H, W, 4) arrays."""
+    keep = [np.ascontiguousarray(t, np.uint8) for t in textures]
+    for t in keep:
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise ValueError("textures are (height, width, 4) RGBA8 arrays")
+    arr = (_TextureDesc * max(1, len(keep)))()
+    for i, t in enumerate(keep):
+        arr[i] = _TextureDesc(t.shape[1], t.shape[0], t.ctypes.data)
+    return arr, keep
+
+
 class _SceneDesc(C.Structure):
     _fields_ = [("cameras", C.c_void_p), ("camera_count", C.c_uint32), ("light_count", C.c_uint32),
                 ("lights", C.c_void_p), ("materials", C.c_void_p), ("material_count", C.c_uint32),
@@ -153,6 +169,7 @@ SIGNATURES = {
     "hpt_set_profiling": [C.c_void_p, C.c_int],
     "hpt_get_stage_stats": [C.c_void_p, C.c_int, C.POINTER(_StageStats)],
     "hpt_set_frames_in_flight": [C.c_void_p, C.c_uint32],
+    "hpt_set_textures": [C.c_void_p, C.POINTER(_TextureDesc), C.c_uint32],
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
 }
 
@@ -217,6 +234,7 @@ class Scene:
     ground_color: tuple = (0.5, 0.5, 0.5)
     sky_color: tuple = (0.5, 0.5, 0.5)
     env_intensity: float = 1.0
+    textures: list = field(default_factory=list)  # RGBA8 (H, W, 4) arrays the *_map_index fields index
 
     @property
     def triangle_count(self) -> int:
@@ -294,15 +312,22 @@ class HalaRenderer:
 
     # -- set-up (hala-pathtracer.rs:195-213)
     def set_scene(self, scene: Scene) -> None:
+        """set_scene (hala-pathtracer.rs:198): the scene's arrays, its
+        texture table and its environment."""
         d, keep = scene.to_desc()
         _check(self._lib.hpt_set_scene(self._h, C.byref(d)))
         del keep
+        self.set_textures(scene.textures)
         if scene.env_rgb is not None:
             self.set_envmap(scene.env_rgb, scene.env_rotation)
         else:
             self.set_ground_color(scene.ground_color)
             self.set_sky_color(scene.sky_color)
         self.set_env_intensity(scene.env_intensity)
+
+    def set_textures(self, textures) -> None:
+        arr, keep = texture_descs(textures)
+        _check(self._lib.hpt_set_textures(self._h, arr, len(keep)))
 
     def set_envmap(self, rgb: np.ndarray, rotation: float = 0.0) -> None:
         a = np.ascontiguousarray(rgb, np.float32)

@@ -183,6 +183,64 @@ def cornell_box(width=512, height=512):
     return hpt.Scene(cam, lights, mats, prims, ground_color=(0, 0, 0), sky_color=(0, 0, 0))
 
 
+def checker_texture(n=8, size=64, a=(230, 230, 230), b=(40, 60, 200)):
+    """RGBA8 checkerboard of n x n squares."""
+    y, x = np.mgrid[0:size, 0:size]
+    on = ((x * n // size) + (y * n // size)) % 2 == 0
+    t = np.empty((size, size, 4), np.uint8)
+    t[..., :3] = np.where(on[..., None], np.array(a, np.uint8), np.array(b, np.uint8))
+    t[..., 3] = 255
+    return t
+
+
+def bump_normal_map(size=64, freq=4, amp=0.6):
+    """RGBA8 tangent-space normal map of sinusoidal bumps (z stored at 1.0 so
+    that the reference's z * 5 before the remap keeps it facing out)."""
+    y, x = np.mgrid[0:size, 0:size] / size
+    nx = amp * np.cos(2 * math.pi * freq * x)
+    ny = amp * np.cos(2 * math.pi * freq * y)
+    t = np.empty((size, size, 4), np.uint8)
+    t[..., 0] = np.clip((nx * 0.5 + 0.5) * 255 + 0.5, 0, 255)
+    t[..., 1] = np.clip((ny * 0.5 + 0.5) * 255 + 0.5, 0, 255)
+    t[..., 2] = 255
+    t[..., 3] = 255
+    return t
+
+
+def metal_rough_texture(size=32):
+    """RGBA8 metallic (blue) / roughness (green) ramp: metallic along u,
+    roughness along v."""
+    y, x = np.mgrid[0:size, 0:size] / (size - 1)
+    t = np.zeros((size, size, 4), np.uint8)
+    t[..., 1] = (0.1 + 0.8 * y) * 255
+    t[..., 2] = x * 255
+    t[..., 3] = 255
+    return t
+
+
+def textured_cornell_box(width=512, height=512):
+    """The Cornell box with material maps: a checkerboard base-colour map on
+    the floor, a bump normal map on the back wall, an emission map on the
+    ceiling, and a Disney tall box whose metallic/roughness come from a map."""
+    sc = cornell_box(width, height)
+    mats = np.concatenate([sc.materials, hpt.new_materials(4)])
+    diffuse_material(mats[3], (1.0, 1.0, 1.0))
+    mats[3]["base_color_map_index"] = 0
+    diffuse_material(mats[4], (0.73, 0.73, 0.73))
+    mats[4]["normal_map_index"] = 1
+    diffuse_material(mats[5], (0.73, 0.73, 0.73))
+    mats[5]["emission_map_index"] = 0
+    disney_material(mats[6], (0.8, 0.6, 0.3), metallic=0.0, roughness=0.5)
+    mats[6]["metallic_roughness_map_index"] = 2
+    sc.materials = mats
+    sc.primitives[0].material_index = 3   # floor
+    sc.primitives[2].material_index = 4   # back wall
+    sc.primitives[1].material_index = 5   # ceiling
+    sc.primitives[6].material_index = 6   # tall box
+    sc.textures = [checker_texture(), bump_normal_map(), metal_rough_texture()]
+    return sc
+
+
 def sky_env_map(width=2048, height=1024, seed=0, sun_elevation_deg=35.0, sun_azimuth=0.3,
                 sun_radiance=1.0e4, sun_diameter_deg=0.5):
     """Lat-long sky: gradient from horizon white to zenith blue, a dim ground

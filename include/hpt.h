@@ -270,6 +270,22 @@ HPT_API hpt_status hpt_set_exposure_value(hpt_renderer* r, float ev);
 HPT_API hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t width,
                                       uint32_t height);
 
+/* Material textures: the g_textures[] table (inc/global.glsl:48) that the
+ * Material *_map_index fields index (pathtracer.rgen.glsl:122-146), which the
+ * reference renderer fills from the scene's glTF images inside set_scene.
+ * Each texture is width*height RGBA8 texels, row 0 at v = 0.  Sampling is
+ * bilinear with REPEAT in u and v at level 0; base-colour and emission maps
+ * are decoded from sRGB per texel, metallic-roughness (B = metallic,
+ * G = roughness) and normal maps are read as linear.  Replaces the whole
+ * table; the materials' indices are checked against it at hpt_commit. */
+typedef struct hpt_texture_desc {
+  uint32_t width;
+  uint32_t height;
+  const uint8_t* rgba8;
+} hpt_texture_desc;
+
+HPT_API hpt_status hpt_set_textures(hpt_renderer* r, const hpt_texture_desc* textures, uint32_t count);
+
 /* HalaRenderer::commit (hala-pathtracer.rs:213): builds the BVH, uploads
  * everything and resets accumulation. */
 HPT_API hpt_status hpt_commit(hpt_renderer* r);

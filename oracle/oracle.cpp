@@ -62,6 +62,9 @@ struct or_scene {
   vec3 ground{0.5f, 0.5f, 0.5f}, sky{0.5f, 0.5f, 0.5f};
   float env_rotation = 0.0f, env_intensity = 1.0f, env_total = 0.0f;
   BlueNoise bn;
+  // Material textures (g_textures[], inc/global.glsl:48): RGBA8 rows, v = 0 first.
+  std::vector<std::vector<uint8_t>> tex;
+  std::vector<uint32_t> tex_w, tex_h;
 };
 
 namespace {
@@ -402,6 +405,71 @@ void trace_payload(const or_scene& s, const Ray& r, State& st) {
   st.tex_coord = tex_coord;
 }
 
+// ------------------------------------------------------------ textures
+// 8-bit sRGB code -> linear (IEC 61966-2-1 in double, rounded to float).
+float srgb_code_to_linear(int code) {
+  const double c = code / 255.0;
+  return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+
+// textureLod(g_textures[t], uv, 0): bilinear, REPEAT in u and v, texel
+// centres at (i + 0.5) / W; colour maps decode sRGB per texel before
+// filtering, data maps read c / 255.  (The sampler state lives in the
+// unvendored renderer; this is the build's stated choice.)
+vec4 texture_lod(const or_scene& s, uint32_t t, vec2 uv, bool srgb) {
+  const int w = (int)s.tex_w[t], h = (int)s.tex_h[t];
+  const float x = uv.x * (float)w - 0.5f;
+  const float y = uv.y * (float)h - 0.5f;
+  const float fx0 = std::floor(x), fy0 = std::floor(y);
+  const float fx = x - fx0, fy = y - fy0;
+  int i0 = (int)fx0 % w;
+  if (i0 < 0) i0 += w;
+  int j0 = (int)fy0 % h;
+  if (j0 < 0) j0 += h;
+  const int i1 = i0 + 1 == w ? 0 : i0 + 1;
+  const int j1 = j0 + 1 == h ? 0 : j0 + 1;
+  auto texel = [&](int i, int j) {
+    const uint8_t* c = s.tex[t].data() + ((size_t)j * w + i) * 4;
+    if (srgb)
+      return vec4(srgb_code_to_linear(c[0]), srgb_code_to_linear(c[1]), srgb_code_to_linear(c[2]),
+                  (float)c[3] / 255.0f);
+    return vec4((float)c[0] / 255.0f, (float)c[1] / 255.0f, (float)c[2] / 255.0f, (float)c[3] / 255.0f);
+  };
+  auto mix4 = [](vec4 p, vec4 q, float a) {
+    return vec4(mixf(p.x, q.x, a), mixf(p.y, q.y, a), mixf(p.z, q.z, a), mixf(p.w, q.w, a));
+  };
+  return mix4(mix4(texel(i0, j0), texel(i1, j0), fx), mix4(texel(i0, j1), texel(i1, j1), fx), fy);
+}
+
+// pathtracer.rgen.glsl:121-146: base colour, metallic (b) / roughness (g),
+// emission and the normal map (z * 5 before the remap; the bitangent
+// follows the new normal).
+void apply_texture_maps(const or_scene& s, vec3 dir, Material& mat, State& st) {
+  if (mat.base_color_map_index != INVALID_INDEX) {
+    const vec4 c = texture_lod(s, mat.base_color_map_index, st.tex_coord, true);
+    mat.base_color = vec3(c.x, c.y, c.z);
+  }
+  if (mat.metallic_roughness_map_index != INVALID_INDEX) {
+    const vec4 mr = texture_lod(s, mat.metallic_roughness_map_index, st.tex_coord, false);
+    mat.metallic = mr.z;
+    mat.roughness = mr.y;
+  }
+  if (mat.emission_map_index != INVALID_INDEX) {
+    const vec4 c = texture_lod(s, mat.emission_map_index, st.tex_coord, true);
+    mat.emission = vec3(c.x, c.y, c.z);
+  }
+  if (mat.normal_map_index != INVALID_INDEX) {
+    const vec4 c = texture_lod(s, mat.normal_map_index, st.tex_coord, false);
+    vec3 nts = vec3(c.x, c.y, c.z);
+    nts.z *= 5.0f;
+    nts = normalize(nts * 2.0f - 1.0f);
+    const vec3 original = st.normal;
+    st.normal = normalize(st.tangent * nts.x + st.bitangent * nts.y + st.normal * nts.z);
+    st.ffnormal = dot(original, dir) <= 0.0f ? st.normal : -st.normal;
+    st.bitangent = cross(st.normal, st.tangent);
+  }
+}
+
 // ------------------------------------------------------- integrator
 struct Variant {
   bool medium, vol_mis, transparent;
@@ -615,9 +683,9 @@ PathOut trace_path(const or_scene& s, const Globals& g, const Variant& var, uint
       break;
     }
 
-    const Material& mat = s.materials[pl.state.material_index];
+    Material mat = s.materials[pl.state.material_index];
     pl.state.eta = dot(pl.ray.direction, pl.state.normal) < 0.0f ? (1.0f / mat.ior) : mat.ior;
-    // Texture maps (pathtracer.rgen.glsl:122-146) are rejected at scene set-up.
+    apply_texture_maps(s, pl.ray.direction, mat, pl.state);  // pathtracer.rgen.glsl:121-146
 
     if (pl.state.depth == 0) {
       albedo = mat.base_color;
@@ -900,6 +968,29 @@ int or_scene_set_blue_noise(or_scene* s, const uint8_t* rgba8, uint32_t w, uint3
   if (!s || !rgba8 || w != 1024 || h != 1024) return 1;
   s->bn.rgba.assign(rgba8, rgba8 + (size_t)w * h * 4);
   return 0;
+}
+
+int or_scene_set_textures(or_scene* s, const hpt_texture_desc* t, uint32_t count) {
+  if (!s || (count && !t)) return 1;
+  s->tex.clear();
+  s->tex_w.clear();
+  s->tex_h.clear();
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!t[i].rgba8 || !t[i].width || !t[i].height) return 1;
+    s->tex.emplace_back(t[i].rgba8, t[i].rgba8 + (size_t)t[i].width * t[i].height * 4);
+    s->tex_w.push_back(t[i].width);
+    s->tex_h.push_back(t[i].height);
+  }
+  return 0;
+}
+
+float or_texture_sample(const or_scene* s, uint32_t t, float u, float v, int srgb, float* rgba) {
+  const vec4 c = texture_lod(*s, t, vec2(u, v), srgb != 0);
+  rgba[0] = c.x;
+  rgba[1] = c.y;
+  rgba[2] = c.z;
+  rgba[3] = c.w;
+  return c.x;
 }
 
 uint32_t or_scene_triangle_count(const or_scene* s) { return (uint32_t)s->tri_prim.size(); }

@@ -34,6 +34,10 @@ int or_scene_set_env_map(or_scene* s, const float* rgb, uint32_t w, uint32_t h, 
 void or_scene_set_env_sky(or_scene* s, const float ground[3], const float sky[3]);
 void or_scene_set_env_intensity(or_scene* s, float intensity);
 int or_scene_set_blue_noise(or_scene* s, const uint8_t* rgba8, uint32_t w, uint32_t h);
+/* Material texture table (hpt_set_textures semantics); 0 on success. */
+int or_scene_set_textures(or_scene* s, const hpt_texture_desc* textures, uint32_t count);
+/* One look-up of texture t at (u, v) into rgba[4]; srgb selects the colour-map decode. */
+float or_texture_sample(const or_scene* s, uint32_t t, float u, float v, int srgb, float* rgba);
 uint32_t or_scene_triangle_count(const or_scene* s);
 
 /* Ray queries. */

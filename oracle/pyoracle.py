@@ -47,6 +47,8 @@ def lib() -> C.CDLL:
             "or_scene_set_env_map": ([vp, vp, u32, u32, f32], i32),
             "or_scene_set_env_sky": ([vp, vp, vp], None), "or_scene_set_env_intensity": ([vp, f32], None),
             "or_scene_set_blue_noise": ([vp, vp, u32, u32], i32), "or_scene_triangle_count": ([vp], u32),
+            "or_scene_set_textures": ([vp, vp, u32], i32),
+            "or_texture_sample": ([vp, u32, f32, f32, i32, vp], f32),
             "or_trace_closest_brute": ([vp, vp, u32, vp], None), "or_trace_closest": ([vp, vp, u32, vp], None),
             "or_trace_any": ([vp, vp, u32, vp], None), "or_trace_any_brute": ([vp, vp, u32, vp], None),
             "or_trace_closest_fp64": ([vp, vp, u32, vp, vp, vp], None),
@@ -107,6 +109,14 @@ class OracleScene:
         else:
             L.or_scene_set_env_sky(self._p, f3((scene.ground_color)), f3((scene.sky_color)))
         L.or_scene_set_env_intensity(self._p, scene.env_intensity)
+        arr, keep = hpt.texture_descs(scene.textures)
+        if L.or_scene_set_textures(self._p, C.cast(arr, C.c_void_p), len(keep)) != 0:
+            raise RuntimeError("or_scene_set_textures failed")
+
+    def texture_sample(self, t: int, u: float, v: float, srgb: bool) -> np.ndarray:
+        out = np.zeros(4, np.float32)
+        lib().or_texture_sample(self._p, t, u, v, int(srgb), P(out))
+        return out
 
     def close(self):
         if self._p:

@@ -8,6 +8,8 @@ Converged: GPU frames [0, N) against oracle frames [N, 2N) (independent
 samples): per-pixel means agree within 3 sigma of the Monte Carlo error.
 Closed form: a closed emitting Lambert sphere gives Le * sum_k rho^k.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -239,5 +241,53 @@ def test_deep_tree_render_per_sample():
     ref, _, _ = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] deep tree per-sample agreement {agree:.4f}", flush=True)
+    assert agree > 0.97
+    r.close()
+
+
+def test_textured_materials_per_sample_and_converged():
+    # Base-colour, normal, emission and metallic-roughness maps
+    # (pathtracer.rgen.glsl:121-146) against the oracle's look-ups.
+    scene = hpt_scenes.textured_cornell_box()
+    w = h = 64
+    r = _renderer(scene, w, h)
+    o = pyoracle.OracleScene(scene)
+    st = pyoracle.settings(w, h, 8, 3)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    ref, alb, nrm = o.trace_paths(st, 0)
+    agree = _per_sample_agreement(gpu, ref)
+    print(f"[gpu] textured per-sample agreement {agree:.4f}", flush=True)
+    assert agree > 0.97
+    # first-hit AOVs after the maps: the texture coordinate is interpolated
+    # with fused multiply-adds on the GPU, so allow a few ulps of weight.
+    assert np.allclose(r.read_image(hpt.IMAGE_ALBEDO)[..., :3], alb, atol=1e-4)
+    assert np.allclose(r.read_image(hpt.IMAGE_NORMAL)[..., :3], nrm, atol=1e-4)
+    r.close()
+    _stat_test(scene, w, h, 48)
+
+
+def test_texture_index_checked_at_commit():
+    scene = hpt_scenes.textured_cornell_box()
+    scene.textures = scene.textures[:2]  # the metallic-roughness map (index 2) is missing
+    r = hpt.HalaRenderer(16, 16, max_depth=4, rr_depth=2)
+    r.set_scene(scene)
+    with pytest.raises(hpt.HptError, match="texture index"):
+        r.commit()
+    r.close()
+
+
+def test_reference_cornell_asset_per_sample():
+    import gltf_scene
+
+    scene = gltf_scene.load_gltf(os.path.join(os.path.dirname(__file__), "golden", "gltf",
+                                              "cornell-box.gltf")).scene
+    w, h = 80, 60
+    r = _renderer(scene, w, h, max_depth=8, rr_depth=4)
+    o = pyoracle.OracleScene(scene)
+    st = pyoracle.settings(w, h, 8, 4)
+    r.render(1)
+    agree = _per_sample_agreement(r.read_image(hpt.IMAGE_ACCUM)[..., :3], o.trace_paths(st, 0)[0])
+    print(f"[gpu] reference cornell asset per-sample agreement {agree:.4f}", flush=True)
     assert agree > 0.97
     r.close()
