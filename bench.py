@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -162,10 +163,63 @@ def time_to_relmse(make_renderer, ref_spp, threshold, chunk, max_spp):
             "curve": curve[:: max(1, len(curve) // 16)] + ([curve[-1]] if len(curve) % 16 else [])}
 
 
+def serving_scenes():
+    """The 8 resident scenes of the batched/serving workload: the Cornell box,
+    the Disney grid, the large-mesh scene with lights and a medium, and five
+    random triangle soups of 1e5..2e6 triangles (log-uniform, seeded)."""
+    rng = np.random.default_rng(0)
+    scenes = [("cornell", hpt_scenes.cornell_box(), 0),
+              ("disney_grid", hpt_scenes.disney_grid(seed=0), 0),
+              ("icosphere_lights_medium", hpt_scenes.displaced_icosphere_lights_medium(level=8),
+               hpt_scenes.MEDIUM_SCENE_FEATURES)]
+    for i in range(5):
+        n = int(round(math.exp(rng.uniform(math.log(1e5), math.log(2e6)))))
+        scenes.append((f"soup_{n}", hpt_scenes.triangle_soup(n, seed=100 + i), 0))
+    return scenes
+
+
+def serving_bench(args):
+    import serve
+
+    log("building and committing the 8 resident scenes")
+    resident = {}
+    for name, sc, feat in serving_scenes():
+        rs = serve.ResidentScene(name, sc, feat)
+        resident[name] = rs
+        log(f"  {name}: {sc.triangle_count} triangles, commit {rs.commit_seconds:.2f} s, "
+            f"{rs.scene_bytes / 2**20:.0f} MiB")
+    jobs = serve.poisson_jobs(args.jobs, args.rate, sorted(resident), seed=0)
+    offered = sum(j.width * j.height * j.spp for j in jobs)
+    log(f"{len(jobs)} jobs, {offered / 1e9:.2f} Gsamples offered over {jobs[-1].arrival:.2f} s of arrivals")
+    srv = serve.Server(resident, int(args.budget_gib * 2**30))
+    t0 = time.perf_counter()
+    res = srv.run(jobs)
+    makespan = time.perf_counter() - t0
+    summ = serve.summarize(res, makespan)
+    log(f"served: {summ}")
+    for rs in resident.values():
+        rs.close()
+    out = {
+        "metric": "Msamples/s", "value": summ["msamples_per_s"], "unit": "Msamples/s", "n_gpus": 1,
+        "steps": len(jobs), "warmup": 0, "ms_per_step": round(makespan / len(jobs) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serving", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
+                   "resident_scenes": {k: v.scene.triangle_count for k, v in resident.items()},
+                   "budget_gib": args.budget_gib, "offered_gsamples": round(offered / 1e9, 3),
+                   "arrival_span_s": round(jobs[-1].arrival, 3)},
+        "serving": summ, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="disney_grid_env_1080p")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving"], default="disney_grid_env_1080p")
+    ap.add_argument("--jobs", type=int, default=64, help="serving: number of jobs")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrivals per second")
+    ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: device memory for admitted jobs")
     ap.add_argument("--steps", type=int, default=None, help="frames (default: the workload's spp)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--width", type=int, default=None)
@@ -188,6 +242,9 @@ def main():
               f"described in DESIGN.md and not measured here.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        serving_bench(args)
+        return
     wl = WORKLOADS[args.workload]
     W = args.width or wl["width"]
     H = args.height or wl["height"]

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -69,6 +70,14 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
   if (!v.empty()) check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
 }
 
+// The committed scene on the device: BVH, shading arrays, environment,
+// textures, blue noise.
+struct SceneDevice {
+  uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
+  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table,
+      d_env_texels, d_tex_texels, d_tex_info, d_srgb_lut, d_blue_noise;
+};
+
 struct StageTimer {
   hipEvent_t a, b;
   int stage;
@@ -101,11 +110,9 @@ struct hpt_renderer {
   // Material textures: one RGBA8 pool, per-texture offsets and sizes.
   std::vector<uchar4> tex_texels;
   std::vector<TexInfo> tex_info;
-  // BVH.
-  uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
-  // Device.
-  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table, d_env_texels, d_tex_texels, d_tex_info, d_srgb_lut,
-      d_blue_noise;
+  // Device copy of the committed scene; shared by handles that render the
+  // same scene (hpt_share_scene), freed with the last of them.
+  std::shared_ptr<SceneDevice> sd;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
   DevBuf d_accum, d_albedo, d_normal, d_final;
@@ -214,21 +221,21 @@ void default_blue_noise(std::vector<uchar4>& bn) {
 
 SceneView scene_view(const hpt_renderer* r) {
   SceneView sv;
-  sv.nodes = r->d_nodes.as<Node8>();
-  sv.ltris = r->d_ltris.as<LeafTri>();
-  sv.tri_shade = r->d_tri_shade.as<TriShade>();
-  sv.prims = r->d_prims.as<PrimInfo>();
-  sv.materials = r->d_materials.as<hpt_material>();
-  sv.lights = r->d_lights.as<hpt_light>();
-  sv.cameras = r->d_cameras.as<hpt_camera>();
-  sv.env_rgb = r->d_env_rgb.as<float4>();
-  sv.env_texels = r->d_env_texels.as<EnvTexel>();
-  sv.blue_noise = r->d_blue_noise.as<uchar4>();
-  sv.tex_texels = r->d_tex_texels.as<uchar4>();
-  sv.tex_info = r->d_tex_info.as<TexInfo>();
-  sv.srgb_lut = r->d_srgb_lut.as<float>();
-  sv.node_count = r->node_count;
-  sv.tri_count = r->leaf_tri_count;
+  sv.nodes = r->sd->d_nodes.as<Node8>();
+  sv.ltris = r->sd->d_ltris.as<LeafTri>();
+  sv.tri_shade = r->sd->d_tri_shade.as<TriShade>();
+  sv.prims = r->sd->d_prims.as<PrimInfo>();
+  sv.materials = r->sd->d_materials.as<hpt_material>();
+  sv.lights = r->sd->d_lights.as<hpt_light>();
+  sv.cameras = r->sd->d_cameras.as<hpt_camera>();
+  sv.env_rgb = r->sd->d_env_rgb.as<float4>();
+  sv.env_texels = r->sd->d_env_texels.as<EnvTexel>();
+  sv.blue_noise = r->sd->d_blue_noise.as<uchar4>();
+  sv.tex_texels = r->sd->d_tex_texels.as<uchar4>();
+  sv.tex_info = r->sd->d_tex_info.as<TexInfo>();
+  sv.srgb_lut = r->sd->d_srgb_lut.as<float>();
+  sv.node_count = r->sd->node_count;
+  sv.tri_count = r->sd->leaf_tri_count;
   sv.num_lights = r->num_lights;
   sv.material_count = (uint32_t)r->materials.size();
   return sv;
@@ -255,8 +262,8 @@ PathState path_state(const hpt_renderer* r) {
 // more levels than the short stack holds.
 Spill ensure_spill(hpt_renderer* r, uint32_t threads) {
   Spill sp{nullptr, 0};
-  if (r->bvh_depth <= (uint32_t)kShortStack) return sp;
-  const size_t entries = (size_t)(r->bvh_depth - kShortStack) * threads;
+  if (r->sd->bvh_depth <= (uint32_t)kShortStack) return sp;
+  const size_t entries = (size_t)(r->sd->bvh_depth - kShortStack) * threads;
   if (r->d_spill.bytes < entries * sizeof(uint2)) r->d_spill.alloc(entries * sizeof(uint2));
   sp.buf = r->d_spill.as<uint2>();
   sp.stride = threads;
@@ -441,11 +448,30 @@ void upload_env(hpt_renderer* r, hipStream_t s) {
     r->env_table.assign(1, make_float4(1, 0, 0, 0));
   }
   const uint32_t w = r->env_type == HPT_ENV_MAP ? r->env_w : 1, h = r->env_type == HPT_ENV_MAP ? r->env_h : 1;
-  upload(r->d_env_rgb, r->env_rgb, s);
-  upload(r->d_env_table, r->env_table, s);
-  r->d_env_texels.alloc((size_t)w * h * sizeof(EnvTexel));
-  launch_env_texels(s, r->d_env_rgb.as<float4>(), r->d_env_table.as<float4>(), w, h, r->d_env_texels.as<EnvTexel>());
+  upload(r->sd->d_env_rgb, r->env_rgb, s);
+  upload(r->sd->d_env_table, r->env_table, s);
+  r->sd->d_env_texels.alloc((size_t)w * h * sizeof(EnvTexel));
+  launch_env_texels(s, r->sd->d_env_rgb.as<float4>(), r->sd->d_env_table.as<float4>(), w, h, r->sd->d_env_texels.as<EnvTexel>());
   check(hipGetLastError(), "environment records");
+}
+
+// Path state, queues, counters and images of a committed handle, zeroed.
+void alloc_frame_state(hpt_renderer* r) {
+  hipStream_t s = r->stream;
+  const uint32_t n = r->info.width * r->info.height;
+  r->npix = n;
+  alloc_paths(r, batch_frames_for(r));
+  const size_t f4 = (size_t)n * 16;
+  r->d_counts.alloc(3 * kSeg * kSegStride * sizeof(uint32_t));
+  check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
+  r->d_accum.alloc(f4);
+  r->d_albedo.alloc(f4);
+  r->d_normal.alloc(f4);
+  r->d_final.alloc((size_t)n * 4);
+  check(hipMemsetAsync(r->d_accum.p, 0, f4, s), "hipMemsetAsync");
+  check(hipMemsetAsync(r->d_albedo.p, 0, f4, s), "hipMemsetAsync");
+  check(hipMemsetAsync(r->d_normal.p, 0, f4, s), "hipMemsetAsync");
+  check(hipMemsetAsync(r->d_final.p, 0, (size_t)n * 4, s), "hipMemsetAsync");
 }
 
 void require_committed(const hpt_renderer* r) {
@@ -608,6 +634,9 @@ hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t w, uint32
     r->env_total = (float)total;
     r->env_w = w;
     r->env_h = h;
+    need(!(r->committed && r->sd.use_count() > 1),
+         "the environment map belongs to a scene shared with other handles; set it before sharing",
+         HPT_ERROR_STATE);
     r->env_rotation = rotation;
     r->env_type = HPT_ENV_MAP;
     if (r->committed) {
@@ -651,11 +680,14 @@ hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t w,
   return guarded([&] {
     need(r && rgba8, "null argument");
     need(w == 1024 && h == 1024, "the blue-noise texture must be 1024x1024 (rand4blue wraps at 1024)");
+    need(!(r->committed && r->sd.use_count() > 1),
+         "the blue-noise texture belongs to a scene shared with other handles; set it before sharing",
+         HPT_ERROR_STATE);
     r->blue_noise.resize(1024 * 1024);
     std::memcpy(r->blue_noise.data(), rgba8, 1024 * 1024 * 4);
     if (r->committed) {
       set_device(r);
-      upload(r->d_blue_noise, r->blue_noise, r->stream);
+      upload(r->sd->d_blue_noise, r->blue_noise, r->stream);
     }
   });
 }
@@ -696,17 +728,18 @@ hpt_status hpt_commit(hpt_renderer* r) {
     BvhBuildResult bvh;
     std::string err;
     need(build_bvh8(r->tri_pos.data(), (uint32_t)(r->tri_pos.size() / 9), bvh, err), err);
-    r->node_count = (uint32_t)bvh.nodes.size();
-    r->leaf_tri_count = (uint32_t)bvh.tris.size();
-    r->bvh_depth = bvh.depth;
+    r->sd = std::make_shared<SceneDevice>();  // handles sharing the previous scene keep it
+    r->sd->node_count = (uint32_t)bvh.nodes.size();
+    r->sd->leaf_tri_count = (uint32_t)bvh.tris.size();
+    r->sd->bvh_depth = bvh.depth;
     hipStream_t s = r->stream;
-    upload(r->d_nodes, bvh.nodes, s);
-    upload(r->d_ltris, bvh.tris, s);
-    upload(r->d_tri_shade, r->tri_shade, s);
-    upload(r->d_prims, r->prims, s);
-    upload(r->d_materials, r->materials, s);
-    upload(r->d_lights, r->lights, s);
-    upload(r->d_cameras, r->cameras, s);
+    upload(r->sd->d_nodes, bvh.nodes, s);
+    upload(r->sd->d_ltris, bvh.tris, s);
+    upload(r->sd->d_tri_shade, r->tri_shade, s);
+    upload(r->sd->d_prims, r->prims, s);
+    upload(r->sd->d_materials, r->materials, s);
+    upload(r->sd->d_lights, r->lights, s);
+    upload(r->sd->d_cameras, r->cameras, s);
     upload_env(r, s);
     const uint32_t ntex = (uint32_t)r->tex_info.size();
     for (const hpt_material& m : r->materials)
@@ -717,29 +750,84 @@ hpt_status hpt_commit(hpt_renderer* r) {
     // stream synchronisation at the end of this call.
     const std::vector<uchar4> no_texels(1);
     const std::vector<TexInfo> no_info(1);
-    upload(r->d_tex_texels, r->tex_texels.empty() ? no_texels : r->tex_texels, s);
-    upload(r->d_tex_info, r->tex_info.empty() ? no_info : r->tex_info, s);
+    upload(r->sd->d_tex_texels, r->tex_texels.empty() ? no_texels : r->tex_texels, s);
+    upload(r->sd->d_tex_info, r->tex_info.empty() ? no_info : r->tex_info, s);
     std::vector<float> lut(256);
     for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear(i);
-    upload(r->d_srgb_lut, lut, s);
-    upload(r->d_blue_noise, r->blue_noise, s);
-    const uint32_t n = r->info.width * r->info.height;
-    r->npix = n;
-    alloc_paths(r, batch_frames_for(r));
-    const size_t f4 = (size_t)n * 16;
-    r->d_counts.alloc(3 * kSeg * kSegStride * sizeof(uint32_t));
-    check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
-    r->d_accum.alloc(f4);
-    r->d_albedo.alloc(f4);
-    r->d_normal.alloc(f4);
-    r->d_final.alloc((size_t)n * 4);
-    check(hipMemsetAsync(r->d_accum.p, 0, f4, s), "hipMemsetAsync");
-    check(hipMemsetAsync(r->d_albedo.p, 0, f4, s), "hipMemsetAsync");
-    check(hipMemsetAsync(r->d_normal.p, 0, f4, s), "hipMemsetAsync");
-    check(hipMemsetAsync(r->d_final.p, 0, (size_t)n * 4, s), "hipMemsetAsync");
+    upload(r->sd->d_srgb_lut, lut, s);
+    upload(r->sd->d_blue_noise, r->blue_noise, s);
+    alloc_frame_state(r);
     check(hipStreamSynchronize(s), "commit");
     r->frame_index = 0;
     r->committed = true;
+  });
+}
+
+hpt_status hpt_share_scene(hpt_renderer* dst, const hpt_renderer* src) {
+  return guarded([&] {
+    need(dst && src, "null renderer");
+    need(src->committed, "the source renderer has no committed scene", HPT_ERROR_STATE);
+    need(dst != src, "a renderer cannot share its own scene");
+    need(dst->info.device == src->info.device, "renderers on different devices cannot share a scene");
+    set_device(dst);
+    // Host-side copies the frame parameters and the ray-query decoding read.
+    dst->cameras = src->cameras;
+    dst->camera_count = src->camera_count;
+    dst->lights = src->lights;
+    dst->num_lights = src->num_lights;
+    dst->materials = src->materials;
+    dst->tri_prim = src->tri_prim;
+    dst->prims = src->prims;
+    dst->env_type = src->env_type;
+    std::memcpy(dst->ground, src->ground, sizeof dst->ground);
+    std::memcpy(dst->sky, src->sky, sizeof dst->sky);
+    dst->env_intensity = src->env_intensity;
+    dst->env_rotation = src->env_rotation;
+    dst->env_total = src->env_total;
+    dst->env_w = src->env_w;
+    dst->env_h = src->env_h;
+    dst->tri_pos.clear();
+    dst->tri_shade.clear();
+    dst->sd = src->sd;
+    dst->scene_set = false;  // a later hpt_commit needs hpt_set_scene first
+    alloc_frame_state(dst);
+    check(hipStreamSynchronize(dst->stream), "share scene");
+    dst->frame_index = 0;
+    dst->committed = true;
+  });
+}
+
+hpt_status hpt_query_idle(hpt_renderer* r, int* idle) {
+  return guarded([&] {
+    need(r && idle, "null argument");
+    set_device(r);
+    const hipError_t e = hipStreamQuery(r->stream);
+    if (e == hipErrorNotReady) {
+      *idle = 0;
+      return;
+    }
+    check(e, "hipStreamQuery");
+    *idle = 1;
+  });
+}
+
+hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, uint64_t* scene_bytes) {
+  return guarded([&] {
+    need(r && frame_bytes && scene_bytes, "null argument");
+    uint64_t f = 0, sc = 0;
+    for (const DevBuf* b : {&r->d_rng, &r->d_org, &r->d_dir, &r->d_thr, &r->d_rad, &r->d_alb, &r->d_nrm, &r->d_med,
+                            &r->d_med2, &r->d_nee_e, &r->d_nee_l, &r->d_hit, &r->d_qa, &r->d_qb, &r->d_sh_o,
+                            &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill, &r->d_stats, &r->d_accum,
+                            &r->d_albedo, &r->d_normal, &r->d_final})
+      f += b->bytes;
+    if (r->sd)
+      for (const DevBuf* b : {&r->sd->d_nodes, &r->sd->d_ltris, &r->sd->d_tri_shade, &r->sd->d_prims,
+                              &r->sd->d_materials, &r->sd->d_lights, &r->sd->d_cameras, &r->sd->d_env_rgb,
+                              &r->sd->d_env_table, &r->sd->d_env_texels, &r->sd->d_tex_texels, &r->sd->d_tex_info,
+                              &r->sd->d_srgb_lut, &r->sd->d_blue_noise})
+        sc += b->bytes;
+    *frame_bytes = f;
+    *scene_bytes = sc;
   });
 }
 
@@ -913,9 +1001,9 @@ hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info) {
   return guarded([&] {
     need(r && info, "null argument");
     require_committed(r);
-    info->node_count = r->node_count;
-    info->triangle_count = r->leaf_tri_count;
-    info->max_depth = r->bvh_depth;
+    info->node_count = r->sd->node_count;
+    info->triangle_count = r->sd->leaf_tri_count;
+    info->max_depth = r->sd->bvh_depth;
     info->_pad = 0;
   });
 }
@@ -924,11 +1012,11 @@ hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes, void*
   return guarded([&] {
     need(r, "null renderer");
     require_committed(r);
-    need(node_bytes == (size_t)r->node_count * sizeof(Node8), "node buffer size mismatch");
-    need(tri_bytes == (size_t)r->leaf_tri_count * sizeof(LeafTri), "triangle buffer size mismatch");
+    need(node_bytes == (size_t)r->sd->node_count * sizeof(Node8), "node buffer size mismatch");
+    need(tri_bytes == (size_t)r->sd->leaf_tri_count * sizeof(LeafTri), "triangle buffer size mismatch");
     set_device(r);
-    if (node_bytes) check(hipMemcpy(nodes, r->d_nodes.p, node_bytes, hipMemcpyDeviceToHost), "export nodes");
-    if (tri_bytes) check(hipMemcpy(tris, r->d_ltris.p, tri_bytes, hipMemcpyDeviceToHost), "export tris");
+    if (node_bytes) check(hipMemcpy(nodes, r->sd->d_nodes.p, node_bytes, hipMemcpyDeviceToHost), "export nodes");
+    if (tri_bytes) check(hipMemcpy(tris, r->sd->d_ltris.p, tri_bytes, hipMemcpyDeviceToHost), "export tris");
   });
 }
 

@@ -170,6 +170,9 @@ SIGNATURES = {
     "hpt_get_stage_stats": [C.c_void_p, C.c_int, C.POINTER(_StageStats)],
     "hpt_set_frames_in_flight": [C.c_void_p, C.c_uint32],
     "hpt_set_textures": [C.c_void_p, C.POINTER(_TextureDesc), C.c_uint32],
+    "hpt_share_scene": [C.c_void_p, C.c_void_p],
+    "hpt_query_idle": [C.c_void_p, C.POINTER(C.c_int)],
+    "hpt_get_memory_usage": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
 }
 
@@ -324,6 +327,21 @@ class HalaRenderer:
             self.set_ground_color(scene.ground_color)
             self.set_sky_color(scene.sky_color)
         self.set_env_intensity(scene.env_intensity)
+
+    def share_scene(self, src: "HalaRenderer") -> None:
+        """Renders src's committed scene (shared device copy); commits self."""
+        _check(self._lib.hpt_share_scene(self._h, src._h))
+
+    def is_idle(self) -> bool:
+        v = C.c_int()
+        _check(self._lib.hpt_query_idle(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def memory_usage(self) -> tuple:
+        """(frame bytes, scene bytes) of device memory."""
+        f, sc = C.c_uint64(), C.c_uint64()
+        _check(self._lib.hpt_get_memory_usage(self._h, C.byref(f), C.byref(sc)))
+        return f.value, sc.value
 
     def set_textures(self, textures) -> None:
         arr, keep = texture_descs(textures)

@@ -389,6 +389,30 @@ typedef struct hpt_stage_stats {
 HPT_API hpt_status hpt_set_profiling(hpt_renderer* r, int enable);
 HPT_API hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out);
 
+/* ------------------------------------------------------------------------ */
+/* Serving (no counterpart in the reference, which renders one scene per   */
+/* process).  Many jobs over a few resident scenes share each scene's      */
+/* device copy; each job is a handle with its own stream, images and path  */
+/* state.                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Makes `dst` render the committed scene of `src` without another BVH build
+ * or upload: the device copy (BVH, shading arrays, environment, textures,
+ * blue noise) is shared and freed with the last handle using it.  `dst`
+ * keeps its own size, depths, tonemap, features and exposure; the sky
+ * colours and environment intensity are copied from `src` and may be set per
+ * handle afterwards.  `dst` becomes committed with cleared images.  While a
+ * scene is shared, hpt_set_env_map and hpt_set_blue_noise on any of its
+ * handles fail with HPT_ERROR_STATE. */
+HPT_API hpt_status hpt_share_scene(hpt_renderer* dst, const hpt_renderer* src);
+
+/* *idle = 1 when all work queued on the handle's stream has finished. */
+HPT_API hpt_status hpt_query_idle(hpt_renderer* r, int* idle);
+
+/* Device bytes the handle holds for its frames (path state, queues, images)
+ * and for its (possibly shared) scene. */
+HPT_API hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, uint64_t* scene_bytes);
+
 /* Frames traced together by hpt_render (no counterpart in the reference,
  * whose render loop submits one frame per command buffer).  A batch of k
  * frames is one wavefront of k * width * height paths; the accumulation

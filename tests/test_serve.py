@@ -1,0 +1,135 @@
+"""The serving loop: admission by device-memory footprint, continuous
+batching of jobs over shared resident scenes, and results that do not
+depend on the mix.  Host logic runs on the CPU against fake handles; the
+GPU tests run real jobs through the C ABI."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import hpt
+import hpt_scenes
+import serve
+
+
+class _FakeRenderer:
+    """Stands in for a job handle: renders by sleeping."""
+
+    live = 0
+    peak = 0
+    lock = threading.Lock()
+
+    def __init__(self, spec):
+        self.spec = spec
+        with _FakeRenderer.lock:
+            _FakeRenderer.live += 1
+            _FakeRenderer.peak = max(_FakeRenderer.peak, _FakeRenderer.live)
+
+    def render(self, n):
+        time.sleep(0.002 * n)
+
+    def wait_idle(self):
+        pass
+
+    def read_image(self, which):
+        return np.full((self.spec.height, self.spec.width, 4), self.spec.job_id, np.float32)
+
+    def close(self):
+        with _FakeRenderer.lock:
+            _FakeRenderer.live -= 1
+
+
+class _FakeServer(serve.Server):
+    def _make_job(self, spec):
+        if spec.scene == "broken":
+            raise RuntimeError("device lost")
+        return _FakeRenderer(spec)
+
+
+def _fake_scenes():
+    return {"a": None, "b": None, "broken": None}
+
+
+def test_admission_keeps_the_footprint_within_budget():
+    jobs = serve.poisson_jobs(40, rate=400.0, scene_names=["a", "b"], seed=1,
+                              resolutions=[(64, 64), (128, 96), (256, 128)], spp_range=(4, 40))
+    budget = 3 * serve.footprint(serve.JobSpec(0, "a", 256, 128, 40, 8))
+    srv = _FakeServer(_fake_scenes(), budget, keep_images=True)
+    t0 = time.perf_counter()
+    res = srv.run(jobs)
+    makespan = time.perf_counter() - t0
+    assert len(res) == 40 and all(r.error is None for r in res)
+    # at every admission instant the admitted, unfinished jobs fit the budget
+    for r in res:
+        live = [q for q in res if q.admitted <= r.admitted < q.finished]
+        assert sum(serve.footprint(q.spec) for q in live) <= budget
+    # admitted in arrival order (first come, first served)
+    order = sorted(res, key=lambda q: q.admitted)
+    assert [q.spec.job_id for q in order] == sorted(q.spec.job_id for q in res)
+    for r in res:
+        assert r.image[0, 0, 0] == r.spec.job_id and r.latency >= 0
+    s = serve.summarize(res, makespan)
+    assert s["jobs"] == 40 and s["failed"] == 0 and s["p99_latency_s"] >= s["p50_latency_s"]
+
+
+def test_a_failing_job_is_isolated():
+    jobs = [serve.JobSpec(i, "broken" if i == 2 else "a", 32, 32, 8, 8) for i in range(5)]
+    res = _FakeServer(_fake_scenes(), 1 << 40).run(jobs)
+    bad = [r for r in res if r.error]
+    assert len(res) == 5 and len(bad) == 1 and bad[0].job_id == 2 and "device lost" in bad[0].error
+
+
+def test_job_larger_than_the_budget_is_refused():
+    with pytest.raises(ValueError, match="above the budget"):
+        _FakeServer(_fake_scenes(), 1000).run([serve.JobSpec(0, "a", 64, 64, 8, 8)])
+
+
+def test_poisson_job_distribution():
+    jobs = serve.poisson_jobs(4000, rate=20.0, scene_names=list("abcdefgh"), seed=3)
+    gaps = np.diff([0.0] + [j.arrival for j in jobs])
+    assert abs(gaps.mean() - 1 / 20) < 0.005 and abs(gaps.std() - 1 / 20) < 0.006  # exponential
+    spp = np.log([j.spp for j in jobs])
+    assert spp.min() >= np.log(16) - 0.05 and spp.max() <= np.log(512) + 0.05
+    assert abs(np.median(spp) - 0.5 * (np.log(16) + np.log(512))) < 0.1  # log-uniform
+    assert {j.max_depth for j in jobs} == set(range(8, 13))
+    assert {(j.width, j.height) for j in jobs} == {(256, 256), (512, 512), (1024, 768), (1920, 1080)}
+
+
+@pytest.mark.gpu
+def test_results_identical_alone_and_in_the_mix():
+    scenes = {
+        "cornell": serve.ResidentScene("cornell", hpt_scenes.cornell_box(), 0),
+        "grid": serve.ResidentScene("grid", hpt_scenes.disney_grid(env_width=128, env_height=64, segments=16,
+                                                                   rings=8), 0),
+        "medium": serve.ResidentScene("medium", hpt_scenes.displaced_icosphere_lights_medium(level=2),
+                                      hpt_scenes.MEDIUM_SCENE_FEATURES),
+    }
+    specs = [serve.JobSpec(0, "cornell", 64, 48, 20, 8), serve.JobSpec(1, "grid", 96, 54, 12, 10),
+             serve.JobSpec(2, "medium", 80, 45, 8, 12), serve.JobSpec(3, "cornell", 40, 40, 33, 9),
+             serve.JobSpec(4, "grid", 64, 64, 5, 8), serve.JobSpec(5, "medium", 48, 32, 17, 10)]
+    print("[gpu] serving 6 jobs together", flush=True)
+    mixed = {r.job_id: r for r in serve.Server(scenes, 1 << 34, keep_images=True).run(specs)}
+    assert all(r.error is None for r in mixed.values()), [r.error for r in mixed.values()]
+    for s in specs:
+        print(f"[gpu] job {s.job_id} alone", flush=True)
+        alone = serve.Server(scenes, 1 << 34, keep_images=True).run([s])[0]
+        assert np.array_equal(alone.image, mixed[s.job_id].image)
+    # one of them against a handle that owns its scene (no sharing at all)
+    s = specs[1]
+    r = hpt.HalaRenderer(s.width, s.height, max_depth=s.max_depth, rr_depth=s.rr_depth)
+    r.set_scene(scenes["grid"].scene)
+    r.commit()
+    r.render(s.spp)
+    assert np.array_equal(r.read_image(hpt.IMAGE_ACCUM), mixed[1].image)
+    r.close()
+    # the shared scene's environment cannot change under its jobs
+    j = hpt.HalaRenderer(16, 16, max_depth=4, rr_depth=2)
+    j.share_scene(scenes["grid"].template)
+    with pytest.raises(hpt.HptError, match="shared"):
+        j.set_envmap(np.ones((4, 8, 3), np.float32))
+    f, sc = j.memory_usage()
+    assert f > 0 and sc == scenes["grid"].scene_bytes
+    j.close()
+    for rs in scenes.values():
+        rs.close()
