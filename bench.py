@@ -191,7 +191,7 @@ def serving_bench(args):
     jobs = serve.poisson_jobs(args.jobs, args.rate, sorted(resident), seed=0)
     offered = sum(j.width * j.height * j.spp for j in jobs)
     log(f"{len(jobs)} jobs, {offered / 1e9:.2f} Gsamples offered over {jobs[-1].arrival:.2f} s of arrivals")
-    srv = serve.Server(resident, int(args.budget_gib * 2**30))
+    srv = serve.Server(resident, int(args.budget_gib * 2**30), max_frames=args.job_frames)
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
@@ -206,7 +206,8 @@ def serving_bench(args):
         "data": "synthetic",
         "config": {"workload": "serving", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
                    "resident_scenes": {k: v.scene.triangle_count for k, v in resident.items()},
-                   "budget_gib": args.budget_gib, "offered_gsamples": round(offered / 1e9, 3),
+                   "budget_gib": args.budget_gib, "job_frames_in_flight": args.job_frames,
+                   "offered_gsamples": round(offered / 1e9, 3),
                    "arrival_span_s": round(jobs[-1].arrival, 3)},
         "serving": summ, "roofline": None, "cpu_baseline": None,
     }
@@ -219,7 +220,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving"], default="disney_grid_env_1080p")
     ap.add_argument("--jobs", type=int, default=64, help="serving: number of jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrivals per second")
-    ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: device memory for admitted jobs")
+    ap.add_argument("--budget-gib", type=float, default=192.0, help="serving: device memory for admitted jobs")
+    ap.add_argument("--job-frames", type=int, default=16, help="serving: frames in flight per job (at most)")
     ap.add_argument("--steps", type=int, default=None, help="frames (default: the workload's spp)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--width", type=int, default=None)
