@@ -1,0 +1,669 @@
+// Kernels of the learned denoiser (C ABI in include/hpt_nn.h): BF16 implicit-GEMM
+// convolutions on MFMA with float32 accumulation (forward, data gradient, weight gradient) and
+// the U-Net's elementwise steps.  The reference has no counterpart: its denoise.py shells out to
+// Intel OIDN (denoise.py:7-30).
+//
+// Convolution as GEMM.  For a k x k convolution (zero padding k/2) over NHWC tensors,
+//     out[p][n] = sum_t sum_c in[p + off(t)][c] * W[n][t][c],
+// with p a pixel, t a tap and off(t) its neighbour inside the same image (zero outside).  One
+// wave owns 16*MT pixels x 16*NT output channels and runs v_mfma_f32_16x16x32_bf16: A = 16
+// pixels x 32 K-values, B = 32 K-values x 16 channels.  The K axis is the sequence of
+// (tap, 8-channel segment) pairs; each of the four lane groups of a wave loads one pair per
+// step (16 contiguous bytes of the pixel's channels, 16 contiguous bytes of the weight row), so
+// A and B fill their K slots with the same (lane group, element) -> K assignment and the product
+// does not depend on how the instruction orders K internally.  Any channel count that is a
+// multiple of 8 fits; the last step of a K loop zero-fills the lane groups past the end.
+// The data gradient is the same kernel over dy (float32, rounded to bf16 on load) with the
+// transposed, spatially flipped weights.  The weight gradient contracts over pixels: one wave
+// owns 16*MT output x 16*NT input channels of one tap over a chunk of pixels, writes a partial
+// sum, and a second pass adds the chunks in order (no atomics: the result is deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/hpt_nn.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / 64;
+constexpr int kWgradChunk = 1024;  // pixels per weight-gradient partial sum (a multiple of 32)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bit pattern, round to nearest even (finite inputs).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_value(uint32_t b) { return __uint_as_float(b << 16); }
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+__device__ __forceinline__ bf16x8 load8(const float* p) {
+  const f32x4 a = *reinterpret_cast<const f32x4*>(p);
+  const f32x4 b = *reinterpret_cast<const f32x4*>(p + 4);
+  return __builtin_bit_cast(
+      bf16x8, u32x4{pack_bf16(a.x, a.y), pack_bf16(a.z, a.w), pack_bf16(b.x, b.y), pack_bf16(b.z, b.w)});
+}
+
+// ------------------------------------------------------------------ convolution
+// FORWARD: y = act(out + bias) as bf16 and/or float32, `nstore` channels per pixel.
+// Otherwise (data gradient): yf = out as float32, `nstore` (= the GEMM's N) channels.
+template <typename InT, int KS, int MT, int NT, bool FORWARD>
+__global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, const uint16_t* __restrict__ wgt,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                   float* __restrict__ yf, int P, int h, int w, int cin, int nstore,
+                                                   int relu) {
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int p0 = (blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * (16 * MT);
+  if (p0 >= P) return;  // uniform over the wave; no barriers below
+  const int n0 = blockIdx.y * (16 * NT);
+  const int C8 = cin >> 3, S = KS * KS * C8, K = KS * KS * cin;
+
+  int px[MT], py[MT];
+  bool pv[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) {
+    const int p = p0 + 16 * i + m;
+    pv[i] = p < P;
+    const int pc = pv[i] ? p : 0;
+    const int row = pc / w;
+    px[i] = pc - row * w;
+    py[i] = row % h;
+  }
+  const uint16_t* wrow[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) wrow[j] = wgt + (size_t)(n0 + 16 * j + m) * K;
+
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  int s = g, tap = g / C8, c8 = g - (g / C8) * C8;
+  const int steps = (S + 3) >> 2;
+  for (int step = 0; step < steps; ++step) {
+    const bool sv = s < S;
+    const int ty = tap / KS, tx = tap - ty * KS;
+    const int oy = ty - KS / 2, ox = tx - KS / 2;
+    bf16x8 a[MT], b[NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const bool ok = sv && pv[i] && (unsigned)(py[i] + oy) < (unsigned)h && (unsigned)(px[i] + ox) < (unsigned)w;
+      if (ok)
+        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * cin + c8 * 8);
+      else
+        a[i] = zero8();
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      if (sv)
+        b[j] = load8(wrow[j] + s * 8);
+      else
+        b[j] = zero8();
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    s += 4;
+    c8 += 4;
+    while (c8 >= C8) {
+      c8 -= C8;
+      ++tap;
+    }
+  }
+
+  // Lane (m, g) holds D[4g + r][m]: pixel p0 + 16i + 4g + r, channel n0 + 16j + m.
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = n0 + 16 * j + m;
+    if (co >= nstore) continue;
+    const float bv = FORWARD ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int p = p0 + 16 * i + 4 * g + r;
+        if (p >= P) continue;
+        const size_t o = (size_t)p * nstore + co;
+        if (FORWARD) {
+          float v = acc[i][j][r] + bv;
+          if (relu) v = fmaxf(v, 0.f);
+          if (y) y[o] = (uint16_t)bf16_bits(v);
+          if (yf) yf[o] = v;
+        } else {
+          yf[o] = acc[i][j][r];
+        }
+      }
+  }
+}
+
+template <typename InT, int KS, int MT, int NT, bool FORWARD>
+int go_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P, int h,
+            int wd, int cin, int nstore, int relu, int groups) {
+  const int tiles = (P + 16 * MT - 1) / (16 * MT);
+  const dim3 grid((tiles + kWavesPerBlock - 1) / kWavesPerBlock, groups);
+  hipLaunchKernelGGL((k_conv<InT, KS, MT, NT, FORWARD>), grid, dim3(kThreads), 0, st, in, w, b, y, yf, P, h, wd, cin,
+                     nstore, relu);
+  return (int)hipGetLastError();
+}
+
+// N = nout channels (a multiple of 16) split into `groups` grid rows of NT <= 8 tiles each.
+template <typename InT, int KS, bool FORWARD>
+int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
+                   int h, int wd, int cin, int nout, int nstore, int relu) {
+  const int tiles = nout / 16;
+  int groups = (tiles + 7) / 8;
+  while (tiles % groups) ++groups;
+  switch (tiles / groups) {
+    case 1: return go_conv<InT, KS, 4, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 2: return go_conv<InT, KS, 4, 2, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 3: return go_conv<InT, KS, 4, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 4: return go_conv<InT, KS, 4, 4, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 5: return go_conv<InT, KS, 2, 5, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 6: return go_conv<InT, KS, 2, 6, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 7: return go_conv<InT, KS, 2, 7, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    default: return go_conv<InT, KS, 2, 8, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+  }
+}
+
+template <typename InT, bool FORWARD>
+int launch_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
+                int h, int wd, int cin, int nout, int nstore, int relu, int k) {
+  if (k == 1) return launch_conv_ks<InT, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu);
+  return launch_conv_ks<InT, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu);
+}
+
+// ------------------------------------------------------------------ weight gradient
+// partial[chunk][co][t][ci] = sum over the chunk's pixels p of dy[p][co] * x[p + off(t)][ci].
+// K = pixels: lane group g takes the 8 consecutive pixels base + 8g .. base + 8g + 7 of each
+// 32-pixel step, for A (dy, rows = output channels) and B (x, columns = input channels) alike.
+template <int KS, int MT, int NT>
+__global__ __launch_bounds__(kThreads) void k_wgrad(const float* __restrict__ dy, const uint16_t* __restrict__ x,
+                                                    float* __restrict__ partial, int P, int h, int w, int cin,
+                                                    int cout_pad, int chunks, int co_groups, int ci_groups) {
+  constexpr int taps = KS * KS;
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (item >= chunks * taps * co_groups * ci_groups) return;  // uniform over the wave
+  const int cig = item % ci_groups;
+  item /= ci_groups;
+  const int cog = item % co_groups;
+  item /= co_groups;
+  const int tap = item % taps, chunk = item / taps;
+  const int co0 = cog * 16 * MT, ci0 = cig * 16 * NT;
+  const int oy = tap / KS - KS / 2, ox = tap % KS - KS / 2;
+  const int pbeg = chunk * kWgradChunk;
+  const int pend = min(P, pbeg + kWgradChunk);
+
+  int p = pbeg + 8 * g;  // this lane group's first pixel in the step
+  int row = p / w;
+  int px = p - row * w, py = row % h;
+
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int base = pbeg; base < pend; base += 32) {
+    uint32_t ab[MT][4], bb[NT][4];
+    int qx = px, qy = py;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      float av[MT][2], bv[NT][2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int pi = p + e + u;
+        const bool live = pi < pend;
+        const bool ok = live && (unsigned)(qy + oy) < (unsigned)h && (unsigned)(qx + ox) < (unsigned)w;
+#pragma unroll
+        for (int i = 0; i < MT; ++i) av[i][u] = live ? dy[(size_t)pi * cout_pad + co0 + 16 * i + m] : 0.f;
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          bv[j][u] = ok ? bf16_value(x[(size_t)(pi + oy * w + ox) * cin + ci0 + 16 * j + m]) : 0.f;
+        if (++qx == w) {
+          qx = 0;
+          if (++qy == h) qy = 0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MT; ++i) ab[i][e >> 1] = pack_bf16(av[i][0], av[i][1]);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bb[j][e >> 1] = pack_bf16(bv[j][0], bv[j][1]);
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const bf16x8 a = __builtin_bit_cast(bf16x8, u32x4{ab[i][0], ab[i][1], ab[i][2], ab[i][3]});
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bf16x8 b = __builtin_bit_cast(bf16x8, u32x4{bb[j][0], bb[j][1], bb[j][2], bb[j][3]});
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+      }
+    }
+    p += 32;
+    px += 32;
+    while (px >= w) {
+      px -= w;
+      if (++py == h) py = 0;
+    }
+  }
+
+  // Lane (m, g) holds D[4g + r][m]: output channel co0 + 16i + 4g + r, input channel ci0 + 16j + m.
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int co = co0 + 16 * i + 4 * g + r, ci = ci0 + 16 * j + m;
+        partial[((size_t)(chunk * cout_pad + co) * taps + tap) * cin + ci] = acc[i][j][r];
+      }
+}
+
+// partial_b[chunk][co] = sum over the chunk's pixels of dy[p][co].
+__global__ __launch_bounds__(kThreads) void k_bias_partial(const float* __restrict__ dy, float* __restrict__ partial_b,
+                                                           int P, int cout_pad) {
+  const int co = threadIdx.x;
+  if (co >= cout_pad) return;
+  const int pbeg = blockIdx.x * kWgradChunk, pend = min(P, pbeg + kWgradChunk);
+  float s = 0.f;
+  for (int p = pbeg; p < pend; ++p) s += dy[(size_t)p * cout_pad + co];
+  partial_b[(size_t)blockIdx.x * cout_pad + co] = s;
+}
+
+// dw[co][t][ci] (co < cout) = sum of the chunks in order; db likewise.
+__global__ __launch_bounds__(kThreads) void k_wgrad_reduce(const float* __restrict__ partial,
+                                                           const float* __restrict__ partial_b, float* __restrict__ dw,
+                                                           float* __restrict__ db, int chunks, int cout, int cout_pad,
+                                                           int row) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const size_t n = (size_t)cout * row;
+  if (idx < n) {
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial[(size_t)c * cout_pad * row + idx];
+    dw[idx] = s;
+  } else if (idx < n + (size_t)cout) {
+    const int co = (int)(idx - n);
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial_b[(size_t)c * cout_pad + co];
+    db[co] = s;
+  }
+}
+
+template <int KS, int MT, int NT>
+int go_wgrad(hipStream_t st, const float* dy, const uint16_t* x, float* partial, int P, int h, int wd, int cin,
+             int cout_pad, int chunks) {
+  const int co_groups = cout_pad / (16 * MT), ci_groups = cin / (16 * NT);
+  const long items = (long)chunks * KS * KS * co_groups * ci_groups;
+  hipLaunchKernelGGL((k_wgrad<KS, MT, NT>), dim3((unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock)),
+                     dim3(kThreads), 0, st, dy, x, partial, P, h, wd, cin, cout_pad, chunks, co_groups, ci_groups);
+  return (int)hipGetLastError();
+}
+
+template <int KS>
+int launch_wgrad_ks(hipStream_t st, const float* dy, const uint16_t* x, float* partial, int P, int h, int wd, int cin,
+                    int cout_pad, int chunks) {
+  const bool co2 = (cout_pad / 16) % 2 == 0, ci2 = (cin / 16) % 2 == 0;
+  if (co2 && ci2) return go_wgrad<KS, 2, 2>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks);
+  if (co2) return go_wgrad<KS, 2, 1>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks);
+  if (ci2) return go_wgrad<KS, 1, 2>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks);
+  return go_wgrad<KS, 1, 1>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks);
+}
+
+// ------------------------------------------------------------------ elementwise steps
+__global__ void k_prepare_weights(const float* __restrict__ master, uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                                  int cin, int cout, int cout_pad, int taps) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const size_t row = (size_t)taps * cin;
+  if (idx >= (size_t)cout_pad * row) return;
+  const int co = (int)(idx / row);
+  const int rem = (int)(idx - (size_t)co * row);
+  const int t = rem / cin, ci = rem - t * cin;
+  const uint16_t b = (uint16_t)(co < cout ? bf16_bits(master[(size_t)co * row + rem]) : 0u);
+  if (w) w[idx] = b;
+  if (wt) wt[((size_t)ci * taps + (taps - 1 - t)) * cout_pad + co] = b;
+}
+
+__global__ void k_relu_backward(float* __restrict__ dy, const uint16_t* __restrict__ y, size_t count) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < count && (int16_t)y[i] <= 0) dy[i] = 0.f;
+}
+
+__global__ void k_pool_forward(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w, int c) {
+  const int ho = h / 2, wo = w / 2;
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= (size_t)n * ho * wo * c) return;
+  const int ch = (int)(idx % c);
+  const size_t pix = idx / c;
+  const int xo = (int)(pix % wo);
+  const size_t r = pix / wo;  // img * ho + yo
+  const int yo = (int)(r % ho);
+  const size_t img = r / ho;
+  const size_t i00 = ((img * h + 2 * yo) * w + 2 * xo) * c + ch;
+  const size_t dr = (size_t)w * c;
+  const float s = (bf16_value(x[i00]) + bf16_value(x[i00 + c])) + (bf16_value(x[i00 + dr]) + bf16_value(x[i00 + dr + c]));
+  y[idx] = (uint16_t)bf16_bits(0.25f * s);
+}
+
+__global__ void k_pool_backward(const float* __restrict__ dy, float* __restrict__ dx, int n, int h, int w, int c) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= (size_t)n * h * w * c) return;
+  const int ch = (int)(idx % c);
+  const size_t pix = idx / c;
+  const int xx = (int)(pix % w);
+  const size_t r = pix / w;
+  const int yy = (int)(r % h);
+  const size_t img = r / h;
+  dx[idx] = 0.25f * dy[((img * (h / 2) + yy / 2) * (w / 2) + xx / 2) * c + ch];
+}
+
+__global__ void k_upsample_forward(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int n, int h, int w,
+                                   int c) {
+  const int ho = 2 * h, wo = 2 * w;
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= (size_t)n * ho * wo * c) return;
+  const int ch = (int)(idx % c);
+  const size_t pix = idx / c;
+  const int xo = (int)(pix % wo);
+  const size_t r = pix / wo;
+  const int yo = (int)(r % ho);
+  const size_t img = r / ho;
+  y[idx] = x[((img * h + yo / 2) * w + xo / 2) * c + ch];
+}
+
+__global__ void k_upsample_backward(const float* __restrict__ dy, float* __restrict__ dx, int n, int h, int w, int c) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= (size_t)n * h * w * c) return;
+  const int ch = (int)(idx % c);
+  const size_t pix = idx / c;
+  const int xx = (int)(pix % w);
+  const size_t r = pix / w;
+  const int yy = (int)(r % h);
+  const size_t img = r / h;
+  const size_t i00 = ((img * 2 * h + 2 * yy) * 2 * w + 2 * xx) * c + ch;
+  const size_t dr = (size_t)2 * w * c;
+  dx[idx] = (dy[i00] + dy[i00 + c]) + (dy[i00 + dr] + dy[i00 + dr + c]);
+}
+
+__global__ void k_concat_forward(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
+                                 uint16_t* __restrict__ y, size_t pixels, int ca, int cb) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const int c = ca + cb;
+  if (idx >= pixels * c) return;
+  const size_t p = idx / c;
+  const int ch = (int)(idx - p * c);
+  y[idx] = ch < ca ? a[p * ca + ch] : b[p * cb + ch - ca];
+}
+
+__global__ void k_concat_backward(const float* __restrict__ dy, float* __restrict__ da, float* __restrict__ db,
+                                  size_t pixels, int ca, int cb) {
+  const size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const int c = ca + cb;
+  if (idx >= pixels * c) return;
+  const size_t p = idx / c;
+  const int ch = (int)(idx - p * c);
+  if (ch < ca) {
+    if (da) da[p * ca + ch] = dy[idx];
+  } else if (db) {
+    db[p * cb + ch - ca] = dy[idx];
+  }
+}
+
+__global__ void k_accumulate(float* __restrict__ dx, const float* __restrict__ dy, size_t count) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < count) dx[i] += dy[i];
+}
+
+__global__ void k_make_input(const float4* __restrict__ color, const float4* __restrict__ albedo,
+                             const float4* __restrict__ normal, int src_w, int src_h, int x0, int y0,
+                             uint32_t* __restrict__ x, int h, int w) {
+  const int idx = blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= h * w) return;
+  const int yy = idx / w, xx = idx - yy * w;
+  const int sx = x0 + xx, sy = y0 + yy;
+  float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (sx >= 0 && sy >= 0 && sx < src_w && sy < src_h) {
+    const size_t s = (size_t)sy * src_w + sx;
+    const float4 c = color[s], a = albedo[s], nn = normal[s];
+    v[0] = log1pf(fmaxf(c.x, 0.f));
+    v[1] = log1pf(fmaxf(c.y, 0.f));
+    v[2] = log1pf(fmaxf(c.z, 0.f));
+    v[3] = a.x;
+    v[4] = a.y;
+    v[5] = a.z;
+    v[6] = nn.x * 0.5f + 0.5f;
+    v[7] = nn.y * 0.5f + 0.5f;
+    v[8] = nn.z * 0.5f + 0.5f;
+  }
+  u32x4* o = reinterpret_cast<u32x4*>(x + (size_t)idx * 8);
+  o[0] = u32x4{pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]), pack_bf16(v[4], v[5]), pack_bf16(v[6], v[7])};
+  o[1] = u32x4{pack_bf16(v[8], 0.f), 0u, 0u, 0u};
+}
+
+__global__ __launch_bounds__(kThreads) void k_l1_loss(const float* __restrict__ out, const float* __restrict__ target,
+                                                      float* __restrict__ dy, float* __restrict__ loss_sum,
+                                                      size_t pixels, int cout_pad) {
+  __shared__ float part[kWavesPerBlock];
+  const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const float scale = 1.0f / (3.0f * (float)pixels);
+  float l = 0.f;
+  if (p < pixels) {
+    for (int c = 0; c < 3; ++c) {
+      const float d = out[p * 3 + c] - target[p * 3 + c];
+      l += fabsf(d);
+      dy[p * cout_pad + c] = d > 0.f ? scale : (d < 0.f ? -scale : 0.f);
+    }
+    for (int c = 3; c < cout_pad; ++c) dy[p * cout_pad + c] = 0.f;
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int i = 0; i < kWavesPerBlock; ++i) s += part[i];
+    atomicAdd(loss_sum, s * scale);
+  }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       size_t count, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_write_output(const float* __restrict__ out, float4* __restrict__ dst, int dst_w, int dst_h, int x0,
+                               int y0, int h, int w) {
+  const int idx = blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= h * w) return;
+  const int yy = idx / w, xx = idx - yy * w;
+  const int dx = x0 + xx, dyy = y0 + yy;
+  if (dx < 0 || dyy < 0 || dx >= dst_w || dyy >= dst_h) return;
+  const float* o = out + (size_t)idx * 3;
+  dst[(size_t)dyy * dst_w + dx] = make_float4(fmaxf(expm1f(o[0]), 0.f), fmaxf(expm1f(o[1]), 0.f),
+                                              fmaxf(expm1f(o[2]), 0.f), 1.f);
+}
+
+unsigned blocks_for(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+int bad() { return (int)hipErrorInvalidValue; }
+
+int last() { return (int)hipGetLastError(); }
+
+bool conv_shape_ok(int n, int h, int wd, int cin, int cout_pad, int k) {
+  if (n <= 0 || h <= 0 || wd <= 0 || cin <= 0 || cout_pad <= 0) return false;
+  if (cin % 8 || cout_pad % 16 || (k != 1 && k != 3)) return false;
+  return (long long)n * h * wd < (1ll << 31) && (long long)n * h * wd * (cin > cout_pad ? cin : cout_pad) < (1ll << 40);
+}
+
+int wgrad_chunks(long long P) { return (int)((P + kWgradChunk - 1) / kWgradChunk); }
+
+}  // namespace
+
+extern "C" {
+
+int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y, float* y_f32,
+                        int n, int h, int wd, int cin, int cout, int cout_pad, int k, int relu) {
+  if (!conv_shape_ok(n, h, wd, cin, cout_pad, k) || cout <= 0 || cout > cout_pad || !x || !w || !b || (!y && !y_f32))
+    return bad();
+  return launch_conv<uint16_t, true>((hipStream_t)stream, x, w, b, y, y_f32, n * h * wd, h, wd, cin, cout_pad, cout,
+                                     relu, k);
+}
+
+int hpt_nn_conv_backward_data(void* stream, const float* dy, const uint16_t* wt, float* dx, int n, int h, int wd,
+                              int cin, int cout, int cout_pad, int k) {
+  if (!conv_shape_ok(n, h, wd, cin, cout_pad, k) || cin % 16 || cout <= 0 || cout > cout_pad || !dy || !wt || !dx)
+    return bad();
+  return launch_conv<float, false>((hipStream_t)stream, dy, wt, nullptr, nullptr, dx, n * h * wd, h, wd, cout_pad,
+                                   cin, cin, 0, k);
+}
+
+size_t hpt_nn_wgrad_scratch_floats(int n, int h, int wd, int cin, int cout_pad, int k) {
+  const long long P = (long long)n * h * wd;
+  const size_t chunks = (size_t)wgrad_chunks(P);
+  return chunks * (size_t)cout_pad * (size_t)(k * k) * (size_t)cin + chunks * (size_t)cout_pad;
+}
+
+int hpt_nn_conv_backward_weights(void* stream, const float* dy, const uint16_t* x, float* dw, float* db,
+                                 float* partial, int n, int h, int wd, int cin, int cout, int cout_pad, int k) {
+  if (!conv_shape_ok(n, h, wd, cin, cout_pad, k) || cin % 16 || cout_pad > kThreads || cout <= 0 ||
+      cout > cout_pad || !dy || !x || !dw || !db || !partial)
+    return bad();
+  hipStream_t st = (hipStream_t)stream;
+  const int P = n * h * wd, chunks = wgrad_chunks(P);
+  const int e = k == 1 ? launch_wgrad_ks<1>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks)
+                       : launch_wgrad_ks<3>(st, dy, x, partial, P, h, wd, cin, cout_pad, chunks);
+  if (e) return e;
+  float* partial_b = partial + (size_t)chunks * cout_pad * k * k * cin;
+  hipLaunchKernelGGL(k_bias_partial, dim3(chunks), dim3(kThreads), 0, st, dy, partial_b, P, cout_pad);
+  if (const int e2 = last()) return e2;
+  const int row = k * k * cin;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for((size_t)cout * row + cout)), dim3(kThreads), 0, st, partial,
+                     partial_b, dw, db, chunks, cout, cout_pad, row);
+  return last();
+}
+
+int hpt_nn_prepare_weights(void* stream, const float* master, uint16_t* w, uint16_t* wt, int cin, int cout,
+                           int cout_pad, int k) {
+  if (!master || (!w && !wt) || cin <= 0 || cout <= 0 || cout > cout_pad || (k != 1 && k != 3)) return bad();
+  hipLaunchKernelGGL(k_prepare_weights, dim3(blocks_for((size_t)cout_pad * k * k * cin)), dim3(kThreads), 0,
+                     (hipStream_t)stream, master, w, wt, cin, cout, cout_pad, k * k);
+  return last();
+}
+
+int hpt_nn_relu_backward(void* stream, float* dy, const uint16_t* y, size_t count) {
+  if (!count) return 0;
+  if (!dy || !y) return bad();
+  hipLaunchKernelGGL(k_relu_backward, dim3(blocks_for(count)), dim3(kThreads), 0, (hipStream_t)stream, dy, y, count);
+  return last();
+}
+
+int hpt_nn_pool_forward(void* stream, const uint16_t* x, uint16_t* y, int n, int h, int wd, int c) {
+  if (!x || !y || n <= 0 || h <= 0 || wd <= 0 || c <= 0 || (h | wd) & 1) return bad();
+  hipLaunchKernelGGL(k_pool_forward, dim3(blocks_for((size_t)n * (h / 2) * (wd / 2) * c)), dim3(kThreads), 0,
+                     (hipStream_t)stream, x, y, n, h, wd, c);
+  return last();
+}
+
+int hpt_nn_pool_backward(void* stream, const float* dy, float* dx, int n, int h, int wd, int c) {
+  if (!dy || !dx || n <= 0 || h <= 0 || wd <= 0 || c <= 0 || (h | wd) & 1) return bad();
+  hipLaunchKernelGGL(k_pool_backward, dim3(blocks_for((size_t)n * h * wd * c)), dim3(kThreads), 0,
+                     (hipStream_t)stream, dy, dx, n, h, wd, c);
+  return last();
+}
+
+int hpt_nn_upsample_forward(void* stream, const uint16_t* x, uint16_t* y, int n, int h, int wd, int c) {
+  if (!x || !y || n <= 0 || h <= 0 || wd <= 0 || c <= 0) return bad();
+  hipLaunchKernelGGL(k_upsample_forward, dim3(blocks_for((size_t)n * 4 * h * wd * c)), dim3(kThreads), 0,
+                     (hipStream_t)stream, x, y, n, h, wd, c);
+  return last();
+}
+
+int hpt_nn_upsample_backward(void* stream, const float* dy, float* dx, int n, int h, int wd, int c) {
+  if (!dy || !dx || n <= 0 || h <= 0 || wd <= 0 || c <= 0) return bad();
+  hipLaunchKernelGGL(k_upsample_backward, dim3(blocks_for((size_t)n * h * wd * c)), dim3(kThreads), 0,
+                     (hipStream_t)stream, dy, dx, n, h, wd, c);
+  return last();
+}
+
+int hpt_nn_concat_forward(void* stream, const uint16_t* a, const uint16_t* b, uint16_t* y, size_t pixels, int ca,
+                          int cb) {
+  if (!pixels) return 0;
+  if (!a || !b || !y || ca <= 0 || cb <= 0) return bad();
+  hipLaunchKernelGGL(k_concat_forward, dim3(blocks_for(pixels * (ca + cb))), dim3(kThreads), 0, (hipStream_t)stream,
+                     a, b, y, pixels, ca, cb);
+  return last();
+}
+
+int hpt_nn_concat_backward(void* stream, const float* dy, float* da, float* db, size_t pixels, int ca, int cb) {
+  if (!pixels) return 0;
+  if (!dy || ca <= 0 || cb <= 0) return bad();
+  hipLaunchKernelGGL(k_concat_backward, dim3(blocks_for(pixels * (ca + cb))), dim3(kThreads), 0, (hipStream_t)stream,
+                     dy, da, db, pixels, ca, cb);
+  return last();
+}
+
+int hpt_nn_accumulate(void* stream, float* dx, const float* dy, size_t count) {
+  if (!count) return 0;
+  if (!dx || !dy) return bad();
+  hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(count)), dim3(kThreads), 0, (hipStream_t)stream, dx, dy, count);
+  return last();
+}
+
+int hpt_nn_make_input(void* stream, const float* color, const float* albedo, const float* normal, int src_w, int src_h,
+                      int x0, int y0, uint16_t* x, int h, int wd) {
+  if (!color || !albedo || !normal || !x || src_w <= 0 || src_h <= 0 || h <= 0 || wd <= 0 ||
+      (long long)h * wd >= (1ll << 31))
+    return bad();
+  hipLaunchKernelGGL(k_make_input, dim3(blocks_for((size_t)h * wd)), dim3(kThreads), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(color), reinterpret_cast<const float4*>(albedo),
+                     reinterpret_cast<const float4*>(normal), src_w, src_h, x0, y0, reinterpret_cast<uint32_t*>(x), h,
+                     wd);
+  return last();
+}
+
+int hpt_nn_l1_loss(void* stream, const float* out, const float* target_log, float* dy, float* loss_sum, size_t pixels,
+                   int cout_pad) {
+  if (!pixels) return 0;
+  if (!out || !target_log || !dy || !loss_sum || cout_pad < 3) return bad();
+  hipLaunchKernelGGL(k_l1_loss, dim3(blocks_for(pixels)), dim3(kThreads), 0, (hipStream_t)stream, out, target_log, dy,
+                     loss_sum, pixels, cout_pad);
+  return last();
+}
+
+int hpt_nn_adam(void* stream, float* p, const float* g, float* m, float* v, size_t count, float lr, float beta1,
+                float beta2, float eps, int t) {
+  if (!count) return 0;
+  if (!p || !g || !m || !v || t < 1) return bad();
+  const float c1 = 1.f - (float)std::pow((double)beta1, (double)t);
+  const float c2 = 1.f - (float)std::pow((double)beta2, (double)t);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(count)), dim3(kThreads), 0, (hipStream_t)stream, p, g, m, v, count, lr,
+                     beta1, beta2, eps, c1, c2);
+  return last();
+}
+
+int hpt_nn_write_output(void* stream, const float* out, float* dst, int dst_w, int dst_h, int x0, int y0, int h,
+                        int wd) {
+  if (!out || !dst || dst_w <= 0 || dst_h <= 0 || h <= 0 || wd <= 0 || (long long)h * wd >= (1ll << 31)) return bad();
+  hipLaunchKernelGGL(k_write_output, dim3(blocks_for((size_t)h * wd)), dim3(kThreads), 0, (hipStream_t)stream, out,
+                     reinterpret_cast<float4*>(dst), dst_w, dst_h, x0, y0, h, wd);
+  return last();
+}
+
+}  // extern "C"

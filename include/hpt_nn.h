@@ -1,0 +1,103 @@
+/*
+ * hpt_nn.h — kernels of the learned denoiser (no counterpart in the
+ * reference, whose denoise.py shells out to Intel OIDN): a U-Net over the
+ * renderer's colour, albedo and normal images, trained and run on MI355X.
+ *
+ * Tensors are NHWC in device memory.  Activations are bf16 (uint16 bit
+ * patterns); gradients, weights' masters, biases and optimizer state are
+ * float32.  A convolution weight tensor is [Cout][k*k][Cin] (Cin innermost),
+ * the layout the MFMA fragments read; Cin and the padded Cout are multiples
+ * of 16.  Every launcher queues work on `stream` (a hipStream_t, 0 = the
+ * null stream) and returns 0, or a non-zero HIP error code for a bad launch.
+ */
+#ifndef HPT_NN_H
+#define HPT_NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPT_NN_API __attribute__((visibility("default")))
+
+/* y = act(conv_k(x, w) + b): k in {1, 3} (zero padding k/2), x bf16
+ * [n][h][w][cin], w bf16 [cout_pad][k*k][cin], y bf16 [n][h][w][cout] with
+ * cout <= cout_pad; relu != 0 applies max(., 0).  If y_f32 is non-null the
+ * pre-rounding values are also written there as float32 [n][h][w][cout]
+ * (the network output); y may then be null. */
+HPT_NN_API int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16_t* w, const float* b,
+                                   uint16_t* y, float* y_f32, int n, int h, int wd, int cin, int cout,
+                                   int cout_pad, int k, int relu);
+
+/* dx = conv_k^T(dy, w): dy float32 [n][h][w][cout_pad] (channels >= cout
+ * zero, already multiplied by the activation's derivative), wt bf16
+ * [cin][k*k][cout_pad] = w transposed and spatially flipped
+ * (hpt_nn_prepare_weights), dx float32 [n][h][w][cin]. */
+HPT_NN_API int hpt_nn_conv_backward_data(void* stream, const float* dy, const uint16_t* wt, float* dx, int n,
+                                         int h, int wd, int cin, int cout, int cout_pad, int k);
+
+/* dw[co][t][ci] = sum over pixels of dy[p][co] * x[p + offset(t)][ci] and
+ * db[co] = sum of dy[p][co] for co < cout, float32, written (not
+ * accumulated); dy as for hpt_nn_conv_backward_data.  `partial` is scratch
+ * of hpt_nn_wgrad_scratch_floats(...) floats. */
+HPT_NN_API int hpt_nn_conv_backward_weights(void* stream, const float* dy, const uint16_t* x, float* dw, float* db,
+                                            float* partial, int n, int h, int wd, int cin, int cout, int cout_pad,
+                                            int k);
+HPT_NN_API size_t hpt_nn_wgrad_scratch_floats(int n, int h, int wd, int cin, int cout_pad, int k);
+
+/* Master float32 weights [cout][k*k][cin] -> bf16 w [cout_pad][k*k][cin]
+ * (rows >= cout zero) and bf16 wt [cin][k*k][cout_pad] (flipped taps). */
+HPT_NN_API int hpt_nn_prepare_weights(void* stream, const float* master, uint16_t* w, uint16_t* wt, int cin,
+                                      int cout, int cout_pad, int k);
+
+/* dy *= (y > 0) for a ReLU output y (bf16); count elements. */
+HPT_NN_API int hpt_nn_relu_backward(void* stream, float* dy, const uint16_t* y, size_t count);
+
+/* 2x2 average pooling and its gradient; nearest 2x upsampling and its
+ * gradient (sum of the four children).  (h, wd) is always the size of the
+ * forward input x: pooling maps h x wd (even) to h/2 x wd/2, upsampling h x
+ * wd to 2h x 2wd. */
+HPT_NN_API int hpt_nn_pool_forward(void* stream, const uint16_t* x, uint16_t* y, int n, int h, int wd, int c);
+HPT_NN_API int hpt_nn_pool_backward(void* stream, const float* dy, float* dx, int n, int h, int wd, int c);
+HPT_NN_API int hpt_nn_upsample_forward(void* stream, const uint16_t* x, uint16_t* y, int n, int h, int wd, int c);
+HPT_NN_API int hpt_nn_upsample_backward(void* stream, const float* dy, float* dx, int n, int h, int wd, int c);
+
+/* y[p] = concat(a[p] (ca channels), b[p] (cb channels)); the gradient split. */
+HPT_NN_API int hpt_nn_concat_forward(void* stream, const uint16_t* a, const uint16_t* b, uint16_t* y,
+                                     size_t pixels, int ca, int cb);
+HPT_NN_API int hpt_nn_concat_backward(void* stream, const float* dy, float* da, float* db, size_t pixels, int ca,
+                                      int cb);
+
+/* dx += dy elementwise (a skip path's gradient joining the main path). */
+HPT_NN_API int hpt_nn_accumulate(void* stream, float* dx, const float* dy, size_t count);
+
+/* Network input of one image from renderer images (float4 per pixel,
+ * src_w x src_h): the h x wd window at (x0, y0), zero outside the source,
+ * channels log(1 + colour), albedo, normal * 0.5 + 0.5, zeros up to 16, as
+ * bf16 into x [h][wd][16] (one image of a batch). */
+HPT_NN_API int hpt_nn_make_input(void* stream, const float* color, const float* albedo, const float* normal,
+                                 int src_w, int src_h, int x0, int y0, uint16_t* x, int h, int wd);
+
+/* L1 loss on log(1 + colour): out = network output float32 [p][3],
+ * target_log = log(1 + reference colour) float32 [p][3]; dy [p][cout_pad]
+ * = sign(out - target) / (3 * pixels) (channels >= 3 zero); the mean |.| is
+ * added to *loss_sum (float32, device). */
+HPT_NN_API int hpt_nn_l1_loss(void* stream, const float* out, const float* target_log, float* dy,
+                              float* loss_sum, size_t pixels, int cout_pad);
+
+/* Adam (Kingma & Ba 2015) on float32: step t >= 1. */
+HPT_NN_API int hpt_nn_adam(void* stream, float* p, const float* g, float* m, float* v, size_t count, float lr,
+                           float beta1, float beta2, float eps, int t);
+
+/* Denoised colour: expm1 of the network output [p][3] written into float4
+ * images of width dst_w at (x0, y0) for the h x wd region, alpha 1. */
+HPT_NN_API int hpt_nn_write_output(void* stream, const float* out, float* dst, int dst_w, int dst_h, int x0,
+                                   int y0, int h, int wd);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPT_NN_H */

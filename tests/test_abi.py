@@ -81,3 +81,33 @@ def test_pfm_round_trip(tmp_path):
     p = tmp_path / "x.pfm"
     hpt.write_pfm(str(p), img)
     assert np.array_equal(hpt.read_pfm(str(p)), img)
+
+
+NN_HEADER = os.path.join(ROOT, "include", "hpt_nn.h")
+
+
+def test_denoiser_header_symbols_are_exported_and_typed():
+    import denoiser
+
+    names = sorted(set(re.findall(r"HPT_NN_API\s+[\w\s\*]+?\b(hpt_nn_\w+)\s*\(", open(NN_HEADER).read())))
+    assert len(names) == 17
+    assert set(names) == set(denoiser.NN_SIGNATURES)
+    lib = denoiser.nn_library()
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_denoiser_entry_points_reject_bad_shapes_without_a_device():
+    import denoiser
+
+    lib = denoiser.nn_library()
+    # no launch happens: the argument checks come first (1 = hipErrorInvalidValue)
+    assert lib.hpt_nn_conv_forward(None, None, None, None, None, None, 1, 8, 8, 16, 32, 32, 3, 1) == 1
+    assert lib.hpt_nn_conv_forward(None, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), None,
+                                   1, 8, 8, 12, 32, 32, 3, 1) == 1   # cin not a multiple of 8
+    assert lib.hpt_nn_conv_forward(None, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), None,
+                                   1, 8, 8, 16, 32, 32, 5, 1) == 1   # only 1x1 and 3x3
+    assert lib.hpt_nn_pool_forward(None, C.c_void_p(16), C.c_void_p(16), 1, 7, 8, 16) == 1  # odd height
+    assert lib.hpt_nn_adam(None, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), 4, 0.1, 0.9,
+                           0.999, 1e-8, 0) == 1  # step 0
+    assert lib.hpt_nn_wgrad_scratch_floats(1, 32, 32, 16, 32, 3) == 1 * 32 * 9 * 16 + 32

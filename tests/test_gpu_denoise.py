@@ -1,0 +1,327 @@
+"""Denoiser kernels (include/hpt_nn.h) on the GPU against the numpy oracle
+(oracle/unet_ref.py): each convolution pass within the float32 summation
+bound, elementwise steps bit-exact, the whole network's output and gradients
+within bf16 tolerance, determinism, and a training run that lowers the loss."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import unet_ref as U  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+U24 = 2.0 ** -24
+
+
+@pytest.fixture(scope="module")
+def dn():
+    import denoiser
+
+    denoiser.nn_library()
+    return denoiser
+
+
+def _dev_bf16(a):
+    """float32 numpy -> device int16 tensor of the bf16 bit patterns."""
+    bits = (U.bf16(a).view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+    return torch.from_numpy(bits.reshape(-1).copy()).cuda()
+
+
+def _host_bf16(t, shape):
+    bits = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return bits.view(np.float32).reshape(shape)
+
+
+def _dev(a, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32).reshape(-1)).to("cuda", dtype)
+
+
+class _Ptr(C.c_void_p):
+    """A device pointer that keeps its tensor alive until the call returns:
+    a temporary freed while the arguments are evaluated would be handed to
+    the next argument's allocation by torch's caching allocator."""
+
+
+def _p(t):
+    v = _Ptr(t.data_ptr())
+    v.keep = t
+    return v
+
+
+def _call(dn, name, *args):
+    assert getattr(dn.nn_library(), name)(*args) == 0, name
+
+
+def _bound(absx, absw, k, op):
+    """|error| bound of a float32 sum of K products: (K - 1) u sum |terms|."""
+    terms = op(absx, absw, k)
+    K = absx.shape[-1] * k * k
+    return 2 * (K + 1) * U24 * terms + 1e-30
+
+
+CONV_SHAPES = [  # n, h, w, cin, cout, k
+    (2, 8, 12, 16, 32, 3), (1, 16, 16, 224, 96, 3), (1, 8, 8, 80, 32, 3), (2, 16, 8, 32, 3, 1),
+    (1, 6, 10, 48, 48, 3), (1, 5, 7, 112, 48, 3), (1, 4, 4, 128, 128, 3), (3, 7, 9, 160, 64, 3),
+]
+
+
+def _weights(dn, rng, cin, cout, k):
+    cp = dn.pad16(cout)
+    master = rng.normal(0, (2.0 / (k * k * cin)) ** 0.5, (cout, k * k, cin)).astype(np.float32)
+    w = torch.zeros(cp * k * k * cin, dtype=torch.int16, device="cuda")
+    wt = torch.zeros(cin * k * k * cp, dtype=torch.int16, device="cuda")
+    _call(dn, "hpt_nn_prepare_weights", None, _p(_dev(master)), _p(w), _p(wt), cin, cout, cp, k)
+    torch.cuda.synchronize()
+    return master, w, wt, cp
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+def test_prepare_weights_layouts(dn, shape):
+    _, _, _, cin, cout, k = shape
+    rng = np.random.default_rng(1)
+    master, w, wt, cp = _weights(dn, rng, cin, cout, k)
+    wb = U.bf16(master)
+    wpad = np.zeros((cp, k * k, cin), np.float32)
+    wpad[:cout] = wb
+    assert np.array_equal(_host_bf16(w, wpad.shape), wpad)
+    # wt[ci][t'][co] = w[co][k*k - 1 - t'][ci]
+    ref_t = wpad[:, ::-1, :].transpose(2, 1, 0)
+    assert np.array_equal(_host_bf16(wt, (cin, k * k, cp)), ref_t)
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+def test_conv_forward_matches_oracle(dn, shape):
+    n, h, wd, cin, cout, k = shape
+    print(f"conv forward {shape}", flush=True)
+    rng = np.random.default_rng(2)
+    x = U.bf16(rng.normal(0, 1, (n, h, wd, cin)).astype(np.float32))
+    master, w, _, cp = _weights(dn, rng, cin, cout, k)
+    b = rng.normal(0, 0.1, cout).astype(np.float32)
+    y = torch.zeros(n * h * wd * cout, dtype=torch.int16, device="cuda")
+    yf = torch.zeros(n * h * wd * cout, dtype=torch.float32, device="cuda")
+    relu = int(cout > 3)
+    _call(dn, "hpt_nn_conv_forward", None, _p(_dev_bf16(x)), _p(w), _p(_dev(b)), _p(y), _p(yf), n, h, wd, cin, cout,
+          cp, k, relu)
+    torch.cuda.synchronize()
+    wb = U.bf16(master).astype(np.float64)
+    z = U.conv_forward(x.astype(np.float64), wb, b.astype(np.float64), k)
+    ref = np.maximum(z, 0) if relu else z
+    bound = _bound(np.abs(x).astype(np.float64), np.abs(wb), k,
+                   lambda ax, aw, kk: U.conv_forward(ax, aw, np.abs(b).astype(np.float64), kk))
+    got = yf.cpu().numpy().reshape(ref.shape)
+    assert np.all(np.abs(got - ref) <= bound), float(np.max(np.abs(got - ref) - bound))
+    # the bf16 output is the float32 value rounded to nearest even
+    assert np.array_equal(_host_bf16(y, ref.shape), U.bf16(got))
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+def test_conv_backward_data_matches_oracle(dn, shape):
+    n, h, wd, cin, cout, k = shape
+    print(f"conv backward data {shape}", flush=True)
+    rng = np.random.default_rng(3)
+    master, _, wt, cp = _weights(dn, rng, cin, cout, k)
+    dy = np.zeros((n, h, wd, cp), np.float32)
+    dy[..., :cout] = rng.normal(0, 1, (n, h, wd, cout))
+    dx = torch.zeros(n * h * wd * cin, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_conv_backward_data", None, _p(_dev(dy)), _p(wt), _p(dx), n, h, wd, cin, cout, cp, k)
+    torch.cuda.synchronize()
+    dyb = U.bf16(dy[..., :cout]).astype(np.float64)
+    wb = U.bf16(master).astype(np.float64)
+    ref = U.conv_backward_data(dyb, wb, k)
+    bound = 2 * (cout * k * k + 1) * U24 * U.conv_backward_data(np.abs(dyb), np.abs(wb), k) + 1e-30
+    got = dx.cpu().numpy().reshape(ref.shape)
+    assert np.all(np.abs(got - ref) <= bound), float(np.max(np.abs(got - ref) - bound))
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+def test_conv_backward_weights_matches_oracle(dn, shape):
+    n, h, wd, cin, cout, k = shape
+    print(f"conv backward weights {shape}", flush=True)
+    rng = np.random.default_rng(4)
+    cp = dn.pad16(cout)
+    x = U.bf16(rng.normal(0, 1, (n, h, wd, cin)).astype(np.float32))
+    dy = np.zeros((n, h, wd, cp), np.float32)
+    dy[..., :cout] = rng.normal(0, 1, (n, h, wd, cout))
+    lib = dn.nn_library()
+    scratch = torch.zeros(lib.hpt_nn_wgrad_scratch_floats(n, h, wd, cin, cp, k), dtype=torch.float32, device="cuda")
+    dw = torch.zeros(cout * k * k * cin, dtype=torch.float32, device="cuda")
+    db = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_conv_backward_weights", None, _p(_dev(dy)), _p(_dev_bf16(x)), _p(dw), _p(db), _p(scratch), n, h,
+          wd, cin, cout, cp, k)
+    torch.cuda.synchronize()
+    dyb = U.bf16(dy[..., :cout]).astype(np.float64)
+    ref_w, _ = U.conv_backward_weights(dyb, x.astype(np.float64), k)
+    P = n * h * wd
+    bound_w = 2 * (P + 1) * U24 * U.conv_backward_weights(np.abs(dyb), np.abs(x).astype(np.float64), k)[0] + 1e-30
+    got_w = dw.cpu().numpy().reshape(ref_w.shape)
+    assert np.all(np.abs(got_w - ref_w) <= bound_w), float(np.max(np.abs(got_w - ref_w) - bound_w))
+    dy64 = dy[..., :cout].astype(np.float64).reshape(-1, cout)
+    ref_b = dy64.sum(0)
+    bound_b = 2 * (P + 1) * U24 * np.abs(dy64).sum(0)
+    assert np.all(np.abs(db.cpu().numpy() - ref_b) <= bound_b)
+
+
+def test_elementwise_steps_bit_exact(dn):
+    rng = np.random.default_rng(5)
+    n, h, w, c = 2, 6, 10, 48
+    x = U.bf16(rng.normal(0, 1, (n, h, w, c)).astype(np.float32))
+    xd = _dev_bf16(x)
+    st = None
+    y = torch.zeros(n * h * w * c // 4, dtype=torch.int16, device="cuda")
+    _call(dn, "hpt_nn_pool_forward", st, _p(xd), _p(y), n, h, w, c)
+    assert np.array_equal(_host_bf16(y, (n, h // 2, w // 2, c)), U.bf16(U.pool_forward(x)))
+    g = rng.normal(0, 1, (n, h // 2, w // 2, c)).astype(np.float32)
+    dx = torch.zeros(n * h * w * c, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_pool_backward", st, _p(_dev(g)), _p(dx), n, h, w, c)
+    assert np.array_equal(dx.cpu().numpy().reshape(n, h, w, c), U.pool_backward(g))
+    up = torch.zeros(n * h * w * c * 4, dtype=torch.int16, device="cuda")
+    _call(dn, "hpt_nn_upsample_forward", st, _p(xd), _p(up), n, h, w, c)
+    assert np.array_equal(_host_bf16(up, (n, 2 * h, 2 * w, c)), U.upsample_forward(x))
+    g2 = rng.normal(0, 1, (n, 2 * h, 2 * w, c)).astype(np.float32)
+    _call(dn, "hpt_nn_upsample_backward", st, _p(_dev(g2)), _p(dx), n, h, w, c)
+    assert np.array_equal(dx.cpu().numpy().reshape(n, h, w, c), U.upsample_backward(g2))
+    # concat and its split
+    b2 = U.bf16(rng.normal(0, 1, (n, h, w, 32)).astype(np.float32))
+    cat = torch.zeros(n * h * w * 80, dtype=torch.int16, device="cuda")
+    _call(dn, "hpt_nn_concat_forward", st, _p(xd), _p(_dev_bf16(b2)), _p(cat), n * h * w, c, 32)
+    ref_cat = np.concatenate([x, b2], axis=-1)
+    assert np.array_equal(_host_bf16(cat, ref_cat.shape), ref_cat)
+    gc = rng.normal(0, 1, ref_cat.shape).astype(np.float32)
+    da = torch.zeros(n * h * w * c, dtype=torch.float32, device="cuda")
+    dbb = torch.zeros(n * h * w * 32, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_concat_backward", st, _p(_dev(gc)), _p(da), _p(dbb), n * h * w, c, 32)
+    assert np.array_equal(da.cpu().numpy().reshape(x.shape), gc[..., :c])
+    assert np.array_equal(dbb.cpu().numpy().reshape(b2.shape), gc[..., c:])
+    # relu derivative from the stored bf16 output, accumulate
+    yr = U.bf16(np.maximum(rng.normal(0, 1, 1000), 0).astype(np.float32))
+    yr[::7] = -0.0
+    gr = rng.normal(0, 1, 1000).astype(np.float32)
+    gd = _dev(gr)
+    _call(dn, "hpt_nn_relu_backward", st, _p(gd), _p(_dev_bf16(yr)), 1000)
+    assert np.array_equal(gd.cpu().numpy(), gr * (yr > 0))
+    acc = _dev(gr)
+    _call(dn, "hpt_nn_accumulate", st, _p(acc), _p(_dev(yr)), 1000)
+    assert np.array_equal(acc.cpu().numpy(), gr + yr)
+
+
+def test_make_input_loss_adam_and_output(dn):
+    rng = np.random.default_rng(6)
+    H, W = 9, 13
+    color = np.abs(rng.normal(0, 2, (H, W, 4))).astype(np.float32)
+    color[0, 0, 0] = -0.5  # negative radiance is clamped to 0
+    albedo = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    normal = rng.uniform(-1, 1, (H, W, 4)).astype(np.float32)
+    h, w, x0, y0 = 8, 16, 2, 3  # the window sticks out of the image on the right and bottom
+    x = torch.zeros(h * w * 16, dtype=torch.int16, device="cuda")
+    _call(dn, "hpt_nn_make_input", None, _p(_dev(color)), _p(_dev(albedo)), _p(_dev(normal)), W, H, x0, y0, _p(x), h,
+          w)
+    got = _host_bf16(x, (h, w, 16))
+    full = U.make_input(color, albedo, normal)
+    ref = np.zeros((h, w, 16), np.float32)
+    ys, xs = slice(y0, min(H, y0 + h)), slice(x0, min(W, x0 + w))
+    ref[:ys.stop - y0, :xs.stop - x0] = full[ys, xs]
+    ulp = np.abs(got.view(np.int32) - U.bf16(ref).view(np.int32))
+    assert ulp.max() <= 1 << 16  # log1p may differ by an ulp before the bf16 rounding
+
+    P = 300
+    out = rng.normal(0, 1, (P, 3)).astype(np.float32)
+    tgt = rng.normal(0, 1, (P, 3)).astype(np.float32)
+    dy = torch.full((P * 16,), 7.0, dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_l1_loss", None, _p(_dev(out)), _p(_dev(tgt)), _p(dy), _p(loss), P, 16)
+    ref_loss, ref_g = U.l1_loss(out.astype(np.float64), tgt.astype(np.float64))
+    gd = dy.cpu().numpy().reshape(P, 16)
+    assert np.array_equal(gd[:, 3:], np.zeros((P, 13), np.float32))
+    assert np.allclose(gd[:, :3], ref_g, rtol=1e-6, atol=0)
+    assert float(loss.item()) == pytest.approx(ref_loss, rel=1e-5)
+
+    p = rng.normal(0, 1, 500).astype(np.float32)
+    g = rng.normal(0, 1, 500).astype(np.float32)
+    pd, md, vd = _dev(p), torch.zeros(500, device="cuda"), torch.zeros(500, device="cuda")
+    rp, rm, rv = p.astype(np.float64), np.zeros(500), np.zeros(500)
+    for t in (1, 2, 3):
+        gt = g * t
+        _call(dn, "hpt_nn_adam", None, _p(pd), _p(_dev(gt)), _p(md), _p(vd), 500, 1e-2, 0.9, 0.999, 1e-8, t)
+        rp, rm, rv = U.adam(rp, gt.astype(np.float64), rm, rv, 1e-2, 0.9, 0.999, 1e-8, t)
+    assert np.allclose(pd.cpu().numpy(), rp, rtol=1e-5, atol=1e-6)
+
+    o = rng.normal(0, 1, (h * w, 3)).astype(np.float32)
+    dst = torch.full((H * W * 4,), -3.0, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_write_output", None, _p(_dev(o)), _p(dst), W, H, x0, y0, h, w)
+    img = dst.cpu().numpy().reshape(H, W, 4)
+    o3 = o.reshape(h, w, 3)
+    win = img[ys, xs]
+    assert np.allclose(win[..., :3], np.maximum(np.expm1(o3[:ys.stop - y0, :xs.stop - x0]), 0), rtol=1e-5, atol=1e-6)
+    assert np.all(win[..., 3] == 1.0) and np.all(img[:y0] == -3.0)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_unet_forward_backward_match_oracle(dn):
+    n, h, w = 2, 32, 48
+    print("unet full network vs oracle", flush=True)
+    net = dn.UNet(seed=11)
+    rng = np.random.default_rng(12)
+    params = {k: (wt, b + rng.normal(0, 0.02, b.shape).astype(np.float32)) for k, (wt, b) in net.get_params().items()}
+    net.set_params(params)
+    x = U.bf16(np.concatenate([rng.uniform(0, 1, (n, h, w, 9)), np.zeros((n, h, w, 7))], -1).astype(np.float32))
+    tgt = rng.normal(0, 0.5, (n, h, w, 3)).astype(np.float32)
+    out = net.forward_input(_dev_bf16(x), n, h, w, train=True)
+    b = net._buffers(n, h, w, True)
+    b["target"].copy_(_dev(tgt))
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    _call(dn, "hpt_nn_l1_loss", None, _p(out), _p(b["target"]), _p(b["dout"]), _p(loss), n * h * w, 16)
+    net.backward(n, h, w)
+    got_out = out.cpu().numpy().reshape(n, h, w, 3)
+    grads = net.get_grads()
+
+    ref = U.UNet(params, bf16_rounding=True)
+    r_out, cache = ref.forward(x)
+    assert _rel(got_out, r_out) < 1e-2
+    dout = b["dout"].cpu().numpy().reshape(n, h, w, 16)[..., :3]
+    r_grads = ref.backward(dout, cache)
+    for name, (dw, db) in r_grads.items():
+        assert _rel(grads[name][0], dw) < 3e-2, (name, _rel(grads[name][0], dw))
+        assert _rel(grads[name][1], db) < 3e-2, (name, _rel(grads[name][1], db))
+
+    # the backward pass is deterministic: a second run gives the same bits
+    first = net.grads.clone()
+    net.forward_input(_dev_bf16(x), n, h, w, train=True)
+    net.backward(n, h, w)
+    assert torch.equal(first, net.grads)
+
+
+def test_training_lowers_the_loss(dn):
+    """A few hundred Adam steps on a synthetic denoising task: smooth colour
+    fields plus noise whose variance the albedo channel does not explain."""
+    print("unet training on a synthetic task", flush=True)
+    n, h, w = 4, 64, 64
+    rng = np.random.default_rng(13)
+    net = dn.UNet(seed=14)
+    yy, xx = np.mgrid[0:h, 0:w] / 64.0
+
+    def batch():
+        clean = np.stack([np.stack([0.5 + 0.4 * np.sin(6 * xx * rng.uniform(0.5, 1.5) + rng.uniform(0, 6)),
+                                    0.5 + 0.4 * np.cos(5 * yy * rng.uniform(0.5, 1.5) + rng.uniform(0, 6)),
+                                    0.5 + 0.3 * np.sin(4 * (xx + yy) + rng.uniform(0, 6))], -1) for _ in range(n)])
+        noisy = np.maximum(clean + rng.normal(0, 0.3, clean.shape), 0)
+        x = np.zeros((n, h, w, 16), np.float32)
+        x[..., 0:3] = np.log1p(noisy)
+        x[..., 3:6] = 0.5
+        x[..., 6:9] = 0.5
+        return _dev_bf16(x), _dev(np.log1p(clean))
+
+    losses = []
+    for it in range(200):
+        x, t = batch()
+        losses.append(net.train_step(x, t, n, h, w, lr=2e-3))
+    vals = [float(v.item()) for v in losses]
+    first, last = np.mean(vals[:10]), np.mean(vals[-10:])
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert np.all(np.isfinite(vals))
+    assert last < 0.5 * first
