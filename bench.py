@@ -127,21 +127,10 @@ def relmse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
 
 
-def time_to_relmse(make_renderer, ref_spp, threshold, chunk, max_spp):
-    """Renders a ref_spp reference from frames [65536, 65536 + ref_spp) (so it
-    shares no samples with the estimate), then the plain progressive estimate
-    from frame 0 in chunks of `chunk` frames; the render time until the
-    estimate's relMSE first meets `threshold` (relMSE evaluation excluded)."""
-    r = make_renderer()
-    base = 1 << 16
-    r.frame_index = base
-    t0 = time.perf_counter()
-    r.render(ref_spp)
-    r.wait_idle()
-    t_ref = time.perf_counter() - t0
-    ref = r.read_image(hpt.IMAGE_ACCUM)[..., :3].astype(np.float64) * ((base + ref_spp) / ref_spp)
-    r.close()
-    log(f"reference: {ref_spp} spp in {t_ref:.2f} s")
+def time_to_relmse(make_renderer, ref, t_ref, ref_spp, threshold, chunk, max_spp):
+    """The plain progressive estimate from frame 0 in chunks of `chunk`
+    frames against the reference `ref`; the render time until the estimate's
+    relMSE first meets `threshold` (relMSE evaluation excluded)."""
     r = make_renderer()
     curve, t_render, spp, hit = [], 0.0, 0, None
     while spp < max_spp:
@@ -161,6 +150,165 @@ def time_to_relmse(make_renderer, ref_spp, threshold, chunk, max_spp):
     return {"threshold": threshold, "ref_spp": ref_spp, "ref_seconds": round(t_ref, 3),
             "time_to_threshold_s": round(hit[1], 4) if hit else None, "spp_at_threshold": hit[0] if hit else None,
             "curve": curve[:: max(1, len(curve) // 16)] + ([curve[-1]] if len(curve) % 16 else [])}
+
+
+def render_reference(make_renderer, ref_spp):
+    """A ref_spp reference from frames [65536, 65536 + ref_spp), so it shares
+    no samples with an estimate that starts at frame 0."""
+    r = make_renderer()
+    base = 1 << 16
+    r.frame_index = base
+    t0 = time.perf_counter()
+    r.render(ref_spp)
+    r.wait_idle()
+    t_ref = time.perf_counter() - t0
+    ref = r.read_image(hpt.IMAGE_ACCUM)[..., :3].astype(np.float64) * ((base + ref_spp) / ref_spp)
+    r.close()
+    log(f"reference: {ref_spp} spp in {t_ref:.2f} s")
+    return ref, t_ref
+
+
+def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
+    """The denoised estimate at 2, 4, 8, ... spp (progressive accumulation
+    from frame 0): time = render time up to that sample count + one denoiser
+    pass (on the renderer's resident images, same GPU); the first sample
+    count whose denoised image meets `threshold` (relMSE evaluation and the
+    passes of earlier sample counts excluded)."""
+    import denoiser
+
+    r = make_renderer()
+    curve, t_render, spp, hit = [], 0.0, 0, None
+    out = denoiser.DeviceArray(r.height * r.width * 4, np.float32, (r.height, r.width, 4))
+    net.denoise_renderer(r, out)  # allocate the network's buffers outside the timed region
+    denoiser.synchronize()
+    r.reset_accumulation()
+    while spp < max_spp:
+        step = 2 if spp == 0 else spp
+        r.wait_idle()
+        t0 = time.perf_counter()
+        r.render(step)
+        r.wait_idle()
+        t_render += time.perf_counter() - t0
+        spp += step
+        t0 = time.perf_counter()
+        net.denoise_renderer(r, out)
+        denoiser.synchronize()
+        t_den = time.perf_counter() - t0
+        e = relmse(out.numpy(), ref)
+        e_plain = relmse(r.read_image(hpt.IMAGE_ACCUM), ref)
+        curve.append([spp, round(t_render, 4), round(t_den * 1e3, 3), e_plain, e])
+        if hit is None and e <= threshold:
+            hit = (spp, t_render + t_den, t_den)
+            break
+    r.close()
+    log("denoised relMSE: " + ", ".join(f"{c[0]} spp {c[4]:.2e} (plain {c[3]:.2e})" for c in curve))
+    return {"threshold": threshold, "time_to_threshold_s": round(hit[1], 4) if hit else None,
+            "spp_at_threshold": hit[0] if hit else None, "denoise_ms": round(hit[2] * 1e3, 3) if hit else None,
+            "curve_spp_render_s_denoise_ms_plain_denoised": curve}
+
+
+def train_step_time(denoiser, images, W, H, steps):
+    """Seconds per training step (batch 8 of 128x128 crops of `images`,
+    forward + L1 loss + backward + Adam) and its convolution FLOP."""
+    tnet = denoiser.UNet(seed=1)
+    bt = tnet._buffers(8, 128, 128, True)
+    acc = denoiser.DeviceArray(1, np.float32)
+    for i in range(8):
+        tnet.make_input(images, W, H, 128 * i, 256, 128, 128, bt["x"].addr(i * 128 * 128 * 16), 16)
+        tnet.make_target(images[0], W, H, 128 * i, 256, 128, 128, bt["target"].addr(i * 128 * 128 * 3))
+    for _ in range(5):
+        tnet.train_step(bt["x"], bt["target"], 8, 128, 128, 1e-4, loss_sum=acc)
+    denoiser.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tnet.train_step(bt["x"], bt["target"], 8, 128, 128, 1e-4, loss_sum=acc)
+    denoiser.synchronize()
+    return (time.perf_counter() - t0) / steps, 3 * tnet.flops_forward(8, 128, 128)
+
+
+def denoise_bench(args):
+    """Inference throughput of the denoiser on a 1080p render (16 spp of the
+    large-mesh scene with lights and a medium, resident in HBM), per-layer
+    MFMA roofline from HIP events, training-step throughput, and the numpy
+    oracle's forward pass on a host sample."""
+    import denoiser
+
+    W, H = args.width or 1920, args.height or 1080
+    path = args.weights or denoiser.weights_path()
+    net = denoiser.UNet.load(path) if os.path.exists(path) else denoiser.UNet(seed=0)
+    log(f"weights: {path if os.path.exists(path) else 'untrained (seed 0)'}; {net.param_count} parameters")
+    scene = hpt_scenes.displaced_icosphere_lights_medium(level=8)
+    r = hpt.HalaRenderer(W, H, max_depth=12, rr_depth=3, features=hpt_scenes.MEDIUM_SCENE_FEATURES)
+    r.set_scene(scene)
+    r.commit()
+    r.render(16)
+    r.wait_idle()
+    images = tuple(r.image_pointer(k)[0] for k in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL))
+    out = denoiser.DeviceArray(H * W * 4, np.float32, (H, W, 4))
+    for _ in range(max(1, args.warmup)):
+        net.denoise(images, W, H, 16, out)
+    denoiser.synchronize()
+    steps = args.steps or 50
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.denoise(images, W, H, 16, out)
+    denoiser.synchronize()
+    wall = time.perf_counter() - t0
+    hp, wp = -(-H // 16) * 16, -(-W // 16) * 16
+    flops = net.flops_forward(1, hp, wp)
+    prof = net.profile_forward(1, hp, wp)
+    conv_ms = sum(ms for ms, _ in prof.values())
+    name, (ms, fl) = max(prof.items(), key=lambda kv: kv[1][0])
+    log(f"{steps} passes in {wall:.3f} s: {wall / steps * 1e3:.3f} ms/pass, {flops / (wall / steps) / 1e12:.1f} "
+        f"TFLOP/s overall; convolutions {conv_ms:.3f} ms; dominant {name}: {ms:.3f} ms, {fl / ms / 1e9:.1f} TFLOP/s")
+    log("layers: " + ", ".join(f"{k} {v[0]:.3f} ms {v[1] / v[0] / 1e9:.0f} TF/s" for k, v in prof.items()))
+
+    # Training step throughput: batch 8 of 128x128 crops of the same render.
+    t_train, train_flops = None, 0
+    if args.train_steps > 0:
+        t_train, train_flops = train_step_time(denoiser, images, W, H, args.train_steps)
+        log(f"training step (8 x 128x128): {t_train * 1e3:.2f} ms, {train_flops / t_train / 1e12:.1f} TFLOP/s")
+    r.close()
+
+    base = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import unet_ref
+
+        ref_net = unet_ref.UNet(net.get_params(), net.levels, bf16_rounding=False)
+        x = np.random.default_rng(0).uniform(0, 1, (1, 128, 128, 16)).astype(np.float32)
+        ref_net.forward(x)
+        t0, n = time.perf_counter(), 0
+        while time.perf_counter() - t0 < 10.0:
+            ref_net.forward(x)
+            n += 1
+        dt = time.perf_counter() - t0
+        try:
+            from threadpoolctl import threadpool_info
+
+            cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+        except ImportError:
+            cores = os.cpu_count()
+        base = {"value": round(n * 128 * 128 / dt / 1e6, 4), "unit": "Mpixels/s", "cores": cores,
+                "kind": "port", "sample": f"{n} forward passes of the numpy float32 oracle on 128x128 ({dt:.1f} s)"}
+    out_json = {
+        "metric": "denoised Mpixels/s", "value": round(W * H * steps / wall / 1e6, 3), "unit": "Mpixels/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "denoise", "width": W, "height": H, "network_size": [hp, wp],
+                   "levels": list(net.levels), "parameters": net.param_count, "input_spp": 16,
+                   "gflop_per_pass": round(flops / 1e9, 2),
+                   "tflops_overall": round(flops / (wall / steps) / 1e12, 2),
+                   "train_step_ms_8x128x128": round(t_train * 1e3, 3) if t_train else None,
+                   "train_tflops": round(train_flops / t_train / 1e12, 2) if t_train else None,
+                   "layer_ms": {k: round(v[0], 4) for k, v in prof.items()}},
+        "roofline": {"bound": "mfma", "achieved": round(fl / ms / 1e9, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(fl / ms / 1e9 / 2500.0, 4), "traffic": None, "kernel": f"k_conv ({name})",
+                     "avg_launch_ms": round(ms, 4)},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out_json), flush=True)
 
 
 def serving_scenes():
@@ -217,7 +365,12 @@ def serving_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving"], default="disney_grid_env_1080p")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "denoise"], default="disney_grid_env_1080p")
+    ap.add_argument("--denoise", action="store_true",
+                    help="with --relmse: also the time to the threshold of the denoised estimate")
+    ap.add_argument("--weights", default="", help="denoiser weights (default: the committed network)")
+    ap.add_argument("--train-steps", type=int, default=50,
+                    help="denoise: timed training steps (0 skips the training measurement)")
     ap.add_argument("--jobs", type=int, default=64, help="serving: number of jobs")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=192.0, help="serving: device memory for admitted jobs")
@@ -246,6 +399,9 @@ def main():
 
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload == "denoise":
+        denoise_bench(args)
         return
     wl = WORKLOADS[args.workload]
     W = args.width or wl["width"]
@@ -304,7 +460,19 @@ def main():
 
     conv = None
     if args.relmse:
-        conv = time_to_relmse(make_renderer, args.ref_spp, args.threshold, max(1, fif), args.ref_spp // 2)
+        ref, t_ref = render_reference(make_renderer, args.ref_spp)
+        conv = time_to_relmse(make_renderer, ref, t_ref, args.ref_spp, args.threshold, max(1, fif),
+                              args.ref_spp // 2)
+        if args.denoise:
+            import denoiser
+
+            path = args.weights or denoiser.weights_path()
+            den = time_to_relmse_denoised(make_renderer, ref, denoiser.UNet.load(path), args.threshold,
+                                          args.ref_spp // 2)
+            den["weights"] = os.path.relpath(path, ROOT)
+            if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
+                den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
+            conv["denoised"] = den
     base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES)
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
               "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,

@@ -904,6 +904,23 @@ hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t by
   });
 }
 
+hpt_status hpt_get_image_pointer(hpt_renderer* r, hpt_image which, void** ptr, size_t* bytes) {
+  return guarded([&] {
+    need(r && ptr && bytes, "null argument");
+    require_committed(r);
+    const DevBuf* b = nullptr;
+    switch (which) {
+      case HPT_IMAGE_FINAL_RGBA8: b = &r->d_final; break;
+      case HPT_IMAGE_ACCUM_RGBA32F: b = &r->d_accum; break;
+      case HPT_IMAGE_ALBEDO_RGBA32F: b = &r->d_albedo; break;
+      case HPT_IMAGE_NORMAL_RGBA32F: b = &r->d_normal; break;
+      default: need(false, "unknown image");
+    }
+    *ptr = b->p;
+    *bytes = b->bytes;
+  });
+}
+
 static void trace_api(hpt_renderer* r, const hpt_ray* rays, uint32_t n, int any, std::vector<float4>& out,
                       uint32_t* nvis = nullptr, uint32_t* ntri = nullptr) {
   set_device(r);

@@ -271,15 +271,26 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(const float* __restrict__ dy
       }
 }
 
-// partial_b[chunk][co] = sum over the chunk's pixels of dy[p][co].
+// partial_b[chunk][co] = sum over the chunk's pixels of dy[p][co]: the block's
+// threads form kThreads / cout_pad row groups (group g sums pixels g, g + groups,
+// ...; consecutive threads read consecutive channels of a pixel), then thread co
+// adds the groups in order (deterministic).
 __global__ __launch_bounds__(kThreads) void k_bias_partial(const float* __restrict__ dy, float* __restrict__ partial_b,
                                                            int P, int cout_pad) {
-  const int co = threadIdx.x;
-  if (co >= cout_pad) return;
+  __shared__ float part[kThreads];
+  const int groups = kThreads / cout_pad;
+  const int g = threadIdx.x / cout_pad, co = threadIdx.x - g * cout_pad;
   const int pbeg = blockIdx.x * kWgradChunk, pend = min(P, pbeg + kWgradChunk);
   float s = 0.f;
-  for (int p = pbeg; p < pend; ++p) s += dy[(size_t)p * cout_pad + co];
-  partial_b[(size_t)blockIdx.x * cout_pad + co] = s;
+  if (g < groups)
+    for (int p = pbeg + g; p < pend; p += groups) s += dy[(size_t)p * cout_pad + co];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < cout_pad) {
+    float t = 0.f;
+    for (int k = 0; k < groups; ++k) t += part[k * cout_pad + threadIdx.x];
+    partial_b[(size_t)blockIdx.x * cout_pad + threadIdx.x] = t;
+  }
 }
 
 // dw[co][t][ci] (co < cout) = sum of the chunks in order; db likewise.
@@ -425,13 +436,21 @@ __global__ void k_accumulate(float* __restrict__ dx, const float* __restrict__ d
   if (i < count) dx[i] += dy[i];
 }
 
+// Source pixel of window pixel (xx, yy), mirrored as `mirror` asks.
+__device__ __forceinline__ void window_source(int xx, int yy, int w, int h, int x0, int y0, int mirror, int& sx,
+                                              int& sy) {
+  sx = x0 + ((mirror & 1) ? w - 1 - xx : xx);
+  sy = y0 + ((mirror & 2) ? h - 1 - yy : yy);
+}
+
 __global__ void k_make_input(const float4* __restrict__ color, const float4* __restrict__ albedo,
                              const float4* __restrict__ normal, int src_w, int src_h, int x0, int y0,
-                             uint32_t* __restrict__ x, int h, int w) {
+                             uint32_t* __restrict__ x, int h, int w, int mirror, float aux) {
   const int idx = blockIdx.x * kThreads + threadIdx.x;
   if (idx >= h * w) return;
   const int yy = idx / w, xx = idx - yy * w;
-  const int sx = x0 + xx, sy = y0 + yy;
+  int sx, sy;
+  window_source(xx, yy, w, h, x0, y0, mirror, sx, sy);
   float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (sx >= 0 && sy >= 0 && sx < src_w && sy < src_h) {
     const size_t s = (size_t)sy * src_w + sx;
@@ -448,7 +467,29 @@ __global__ void k_make_input(const float4* __restrict__ color, const float4* __r
   }
   u32x4* o = reinterpret_cast<u32x4*>(x + (size_t)idx * 8);
   o[0] = u32x4{pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]), pack_bf16(v[4], v[5]), pack_bf16(v[6], v[7])};
-  o[1] = u32x4{pack_bf16(v[8], 0.f), 0u, 0u, 0u};
+  const float a = (sx >= 0 && sy >= 0 && sx < src_w && sy < src_h) ? aux : 0.f;
+  o[1] = u32x4{pack_bf16(v[8], a), 0u, 0u, 0u};
+}
+
+__global__ void k_make_target(const float4* __restrict__ color, int src_w, int src_h, int x0, int y0,
+                              float* __restrict__ out, int h, int w, int mirror) {
+  const int idx = blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= h * w) return;
+  const int yy = idx / w, xx = idx - yy * w;
+  int sx, sy;
+  window_source(xx, yy, w, h, x0, y0, mirror, sx, sy);
+  float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (sx >= 0 && sy >= 0 && sx < src_w && sy < src_h) c = color[(size_t)sy * src_w + sx];
+  out[(size_t)idx * 3 + 0] = log1pf(fmaxf(c.x, 0.f));
+  out[(size_t)idx * 3 + 1] = log1pf(fmaxf(c.y, 0.f));
+  out[(size_t)idx * 3 + 2] = log1pf(fmaxf(c.z, 0.f));
+}
+
+__global__ void k_add_residual(float* __restrict__ out, const uint16_t* __restrict__ x, size_t pixels, int xc) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= pixels * 3) return;
+  const size_t p = i / 3;
+  out[i] += bf16_value(x[p * xc + (i - p * 3)]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_l1_loss(const float* __restrict__ out, const float* __restrict__ target,
@@ -498,6 +539,11 @@ __global__ void k_write_output(const float* __restrict__ out, float4* __restrict
   const float* o = out + (size_t)idx * 3;
   dst[(size_t)dyy * dst_w + dx] = make_float4(fmaxf(expm1f(o[0]), 0.f), fmaxf(expm1f(o[1]), 0.f),
                                               fmaxf(expm1f(o[2]), 0.f), 1.f);
+}
+
+__global__ void k_fill(float* __restrict__ p, float v, size_t count) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < count) p[i] = v;
 }
 
 unsigned blocks_for(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -627,14 +673,31 @@ int hpt_nn_accumulate(void* stream, float* dx, const float* dy, size_t count) {
 }
 
 int hpt_nn_make_input(void* stream, const float* color, const float* albedo, const float* normal, int src_w, int src_h,
-                      int x0, int y0, uint16_t* x, int h, int wd) {
+                      int x0, int y0, uint16_t* x, int h, int wd, int mirror, float aux) {
   if (!color || !albedo || !normal || !x || src_w <= 0 || src_h <= 0 || h <= 0 || wd <= 0 ||
       (long long)h * wd >= (1ll << 31))
     return bad();
   hipLaunchKernelGGL(k_make_input, dim3(blocks_for((size_t)h * wd)), dim3(kThreads), 0, (hipStream_t)stream,
                      reinterpret_cast<const float4*>(color), reinterpret_cast<const float4*>(albedo),
                      reinterpret_cast<const float4*>(normal), src_w, src_h, x0, y0, reinterpret_cast<uint32_t*>(x), h,
-                     wd);
+                     wd, mirror, aux);
+  return last();
+}
+
+int hpt_nn_make_target(void* stream, const float* color, int src_w, int src_h, int x0, int y0, float* out, int h,
+                       int wd, int mirror) {
+  if (!color || !out || src_w <= 0 || src_h <= 0 || h <= 0 || wd <= 0 || (long long)h * wd >= (1ll << 31))
+    return bad();
+  hipLaunchKernelGGL(k_make_target, dim3(blocks_for((size_t)h * wd)), dim3(kThreads), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(color), src_w, src_h, x0, y0, out, h, wd, mirror);
+  return last();
+}
+
+int hpt_nn_add_residual(void* stream, float* out, const uint16_t* x, size_t pixels, int x_channels) {
+  if (!pixels) return 0;
+  if (!out || !x || x_channels < 3) return bad();
+  hipLaunchKernelGGL(k_add_residual, dim3(blocks_for(pixels * 3)), dim3(kThreads), 0, (hipStream_t)stream, out, x,
+                     pixels, x_channels);
   return last();
 }
 
@@ -664,6 +727,56 @@ int hpt_nn_write_output(void* stream, const float* out, float* dst, int dst_w, i
   hipLaunchKernelGGL(k_write_output, dim3(blocks_for((size_t)h * wd)), dim3(kThreads), 0, (hipStream_t)stream, out,
                      reinterpret_cast<float4*>(dst), dst_w, dst_h, x0, y0, h, wd);
   return last();
+}
+
+int hpt_nn_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return bad();
+  *ptr = nullptr;
+  if (!bytes) return 0;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return (int)e;
+  e = hipMemset(*ptr, 0, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+  }
+  return (int)e;
+}
+
+int hpt_nn_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }
+
+int hpt_nn_copy(void* stream, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return 0;
+  if (!dst || !src) return bad();
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream);
+}
+
+int hpt_nn_fill(void* stream, float* p, float value, size_t count) {
+  if (!count) return 0;
+  if (!p) return bad();
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(count)), dim3(kThreads), 0, (hipStream_t)stream, p, value, count);
+  return last();
+}
+
+int hpt_nn_stream_synchronize(void* stream) { return (int)hipStreamSynchronize((hipStream_t)stream); }
+
+int hpt_nn_event_create(void** event) {
+  if (!event) return bad();
+  return (int)hipEventCreate(reinterpret_cast<hipEvent_t*>(event));
+}
+
+int hpt_nn_event_destroy(void* event) { return event ? (int)hipEventDestroy((hipEvent_t)event) : 0; }
+
+int hpt_nn_event_record(void* event, void* stream) {
+  if (!event) return bad();
+  return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
+}
+
+int hpt_nn_event_elapsed(void* start, void* stop, float* ms) {
+  if (!start || !stop || !ms) return bad();
+  const hipError_t e = hipEventSynchronize((hipEvent_t)stop);
+  if (e != hipSuccess) return (int)e;
+  return (int)hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
 }
 
 }  // extern "C"

@@ -1,0 +1,174 @@
+"""Training of the learned denoiser on image pairs the path tracer renders
+itself: for each seeded random scene (hpt_scenes.random_scene), the
+progressive estimate at several low sample counts (colour, albedo, normal)
+and a high-sample reference from a disjoint frame range.  Batches are random
+crops, mirrored at random, cut from the device images by the library's
+kernels; the network, loss and optimizer are those of denoiser.UNet.
+
+    python hala-pathtracer_amd/denoise_train.py --scenes 512 --steps 16000 \
+        --ref-spp 4096 --out hala-pathtracer_amd/weights/unet_denoiser.npz
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+import denoiser  # noqa: E402
+import hpt  # noqa: E402
+import hpt_scenes  # noqa: E402
+
+SPP_LEVELS = (4, 8, 16, 32, 64, 128, 256)
+REF_FRAME_BASE = 1 << 16  # the reference's frames share no samples with the estimate's
+
+
+@dataclass
+class Sample:
+    """One rendered scene: noisy (colour, albedo, normal) per sample count
+    and the reference colour, float4 device images."""
+    seed: int
+    width: int
+    height: int
+    noisy: dict
+    reference: denoiser.DeviceArray
+
+
+def relmse(img: np.ndarray, ref: np.ndarray) -> float:
+    """Mean over pixels and RGB of (I - R)^2 / (R^2 + 0.01)."""
+    img = img[..., :3].astype(np.float64)
+    ref = ref[..., :3].astype(np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def render_reference(r: "hpt.HalaRenderer", spp: int) -> np.ndarray:
+    """`spp` frames from REF_FRAME_BASE on, rescaled from the renderer's
+    running mean (whose weights count from frame 0) to their own mean."""
+    r.reset_accumulation()
+    r.frame_index = REF_FRAME_BASE
+    r.render(spp)
+    r.wait_idle()
+    return r.read_image(hpt.IMAGE_ACCUM) * np.float32((REF_FRAME_BASE + spp) / spp)
+
+
+def render_sample(seed: int, width: int, height: int, spp_levels=SPP_LEVELS, ref_spp: int = 2048,
+                  max_depth: int = 8, device: int = 0) -> Sample:
+    scene, features = hpt_scenes.random_scene(seed, width, height)
+    r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=3, features=features, device=device)
+    try:
+        r.set_scene(scene)
+        r.commit()
+        noisy, done = {}, 0
+        for spp in sorted(spp_levels):
+            r.render(spp - done)
+            done = spp
+            noisy[spp] = denoiser.renderer_images(r)
+        ref = denoiser.DeviceArray.from_numpy(render_reference(r, ref_spp))
+    finally:
+        r.close()
+    return Sample(seed, width, height, noisy, ref)
+
+
+def render_dataset(seeds, width: int, height: int, spp_levels=SPP_LEVELS, ref_spp: int = 2048, log=print,
+                   device: int = 0) -> list:
+    out, t0 = [], time.perf_counter()
+    for i, s in enumerate(seeds):
+        out.append(render_sample(int(s), width, height, spp_levels, ref_spp, device=device))
+        if log and (i + 1) % 16 == 0:
+            log(f"rendered {i + 1}/{len(seeds)} scenes in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def cosine_lr(step: int, steps: int, lr: float, warmup: int = 100) -> float:
+    if step < warmup:
+        return lr * (step + 1) / warmup
+    t = (step - warmup) / max(1, steps - warmup)
+    return lr * (0.02 + 0.98 * 0.5 * (1 + math.cos(math.pi * t)))
+
+
+def train(net: denoiser.UNet, samples: list, steps: int, batch: int = 8, crop: int = 128, lr: float = 1e-3,
+          seed: int = 0, log=print, log_every: int = 100, callback=None) -> list:
+    """Adam on random (scene, sample count, crop, mirror) batches; returns
+    the mean loss of each logging interval (after which `callback(step)` is
+    called when given)."""
+    rng = np.random.default_rng(seed)
+    b = net._buffers(batch, crop, crop, True)
+    x, tgt = b["x"], b["target"]
+    px_in, px_t = crop * crop * denoiser.IN_CHANNELS, crop * crop * 3
+    losses, acc = [], denoiser.DeviceArray(1, np.float32)
+    t0 = time.perf_counter()
+    for it in range(steps):
+        for i in range(batch):
+            s = samples[int(rng.integers(len(samples)))]
+            spp = list(s.noisy)[int(rng.integers(len(s.noisy)))]
+            x0 = int(rng.integers(0, s.width - crop + 1))
+            y0 = int(rng.integers(0, s.height - crop + 1))
+            m = int(rng.integers(4))
+            net.make_input(s.noisy[spp], s.width, s.height, x0, y0, crop, crop, x.addr(i * px_in), spp, m)
+            net.make_target(s.reference, s.width, s.height, x0, y0, crop, crop, tgt.addr(i * px_t), m)
+        net.train_step(x, tgt, batch, crop, crop, cosine_lr(it, steps, lr), loss_sum=acc)
+        if (it + 1) % log_every == 0:
+            losses.append(float(acc.numpy()[0]) / log_every)
+            acc.fill(0.0)
+            if log:
+                log(f"step {it + 1}/{steps} loss {losses[-1]:.4f} ({time.perf_counter() - t0:.1f} s)")
+            if callback:
+                callback(it + 1)
+    return losses
+
+
+def evaluate(net: denoiser.UNet, samples: list) -> dict:
+    """relMSE against the reference of the noisy and the denoised image for
+    every sample count, averaged over `samples`."""
+    res = {}
+    for spp in samples[0].noisy:
+        noisy, den = [], []
+        for s in samples:
+            ref = s.reference.numpy()
+            noisy.append(relmse(s.noisy[spp][0].numpy(), ref))
+            den.append(relmse(net.denoise(s.noisy[spp], s.width, s.height, spp).numpy(), ref))
+        res[spp] = (float(np.mean(noisy)), float(np.mean(den)))
+    return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="Train the denoiser on tracer-rendered pairs.")
+    ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--holdout", type=int, default=16)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=16000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default="", help="continue from these weights")
+    ap.add_argument("--out", default=denoiser.weights_path())
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    # Scene seeds start at 1000: seed 0 is kept out of training.
+    train_seeds = range(1000 + a.seed * 100000, 1000 + a.seed * 100000 + a.scenes)
+    hold_seeds = range(900, 900 + a.holdout)
+    data = render_dataset(train_seeds, a.size, a.size, ref_spp=a.ref_spp, log=log)
+    hold = render_dataset(hold_seeds, a.size, a.size, ref_spp=a.ref_spp, log=None)
+    net = denoiser.UNet.load(a.init) if a.init else denoiser.UNet(seed=a.seed)
+    t0 = time.perf_counter()
+    train(net, data, a.steps, a.batch, a.crop, a.lr, a.seed, log, 500)
+    denoiser.synchronize()
+    log(f"trained {a.steps} steps in {time.perf_counter() - t0:.1f} s")
+    for spp, (n, d) in evaluate(net, hold).items():
+        log(f"held-out relMSE at {spp} spp: noisy {n:.5f} denoised {d:.5f}")
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    net.save(a.out)
+    log(f"saved {a.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

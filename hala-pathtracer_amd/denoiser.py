@@ -5,17 +5,20 @@ and backward).  It takes the place of the reference's offline OIDN step
 (denoise.py:7-30, `oidnDenoise --hdr color.pfm --alb albedo.pfm --nrm
 normal.pfm`): same three inputs, a denoised colour image out.
 
-Network.  Input: log(1 + colour), albedo, normal * 0.5 + 0.5 (9 channels,
-zero-padded to 16).  Encoder: two 3x3 convs + ReLU per level at 32, 48, 64,
-96, 128 channels, 2x2 average pooling between levels.  Decoder: nearest 2x
-upsampling, concatenation [upsampled, skip], two 3x3 convs + ReLU per level.
-Output: a 1x1 conv to log(1 + colour).  Loss: L1 in that log space.
+Network.  Input: log(1 + colour), albedo, normal * 0.5 + 0.5 and the sample
+count's feature log2(spp) / 8 (10 channels, zero-padded to 16).  Encoder: two
+3x3 convs + ReLU per level at 32, 48, 64, 96, 128 channels, 2x2 average
+pooling between levels.  Decoder: nearest 2x upsampling, concatenation
+[upsampled, skip], two 3x3 convs + ReLU per level.  Output: a 1x1 conv whose
+result is added to the input's log(1 + colour) (the network predicts a
+correction of its input).  Loss: L1 in that log space.
 
 Precision.  Float32 master weights, gradients and Adam state; bf16 weights
 and activations on the MFMA; float32 sums.  Images are NHWC on the device.
 
-torch provides device memory and the current stream only; every operation is
-a call into libhpt.so.  There is no CPU path.
+Memory.  Every buffer is device memory of libhpt.so's own HIP runtime
+(DeviceArray over hpt_nn_alloc), and every operation is a call into the
+library on the null stream; there is no CPU path.
 """
 from __future__ import annotations
 
@@ -24,7 +27,6 @@ import math
 import os
 
 import numpy as np
-import torch
 
 import hpt
 
@@ -32,6 +34,7 @@ LEVELS = (32, 48, 64, 96, 128)
 IN_CHANNELS = 16
 OUT_CHANNELS = 3
 OUT_PAD = 16
+STREAM = None  # the null stream: ordered after the renderers' blocking streams
 
 _vp, _i, _sz, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 # Entry points of include/hpt_nn.h with their argument types.
@@ -49,10 +52,21 @@ NN_SIGNATURES = {
     "hpt_nn_concat_forward": [_vp, _vp, _vp, _vp, _sz, _i, _i],
     "hpt_nn_concat_backward": [_vp, _vp, _vp, _vp, _sz, _i, _i],
     "hpt_nn_accumulate": [_vp, _vp, _vp, _sz],
-    "hpt_nn_make_input": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _i, _i],
+    "hpt_nn_make_input": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _i, _i, _i, _f],
+    "hpt_nn_make_target": [_vp, _vp, _i, _i, _i, _i, _vp, _i, _i, _i],
+    "hpt_nn_add_residual": [_vp, _vp, _vp, _sz, _i],
     "hpt_nn_l1_loss": [_vp, _vp, _vp, _vp, _vp, _sz, _i],
     "hpt_nn_adam": [_vp, _vp, _vp, _vp, _vp, _sz, _f, _f, _f, _f, _i],
     "hpt_nn_write_output": [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i],
+    "hpt_nn_alloc": [_sz, C.POINTER(_vp)],
+    "hpt_nn_free": [_vp],
+    "hpt_nn_copy": [_vp, _vp, _vp, _sz],
+    "hpt_nn_fill": [_vp, _vp, _f, _sz],
+    "hpt_nn_stream_synchronize": [_vp],
+    "hpt_nn_event_create": [C.POINTER(_vp)],
+    "hpt_nn_event_destroy": [_vp],
+    "hpt_nn_event_record": [_vp, _vp],
+    "hpt_nn_event_elapsed": [_vp, _vp, C.POINTER(_f)],
 }
 
 _nn = None
@@ -77,26 +91,114 @@ def _call(name, *args):
         raise RuntimeError(f"{name} failed with HIP error {rc}")
 
 
+def synchronize() -> None:
+    """Waits for the denoiser's work (the null stream)."""
+    _call("hpt_nn_stream_synchronize", STREAM)
+
+
+class DeviceArray:
+    """A flat device buffer of `count` elements of a numpy dtype, allocated
+    (zeroed) by libhpt.so and freed with the object."""
+
+    def __init__(self, count: int, dtype, shape=None):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.shape = tuple(shape) if shape is not None else (self.count,)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = C.c_void_p()
+        _call("hpt_nn_alloc", self.nbytes, C.byref(p))
+        self.ptr = int(p.value or 0)
+
+    def __del__(self):
+        if getattr(self, "ptr", 0) and _nn is not None:
+            _nn.hpt_nn_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype, a.shape)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        """Copies host data in at element `offset` (synchronously)."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if offset + a.size > self.count:
+            raise ValueError("upload past the end of the buffer")
+        _call("hpt_nn_copy", STREAM, C.c_void_p(self.addr(offset)), C.c_void_p(a.ctypes.data), a.nbytes)
+        synchronize()
+
+    def copy_from_device(self, src: int, nbytes: int, offset: int = 0) -> None:
+        """Device-to-device copy from address `src` (queued on the stream)."""
+        if offset * self.dtype.itemsize + nbytes > self.nbytes:
+            raise ValueError("copy past the end of the buffer")
+        _call("hpt_nn_copy", STREAM, C.c_void_p(self.addr(offset)), C.c_void_p(src), nbytes)
+
+    def numpy(self) -> np.ndarray:
+        """A host copy, after the queued work has finished."""
+        out = np.empty(self.count, self.dtype)
+        _call("hpt_nn_copy", STREAM, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), self.nbytes)
+        synchronize()
+        return out.reshape(self.shape)
+
+    def fill(self, value: float) -> None:
+        if self.dtype != np.float32:
+            raise TypeError("fill is for float32 buffers")
+        _call("hpt_nn_fill", STREAM, C.c_void_p(self.ptr), value, self.count)
+
+    def addr(self, offset: int = 0) -> int:
+        """Device address of element `offset`."""
+        if not 0 <= offset <= self.count:
+            raise IndexError(offset)
+        return self.ptr + offset * self.dtype.itemsize
+
+
 class _Ptr(C.c_void_p):
-    """A device pointer holding its tensor until the call returns (a tensor
-    freed while the arguments are evaluated could be reused by the next
-    argument's allocation before the kernel reads it)."""
+    """A pointer argument that holds its DeviceArray until the call returns
+    (a temporary freed while the arguments are evaluated would hand its
+    memory to the next argument's allocation)."""
 
 
 def _p(t):
+    """A device pointer argument: a DeviceArray, a raw address (int) or None."""
     if t is None:
         return None
-    v = _Ptr(t.data_ptr())
-    v.keep = t
-    return v
+    if isinstance(t, DeviceArray):
+        v = _Ptr(t.ptr)
+        v.keep = t
+        return v
+    return C.c_void_p(int(t))
 
 
-def _stream():
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+class Event:
+    """A HIP event of the library's runtime."""
+
+    def __init__(self):
+        e = C.c_void_p()
+        _call("hpt_nn_event_create", C.byref(e))
+        self.e = e
+
+    def __del__(self):
+        if getattr(self, "e", None) and self.e.value and _nn is not None:
+            _nn.hpt_nn_event_destroy(self.e)
+
+    def record(self, stream=STREAM) -> None:
+        _call("hpt_nn_event_record", self.e, stream)
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = C.c_float()
+        _call("hpt_nn_event_elapsed", self.e, stop.e, C.byref(ms))
+        return float(ms.value)
 
 
 def pad16(c: int) -> int:
     return (c + 15) // 16 * 16
+
+
+def spp_feature(spp: int) -> float:
+    """The sample-count input channel: log2(spp) / 8."""
+    return math.log2(max(1, spp)) / 8.0
 
 
 def layer_specs(levels=LEVELS):
@@ -114,37 +216,37 @@ def layer_specs(levels=LEVELS):
 
 
 class _Layer:
-    def __init__(self, name, cin, cout, k, relu, offset, device):
+    def __init__(self, name, cin, cout, k, relu, offset):
         self.name, self.cin, self.cout, self.k, self.relu = name, cin, cout, k, relu
         self.cout_pad = pad16(cout)
         self.w_size = cout * k * k * cin
         self.w_off, self.b_off = offset, offset + self.w_size
         self.size = self.w_size + cout
-        self.w = torch.empty(self.cout_pad * k * k * cin, dtype=torch.int16, device=device)
-        self.wt = torch.empty(cin * k * k * self.cout_pad, dtype=torch.int16, device=device)
+        self.w = DeviceArray(self.cout_pad * k * k * cin, np.uint16)
+        self.wt = DeviceArray(cin * k * k * self.cout_pad, np.uint16)
 
 
 class UNet:
-    """The denoising network with its optimizer state, on one GPU."""
+    """The denoising network with its optimizer state, on the current GPU."""
 
-    def __init__(self, levels=LEVELS, seed: int = 0, device: int = 0):
+    def __init__(self, levels=LEVELS, seed: int = 0, residual: bool = True):
         nn_library()
         self.levels = tuple(levels)
-        self.device = torch.device("cuda", device)
+        self.residual = residual
         self.layers = {}
         off = 0
         for name, cin, cout, k, relu in layer_specs(self.levels):
-            self.layers[name] = _Layer(name, cin, cout, k, relu, off, self.device)
+            self.layers[name] = _Layer(name, cin, cout, k, relu, off)
             off += self.layers[name].size
         self.param_count = off
-        self.params = torch.zeros(off, dtype=torch.float32, device=self.device)
-        self.grads = torch.zeros_like(self.params)
-        self.adam_m = torch.zeros_like(self.params)
-        self.adam_v = torch.zeros_like(self.params)
+        self.params = DeviceArray(off, np.float32)
+        self.grads = DeviceArray(off, np.float32)
+        self.adam_m = DeviceArray(off, np.float32)
+        self.adam_v = DeviceArray(off, np.float32)
         self.step_count = 0
         self._bufs = {}
-        self._scratch = torch.empty(0, dtype=torch.float32, device=self.device)
-        self._loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        self._scratch = None
+        self._loss = DeviceArray(1, np.float32)
         self.set_params(self.init_params(seed))
 
     # -------------------------------------------------------- parameters
@@ -163,38 +265,37 @@ class UNet:
             w, b = params[name]
             flat[L.w_off:L.b_off] = np.asarray(w, np.float32).reshape(-1)
             flat[L.b_off:L.b_off + L.cout] = np.asarray(b, np.float32).reshape(-1)
-        self.params.copy_(torch.from_numpy(flat))
+        self.params.upload(flat)
         self.prepare_weights()
 
-    def get_params(self) -> dict:
-        flat = self.params.cpu().numpy()
+    def _unflatten(self, flat: np.ndarray) -> dict:
         return {name: (flat[L.w_off:L.b_off].reshape(L.cout, L.k * L.k, L.cin).copy(),
                        flat[L.b_off:L.b_off + L.cout].copy()) for name, L in self.layers.items()}
 
+    def get_params(self) -> dict:
+        return self._unflatten(self.params.numpy())
+
     def get_grads(self) -> dict:
-        flat = self.grads.cpu().numpy()
-        return {name: (flat[L.w_off:L.b_off].reshape(L.cout, L.k * L.k, L.cin).copy(),
-                       flat[L.b_off:L.b_off + L.cout].copy()) for name, L in self.layers.items()}
+        return self._unflatten(self.grads.numpy())
 
     def prepare_weights(self) -> None:
         """Master float32 weights -> the bf16 copies the convolutions read."""
-        st = _stream()
         for L in self.layers.values():
-            _call("hpt_nn_prepare_weights", st, _p(self.params[L.w_off:]), _p(L.w), _p(L.wt), L.cin, L.cout,
+            _call("hpt_nn_prepare_weights", STREAM, _p(self.params.addr(L.w_off)), _p(L.w), _p(L.wt), L.cin, L.cout,
                   L.cout_pad, L.k)
 
     def save(self, path: str) -> None:
-        np.savez(path, levels=np.array(self.levels), params=self.params.cpu().numpy(),
-                 step=np.array(self.step_count))
+        np.savez(path, levels=np.array(self.levels), params=self.params.numpy(),
+                 step=np.array(self.step_count), residual=np.array(int(self.residual)))
 
     @classmethod
-    def load(cls, path: str, device: int = 0) -> "UNet":
+    def load(cls, path: str) -> "UNet":
         with np.load(path, allow_pickle=False) as f:
-            net = cls(tuple(int(c) for c in f["levels"]), device=device)
+            net = cls(tuple(int(c) for c in f["levels"]), residual=bool(int(f["residual"])))
             params = f["params"]
             if params.size != net.param_count:
                 raise ValueError(f"{path}: {params.size} parameters, the network has {net.param_count}")
-            net.params.copy_(torch.from_numpy(params.astype(np.float32)))
+            net.params.upload(params.astype(np.float32))
             net.step_count = int(f["step"])
         net.prepare_weights()
         return net
@@ -207,9 +308,8 @@ class UNet:
         step = 1 << (len(self.levels) - 1)
         if h % step or w % step:
             raise ValueError(f"image size {w}x{h} is not a multiple of {step}")
-        dev = self.device
-        bf = lambda px, c: torch.empty(px * c, dtype=torch.int16, device=dev)  # noqa: E731
-        f32 = lambda px, c: torch.empty(px * c, dtype=torch.float32, device=dev)  # noqa: E731
+        bf = lambda px, c: DeviceArray(px * c, np.uint16)  # noqa: E731
+        f32 = lambda px, c: DeviceArray(px * c, np.float32)  # noqa: E731
         P = [n * (h >> l) * (w >> l) for l in range(len(self.levels))]
         b = {"x": bf(P[0], IN_CHANNELS), "out": f32(P[0], OUT_CHANNELS)}
         for name, L in self.layers.items():
@@ -233,8 +333,8 @@ class UNet:
             need = max(nn_library().hpt_nn_wgrad_scratch_floats(n, h >> self._level(nm), w >> self._level(nm), L.cin,
                                                                   L.cout_pad, L.k)
                        for nm, L in self.layers.items())
-            if self._scratch.numel() < need:
-                self._scratch = torch.empty(need, dtype=torch.float32, device=dev)
+            if self._scratch is None or self._scratch.count < need:
+                self._scratch = DeviceArray(need, np.float32)
         b["P"] = P
         self._bufs[key] = b
         return b
@@ -245,16 +345,17 @@ class UNet:
 
     # -------------------------------------------------------- passes
     def _conv(self, L, x, y, y_f32, n, h, w):
-        _call("hpt_nn_conv_forward", _stream(), _p(x), _p(L.w), _p(self.params[L.b_off:]), _p(y), _p(y_f32), n, h, w,
+        _call("hpt_nn_conv_forward", STREAM, _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_f32), n, h, w,
               L.cin, L.cout, L.cout_pad, L.k, int(L.relu))
 
-    def forward_input(self, x: torch.Tensor, n: int, h: int, w: int, train: bool = False) -> torch.Tensor:
-        """Runs the network on the bf16 input x ((n*h*w*16) int16 bit
-        patterns, NHWC); returns the float32 output (n*h*w*3)."""
+    def forward_input(self, x: DeviceArray, n: int, h: int, w: int, train: bool = False) -> DeviceArray:
+        """Runs the network on the bf16 input x (n*h*w*16 bit patterns, NHWC;
+        copied into the network's input buffer unless it is that buffer);
+        returns the float32 output buffer (n*h*w*3)."""
         b = self._buffers(n, h, w, train)
-        if x.data_ptr() != b["x"].data_ptr():
-            b["x"].copy_(x.reshape(-1))
-        st = _stream()
+        if x.ptr != b["x"].ptr:
+            b["x"].copy_from_device(x.ptr, b["x"].nbytes)
+        st = STREAM
         Lv = len(self.levels)
         cur = b["x"]
         for l in range(Lv):
@@ -275,17 +376,19 @@ class UNet:
             self._conv(self.layers[f"dec{l}b"], b[f"y_dec{l}a"], b[f"y_dec{l}b"], None, n, hl, wl)
             cur = b[f"y_dec{l}b"]
         self._conv(self.layers["out"], cur, None, b["out"], n, h, w)
+        if self.residual:
+            _call("hpt_nn_add_residual", st, _p(b["out"]), _p(b["x"]), n * h * w, IN_CHANNELS)
         return b["out"]
 
     def _conv_back(self, L, dy, x, y, dx, n, h, w):
         """dy: gradient w.r.t. the layer's output (modified in place by the
         ReLU's derivative); weight and bias gradients into self.grads; the
         input gradient into dx when dx is given."""
-        st = _stream()
+        st = STREAM
         if L.relu:
             _call("hpt_nn_relu_backward", st, _p(dy), _p(y), n * h * w * L.cout)
-        _call("hpt_nn_conv_backward_weights", st, _p(dy), _p(x), _p(self.grads[L.w_off:]), _p(self.grads[L.b_off:]),
-              _p(self._scratch), n, h, w, L.cin, L.cout, L.cout_pad, L.k)
+        _call("hpt_nn_conv_backward_weights", st, _p(dy), _p(x), _p(self.grads.addr(L.w_off)),
+              _p(self.grads.addr(L.b_off)), _p(self._scratch), n, h, w, L.cin, L.cout, L.cout_pad, L.k)
         if dx is not None:
             _call("hpt_nn_conv_backward_data", st, _p(dy), _p(L.wt), _p(dx), n, h, w, L.cin, L.cout, L.cout_pad, L.k)
 
@@ -293,10 +396,10 @@ class UNet:
         """Gradients of the loss whose output gradient is in the "dout"
         buffer (float32, 16 channels per pixel) into self.grads."""
         b = self._buffers(n, h, w, True)
-        st = _stream()
+        st = STREAM
         Lv = len(self.levels)
         lay = self.layers
-        self._conv_back(lay["out"], b["dout"], b[f"y_dec0b"] if Lv > 1 else b["y_enc0b"], None, b["dx_out"], n, h, w)
+        self._conv_back(lay["out"], b["dout"], b["y_dec0b"] if Lv > 1 else b["y_enc0b"], None, b["dx_out"], n, h, w)
         d = b["dx_out"]
         for l in range(Lv - 1):
             hl, wl = h >> l, w >> l
@@ -320,47 +423,104 @@ class UNet:
             self._conv_back(lay[f"enc{l}a"], b[f"dx_enc{l}b"], x_in, b[f"y_enc{l}a"],
                             b[f"dx_enc{l}a"] if l > 0 else None, n, hl, wl)
 
+    def loss(self, out: DeviceArray, target: DeviceArray, n: int, h: int, w: int, loss_sum: DeviceArray) -> None:
+        """L1 loss of `out` against `target` (log colours): its mean is
+        added to loss_sum[0] and its gradient written to the "dout" buffer."""
+        b = self._buffers(n, h, w, True)
+        _call("hpt_nn_l1_loss", STREAM, _p(out), _p(target), _p(b["dout"]), _p(loss_sum), n * h * w, OUT_PAD)
+
     def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         self.step_count += 1
-        _call("hpt_nn_adam", _stream(), _p(self.params), _p(self.grads), _p(self.adam_m), _p(self.adam_v),
+        _call("hpt_nn_adam", STREAM, _p(self.params), _p(self.grads), _p(self.adam_m), _p(self.adam_v),
               self.param_count, lr, beta1, beta2, eps, self.step_count)
         self.prepare_weights()
 
-    def train_step(self, x: torch.Tensor, target_log: torch.Tensor, n: int, h: int, w: int, lr: float) -> torch.Tensor:
+    def train_step(self, x: DeviceArray, target: DeviceArray, n: int, h: int, w: int, lr: float,
+                   loss_sum: DeviceArray | None = None) -> float | None:
         """One step on a batch: forward, L1 loss in log space, backward, Adam.
-        Returns the loss as a device tensor (read it when convenient)."""
+        The loss is added to loss_sum[0] when given (nothing waits); else it
+        is returned (the step is waited for)."""
         b = self._buffers(n, h, w, True)
         out = self.forward_input(x, n, h, w, train=True)
-        if target_log.data_ptr() != b["target"].data_ptr():
-            b["target"].copy_(target_log.reshape(-1))
-        loss = torch.zeros(1, dtype=torch.float32, device=self.device)
-        _call("hpt_nn_l1_loss", _stream(), _p(out), _p(b["target"]), _p(b["dout"]), _p(loss), n * h * w, OUT_PAD)
+        if target.ptr != b["target"].ptr:
+            b["target"].copy_from_device(target.ptr, b["target"].nbytes)
+        acc = loss_sum
+        if acc is None:
+            acc = self._loss
+            acc.fill(0.0)
+        self.loss(out, b["target"], n, h, w, acc)
         self.backward(n, h, w)
         self.adam_step(lr)
-        return loss
+        return None if loss_sum is not None else float(acc.numpy()[0])
 
     # -------------------------------------------------------- inference
-    def make_input(self, color, albedo, normal, x0: int, y0: int, h: int, w: int, out: torch.Tensor | None = None):
-        """Network input (h*w*16 bf16) of the h x w window at (x0, y0) of
-        float4 device images (H, W, 4)."""
-        sh, sw = color.shape[:2]
-        if out is None:
-            out = torch.empty(h * w * IN_CHANNELS, dtype=torch.int16, device=self.device)
-        _call("hpt_nn_make_input", _stream(), _p(color), _p(albedo), _p(normal), sw, sh, x0, y0, _p(out), h, w)
+    def make_input(self, images, src_w: int, src_h: int, x0: int, y0: int, h: int, w: int, out, spp: int,
+                   mirror: int = 0):
+        """Network input (h*w*16 bf16) into `out` (a buffer or an address)
+        of the h x w window at (x0, y0) of (colour, albedo, normal): float4
+        device images of src_w x src_h averaged over `spp` samples, given as
+        DeviceArrays or addresses."""
+        color, albedo, normal = images
+        _call("hpt_nn_make_input", STREAM, _p(color), _p(albedo), _p(normal), src_w, src_h, x0, y0, _p(out), h, w,
+              mirror, spp_feature(spp))
         return out
 
-    def denoise(self, color: torch.Tensor, albedo: torch.Tensor, normal: torch.Tensor) -> torch.Tensor:
-        """Denoised colour (H, W, 4) of float4 device images (H, W, 4); the
-        network runs on the image padded up to a multiple of 16."""
-        H, W = color.shape[:2]
+    def make_target(self, color, src_w: int, src_h: int, x0: int, y0: int, h: int, w: int, out, mirror: int = 0):
+        """log(1 + colour) of the same window of a reference image into `out`
+        (h*w*3 float32)."""
+        _call("hpt_nn_make_target", STREAM, _p(color), src_w, src_h, x0, y0, _p(out), h, w, mirror)
+        return out
+
+    def denoise(self, images, width: int, height: int, spp: int, out: DeviceArray | None = None) -> DeviceArray:
+        """Denoised colour (height, width, 4) float32 of (colour, albedo,
+        normal) float4 device images (DeviceArrays or addresses) of `spp`
+        samples per pixel; the network runs on the image padded up to a
+        multiple of 16 (zero features outside)."""
         step = 1 << (len(self.levels) - 1)
-        h, w = -(-H // step) * step, -(-W // step) * step
+        h, w = -(-height // step) * step, -(-width // step) * step
         b = self._buffers(1, h, w, False)
-        self.make_input(color, albedo, normal, 0, 0, h, w, out=b["x"])
-        out = self.forward_input(b["x"], 1, h, w)
-        dst = torch.empty((H, W, 4), dtype=torch.float32, device=self.device)
-        _call("hpt_nn_write_output", _stream(), _p(out), _p(dst), W, H, 0, 0, h, w)
-        return dst
+        self.make_input(images, width, height, 0, 0, h, w, b["x"], spp)
+        res = self.forward_input(b["x"], 1, h, w)
+        if out is None:
+            out = DeviceArray(height * width * 4, np.float32, (height, width, 4))
+        _call("hpt_nn_write_output", STREAM, _p(res), _p(out), width, height, 0, 0, h, w)
+        return out
+
+    def denoise_renderer(self, r: "hpt.HalaRenderer", out: DeviceArray | None = None) -> DeviceArray:
+        """Denoises a renderer's accumulated images in place on the device
+        (the role of denoise.py's oidnDenoise call on the saved PFMs)."""
+        r.wait_idle()
+        images = tuple(r.image_pointer(k)[0] for k in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL))
+        return self.denoise(images, r.width, r.height, max(1, r.frame_index), out)
+
+    def profile_forward(self, n: int, h: int, w: int, reps: int = 5) -> dict:
+        """{layer: (average ms, algorithmic FLOP)} of each convolution of the
+        forward pass at n x h x w, timed with HIP events on the stream the
+        kernels run on; the input buffer is used as it is."""
+        b = self._buffers(n, h, w, False)
+        ev = {name: [] for name in self.layers}
+        orig = self._conv
+
+        def timed(L, x, y, y_f32, nn, hh, ww):
+            s, e = Event(), Event()
+            s.record()
+            orig(L, x, y, y_f32, nn, hh, ww)
+            e.record()
+            ev[L.name].append((s, e))
+
+        self._conv = timed
+        try:
+            for _ in range(reps):
+                self.forward_input(b["x"], n, h, w)
+        finally:
+            del self._conv
+        synchronize()
+        out = {}
+        for name, L in self.layers.items():
+            l = self._level(name)
+            flops = 2 * n * (h >> l) * (w >> l) * L.k * L.k * L.cin * L.cout
+            out[name] = (sum(s.elapsed_ms(e) for s, e in ev[name]) / len(ev[name]), flops)
+        return out
 
     def flops_forward(self, n: int, h: int, w: int) -> int:
         """Multiply-adds x 2 of the convolutions in one forward pass."""
@@ -371,12 +531,18 @@ class UNet:
         return total
 
 
-def read_renderer_images(r: "hpt.HalaRenderer", device: int = 0):
-    """(colour, albedo, normal) float4 device tensors of a renderer's
-    accumulated images."""
-    dev = torch.device("cuda", device)
-    return tuple(torch.from_numpy(np.ascontiguousarray(r.read_image(k))).to(dev)
-                 for k in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL))
+def renderer_images(r: "hpt.HalaRenderer") -> tuple:
+    """Copies of a renderer's accumulated (colour, albedo, normal) images
+    (after wait_idle), device to device."""
+    r.wait_idle()
+    out = []
+    for k in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL):
+        ptr, nbytes = r.image_pointer(k)
+        d = DeviceArray(r.height * r.width * 4, np.float32, (r.height, r.width, 4))
+        d.copy_from_device(ptr, nbytes)
+        out.append(d)
+    synchronize()  # the renderer may overwrite its images once it renders again
+    return tuple(out)
 
 
 def weights_path() -> str:

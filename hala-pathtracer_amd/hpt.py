@@ -174,6 +174,7 @@ SIGNATURES = {
     "hpt_query_idle": [C.c_void_p, C.POINTER(C.c_int)],
     "hpt_get_memory_usage": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
+    "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
 }
 
 _lib = None
@@ -405,6 +406,13 @@ class HalaRenderer:
             out = np.empty((self.height, self.width, 4), np.float32)
         _check(self._lib.hpt_read_image(self._h, which, _ptr(out), out.nbytes))
         return out
+
+    def image_pointer(self, which: int) -> tuple:
+        """(device address, bytes) of a resident image; read it only after
+        wait_idle()."""
+        p, n = C.c_void_p(), C.c_size_t()
+        _check(self._lib.hpt_get_image_pointer(self._h, which, C.byref(p), C.byref(n)))
+        return int(p.value or 0), int(n.value)
 
     def save_images(self, prefix: str) -> None:
         """HalaRenderer::save_images (hala-pathtracer.rs:251): colour, albedo

@@ -515,3 +515,176 @@ def furnace_sphere(albedo=0.5, emission=1.0, segments=48, rings=24):
     cam = look_at_camera((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), yfov_deg=60.0)
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)],
                      ground_color=(0, 0, 0), sky_color=(0, 0, 0))
+
+
+# ------------------------------------------------------------------ training scenes
+def _random_material(m, rng, textures):
+    """Lambert/Oren-Nayar or Disney with random lobes; sometimes a checker
+    base-colour map or a bump normal map from `textures`."""
+    color = tuple(rng.uniform(0.03, 0.95, 3) * (rng.uniform(0.5, 1.0) if rng.uniform() < 0.5 else 1.0))
+    if rng.uniform() < 0.3:
+        diffuse_material(m, color, roughness=float(rng.uniform(0.0, 1.0)))
+    else:
+        metallic = float(rng.choice([0.0, 1.0, rng.uniform()], p=[0.5, 0.25, 0.25]))
+        transmission = float(rng.uniform(0.6, 1.0)) if rng.uniform() < 0.12 and metallic < 0.5 else 0.0
+        disney_material(m, color, metallic=metallic, roughness=float(rng.uniform(0.02, 1.0)),
+                        anisotropic=float(rng.uniform() * (rng.uniform() < 0.3)),
+                        sheen=float(rng.uniform() * (rng.uniform() < 0.3)),
+                        clearcoat=float(rng.uniform() * (rng.uniform() < 0.3)),
+                        specular_tint=float(rng.uniform() * (rng.uniform() < 0.3)),
+                        specular_transmission=transmission, ior=float(rng.uniform(1.3, 1.8)))
+    if textures is not None and rng.uniform() < 0.2:
+        m["base_color"] = (1.0, 1.0, 1.0)
+        m["base_color_map_index"] = 0 if rng.uniform() < 0.5 else 2
+    if textures is not None and rng.uniform() < 0.1:
+        m["normal_map_index"] = 1
+
+
+def _random_lights(rng, center, extent, count):
+    lights = np.zeros(count, hpt.LIGHT_DTYPE)
+    for i in range(count):
+        kind = int(rng.choice([hpt.LIGHT_POINT, hpt.LIGHT_SPOT, hpt.LIGHT_DIRECTIONAL, hpt.LIGHT_QUAD,
+                               hpt.LIGHT_SPHERE], p=[0.2, 0.15, 0.15, 0.3, 0.2]))
+        pos = center + rng.uniform(-1, 1, 3) * extent * np.array([1.0, 1.0, 0.5]) + np.array([0, 0, extent])
+        tint = rng.uniform(0.6, 1.0, 3)
+        power = float(np.exp(rng.uniform(np.log(2.0), np.log(40.0))))
+        lights[i]["type"] = kind
+        lights[i]["position"] = pos
+        if kind == hpt.LIGHT_SPOT:
+            axis = center - pos
+            lights[i]["u"] = axis / np.linalg.norm(axis)
+            inner = rng.uniform(10, 30)
+            lights[i]["v"] = (math.cos(math.radians(inner)), math.cos(math.radians(inner + rng.uniform(5, 15))), 0.0)
+            lights[i]["intensity"] = tint * power
+        elif kind == hpt.LIGHT_DIRECTIONAL:
+            d = np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), -rng.uniform(0.3, 1.0)])
+            lights[i]["u"] = d / np.linalg.norm(d)
+            lights[i]["intensity"] = tint * power / 15.0
+        elif kind == hpt.LIGHT_QUAD:
+            a, b = rng.uniform(0.1, 0.8) * extent, rng.uniform(0.1, 0.8) * extent
+            lights[i]["position"] = pos - np.array([a / 2, b / 2, 0])
+            lights[i]["u"] = (a, 0.0, 0.0)
+            lights[i]["v"] = (0.0, b, 0.0)
+            lights[i]["area"] = a * b
+            lights[i]["intensity"] = tint * power / max(0.2, a * b)
+        elif kind == hpt.LIGHT_SPHERE:
+            r = rng.uniform(0.05, 0.3) * extent
+            lights[i]["radius"] = r
+            lights[i]["area"] = 4 * math.pi * r * r
+            lights[i]["intensity"] = tint * power / max(0.5, 4 * math.pi * r * r)
+        else:
+            lights[i]["intensity"] = tint * power
+    return lights
+
+
+def random_scene(seed, width=256, height=256):
+    """A seeded random scene for training the denoiser: (scene, features).
+    One of three layouts -- a walled room with objects and ceiling lights, an
+    outdoor ground plane under a sky map with a sun, or a displaced
+    icosphere with mixed lights -- filled with random spheres, boxes and
+    materials (Lambert, Oren-Nayar, Disney with metal, clearcoat, sheen and
+    glass lobes, textured or bump-mapped), sometimes a participating medium
+    and a thin-lens camera."""
+    rng = np.random.default_rng(seed)
+    layout = int(rng.choice(3, p=[0.35, 0.35, 0.3]))
+    textures = [checker_texture(int(rng.integers(4, 16)), 64, tuple(int(c) for c in rng.integers(120, 255, 3)),
+                                tuple(int(c) for c in rng.integers(0, 120, 3))),
+                bump_normal_map(64, int(rng.integers(2, 8)), float(rng.uniform(0.2, 0.8))),
+                checker_texture(int(rng.integers(2, 6)), 64, tuple(int(c) for c in rng.integers(60, 255, 3)),
+                                tuple(int(c) for c in rng.integers(60, 255, 3)))]
+    n_obj = int(rng.integers(3, 12))
+    mats = hpt.new_materials(n_obj + 8)
+    for i in range(n_obj + 8):
+        _random_material(mats[i], rng, textures)
+    prims, features = [], 0
+    env, sky, ground = None, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
+    if layout == 0:      # room
+        s = float(rng.uniform(1.5, 3.0))
+        for k, (p0, e1, e2) in enumerate([((-s, -s, 0), (2 * s, 0, 0), (0, 2 * s, 0)),
+                                          ((-s, -s, 2 * s), (0, 2 * s, 0), (2 * s, 0, 0)),
+                                          ((-s, s, 0), (0, 0, 2 * s), (2 * s, 0, 0)),
+                                          ((-s, -s, 0), (0, 2 * s, 0), (0, 0, 2 * s)),
+                                          ((s, -s, 0), (0, 0, 2 * s), (0, 2 * s, 0))]):
+            diffuse_material(mats[n_obj + k], tuple(rng.uniform(0.1, 0.9, 3)), roughness=float(rng.uniform(0, 0.5)))
+            if rng.uniform() < 0.2:
+                mats[n_obj + k]["base_color"] = (1.0, 1.0, 1.0)
+                mats[n_obj + k]["base_color_map_index"] = 0
+            v, ix = quad_mesh(p0, e1, e2)
+            prims.append(hpt.Primitive(v, ix, n_obj + k))
+        lights = _random_lights(rng, np.array([0.0, 0.0, 0.0]), s, int(rng.integers(1, 4)))
+        for L in lights:  # keep lights inside the room
+            L["position"] = np.clip(L["position"], -0.9 * s, 0.9 * s) + np.array([0, 0, s])
+        eye = np.array([rng.uniform(-0.5, 0.5) * s, -0.95 * s, rng.uniform(0.4, 1.4) * s])
+        target = np.array([rng.uniform(-0.3, 0.3) * s, 0.0, rng.uniform(0.2, 0.8) * s])
+        spread, zmin = 0.8 * s, 0.0
+    elif layout == 1:    # outdoor
+        diffuse_material(mats[n_obj], tuple(rng.uniform(0.2, 0.7, 3)), roughness=float(rng.uniform(0, 1)))
+        if rng.uniform() < 0.4:
+            mats[n_obj]["base_color"] = (1.0, 1.0, 1.0)
+            mats[n_obj]["base_color_map_index"] = 2
+        v, ix = quad_mesh((-30, -30, 0), (60, 0, 0), (0, 60, 0))
+        prims.append(hpt.Primitive(v, ix, n_obj))
+        env = sky_env_map(512, 256, seed, sun_elevation_deg=float(rng.uniform(5, 80)),
+                          sun_azimuth=float(rng.uniform()), sun_radiance=float(rng.uniform(2e3, 2e4)),
+                          sun_diameter_deg=float(rng.uniform(0.5, 3.0)))
+        lights = _random_lights(rng, np.zeros(3), 2.0, int(rng.integers(0, 2)))
+        d = rng.uniform(3.0, 7.0)
+        a = rng.uniform(0, 2 * math.pi)
+        eye = np.array([d * math.cos(a), d * math.sin(a), rng.uniform(0.5, 4.0)])
+        target = np.array([0.0, 0.0, rng.uniform(0.2, 0.8)])
+        spread, zmin = 2.0, 0.0
+    else:                # displaced icosphere with mixed lights
+        v, f = icosphere(int(rng.integers(4, 8)))
+        d = 1.0 + rng.uniform(0.03, 0.15) * fbm(v * rng.uniform(1.0, 3.0), seed=seed)
+        r0 = rng.uniform(0.5, 0.9)
+        sv, sf = smooth_mesh(v * d[:, None] * r0 + np.array([0.0, 0.0, r0 + 0.1]), f)
+        prims.append(hpt.Primitive(sv, sf, 0))
+        diffuse_material(mats[n_obj], tuple(rng.uniform(0.3, 0.7, 3)), roughness=float(rng.uniform(0, 0.5)))
+        v, ix = quad_mesh((-6, -6, 0), (12, 0, 0), (0, 12, 0))
+        prims.append(hpt.Primitive(v, ix, n_obj))
+        lights = _random_lights(rng, np.array([0.0, 0.0, 0.5]), 2.0, int(rng.integers(2, 6)))
+        sky = tuple(rng.uniform(0.0, 0.2, 3))
+        ground = tuple(rng.uniform(0.0, 0.08, 3))
+        eye = np.array([rng.uniform(-2, 2), -rng.uniform(3.0, 5.0), rng.uniform(0.8, 2.5)])
+        target = np.array([0.0, 0.0, r0])
+        spread, zmin = 2.0, 0.0
+    # objects
+    for i in range(1 if layout == 2 else 0, n_obj):
+        c = np.array([rng.uniform(-spread, spread), rng.uniform(-spread, spread), zmin])
+        size = float(rng.uniform(0.15, 0.5)) * (spread / 2.0 if layout == 0 else 1.0)
+        if rng.uniform() < 0.55:
+            c[2] += size
+            v, ix = uv_sphere_mesh(c, size, 32, 16)
+            prims.append(hpt.Primitive(v, ix, i))
+        else:
+            cv, ci = cube_mesh()
+            ang = rng.uniform(0, math.pi)
+            prims.append(hpt.Primitive(cv, ci, i, trs((c[0], c[1], size), (0.0, 0.0, math.sin(ang / 2), math.cos(ang / 2)),
+                                                   (size, size * rng.uniform(0.5, 1.5), size))))
+    if rng.uniform() < 0.15:  # an emissive object
+        k = int(rng.integers(0, n_obj))
+        mats[k]["emission"] = tuple(rng.uniform(0.5, 1.0, 3) * rng.uniform(1.0, 8.0))
+    if rng.uniform() < (0.5 if layout == 2 else 0.2):  # a participating medium behind an invisible boundary
+        mi = n_obj + 6
+        diffuse_material(mats[mi], (1.0, 1.0, 1.0))
+        mats[mi]["opacity"] = 0.0
+        mats[mi]["medium_type"] = hpt.MEDIUM_SCATTER if rng.uniform() < 0.7 else hpt.MEDIUM_ABSORB
+        mats[mi]["medium_density"] = float(rng.uniform(0.2, 2.0))
+        mats[mi]["medium_color"] = tuple(rng.uniform(0.3, 1.0, 3))
+        mats[mi]["medium_anisotropy"] = float(rng.uniform(-0.3, 0.6))
+        c = np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), 0.0]) * spread * 0.6
+        r = float(rng.uniform(0.3, 0.8)) * (spread / 2.0)
+        c[2] = r + 0.05
+        v, ix = uv_sphere_mesh(c, r, 32, 16)
+        prims.append(hpt.Primitive(v, ix, mi))
+        features = MEDIUM_SCENE_FEATURES
+    if len(lights) == 0 and env is None:
+        lights = _random_lights(rng, np.zeros(3), 2.0, 1)
+    # A narrow field of view in part of the scenes gives the pixel scale of
+    # full-HD frames on these small training images.
+    yfov = float(rng.uniform(30, 60)) if rng.uniform() < 0.6 else float(rng.uniform(5, 15))
+    aperture = float(rng.uniform(0.01, 0.06)) if rng.uniform() < (0.5 if layout == 2 else 0.2) else 0.0
+    cam = look_at_camera(eye, target, yfov_deg=yfov, aperture=aperture,
+                         focal_distance=float(np.linalg.norm(target - eye)))
+    sc = hpt.Scene(cam, lights, mats, prims, env_rgb=env, ground_color=ground, sky_color=sky, textures=textures)
+    return sc, features

@@ -313,6 +313,13 @@ HPT_API hpt_status hpt_get_frame_index(hpt_renderer* r, uint64_t* frame_index);
  * HalaRenderer::save_images, hala-pathtracer.rs:251).  Synchronises. */
 HPT_API hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t bytes);
 
+/* Device address and size of one resident image, for consumers on the same
+ * GPU (the learned denoiser reads colour, albedo and normal in place instead
+ * of the PFM files denoise.py:18-19 hands to OIDN).  No counterpart in the
+ * reference.  The address stays valid until hpt_destroy; work queued on the
+ * handle's stream must have finished (hpt_wait_idle) before it is read. */
+HPT_API hpt_status hpt_get_image_pointer(hpt_renderer* r, hpt_image which, void** ptr, size_t* bytes);
+
 /* Closest-hit query over the committed scene: the semantics of traceRayEXT
  * with gl_RayFlagsOpaqueEXT (pathtracer.rgen.glsl:56-68): triangles are
  * double-sided, analytic quad/sphere lights are hit through

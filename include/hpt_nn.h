@@ -75,10 +75,24 @@ HPT_NN_API int hpt_nn_accumulate(void* stream, float* dx, const float* dy, size_
 
 /* Network input of one image from renderer images (float4 per pixel,
  * src_w x src_h): the h x wd window at (x0, y0), zero outside the source,
- * channels log(1 + colour), albedo, normal * 0.5 + 0.5, zeros up to 16, as
- * bf16 into x [h][wd][16] (one image of a batch). */
+ * channels log(1 + max(colour, 0)), albedo, normal * 0.5 + 0.5, `aux` (a
+ * per-image constant such as the sample count's feature; 0 outside the
+ * source), zeros up to 16, as bf16 into x [h][wd][16] (one image of a
+ * batch).  `mirror` bit 0 flips the window left-right, bit 1 top-bottom
+ * (training augmentation). */
 HPT_NN_API int hpt_nn_make_input(void* stream, const float* color, const float* albedo, const float* normal,
-                                 int src_w, int src_h, int x0, int y0, uint16_t* x, int h, int wd);
+                                 int src_w, int src_h, int x0, int y0, uint16_t* x, int h, int wd, int mirror,
+                                 float aux);
+
+/* Training target of one image: log(1 + max(colour, 0)) of the same window
+ * (and mirror) of a float4 reference image into out float32 [h][wd][3]. */
+HPT_NN_API int hpt_nn_make_target(void* stream, const float* color, int src_w, int src_h, int x0, int y0,
+                                  float* out, int h, int wd, int mirror);
+
+/* out[p][c] += x[p][c] for c < 3: the network input's log colour (bf16,
+ * x_channels per pixel) added to the output, so the network predicts a
+ * correction of its input (no gradient flows to x). */
+HPT_NN_API int hpt_nn_add_residual(void* stream, float* out, const uint16_t* x, size_t pixels, int x_channels);
 
 /* L1 loss on log(1 + colour): out = network output float32 [p][3],
  * target_log = log(1 + reference colour) float32 [p][3]; dy [p][cout_pad]
@@ -91,10 +105,28 @@ HPT_NN_API int hpt_nn_l1_loss(void* stream, const float* out, const float* targe
 HPT_NN_API int hpt_nn_adam(void* stream, float* p, const float* g, float* m, float* v, size_t count, float lr,
                            float beta1, float beta2, float eps, int t);
 
-/* Denoised colour: expm1 of the network output [p][3] written into float4
- * images of width dst_w at (x0, y0) for the h x wd region, alpha 1. */
+/* Denoised colour: max(expm1(network output), 0) of [p][3] written into a
+ * float4 image of dst_w x dst_h at (x0, y0) for the h x wd region (pixels
+ * outside the image skipped), alpha 1. */
 HPT_NN_API int hpt_nn_write_output(void* stream, const float* out, float* dst, int dst_w, int dst_h, int x0,
                                    int y0, int h, int wd);
+
+/* Device memory, copies and timing for the denoiser's host driver, on the
+ * HIP runtime this library links (a process must not mix in a second HIP
+ * runtime, such as the one a PyTorch wheel bundles, for these buffers).
+ * hpt_nn_alloc returns zeroed memory on the current device. */
+HPT_NN_API int hpt_nn_alloc(size_t bytes, void** ptr);
+HPT_NN_API int hpt_nn_free(void* ptr);
+/* hipMemcpyAsync with the direction inferred from the addresses. */
+HPT_NN_API int hpt_nn_copy(void* stream, void* dst, const void* src, size_t bytes);
+/* p[i] = value for count floats. */
+HPT_NN_API int hpt_nn_fill(void* stream, float* p, float value, size_t count);
+HPT_NN_API int hpt_nn_stream_synchronize(void* stream);
+HPT_NN_API int hpt_nn_event_create(void** event);
+HPT_NN_API int hpt_nn_event_destroy(void* event);
+HPT_NN_API int hpt_nn_event_record(void* event, void* stream);
+/* Milliseconds between two recorded events (synchronises on the second). */
+HPT_NN_API int hpt_nn_event_elapsed(void* start, void* stop, float* ms);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 
 LEVELS = (32, 48, 64, 96, 128)
-IN_CHANNELS = 16     # log(1 + colour), albedo, normal * 0.5 + 0.5, zero padding
+IN_CHANNELS = 16     # log(1 + colour), albedo, normal * 0.5 + 0.5, sample-count feature, zeros
 OUT_CHANNELS = 3
 
 
@@ -106,13 +106,14 @@ def upsample_backward(dy):
     return (v[:, :, 0, :, 0] + v[:, :, 0, :, 1]) + (v[:, :, 1, :, 0] + v[:, :, 1, :, 1])
 
 
-def make_input(color, albedo, normal):
+def make_input(color, albedo, normal, aux=0.0):
     """(H, W, 4) float images -> (H, W, 16) network input."""
     h, w = color.shape[:2]
     x = np.zeros((h, w, IN_CHANNELS), np.float32)
     x[..., 0:3] = np.log1p(np.maximum(color[..., :3], 0).astype(np.float32))
     x[..., 3:6] = albedo[..., :3]
     x[..., 6:9] = normal[..., :3] * np.float32(0.5) + np.float32(0.5)
+    x[..., 9] = aux
     return x
 
 
@@ -160,10 +161,11 @@ def init_params(levels=LEVELS, seed=0, dtype=np.float32):
 
 
 class UNet:
-    def __init__(self, params, levels=LEVELS, bf16_rounding=True):
+    def __init__(self, params, levels=LEVELS, bf16_rounding=True, residual=True):
         self.params = params
         self.levels = tuple(levels)
         self.rb = bf16_rounding
+        self.residual = residual
         self.specs = {s[0]: s for s in layer_specs(levels)}
 
     def _conv(self, name, x, cache):
@@ -195,6 +197,8 @@ class UNet:
             cur = self._conv(f"dec{l}a", np.concatenate([u, skips[l]], axis=-1), cache)
             cur = self._conv(f"dec{l}b", cur, cache)
         out = self._conv("out", cur, cache)
+        if self.residual:  # the output corrects the input's log colour
+            out = out + _r(x, self.rb)[..., :OUT_CHANNELS]
         return out, cache
 
     def _conv_back(self, name, dy, cache, grads, need_dx=True):

@@ -145,7 +145,7 @@ def _torch_unet(params, levels, x):
     for l in range(len(levels) - 2, -1, -1):
         u = F.interpolate(cur, scale_factor=2, mode="nearest")
         cur = conv(f"dec{l}b", conv(f"dec{l}a", torch.cat([u, skips[l]], dim=1)))
-    out = conv("out", cur, relu=False)
+    out = conv("out", cur, relu=False) + t[:, :3]
     return out, tp
 
 
