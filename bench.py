@@ -92,6 +92,20 @@ def profiled_traffic(workload, frames_in_flight):
     return None
 
 
+def profiled_conv_traffic(layer):
+    """HBM bytes per launch of one denoiser convolution at 1080p from the
+    committed rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated);
+    None when absent."""
+    path = os.path.join(ROOT, "profiles", "k_conv_denoise_1080p.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"traffic_bytes_per_launch k_conv (\w+) = (\d+)", line)
+        if m and m.group(1) == layer:
+            return int(m.group(2))
+    return None
+
+
 def copy_calibration(mib=1024, reps=4):
     """Plain device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -304,7 +318,9 @@ def denoise_bench(args):
                    "train_tflops": round(train_flops / t_train / 1e12, 2) if t_train else None,
                    "layer_ms": {k: round(v[0], 4) for k, v in prof.items()}},
         "roofline": {"bound": "mfma", "achieved": round(fl / ms / 1e9, 2), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(fl / ms / 1e9 / 2500.0, 4), "traffic": None, "kernel": f"k_conv ({name})",
+                     "frac": round(fl / ms / 1e9 / 2500.0, 4),
+                     "traffic": profiled_conv_traffic(name) if (W, H) == (1920, 1080) else None,
+                     "kernel": f"k_conv ({name})",
                      "avg_launch_ms": round(ms, 4)},
         "cpu_baseline": base,
     }

@@ -30,6 +30,16 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 constexpr int kWgradChunk = 1024;  // pixels per weight-gradient partial sum (a multiple of 32)
+constexpr unsigned kXcds = 8;      // MI355X: workgroups are dealt round-robin over 8 XCDs, each with its own L2
+
+// Block b runs on XCD b % 8; this maps it to tile (b % 8) * (grid / 8) + b / 8
+// so that each XCD walks a contiguous range of pixel tiles and a tile's
+// neighbouring image rows (the 3x3 halo) are in the same L2.  The grid is a
+// multiple of 8 (xcd_grid), so the map is a bijection.
+__device__ __forceinline__ unsigned xcd_block() {
+  return (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+}
+inline unsigned xcd_grid(unsigned blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -63,8 +73,9 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
                                                    float* __restrict__ yf, int P, int h, int w, int cin, int nstore,
                                                    int relu) {
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-  const int p0 = (blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * (16 * MT);
-  if (p0 >= P) return;  // uniform over the wave; no barriers below
+  const long long p0l = ((long long)xcd_block() * kWavesPerBlock + (threadIdx.x >> 6)) * (16 * MT);
+  if (p0l >= P) return;  // uniform over the wave; no barriers below
+  const int p0 = (int)p0l;
   const int n0 = blockIdx.y * (16 * NT);
   const int C8 = cin >> 3, S = KS * KS * C8, K = KS * KS * cin;
 
@@ -152,7 +163,7 @@ template <typename InT, int KS, int MT, int NT, bool FORWARD>
 int go_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P, int h,
             int wd, int cin, int nstore, int relu, int groups) {
   const int tiles = (P + 16 * MT - 1) / (16 * MT);
-  const dim3 grid((tiles + kWavesPerBlock - 1) / kWavesPerBlock, groups);
+  const dim3 grid(xcd_grid((tiles + kWavesPerBlock - 1) / kWavesPerBlock), groups);
   hipLaunchKernelGGL((k_conv<InT, KS, MT, NT, FORWARD>), grid, dim3(kThreads), 0, st, in, w, b, y, yf, P, h, wd, cin,
                      nstore, relu);
   return (int)hipGetLastError();
