@@ -355,7 +355,14 @@ def serving_bench(args):
     jobs = serve.poisson_jobs(args.jobs, args.rate, sorted(resident), seed=0)
     offered = sum(j.width * j.height * j.spp for j in jobs)
     log(f"{len(jobs)} jobs, {offered / 1e9:.2f} Gsamples offered over {jobs[-1].arrival:.2f} s of arrivals")
-    srv = serve.Server(resident, int(args.budget_gib * 2**30), max_frames=args.job_frames)
+    tuning, t_tune = None, 0.0
+    if args.tune:
+        t0 = time.perf_counter()
+        tuning = serve.tune_classes(resident, jobs, max_frames=args.job_frames, log=log)
+        t_tune = time.perf_counter() - t0
+        log(f"autotuned {len(tuning.entries)} (scene, resolution) classes in {t_tune:.1f} s")
+    srv = serve.Server(resident, int(args.budget_gib * 2**30), max_frames=args.job_frames, tuning=tuning,
+                       launch=hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
@@ -371,6 +378,9 @@ def serving_bench(args):
         "config": {"workload": "serving", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
                    "resident_scenes": {k: v.scene.triangle_count for k, v in resident.items()},
                    "budget_gib": args.budget_gib, "job_frames_in_flight": args.job_frames,
+                   "autotuned": {k: [v.frames_in_flight, "graph" if v.launch == hpt.LAUNCH_GRAPH else "eager"]
+                                 for k, v in tuning.entries.items()} if tuning else None,
+                   "autotune_seconds": round(t_tune, 2), "launch": args.launch,
                    "offered_gsamples": round(offered / 1e9, 3),
                    "arrival_span_s": round(jobs[-1].arrival, 3)},
         "serving": summ, "roofline": None, "cpu_baseline": None,
@@ -391,11 +401,15 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=192.0, help="serving: device memory for admitted jobs")
     ap.add_argument("--job-frames", type=int, default=16, help="serving: frames in flight per job (at most)")
+    ap.add_argument("--tune", action="store_true",
+                    help="serving: autotune frames in flight and launch mode per (scene, resolution) first")
     ap.add_argument("--steps", type=int, default=None, help="frames (default: the workload's spp)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
+                    help="full batches replayed as captured HIP graphs (default) or every kernel launched")
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced together per batch (0: sized by the library)")
     ap.add_argument("--relmse", action="store_true",
@@ -433,6 +447,7 @@ def main():
         rr.set_scene(scene)
         rr.frames_in_flight = args.frames_in_flight
         rr.commit()
+        rr.set_launch_mode(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
         return rr
 
     t0 = time.perf_counter()
@@ -449,19 +464,31 @@ def main():
     tris_per_ray = ext["tri_tests"] / max(1, ext["rays"])
     log(f"warm-up: extend {ext['rays']} rays, {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray")
 
-    # Timed region: `steps` frames from a fresh accumulation.
+    # Timed region: `steps` frames from a fresh accumulation, submitted the way
+    # the renderer submits them (full batches as graph replays by default).
+    r.reset_accumulation()
+    r.set_profiling(0)
+    r.wait_idle()
+    t0 = time.perf_counter()
+    r.render(steps)
+    r.wait_idle()
+    wall = time.perf_counter() - t0
+    img = r.read_image(hpt.IMAGE_ACCUM)
+    assert np.isfinite(img).all()
+    graph_launches = r.launch_stats()[0]
+    # Kernel timings: the same frames again with every stage bracketed by HIP
+    # events on the renderer's stream (eager launches: an event cannot sit
+    # inside a replayed graph).  Same kernels, same work.
     r.reset_accumulation()
     r.set_profiling(hpt.PROFILE_TIMING)
     r.wait_idle()
     t0 = time.perf_counter()
     r.render(steps)
     r.wait_idle()
-    wall = time.perf_counter() - t0
+    profiled_wall = time.perf_counter() - t0
     stats = {s: r.stage_stats(s) for s in range(5)}
     ext = stats[hpt.STAGE_EXTEND]
     shadow = stats[hpt.STAGE_SHADOW]
-    img = r.read_image(hpt.IMAGE_ACCUM)
-    assert np.isfinite(img).all()
     samples = W * H * steps
     msps = samples / wall / 1e6
     ext_ms = ext["total_ms"] / max(1, ext["launches"])
@@ -471,7 +498,8 @@ def main():
     rays_total = ext["rays"] + shadow["rays"]
     log("stage totals: " + ", ".join(f"{n} {stats[s]['total_ms']:.1f} ms/{stats[s]['launches']}"
                                        for s, n in enumerate(["camera", "extend", "shade", "shadow", "accum"])))
-    log(f"wall {wall:.3f} s, {msps:.1f} Msamples/s, {rays_total / wall / 1e6:.1f} Mrays/s")
+    log(f"wall {wall:.3f} s ({args.launch}, {graph_launches} graph replays), {msps:.1f} Msamples/s, "
+        f"{rays_total / wall / 1e6:.1f} Mrays/s; eager with stage events {profiled_wall:.3f} s")
     r.close()
 
     conv = None
@@ -492,7 +520,8 @@ def main():
     base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES)
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
               "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,
-              "lights": int(len(scene.lights)), "frames_in_flight": fif,
+              "lights": int(len(scene.lights)), "frames_in_flight": fif, "launch": args.launch,
+              "eager_profiled_wall_s": round(profiled_wall, 4),
               "mrays_per_s": round(rays_total / wall / 1e6, 2)}
     if scene.env_rgb is not None:
         config["env_map"] = [int(scene.env_rgb.shape[1]), int(scene.env_rgb.shape[0])]

@@ -1,0 +1,120 @@
+"""Autotuner of the tracer's launch parameters (no counterpart in the
+reference, whose frame loop has no tunables).
+
+What it tunes: the frames traced together per batch (the wavefront and
+queue size, `hpt_set_frames_in_flight`) and the launch mode (graph replay or
+eager launches) for one (scene, resolution, depth) class.  Larger batches
+fill the GPU with more paths per launch and amortise the short tail bounces;
+they cost k * W * H * 288 bytes of device memory, which the serving loop
+spends from its admission budget.  The tuner measures, it does not model:
+each candidate renders a warm-up batch and then a timed run on the job's own
+scene, and the smallest batch within `tolerance` of the best throughput
+wins (less memory and a finer progress granularity for the same speed).
+
+Results are kept in a TuningCache keyed by (scene name, width, height,
+max_depth) and can be saved as JSON.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import dataclass, field
+
+import hpt
+
+PATH_BYTES = 288
+CANDIDATES = (2, 4, 8, 16, 32, 64)
+
+
+@dataclass
+class Tuning:
+    frames_in_flight: int
+    launch: int
+    msamples_per_s: float
+    table: dict = field(default_factory=dict)  # "k/mode" -> Msamples/s
+    seconds: float = 0.0                       # time the tuning took
+
+
+def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: int | None = None,
+                   candidates=CANDIDATES) -> list:
+    out = []
+    for k in candidates:
+        if k > max_frames:
+            continue
+        if memory_bytes is not None and k * width * height * PATH_BYTES > memory_bytes:
+            continue
+        if k * width * height * 2 > 0xFFFFFFFF:  # 32-bit queue indices
+            continue
+        out.append(k)
+    return out or [1]
+
+
+def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, rr_depth: int = 3, features: int = 0,
+         samples: float = 32e6, max_frames: int = 64, memory_bytes: int | None = None, tolerance: float = 0.03,
+         modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), device: int = 0) -> Tuning:
+    """Measures every candidate batch size (and launch mode) on a handle that
+    shares `template`'s committed scene, each over about `samples` paths
+    (whole batches); returns the choice."""
+    t_start = time.perf_counter()
+    r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=rr_depth, features=features, device=device)
+    table = {}
+    try:
+        r.share_scene(template)
+
+        def measure(k, mode):
+            r.frames_in_flight = k
+            r.set_launch_mode(mode)
+            n = max(1, -(-int(samples) // (width * height * k))) * k
+            r.reset_accumulation()
+            r.render(k)  # warm-up: allocation and graph capture
+            r.wait_idle()
+            t0 = time.perf_counter()
+            r.render(n)
+            r.wait_idle()
+            table[f"{k}/{'graph' if mode == hpt.LAUNCH_GRAPH else 'eager'}"] = \
+                width * height * n / (time.perf_counter() - t0) / 1e6
+
+        # Batch sizes with the first mode, then the other modes at the best size.
+        for k in candidates_for(width, height, max_frames, memory_bytes):
+            measure(k, modes[0])
+        k_best = int(max(table, key=table.get).split("/")[0])
+        for mode in modes[1:]:
+            measure(k_best, mode)
+    finally:
+        r.close()
+    best = max(table.values())
+    ok = [(int(key.split("/")[0]), key.split("/")[1], v) for key, v in table.items() if v >= (1 - tolerance) * best]
+    k, mode, v = min(ok, key=lambda t: (t[0], t[1] != "graph"))
+    return Tuning(k, hpt.LAUNCH_GRAPH if mode == "graph" else hpt.LAUNCH_EAGER, v, table,
+                  time.perf_counter() - t_start)
+
+
+class TuningCache:
+    """(scene name, width, height, max_depth) -> Tuning."""
+
+    def __init__(self):
+        self.entries = {}
+
+    @staticmethod
+    def key(scene: str, width: int, height: int, max_depth: int) -> str:
+        return f"{scene}|{width}x{height}|d{max_depth}"
+
+    def get(self, scene: str, width: int, height: int, max_depth: int):
+        return self.entries.get(self.key(scene, width, height, max_depth))
+
+    def put(self, scene: str, width: int, height: int, max_depth: int, t: Tuning) -> None:
+        self.entries[self.key(scene, width, height, max_depth)] = t
+
+    def save(self, path: str) -> None:
+        with open(path, "w") as f:
+            json.dump({k: vars(v) for k, v in self.entries.items()}, f, indent=1)
+
+    @classmethod
+    def load(cls, path: str) -> "TuningCache":
+        c = cls()
+        if os.path.exists(path):
+            with open(path) as f:
+                for k, v in json.load(f).items():
+                    c.entries[k] = Tuning(**v)
+        return c

@@ -120,6 +120,15 @@ struct hpt_renderer {
   uint32_t batch_cap = 0;     // frames the path buffers hold
   uint32_t frames_in_flight = 0;  // requested batch size, 0 = sized from kPathBudget
   uint64_t frame_index = 0;
+  // Graph replay of whole batches (HPT_LAUNCH_GRAPH): the instantiated batch,
+  // the launch parameters it was captured with, and the device frame counter
+  // it reads and advances.
+  uint32_t launch_mode = HPT_LAUNCH_GRAPH;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey;
+  DevBuf d_frame;
+  bool frame_synced = false;  // d_frame holds frame_index
+  uint64_t graph_launches = 0, graph_captures = 0;
   // Profiling.
   int profiling = 0;
   std::vector<StageTimer> timers;
@@ -220,7 +229,7 @@ void default_blue_noise(std::vector<uchar4>& bn) {
 }
 
 SceneView scene_view(const hpt_renderer* r) {
-  SceneView sv;
+  SceneView sv{};  // zeroed padding: the bytes are compared as a graph key
   sv.nodes = r->sd->d_nodes.as<Node8>();
   sv.ltris = r->sd->d_ltris.as<LeafTri>();
   sv.tri_shade = r->sd->d_tri_shade.as<TriShade>();
@@ -242,7 +251,7 @@ SceneView scene_view(const hpt_renderer* r) {
 }
 
 PathState path_state(const hpt_renderer* r) {
-  PathState ps;
+  PathState ps{};
   ps.rng = r->d_rng.as<uint4>();
   ps.org = r->d_org.as<float4>();
   ps.dir = r->d_dir.as<float4>();
@@ -334,12 +343,17 @@ FrameParams frame_params(const hpt_renderer* r, uint32_t camera_index) {
   return fp;
 }
 
-// Traces `frames` consecutive frames (starting at r->frame_index) as one
-// wavefront of frames * width * height paths.
-void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
+// Traces `frames` consecutive frames (starting at r->frame_index, or at the
+// device counter `frame_ptr` when given) as one wavefront of
+// frames * width * height paths.
+void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const uint32_t* frame_ptr = nullptr) {
   const SceneView sv = scene_view(r);
   FrameParams fp = frame_params(r, camera_index);
   fp.batch_frames = frames;
+  if (frame_ptr) {
+    fp.frame_index = 0;
+    fp.frame_ptr = frame_ptr;
+  }
   const PathState ps = path_state(r);
   const uint32_t n = frames * r->npix;
   const SegGeom g = seg_geom(n);
@@ -402,6 +416,72 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
   launch_accumulate(s, fp, ps, im, r->npix);
   stage_end(r, t);
   check(hipGetLastError(), "accumulate launch");
+}
+
+template <typename T>
+void key_append(std::vector<uint8_t>& k, const T& v) {
+  const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+// Everything a captured batch bakes into its kernel arguments: the scene,
+// frame parameters (but the first frame, which the device counter supplies),
+// path state and queue buffers, batch size and permutation.
+std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
+  std::vector<uint8_t> k;
+  key_append(k, scene_view(r));
+  FrameParams fp = frame_params(r, camera_index);
+  fp.frame_index = 0;
+  key_append(k, fp);
+  key_append(k, path_state(r));
+  for (const DevBuf* b : {&r->d_qa, &r->d_qb, &r->d_sh_o, &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill,
+                          &r->d_accum, &r->d_albedo, &r->d_normal, &r->d_final, &r->d_frame})
+    key_append(k, b->p);
+  key_append(k, frames);
+  key_append(k, r->info.features);
+  return k;
+}
+
+// One full batch as a graph replay: captured on first use (and again when
+// any captured parameter changed), then launched with one call.  The graph
+// ends by advancing the device frame counter, so replays need no updates.
+void render_batch_graph(hpt_renderer* r, uint32_t camera_index, uint32_t frames) {
+  hipStream_t s = r->stream;
+  if (!r->d_frame.p) r->d_frame.alloc(256);
+  const uint32_t n = frames * r->npix;
+  (void)ensure_spill(r, 2 * kSeg * seg_geom(n).seg_cap);  // any allocation happens before capture
+  std::vector<uint8_t> key = graph_key(r, camera_index, frames);
+  if (!r->gexec || key != r->gkey) {
+    if (r->gexec) {
+      check(hipStreamSynchronize(s), "graph");
+      check(hipGraphExecDestroy(r->gexec), "hipGraphExecDestroy");
+      r->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      render_batch(r, camera_index, frames, r->d_frame.as<uint32_t>());
+      launch_frame_counter(s, r->d_frame.as<uint32_t>(), frames, true);
+      check(hipGetLastError(), "frame counter launch");
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&r->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check(e, "hipGraphInstantiate");
+    r->gkey.swap(key);
+    r->graph_captures++;
+  }
+  if (!r->frame_synced) {
+    launch_frame_counter(s, r->d_frame.as<uint32_t>(), (uint32_t)r->frame_index, false);
+    check(hipGetLastError(), "frame counter launch");
+    r->frame_synced = true;
+  }
+  check(hipGraphLaunch(r->gexec, s), "hipGraphLaunch");
+  r->graph_launches++;
 }
 
 // Device memory a batch may take for path state and queues; 1080p fits 32
@@ -846,9 +926,16 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
     if (r->info.max_samples)
       left = r->frame_index >= r->info.max_samples ? 0 : std::min<uint64_t>(left, r->info.max_samples - r->frame_index);
     left = std::min<uint64_t>(left, 0xFFFFFFFFull - r->frame_index);  // frame indices stay 32-bit
+    const bool graph = r->launch_mode == HPT_LAUNCH_GRAPH && !r->profiling &&
+                       !(r->info.features & HPT_FEATURE_TRANSPARENT);
     while (left) {
       const uint32_t k = (uint32_t)std::min<uint64_t>(left, cap);
-      render_batch(r, camera_index, k);
+      if (graph && k == cap) {
+        render_batch_graph(r, camera_index, k);
+      } else {
+        render_batch(r, camera_index, k);
+        r->frame_synced = false;
+      }
       r->frame_index += k;
       left -= k;
     }
@@ -856,10 +943,27 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
   });
 }
 
+hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(mode == HPT_LAUNCH_EAGER || mode == HPT_LAUNCH_GRAPH, "unknown launch mode");
+    r->launch_mode = mode;
+  });
+}
+
+hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures) {
+  return guarded([&] {
+    need(r && graph_launches && graph_captures, "null argument");
+    *graph_launches = r->graph_launches;
+    *graph_captures = r->graph_captures;
+  });
+}
+
 hpt_status hpt_reset_accumulation(hpt_renderer* r) {
   return guarded([&] {
     need(r, "null renderer");
     r->frame_index = 0;
+    r->frame_synced = false;
     if (r->committed) {
       set_device(r);
       check(hipMemsetAsync(r->d_accum.p, 0, r->d_accum.bytes, r->stream), "hipMemsetAsync");
@@ -875,6 +979,7 @@ hpt_status hpt_set_frame_index(hpt_renderer* r, uint64_t frame_index) {
     need(r, "null renderer");
     need(frame_index <= 0xFFFFFFFFull, "frame index must fit the 32-bit UBO field");
     r->frame_index = frame_index;
+    r->frame_synced = false;
   });
 }
 
@@ -1004,6 +1109,7 @@ void hpt_destroy(hpt_renderer* r) {
   if (!r) return;
   (void)hipSetDevice(r->info.device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
   for (auto& t : r->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);

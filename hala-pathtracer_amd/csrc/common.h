@@ -195,6 +195,9 @@ struct FrameParams {
   uint32_t enable_tonemap, enable_aces, use_simple_aces;
   uint32_t num_lights;
   float tan_half_fov;  // tan(camera.yfov * 0.5), evaluated once per frame on the host
+  // When non-null the batch's first frame is read from this device counter
+  // instead of frame_index (a captured batch replayed as a graph advances it).
+  const uint32_t* frame_ptr;
 };
 
 }  // namespace hpt

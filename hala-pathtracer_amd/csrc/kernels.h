@@ -107,6 +107,8 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
                   Spill sp);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc);
+// counter[0] = value, or counter[0] += value when `add` (one thread).
+void launch_frame_counter(hipStream_t s, uint32_t* counter, uint32_t value, bool add);
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im,
                        uint32_t npaths);
 

@@ -66,6 +66,12 @@ __device__ __forceinline__ SegSlot seg_slot(const uint32_t* counts) {
   return t;
 }
 
+// First frame of the batch: the device counter of a graph replay, else the
+// value passed at launch.
+__device__ __forceinline__ uint32_t first_frame(const FrameParams& fp) {
+  return fp.frame_ptr ? *fp.frame_ptr : fp.frame_index;
+}
+
 // ------------------------------------------------------------------ camera
 // pathtracer.rgen.glsl:23-54: seed, camera callable, payload initialisation.
 // A batch holds batch_frames consecutive frames, one path per pixel each.
@@ -74,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp,
   if (p >= npaths) return;
   const uint32_t j = p / fp.pixels, pix = p - j * fp.pixels;
   const uint32_t x = pix % fp.width, y = pix / fp.width;
-  const uint32_t frame = fp.frame_index + j;
+  const uint32_t frame = first_frame(fp) + j;
   f3 o, d;
   camera_ray(sv, fp, frame, x, y, o, d);
   ps.rng[p] = make_uint4(x, y, frame, x + y);
@@ -733,7 +739,8 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= npix) return;
   f3 acc_c = mk3(0.0f), acc_a = mk3(0.0f), acc_n = mk3(0.0f);
-  if (fp.frame_index != 0) {
+  const uint32_t frame0 = first_frame(fp);
+  if (frame0 != 0) {
     const float4 c4 = im.accum[p], a4 = im.albedo[p], n4 = im.normal[p];
     acc_c = mk3(c4.x, c4.y, c4.z);
     acc_a = mk3(a4.x, a4.y, a4.z);
@@ -741,7 +748,7 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(FrameParams fp, PathState
   }
   for (uint32_t j = 0; j < fp.batch_frames; ++j) {
     const uint32_t q = j * npix + p;
-    const uint32_t frame = fp.frame_index + j;
+    const uint32_t frame = frame0 + j;
     const float4 r = ps.rad[q], e = ps.nee_env[q], l = ps.nee_light[q];
     const f3 L = mk3(r.x, r.y, r.z) + mk3(e.x, e.y, e.z) + mk3(l.x, l.y, l.z);
     const float4 a4 = ps.albedo[q], n4 = ps.normal[q];
@@ -903,6 +910,16 @@ void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, ui
 
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im, uint32_t npaths) {
   hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npaths)), dim3(kBlock), 0, s, fp, ps, im, npaths);
+}
+
+// The device frame counter of graph replays: set from the host's value, or
+// advanced by the batch size at the end of a captured batch.
+__global__ void k_frame_counter(uint32_t* counter, uint32_t value, int add) {
+  if (threadIdx.x == 0) counter[0] = add ? counter[0] + value : value;
+}
+
+void launch_frame_counter(hipStream_t s, uint32_t* counter, uint32_t value, bool add) {
+  hipLaunchKernelGGL(k_frame_counter, dim3(1), dim3(64), 0, s, counter, value, add ? 1 : 0);
 }
 
 void launch_trace_rays(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, int any_hit, float4* out,

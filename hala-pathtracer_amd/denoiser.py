@@ -34,7 +34,7 @@ LEVELS = (32, 48, 64, 96, 128)
 IN_CHANNELS = 16
 OUT_CHANNELS = 3
 OUT_PAD = 16
-STREAM = None  # the null stream: ordered after the renderers' blocking streams
+STREAM = None  # the null stream (renderer streams are non-blocking: wait for a renderer before reading its images)
 
 _vp, _i, _sz, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 # Entry points of include/hpt_nn.h with their argument types.

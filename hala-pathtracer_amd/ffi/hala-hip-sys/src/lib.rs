@@ -155,6 +155,8 @@ pub const HPT_IMAGE_FINAL_RGBA8: c_int = 0;
 pub const HPT_IMAGE_ACCUM_RGBA32F: c_int = 1;
 pub const HPT_IMAGE_ALBEDO_RGBA32F: c_int = 2;
 pub const HPT_IMAGE_NORMAL_RGBA32F: c_int = 3;
+pub const HPT_LAUNCH_EAGER: u32 = 0;
+pub const HPT_LAUNCH_GRAPH: u32 = 1;
 
 extern "C" {
     pub fn hpt_create(info: *const hpt_create_info, out: *mut *mut hpt_renderer) -> hpt_status;
@@ -173,6 +175,9 @@ extern "C" {
     pub fn hpt_set_frame_index(r: *mut hpt_renderer, frame_index: u64) -> hpt_status;
     pub fn hpt_set_frames_in_flight(r: *mut hpt_renderer, frames: u32) -> hpt_status;
     pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
+    pub fn hpt_set_launch_mode(r: *mut hpt_renderer, mode: u32) -> hpt_status;
+    pub fn hpt_get_image_pointer(r: *mut hpt_renderer, which: c_int, ptr: *mut *mut c_void, bytes: *mut usize)
+        -> hpt_status;
     pub fn hpt_read_image(r: *mut hpt_renderer, which: c_int, dst: *mut c_void, bytes: usize) -> hpt_status;
     pub fn hpt_wait_idle(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_destroy(r: *mut hpt_renderer);

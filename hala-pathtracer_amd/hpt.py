@@ -35,6 +35,7 @@ IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
 HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
 STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE = range(5)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
+LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 
 STATUS_NAMES = {
     0: "HPT_OK",
@@ -175,6 +176,8 @@ SIGNATURES = {
     "hpt_get_memory_usage": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
     "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
+    "hpt_set_launch_mode": [C.c_void_p, C.c_uint32],
+    "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
 }
 
 _lib = None
@@ -466,6 +469,17 @@ class HalaRenderer:
     @frames_in_flight.setter
     def frames_in_flight(self, k: int) -> None:
         _check(self._lib.hpt_set_frames_in_flight(self._h, k))
+
+    def set_launch_mode(self, mode: int) -> None:
+        """LAUNCH_GRAPH (the default: full batches replayed as captured HIP
+        graphs) or LAUNCH_EAGER (every kernel launched); same images."""
+        _check(self._lib.hpt_set_launch_mode(self._h, mode))
+
+    def launch_stats(self) -> tuple:
+        """(batches replayed as graphs, graphs captured)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(self._lib.hpt_get_launch_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_profiling(self, flags: int) -> None:
         _check(self._lib.hpt_set_profiling(self._h, flags))

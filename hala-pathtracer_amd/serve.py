@@ -63,13 +63,29 @@ class JobResult:
         return self.finished - self.spec.arrival
 
 
-def frames_in_flight(spec: JobSpec, max_frames: int = 32) -> int:
-    return max(1, min(max_frames, spec.spp))
+def frames_in_flight(spec: JobSpec, max_frames: int = 32, tuning=None) -> int:
+    """The job's batch size: the autotuned one for its (scene, resolution)
+    class when `tuning` (an autotune.TuningCache) has it, else max_frames;
+    never more than the job's samples."""
+    k = max_frames
+    if tuning is not None:
+        t = tuning.get(spec.scene, spec.width, spec.height, 0)
+        if t is not None:
+            k = t.frames_in_flight
+    return max(1, min(k, spec.spp))
 
 
-def footprint(spec: JobSpec, max_frames: int = 32) -> int:
+def launch_mode(spec: JobSpec, tuning=None, default: int = hpt.LAUNCH_GRAPH) -> int:
+    if tuning is not None:
+        t = tuning.get(spec.scene, spec.width, spec.height, 0)
+        if t is not None:
+            return t.launch
+    return default
+
+
+def footprint(spec: JobSpec, max_frames: int = 32, tuning=None) -> int:
     px = spec.width * spec.height
-    return px * (frames_in_flight(spec, max_frames) * PATH_BYTES + IMAGE_BYTES)
+    return px * (frames_in_flight(spec, max_frames, tuning) * PATH_BYTES + IMAGE_BYTES)
 
 
 class ResidentScene:
@@ -93,10 +109,12 @@ class ResidentScene:
 
 class Server:
     def __init__(self, scenes: dict, budget_bytes: int, max_frames: int = 32, keep_images: bool = False,
-                 device: int = 0):
+                 device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH):
         self.scenes = scenes                  # name -> ResidentScene
         self.budget = budget_bytes
         self.max_frames = max_frames
+        self.tuning = tuning                  # autotune.TuningCache keyed by (scene, width, height, 0)
+        self.launch = launch                  # launch mode of jobs the tuning does not cover
         self.keep_images = keep_images
         self.device = device
         self._lock = threading.Condition()
@@ -106,15 +124,16 @@ class Server:
         rs = self.scenes[spec.scene]
         r = hpt.HalaRenderer(spec.width, spec.height, max_depth=spec.max_depth, rr_depth=spec.rr_depth,
                              enable_tonemap=True, features=rs.features, device=self.device)
-        r.frames_in_flight = frames_in_flight(spec, self.max_frames)
+        r.frames_in_flight = frames_in_flight(spec, self.max_frames, self.tuning)
         r.share_scene(rs.template)
+        r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
         return r
 
     def _run_job(self, spec: JobSpec, t0: float, admitted: float, results: list, fp: int):
         res = JobResult(spec.job_id, spec, admitted, 0.0)
         try:
             r = self._make_job(spec)
-            k = frames_in_flight(spec, self.max_frames)
+            k = frames_in_flight(spec, self.max_frames, self.tuning)
             done = 0
             while done < spec.spp:
                 n = min(k, spec.spp - done)
@@ -139,15 +158,15 @@ class Server:
         in completion order."""
         specs = sorted(specs, key=lambda s: s.arrival)
         for s in specs:
-            if footprint(s, self.max_frames) > self.budget:
-                raise ValueError(f"job {s.job_id} needs {footprint(s, self.max_frames)} B, above the budget")
+            if footprint(s, self.max_frames, self.tuning) > self.budget:
+                raise ValueError(f"job {s.job_id} needs {footprint(s, self.max_frames, self.tuning)} B, above the budget")
         results, threads = [], []
         t0 = time.perf_counter()
         for spec in specs:
             wait = spec.arrival - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
-            fp = footprint(spec, self.max_frames)
+            fp = footprint(spec, self.max_frames, self.tuning)
             with self._lock:
                 while self._used + fp > self.budget:
                     self._lock.wait()
@@ -159,6 +178,27 @@ class Server:
         for th in threads:
             th.join()
         return results
+
+
+def tune_classes(scenes: dict, specs: list, max_frames: int = 16, tolerance: float = 0.05, log=None):
+    """Autotunes every (scene, resolution) class of `specs` at depth 10 on
+    the resident scenes; returns the TuningCache the Server takes.  For
+    serving, the batch is capped (max_frames) and the smallest one within
+    `tolerance` of the best single-job rate wins: a job's memory is what
+    bounds how many jobs run together, and on a shared GPU concurrency, not
+    one job's batch, keeps the device full."""
+    import autotune
+
+    cache = autotune.TuningCache()
+    for scene, w, h in sorted({(s.scene, s.width, s.height) for s in specs}):
+        rs = scenes[scene]
+        t = autotune.tune(rs.template, w, h, 10, features=rs.features, max_frames=max_frames, tolerance=tolerance)
+        cache.put(scene, w, h, 0, t)
+        if log:
+            log(f"tuned {scene} {w}x{h}: {t.frames_in_flight} frames, "
+                f"{'graph' if t.launch == hpt.LAUNCH_GRAPH else 'eager'}, {t.msamples_per_s:.0f} Msamples/s "
+                f"({t.seconds:.2f} s; " + ", ".join(f"{k} {v:.0f}" for k, v in t.table.items()) + ")")
+    return cache
 
 
 def poisson_jobs(n: int, rate: float, scene_names: list, seed: int = 0, resolutions=None, spp_range=(16, 512),

@@ -301,6 +301,21 @@ HPT_API hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_
  * scene changes); the accumulated images are cleared to zero. */
 HPT_API hpt_status hpt_reset_accumulation(hpt_renderer* r);
 
+/* How hpt_render submits a batch (no counterpart in the reference, which
+ * records one command buffer per frame).  HPT_LAUNCH_GRAPH, the default,
+ * captures a whole batch -- camera, the bounce loop, accumulation -- as a HIP
+ * graph on first use and replays it with one launch for every later full
+ * batch with the same parameters (a device frame counter that the graph
+ * advances supplies the frame index); a change of scene, camera, settings,
+ * batch size or buffers captures it again.  HPT_LAUNCH_EAGER launches each
+ * kernel.  Both give the same images bit for bit.  Partial batches, profiled
+ * renders and the TRANSPARENT permutation (whose bounce count depends on a
+ * host read-back) always launch eagerly. */
+enum { HPT_LAUNCH_EAGER = 0, HPT_LAUNCH_GRAPH = 1 };
+HPT_API hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode);
+/* Batches replayed as graphs and graphs captured since the handle was made. */
+HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
+
 /* Sets the frame counter (the UBO frame_index) the next frame renders with.
  * The RNG is a pure function of (pixel, frame) (common/wltcRS.glsl:9-19), so
  * a job resumes from its accumulated images plus this counter. */

@@ -133,3 +133,40 @@ def test_results_identical_alone_and_in_the_mix():
     j.close()
     for rs in scenes.values():
         rs.close()
+
+
+def test_autotune_candidates_and_cache(tmp_path):
+    import autotune
+
+    # memory and 32-bit queue limits prune the candidates
+    assert autotune.candidates_for(1920, 1080, max_frames=64) == [2, 4, 8, 16, 32, 64]
+    assert autotune.candidates_for(1920, 1080, max_frames=16) == [2, 4, 8, 16]
+    assert autotune.candidates_for(1920, 1080, memory_bytes=8 * 1920 * 1080 * 288) == [2, 4, 8]
+    assert autotune.candidates_for(65536, 65536) == [1]
+    c = autotune.TuningCache()
+    c.put("cornell", 256, 256, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 900.0, {"32/graph": 900.0}, 0.5))
+    path = str(tmp_path / "tuning.json")
+    c.save(path)
+    d = autotune.TuningCache.load(path)
+    assert d.get("cornell", 256, 256, 0).frames_in_flight == 32
+    # the server takes the tuned batch size (never above the job's samples) and its footprint
+    spec = serve.JobSpec(0, "cornell", 256, 256, 100, 8)
+    assert serve.frames_in_flight(spec, 16, d) == 32
+    assert serve.frames_in_flight(serve.JobSpec(1, "cornell", 256, 256, 10, 8), 16, d) == 10
+    assert serve.frames_in_flight(serve.JobSpec(2, "other", 256, 256, 100, 8), 16, d) == 16
+    assert serve.footprint(spec, 16, d) == 256 * 256 * (32 * serve.PATH_BYTES + serve.IMAGE_BYTES)
+
+
+@pytest.mark.gpu
+def test_autotune_picks_a_measured_candidate():
+    import autotune
+
+    rs = serve.ResidentScene("cornell", hpt_scenes.cornell_box(), 0)
+    try:
+        t = autotune.tune(rs.template, 128, 128, 6, max_frames=16, samples=2e6)
+    finally:
+        rs.close()
+    assert set(t.table) >= {"2/graph", "4/graph", "8/graph", "16/graph"}
+    best = max(t.table.values())
+    assert t.msamples_per_s >= 0.97 * best - 1e-9
+    assert t.frames_in_flight in (2, 4, 8, 16) and t.launch in (hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER)
