@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "bvh_build.h"
+#include "bvh_gpu.h"
 #include "common.h"
 #include "kernels.h"
 
@@ -62,6 +64,12 @@ struct DevBuf {
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+  // Takes ownership of device memory from hipMalloc.
+  void adopt(void* q, size_t n) {
+    release();
+    p = q;
+    bytes = n;
+  }
 };
 
 template <typename T>
@@ -124,6 +132,9 @@ struct hpt_renderer {
   // the launch parameters it was captured with, and the device frame counter
   // it reads and advances.
   uint32_t launch_mode = HPT_LAUNCH_GRAPH;
+  // BVH construction at commit: the requested builder, the one that ran and its time.
+  uint32_t bvh_builder = HPT_BVH_GPU, bvh_builder_used = HPT_BVH_HOST;
+  double bvh_build_ms = 0.0;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint8_t> gkey;
   DevBuf d_frame;
@@ -805,16 +816,43 @@ hpt_status hpt_commit(hpt_renderer* r) {
     need(r, "null renderer");
     need(r->scene_set, "hpt_set_scene has not been called", HPT_ERROR_STATE);
     set_device(r);
-    BvhBuildResult bvh;
-    std::string err;
-    need(build_bvh8(r->tri_pos.data(), (uint32_t)(r->tri_pos.size() / 9), bvh, err), err);
-    r->sd = std::make_shared<SceneDevice>();  // handles sharing the previous scene keep it
-    r->sd->node_count = (uint32_t)bvh.nodes.size();
-    r->sd->leaf_tri_count = (uint32_t)bvh.tris.size();
-    r->sd->bvh_depth = bvh.depth;
     hipStream_t s = r->stream;
-    upload(r->sd->d_nodes, bvh.nodes, s);
-    upload(r->sd->d_ltris, bvh.tris, s);
+    const uint32_t ntri = (uint32_t)(r->tri_pos.size() / 9);
+    auto sd = std::make_shared<SceneDevice>();
+    const auto t_build = std::chrono::steady_clock::now();
+    bool on_gpu = false;
+    if (r->bvh_builder == HPT_BVH_GPU && ntri > 0) {
+      // The same tree as the host build, made on the device; a large node whose
+      // centroids all coincide falls back to the host.
+      DevBuf d_pos;
+      upload(d_pos, r->tri_pos, s);
+      GpuBvh g;
+      std::string err;
+      if (build_bvh8_gpu(d_pos.as<float>(), ntri, s, g, err)) {
+        sd->d_nodes.adopt(g.nodes, (size_t)g.node_count * sizeof(Node8));
+        sd->d_ltris.adopt(g.tris, (size_t)ntri * sizeof(LeafTri));
+        sd->node_count = g.node_count;
+        sd->leaf_tri_count = ntri;
+        sd->bvh_depth = g.depth;
+        on_gpu = true;
+      } else {
+        need(err.find("not finite") == std::string::npos, err);
+      }
+    }
+    if (!on_gpu) {
+      BvhBuildResult bvh;
+      std::string err;
+      need(build_bvh8(r->tri_pos.data(), ntri, bvh, err), err);
+      sd->node_count = (uint32_t)bvh.nodes.size();
+      sd->leaf_tri_count = (uint32_t)bvh.tris.size();
+      sd->bvh_depth = bvh.depth;
+      upload(sd->d_nodes, bvh.nodes, s);
+      upload(sd->d_ltris, bvh.tris, s);
+      check(hipStreamSynchronize(s), "upload");  // bvh is a local
+    }
+    r->bvh_builder_used = on_gpu ? HPT_BVH_GPU : HPT_BVH_HOST;
+    r->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
+    r->sd = sd;  // handles sharing the previous scene keep it
     upload(r->sd->d_tri_shade, r->tri_shade, s);
     upload(r->sd->d_prims, r->prims, s);
     upload(r->sd->d_materials, r->materials, s);
@@ -940,6 +978,23 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
       left -= k;
     }
     if (r->profiling & 1) collect_timers(r);
+  });
+}
+
+hpt_status hpt_set_bvh_builder(hpt_renderer* r, uint32_t builder) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(builder == HPT_BVH_HOST || builder == HPT_BVH_GPU, "unknown BVH builder");
+    r->bvh_builder = builder;
+  });
+}
+
+hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, double* build_ms) {
+  return guarded([&] {
+    need(r && builder_used && build_ms, "null argument");
+    require_committed(r);
+    *builder_used = r->bvh_builder_used;
+    *build_ms = r->bvh_build_ms;
   });
 }
 

@@ -1,7 +1,9 @@
 // bvh_build.cpp — host build of the 8-wide compressed BVH (Node8/LeafTri).
 //
 // Binary SAH tree over triangle centroids (32 bins per axis), leaves of at
-// most 3 triangles, collapsed greedily to 8-wide nodes and quantised to 8 bits
+// most 3 triangles (the split stable, so a leaf lists its triangles in id
+// order: the GPU build in bvh_gpu.hip reproduces this tree byte for byte),
+// collapsed greedily to 8-wide nodes and quantised to 8 bits
 // per plane (the compressed-wide-BVH node of Ylitie, Karras and Laine, "Efficient
 // Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017,
 // with this project's own field layout; see common.h).
@@ -146,7 +148,7 @@ struct Builder {
             const float ext = cb.hi[best_axis] - cb.lo[best_axis];
             const float scale = (float)kBins / ext;
             auto beg = order.begin() + it.first, end = beg + it.count;
-            auto m = std::partition(beg, end, [&](uint32_t t) {
+            auto m = std::stable_partition(beg, end, [&](uint32_t t) {
               int b = (int)((cen[(size_t)t * 3 + best_axis] - cb.lo[best_axis]) * scale);
               b = std::min(std::max(b, 0), kBins - 1);
               return b < best_split;

@@ -36,6 +36,7 @@ HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
 STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE = range(5)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
+BVH_HOST, BVH_GPU = 0, 1
 
 STATUS_NAMES = {
     0: "HPT_OK",
@@ -177,6 +178,8 @@ SIGNATURES = {
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
     "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
     "hpt_set_launch_mode": [C.c_void_p, C.c_uint32],
+    "hpt_set_bvh_builder": [C.c_void_p, C.c_uint32],
+    "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
 }
 
@@ -469,6 +472,17 @@ class HalaRenderer:
     @frames_in_flight.setter
     def frames_in_flight(self, k: int) -> None:
         _check(self._lib.hpt_set_frames_in_flight(self._h, k))
+
+    def set_bvh_builder(self, builder: int) -> None:
+        """BVH_GPU (the default: binned SAH and 8-wide collapse on the device)
+        or BVH_HOST; both build the same bytes.  Applies at the next commit."""
+        _check(self._lib.hpt_set_bvh_builder(self._h, builder))
+
+    def bvh_build_stats(self) -> tuple:
+        """(builder used by the last commit, its wall time in ms)."""
+        b, ms = C.c_uint32(), C.c_double()
+        _check(self._lib.hpt_get_bvh_build_stats(self._h, C.byref(b), C.byref(ms)))
+        return b.value, ms.value
 
     def set_launch_mode(self, mode: int) -> None:
         """LAUNCH_GRAPH (the default: full batches replayed as captured HIP

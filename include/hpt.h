@@ -312,6 +312,17 @@ HPT_API hpt_status hpt_reset_accumulation(hpt_renderer* r);
  * renders and the TRANSPARENT permutation (whose bounce count depends on a
  * host read-back) always launch eagerly. */
 enum { HPT_LAUNCH_EAGER = 0, HPT_LAUNCH_GRAPH = 1 };
+
+/* Where hpt_commit builds the BVH (no counterpart in the reference, whose
+ * acceleration structures the Vulkan driver builds).  HPT_BVH_GPU, the
+ * default, runs the binned-SAH build and the 8-wide collapse on the device;
+ * HPT_BVH_HOST runs the same algorithm on the CPU.  Both produce the same
+ * bytes; the device build hands over to the host one when a node of more
+ * than 1024 triangles has all centroids at one point. */
+enum { HPT_BVH_HOST = 0, HPT_BVH_GPU = 1 };
+HPT_API hpt_status hpt_set_bvh_builder(hpt_renderer* r, uint32_t builder);
+/* The builder the last hpt_commit used and its wall time in milliseconds. */
+HPT_API hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, double* build_ms);
 HPT_API hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode);
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
