@@ -445,12 +445,20 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4&
 }
 
 // One iteration of the bounce loop, pathtracer.rgen.glsl:70-256.  The
-// occupancy target trades register spills for latency hiding.
+// occupancy target trades register spills for latency hiding: 3 waves per
+// SIMD for the plain permutations, 5 for VOL_MIS, whose inline transmittance
+// walks are latency-bound traversals (measured on the 1080p icosphere scene
+// with a medium: 3 -> 577, 4 -> 626, 5 -> 633, 6 -> 618, 8 -> 556
+// Msamples/s).
 #ifndef HPT_SHADE_WAVES_PER_EU
 #define HPT_SHADE_WAVES_PER_EU 3
 #endif
+#ifndef HPT_SHADE_VMIS_WAVES_PER_EU
+#define HPT_SHADE_VMIS_WAVES_PER_EU 5
+#endif
 template <bool MED, bool VMIS, bool TRANSP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_SHADE_WAVES_PER_EU)))
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(VMIS ? HPT_SHADE_VMIS_WAVES_PER_EU : HPT_SHADE_WAVES_PER_EU)))
     k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
             Spill sp) {
   __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
