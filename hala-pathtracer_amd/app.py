@@ -247,6 +247,8 @@ def run(config_path: str, frames: int, out_dir: str = "./out", device: int = 0) 
     stem = os.path.splitext(os.path.basename(config.scene_file))[0]
     prefix = os.path.join(out_dir, stem)
     r.save_images(prefix)  # hala-pathtracer.rs:245-251
+    with open(prefix + "_spp.txt", "w") as f:  # the sample count the denoiser takes as an input
+        f.write(f"{r.frame_index}\n")
     save_final_png(prefix + "_final.png", r.read_image(hpt.IMAGE_FINAL_RGBA8))
     r.close()
     return prefix

@@ -157,6 +157,8 @@ pub const HPT_IMAGE_ALBEDO_RGBA32F: c_int = 2;
 pub const HPT_IMAGE_NORMAL_RGBA32F: c_int = 3;
 pub const HPT_LAUNCH_EAGER: u32 = 0;
 pub const HPT_LAUNCH_GRAPH: u32 = 1;
+pub const HPT_BVH_HOST: u32 = 0;
+pub const HPT_BVH_GPU: u32 = 1;
 
 extern "C" {
     pub fn hpt_create(info: *const hpt_create_info, out: *mut *mut hpt_renderer) -> hpt_status;
@@ -176,12 +178,66 @@ extern "C" {
     pub fn hpt_set_frames_in_flight(r: *mut hpt_renderer, frames: u32) -> hpt_status;
     pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
     pub fn hpt_set_launch_mode(r: *mut hpt_renderer, mode: u32) -> hpt_status;
+    pub fn hpt_set_bvh_builder(r: *mut hpt_renderer, builder: u32) -> hpt_status;
     pub fn hpt_get_image_pointer(r: *mut hpt_renderer, which: c_int, ptr: *mut *mut c_void, bytes: *mut usize)
         -> hpt_status;
     pub fn hpt_read_image(r: *mut hpt_renderer, which: c_int, dst: *mut c_void, bytes: usize) -> hpt_status;
     pub fn hpt_wait_idle(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_destroy(r: *mut hpt_renderer);
     pub fn hpt_last_error() -> *const c_char;
+}
+
+// The learned denoiser's kernels (include/hpt_nn.h), the replacement of the
+// `oidnDenoise` step of denoise.py.  Streams are hipStream_t (null = the null
+// stream); device pointers come from hpt_nn_alloc or hpt_get_image_pointer.
+extern "C" {
+    pub fn hpt_nn_conv_forward(stream: *mut c_void, x: *const u16, w: *const u16, b: *const f32, y: *mut u16,
+        y_f32: *mut f32, n: c_int, h: c_int, wd: c_int, cin: c_int, cout: c_int, cout_pad: c_int, k: c_int,
+        relu: c_int) -> c_int;
+    pub fn hpt_nn_conv_backward_data(stream: *mut c_void, dy: *const f32, wt: *const u16, dx: *mut f32, n: c_int,
+        h: c_int, wd: c_int, cin: c_int, cout: c_int, cout_pad: c_int, k: c_int) -> c_int;
+    pub fn hpt_nn_conv_backward_weights(stream: *mut c_void, dy: *const f32, x: *const u16, dw: *mut f32,
+        db: *mut f32, partial: *mut f32, n: c_int, h: c_int, wd: c_int, cin: c_int, cout: c_int, cout_pad: c_int,
+        k: c_int) -> c_int;
+    pub fn hpt_nn_wgrad_scratch_floats(n: c_int, h: c_int, wd: c_int, cin: c_int, cout_pad: c_int, k: c_int) -> usize;
+    pub fn hpt_nn_prepare_weights(stream: *mut c_void, master: *const f32, w: *mut u16, wt: *mut u16, cin: c_int,
+        cout: c_int, cout_pad: c_int, k: c_int) -> c_int;
+    pub fn hpt_nn_relu_backward(stream: *mut c_void, dy: *mut f32, y: *const u16, count: usize) -> c_int;
+    pub fn hpt_nn_pool_forward(stream: *mut c_void, x: *const u16, y: *mut u16, n: c_int, h: c_int, wd: c_int,
+        c: c_int) -> c_int;
+    pub fn hpt_nn_pool_backward(stream: *mut c_void, dy: *const f32, dx: *mut f32, n: c_int, h: c_int, wd: c_int,
+        c: c_int) -> c_int;
+    pub fn hpt_nn_upsample_forward(stream: *mut c_void, x: *const u16, y: *mut u16, n: c_int, h: c_int, wd: c_int,
+        c: c_int) -> c_int;
+    pub fn hpt_nn_upsample_backward(stream: *mut c_void, dy: *const f32, dx: *mut f32, n: c_int, h: c_int,
+        wd: c_int, c: c_int) -> c_int;
+    pub fn hpt_nn_concat_forward(stream: *mut c_void, a: *const u16, b: *const u16, y: *mut u16, pixels: usize,
+        ca: c_int, cb: c_int) -> c_int;
+    pub fn hpt_nn_concat_backward(stream: *mut c_void, dy: *const f32, da: *mut f32, db: *mut f32, pixels: usize,
+        ca: c_int, cb: c_int) -> c_int;
+    pub fn hpt_nn_accumulate(stream: *mut c_void, dx: *mut f32, dy: *const f32, count: usize) -> c_int;
+    pub fn hpt_nn_make_input(stream: *mut c_void, color: *const f32, albedo: *const f32, normal: *const f32,
+        src_w: c_int, src_h: c_int, x0: c_int, y0: c_int, x: *mut u16, h: c_int, wd: c_int, mirror: c_int,
+        aux: f32) -> c_int;
+    pub fn hpt_nn_make_target(stream: *mut c_void, color: *const f32, src_w: c_int, src_h: c_int, x0: c_int,
+        y0: c_int, out: *mut f32, h: c_int, wd: c_int, mirror: c_int) -> c_int;
+    pub fn hpt_nn_add_residual(stream: *mut c_void, out: *mut f32, x: *const u16, pixels: usize,
+        x_channels: c_int) -> c_int;
+    pub fn hpt_nn_l1_loss(stream: *mut c_void, out: *const f32, target_log: *const f32, dy: *mut f32,
+        loss_sum: *mut f32, pixels: usize, cout_pad: c_int) -> c_int;
+    pub fn hpt_nn_adam(stream: *mut c_void, p: *mut f32, g: *const f32, m: *mut f32, v: *mut f32, count: usize,
+        lr: f32, beta1: f32, beta2: f32, eps: f32, t: c_int) -> c_int;
+    pub fn hpt_nn_write_output(stream: *mut c_void, out: *const f32, dst: *mut f32, dst_w: c_int, dst_h: c_int,
+        x0: c_int, y0: c_int, h: c_int, wd: c_int) -> c_int;
+    pub fn hpt_nn_alloc(bytes: usize, ptr: *mut *mut c_void) -> c_int;
+    pub fn hpt_nn_free(ptr: *mut c_void) -> c_int;
+    pub fn hpt_nn_copy(stream: *mut c_void, dst: *mut c_void, src: *const c_void, bytes: usize) -> c_int;
+    pub fn hpt_nn_fill(stream: *mut c_void, p: *mut f32, value: f32, count: usize) -> c_int;
+    pub fn hpt_nn_stream_synchronize(stream: *mut c_void) -> c_int;
+    pub fn hpt_nn_event_create(event: *mut *mut c_void) -> c_int;
+    pub fn hpt_nn_event_destroy(event: *mut c_void) -> c_int;
+    pub fn hpt_nn_event_record(event: *mut c_void, stream: *mut c_void) -> c_int;
+    pub fn hpt_nn_event_elapsed(start: *mut c_void, stop: *mut c_void, ms: *mut f32) -> c_int;
 }
 
 fn check(status: hpt_status) -> anyhow::Result<()> {

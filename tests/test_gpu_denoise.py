@@ -369,3 +369,31 @@ def test_trained_network_beats_the_plain_estimate(dn):
         print(f"  relMSE plain {plain:.5f} denoised {den:.5f}", flush=True)
         gains.append(plain / den)
     assert min(gains) > 2.0 and float(np.exp(np.mean(np.log(gains)))) > 4.0, gains
+
+
+def test_denoise_cli_replaces_oidn_on_saved_images(dn, tmp_path):
+    """denoise.py's flow (find *_color.pfm with _albedo/_normal, write
+    _final.pfm) through the learned denoiser: the written image is the
+    network's output for those images, and it is closer to a reference than
+    the noisy colour."""
+    import denoise_cli
+    import denoise_train
+    import hpt
+
+    out = tmp_path / "out" / "scene"
+    out.mkdir(parents=True)
+    s = denoise_train.render_sample(901, 128, 128, spp_levels=(16,), ref_spp=1024)
+    prefix = str(out / "cornellish")
+    for img, suffix in zip(s.noisy[16], ("color", "albedo", "normal")):
+        hpt.write_pfm(f"{prefix}_{suffix}.pfm", img.numpy()[..., :3])
+    with open(prefix + "_spp.txt", "w") as f:
+        f.write("16\n")
+    (tmp_path / "out" / "lonely_color.pfm").write_bytes(b"")  # no albedo/normal: skipped
+    assert denoise_cli.main(["--dir", str(tmp_path / "out")]) == 0
+    final = hpt.read_pfm(prefix + "_final.pfm")
+    net = dn.UNet.load(dn.weights_path())
+    direct = net.denoise(s.noisy[16], 128, 128, 16).numpy()[..., :3]
+    assert np.allclose(final, direct, rtol=1e-6, atol=1e-6)
+    ref = s.reference.numpy()
+    assert denoise_train.relmse(final, ref) < denoise_train.relmse(s.noisy[16][0].numpy(), ref)
+    assert not (tmp_path / "out" / "lonely_final.pfm").exists()
