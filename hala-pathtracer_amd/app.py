@@ -189,7 +189,9 @@ def read_env(path: str) -> np.ndarray:
         return read_hdr(path)
     if ext == ".pfm":
         return hpt.read_pfm(path)
-    raise ValueError(f"{path}: environment maps are read from .hdr or .pfm (no EXR decoder in this build)")
+    if ext == ".exr":
+        return hpt.read_exr(path)
+    raise ValueError(f"{path}: environment maps are read from .exr, .hdr or .pfm")
 
 
 # ----------------------------------------------------------------- running

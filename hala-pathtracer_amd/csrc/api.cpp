@@ -21,6 +21,7 @@
 #include "bvh_build.h"
 #include "bvh_gpu.h"
 #include "common.h"
+#include "exr.h"
 #include "kernels.h"
 
 namespace {
@@ -571,6 +572,19 @@ void require_committed(const hpt_renderer* r) {
 
 }  // namespace
 
+namespace {
+
+ExrImage decode_exr_or_throw(const void* data, size_t size, bool pixels) {
+  ExrImage img;
+  std::string err;
+  const ExrResult res = decode_exr(static_cast<const uint8_t*>(data), size, pixels, img, err);
+  if (res != kExrOk)
+    throw Fail(res == kExrUnsupported ? HPT_ERROR_UNSUPPORTED : HPT_ERROR_INVALID_ARGUMENT, "OpenEXR: " + err);
+  return img;
+}
+
+}  // namespace
+
 extern "C" {
 
 hpt_status hpt_create(const hpt_create_info* info, hpt_renderer** out) {
@@ -693,6 +707,26 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
     r->camera_count = d->camera_count;
     r->scene_set = true;
     r->committed = false;
+  });
+}
+
+hpt_status hpt_exr_info(const void* data, size_t size, uint32_t* width, uint32_t* height) {
+  return guarded([&] {
+    need(data && width && height, "null argument");
+    const ExrImage img = decode_exr_or_throw(data, size, false);
+    *width = img.width;
+    *height = img.height;
+  });
+}
+
+hpt_status hpt_exr_decode_rgb(const void* data, size_t size, float* rgb, uint32_t width, uint32_t height) {
+  return guarded([&] {
+    need(data && rgb, "null argument");
+    const ExrImage img = decode_exr_or_throw(data, size, true);
+    need(img.width == width && img.height == height,
+         "OpenEXR image is " + std::to_string(img.width) + "x" + std::to_string(img.height) + ", not " +
+             std::to_string(width) + "x" + std::to_string(height));
+    std::memcpy(rgb, img.rgb.data(), img.rgb.size() * sizeof(float));
   });
 }
 

@@ -164,6 +164,8 @@ extern "C" {
     pub fn hpt_create(info: *const hpt_create_info, out: *mut *mut hpt_renderer) -> hpt_status;
     pub fn hpt_set_scene(r: *mut hpt_renderer, scene: *const hpt_scene_desc) -> hpt_status;
     pub fn hpt_set_env_map(r: *mut hpt_renderer, rgb: *const f32, width: u32, height: u32, rotation: f32) -> hpt_status;
+    pub fn hpt_exr_info(data: *const u8, size: usize, width: *mut u32, height: *mut u32) -> hpt_status;
+    pub fn hpt_exr_decode_rgb(data: *const u8, size: usize, rgb: *mut f32, width: u32, height: u32) -> hpt_status;
     pub fn hpt_set_ground_color(r: *mut hpt_renderer, rgba: *const f32) -> hpt_status;
     pub fn hpt_set_sky_color(r: *mut hpt_renderer, rgba: *const f32) -> hpt_status;
     pub fn hpt_set_env_intensity(r: *mut hpt_renderer, intensity: f32) -> hpt_status;
@@ -326,6 +328,16 @@ impl HalaRenderer {
     pub fn set_envmap(&mut self, rgb: &[f32], width: u32, height: u32, rotation: f32) -> anyhow::Result<()> {
         anyhow::ensure!(rgb.len() == (width * height * 3) as usize, "envmap size mismatch");
         check(unsafe { hpt_set_env_map(self.raw, rgb.as_ptr(), width, height, rotation) })
+    }
+
+    /// hala-pathtracer.rs:203 as the reference calls it, with an OpenEXR path.
+    pub fn set_envmap_exr(&mut self, path: &str, rotation: f32) -> anyhow::Result<()> {
+        let bytes = std::fs::read(path)?;
+        let (mut width, mut height) = (0u32, 0u32);
+        check(unsafe { hpt_exr_info(bytes.as_ptr(), bytes.len(), &mut width, &mut height) })?;
+        let mut rgb = vec![0f32; (width as usize) * (height as usize) * 3];
+        check(unsafe { hpt_exr_decode_rgb(bytes.as_ptr(), bytes.len(), rgb.as_mut_ptr(), width, height) })?;
+        self.set_envmap(&rgb, width, height, rotation)
     }
 
     pub fn set_ground_color(&mut self, c: [f32; 4]) -> anyhow::Result<()> {

@@ -149,6 +149,8 @@ SIGNATURES = {
     "hpt_create": [C.POINTER(_CreateInfo), C.POINTER(C.c_void_p)],
     "hpt_set_scene": [C.c_void_p, C.POINTER(_SceneDesc)],
     "hpt_set_env_map": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float],
+    "hpt_exr_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "hpt_exr_decode_rgb": [C.c_char_p, C.c_size_t, C.c_void_p, C.c_uint32, C.c_uint32],
     "hpt_set_ground_color": [C.c_void_p, C.POINTER(C.c_float)],
     "hpt_set_sky_color": [C.c_void_p, C.POINTER(C.c_float)],
     "hpt_set_env_intensity": [C.c_void_p, C.c_float],
@@ -528,6 +530,22 @@ def write_pfm(path: str, rgb: np.ndarray) -> None:
     with open(path, "wb") as f:
         f.write(b"PF\n%d %d\n-1.0\n" % (a.shape[1], a.shape[0]))
         f.write(a.astype("<f4").tobytes())
+
+
+def decode_exr(data: bytes) -> np.ndarray:
+    """An OpenEXR file's bytes as (H, W, 3) float32 RGB, row 0 at the top
+    (hpt_exr_info + hpt_exr_decode_rgb; host code, no device)."""
+    lib = load_library()
+    w, h = C.c_uint32(), C.c_uint32()
+    _check(lib.hpt_exr_info(data, len(data), C.byref(w), C.byref(h)))
+    out = np.empty((h.value, w.value, 3), np.float32)
+    _check(lib.hpt_exr_decode_rgb(data, len(data), _ptr(out), w.value, h.value))
+    return out
+
+
+def read_exr(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        return decode_exr(f.read())
 
 
 def read_pfm(path: str) -> np.ndarray:

@@ -253,6 +253,20 @@ HPT_API hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* scene);
 HPT_API hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t width,
                                    uint32_t height, float rotation);
 
+/* OpenEXR decoding for hpt_set_env_map: the reference's configurations name
+ * OpenEXR HDRIs (`hdri_file` in conf/dragon.yaml and its siblings), which
+ * HalaRenderer::set_envmap decodes from the path it is given
+ * (hala-pathtracer.rs:202-203).  These take the file's bytes.  Read: single-part scanline files, NONE, RLE, ZIPS,
+ * ZIP or PIZ compression, UINT, HALF or FLOAT channels without subsampling;
+ * RGB from the channels R, G and B (a missing one reads 0), else Y copied to
+ * all three.  Other files fail with HPT_ERROR_UNSUPPORTED, malformed ones
+ * with HPT_ERROR_INVALID_ARGUMENT.  Host code only: no device is used. */
+HPT_API hpt_status hpt_exr_info(const void* data, size_t size, uint32_t* width, uint32_t* height);
+/* Writes height*width*3 floats, row 0 = the top line of the data window
+ * (theta 0 of an equirectangular map). */
+HPT_API hpt_status hpt_exr_decode_rgb(const void* data, size_t size, float* rgb, uint32_t width,
+                                      uint32_t height);
+
 /* HalaRenderer::set_ground_color / set_sky_color (hala-pathtracer.rs:205-206).
  * Either selects ENV_TYPE_SKY (env_sky_eval.rcall.glsl:12-15). */
 HPT_API hpt_status hpt_set_ground_color(hpt_renderer* r, const float rgba[4]);
