@@ -42,17 +42,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s=12.0):
+def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s=12.0, threads=0):
     """Oracle throughput on the host: centred pixel rows of frame 0, grown
     until the sample takes a quarter of the budget; once the whole image
-    fits, whole frames 0, 1, ... until the budget is spent.  All host threads
-    of this process's CPU share (at most 16)."""
+    fits, whole frames 0, 1, ... until the budget is spent.  `threads` 0:
+    all host threads of this process's CPU share (at most 16)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     o = pyoracle.OracleScene(scene)
     st = pyoracle.settings(width, height, max_depth, rr_depth, features)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = threads or max(1, min(16, len(os.sched_getaffinity(0))))
     rows = 4
     while True:
         y0 = height // 2 - rows // 2
@@ -408,6 +408,7 @@ def main():
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads for cpu_baseline (0: all, at most 16)")
     ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
     ap.add_argument("--frames-in-flight", type=int, default=0,
@@ -517,7 +518,7 @@ def main():
             if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
                 den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
             conv["denoised"] = den
-    base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES)
+    base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES, threads=args.cpu_threads)
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
               "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,
               "lights": int(len(scene.lights)), "frames_in_flight": fif, "launch": args.launch,

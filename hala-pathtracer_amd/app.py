@@ -223,7 +223,9 @@ def create_renderer(config: AppConfig, device: int = 0):
                          features=scene_features(scene), device=device)
     r.set_scene(scene)  # with the glTF images as its texture table
     if config.hdri_file:
-        r.set_envmap(read_env(config.hdri_file), config.hdri_rotation)
+        # hdri_rotation is in degrees (conf/dragon.yaml:14 holds 77.8); the
+        # UBO's env_rotation is a u offset in turns (env_map_eval.rcall.glsl:18).
+        r.set_envmap(read_env(config.hdri_file), config.hdri_rotation / 360.0)
     else:
         r.set_ground_color(config.ground_color)
         r.set_sky_color([c * 4.0 for c in config.sky_color])  # hala-pathtracer.rs:206
