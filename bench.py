@@ -409,6 +409,9 @@ def main():
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads for cpu_baseline (0: all, at most 16)")
+    ap.add_argument("--shade-sort", default="auto",
+                    help="material-sorted shading from this bounce on ('off': never; 'auto': the autotuner's "
+                         "choice among off and bounces 0-2 at this batch size, measured before the timed region)")
     ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
     ap.add_argument("--frames-in-flight", type=int, default=0,
@@ -449,13 +452,28 @@ def main():
         rr.frames_in_flight = args.frames_in_flight
         rr.commit()
         rr.set_launch_mode(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
+        rr.set_shade_sort(shade_sort)
         return rr
 
+    shade_sort = hpt.SHADE_SORT_OFF if args.shade_sort in ("off", "auto") else int(args.shade_sort)
     t0 = time.perf_counter()
     r = make_renderer()
     log(f"create + commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
         f"{r.frames_in_flight} frames in flight")
     fif = r.frames_in_flight
+    sort_table = None
+    if args.shade_sort == "auto":
+        import autotune
+
+        t0 = time.perf_counter()
+        tuned = autotune.tune(r, W, H, DEPTH, RR, FEATURES, samples=4 * fif * W * H, candidates=(fif,),
+                              max_frames=fif, modes=(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER,),
+                              tolerance=0.05)
+        shade_sort = tuned.shade_sort
+        sort_table = {k: round(v, 1) for k, v in tuned.table.items()}
+        r.set_shade_sort(shade_sort)
+        log(f"shade sort autotuned in {time.perf_counter() - t0:.1f} s: "
+            f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'} ({sort_table})")
 
     # Visit counts per ray from a counted warm-up (separate kernel variant).
     r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
@@ -522,6 +540,8 @@ def main():
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
               "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,
               "lights": int(len(scene.lights)), "frames_in_flight": fif, "launch": args.launch,
+              "shade_sort": "off" if shade_sort == hpt.SHADE_SORT_OFF else shade_sort,
+              "shade_sort_tuning": sort_table,
               "eager_profiled_wall_s": round(profiled_wall, 4),
               "mrays_per_s": round(rays_total / wall / 1e6, 2)}
     if scene.env_rgb is not None:

@@ -2,14 +2,17 @@
 reference, whose frame loop has no tunables).
 
 What it tunes: the frames traced together per batch (the wavefront and
-queue size, `hpt_set_frames_in_flight`) and the launch mode (graph replay or
-eager launches) for one (scene, resolution, depth) class.  Larger batches
+queue size, `hpt_set_frames_in_flight`), the launch mode (graph replay or
+eager launches) and the shading sort threshold (the first bounce whose paths
+are grouped by material before shading, `hpt_set_shade_sort`, or never) for
+one (scene, resolution, depth) class.  Larger batches
 fill the GPU with more paths per launch and amortise the short tail bounces;
 they cost k * W * H * 288 bytes of device memory, which the serving loop
 spends from its admission budget.  The tuner measures, it does not model:
 each candidate renders a warm-up batch and then a timed run on the job's own
 scene, and the smallest batch within `tolerance` of the best throughput
-wins (less memory and a finer progress granularity for the same speed).
+wins (less memory and a finer progress granularity for the same speed),
+then graph replay, then no sort (it costs 5 bytes per path).
 
 Results are kept in a TuningCache keyed by (scene name, width, height,
 max_depth) and can be saved as JSON.
@@ -24,7 +27,9 @@ from dataclasses import dataclass, field
 import hpt
 
 PATH_BYTES = 288
+SORT_BYTES = 5  # key byte and sorted entry per queue slot while shading sort is on
 CANDIDATES = (2, 4, 8, 16, 32, 64)
+SORT_CANDIDATES = (0, 1, 2)
 
 
 @dataclass
@@ -32,8 +37,9 @@ class Tuning:
     frames_in_flight: int
     launch: int
     msamples_per_s: float
-    table: dict = field(default_factory=dict)  # "k/mode" -> Msamples/s
+    table: dict = field(default_factory=dict)  # "k/mode[/sN]" -> Msamples/s
     seconds: float = 0.0                       # time the tuning took
+    shade_sort: int = hpt.SHADE_SORT_OFF       # first sorted bounce
 
 
 def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: int | None = None,
@@ -52,19 +58,21 @@ def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: 
 
 def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, rr_depth: int = 3, features: int = 0,
          samples: float = 32e6, max_frames: int = 64, memory_bytes: int | None = None, tolerance: float = 0.03,
-         modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), device: int = 0) -> Tuning:
-    """Measures every candidate batch size (and launch mode) on a handle that
-    shares `template`'s committed scene, each over about `samples` paths
-    (whole batches); returns the choice."""
+         modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), sorts=SORT_CANDIDATES, candidates=CANDIDATES,
+         device: int = 0) -> Tuning:
+    """Measures every candidate batch size (then launch mode, then shading
+    sort threshold) on a handle that shares `template`'s committed scene,
+    each over about `samples` paths (whole batches); returns the choice."""
     t_start = time.perf_counter()
     r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=rr_depth, features=features, device=device)
     table = {}
     try:
         r.share_scene(template)
 
-        def measure(k, mode):
+        def measure(k, mode, sort=hpt.SHADE_SORT_OFF):
             r.frames_in_flight = k
             r.set_launch_mode(mode)
+            r.set_shade_sort(sort)
             n = max(1, -(-int(samples) // (width * height * k))) * k
             r.reset_accumulation()
             r.render(k)  # warm-up: allocation and graph capture
@@ -72,22 +80,32 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
             t0 = time.perf_counter()
             r.render(n)
             r.wait_idle()
-            table[f"{k}/{'graph' if mode == hpt.LAUNCH_GRAPH else 'eager'}"] = \
-                width * height * n / (time.perf_counter() - t0) / 1e6
+            key = f"{k}/{'graph' if mode == hpt.LAUNCH_GRAPH else 'eager'}"
+            if sort != hpt.SHADE_SORT_OFF:
+                key += f"/s{sort}"
+            table[key] = width * height * n / (time.perf_counter() - t0) / 1e6
 
         # Batch sizes with the first mode, then the other modes at the best size.
-        for k in candidates_for(width, height, max_frames, memory_bytes):
+        for k in candidates_for(width, height, max_frames, memory_bytes, candidates):
             measure(k, modes[0])
         k_best = int(max(table, key=table.get).split("/")[0])
         for mode in modes[1:]:
             measure(k_best, mode)
+        k_best, mode_best = max(table, key=table.get).split("/")[:2]
+        for sort in sorts:
+            measure(int(k_best), hpt.LAUNCH_GRAPH if mode_best == "graph" else hpt.LAUNCH_EAGER, sort)
     finally:
         r.close()
     best = max(table.values())
-    ok = [(int(key.split("/")[0]), key.split("/")[1], v) for key, v in table.items() if v >= (1 - tolerance) * best]
-    k, mode, v = min(ok, key=lambda t: (t[0], t[1] != "graph"))
+    ok = []
+    for key, v in table.items():
+        if v >= (1 - tolerance) * best:
+            parts = key.split("/")
+            sort = int(parts[2][1:]) if len(parts) > 2 else hpt.SHADE_SORT_OFF
+            ok.append((int(parts[0]), parts[1], sort, v))
+    k, mode, sort, v = min(ok, key=lambda t: (t[0], t[1] != "graph", t[2] != hpt.SHADE_SORT_OFF, -t[2]))
     return Tuning(k, hpt.LAUNCH_GRAPH if mode == "graph" else hpt.LAUNCH_EAGER, v, table,
-                  time.perf_counter() - t_start)
+                  time.perf_counter() - t_start, sort)
 
 
 class TuningCache:

@@ -83,8 +83,8 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
 // textures, blue noise.
 struct SceneDevice {
   uint32_t node_count = 0, leaf_tri_count = 0, bvh_depth = 0;
-  DevBuf d_nodes, d_ltris, d_tri_shade, d_prims, d_materials, d_lights, d_cameras, d_env_rgb, d_env_table,
-      d_env_texels, d_tex_texels, d_tex_info, d_srgb_lut, d_blue_noise;
+  DevBuf d_nodes, d_ltris, d_tri_shade, d_tri_key, d_prims, d_materials, d_lights, d_cameras, d_env_rgb,
+      d_env_table, d_env_texels, d_tex_texels, d_tex_info, d_srgb_lut, d_blue_noise;
 };
 
 struct StageTimer {
@@ -124,6 +124,10 @@ struct hpt_renderer {
   std::shared_ptr<SceneDevice> sd;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
+  // Material-sorted shading from bounce shade_sort_from on (HPT_SHADE_SORT_OFF:
+  // never): one key byte and one sorted entry per queue slot.
+  uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
+  DevBuf d_sort_keys, d_sort_items;
   DevBuf d_accum, d_albedo, d_normal, d_final;
   uint32_t npix = 0;          // width * height
   uint32_t batch_cap = 0;     // frames the path buffers hold
@@ -245,6 +249,7 @@ SceneView scene_view(const hpt_renderer* r) {
   sv.nodes = r->sd->d_nodes.as<Node8>();
   sv.ltris = r->sd->d_ltris.as<LeafTri>();
   sv.tri_shade = r->sd->d_tri_shade.as<TriShade>();
+  sv.tri_key = r->sd->d_tri_key.as<uint8_t>();
   sv.prims = r->sd->d_prims.as<PrimInfo>();
   sv.materials = r->sd->d_materials.as<hpt_material>();
   sv.lights = r->sd->d_lights.as<hpt_light>();
@@ -370,7 +375,8 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   const uint32_t n = frames * r->npix;
   const SegGeom g = seg_geom(n);
   uint32_t* counts = r->d_counts.as<uint32_t>();
-  // Segment counters: path queues A and B (ping-pong), then the shadow queue.
+  // Segment counters: path queues A and B (ping-pong), the shadow queue, the
+  // sorted shading queue, then the sort's key counts and cursors.
   const Queue qbuf[2] = {{r->d_qa.as<uint32_t>(), counts, g.seg_cap},
                          {r->d_qb.as<uint32_t>(), counts + kSeg * kSegStride, g.seg_cap}};
   const ShadowQueue sq{r->d_sh_o.as<float4>(), r->d_sh_d.as<float4>(), r->d_sh_c.as<float4>(),
@@ -378,6 +384,8 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   const Spill sp = ensure_spill(r, 2 * kSeg * g.seg_cap);  // threads of the widest launch (shadow)
   hipStream_t s = r->stream;
   StageTimer* t;
+  const ShadeSort ss{r->d_sort_keys.as<uint8_t>(), r->d_sort_items.as<uint32_t>(), counts + 3 * kSeg * kSegStride,
+                     counts + 4 * kSeg * kSegStride};
   auto* stats = r->profiling ? r->d_stats.as<unsigned long long>() : nullptr;
   const bool count = (r->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
   const StageCounters sc_extend{stats ? stats + kStatsPerStage * HPT_STAGE_EXTEND : nullptr, count};
@@ -407,11 +415,17 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
       if (live == 0 || b > 4096) break;
     }
     const Queue& qout = qbuf[out];
+    const bool sort = b >= r->shade_sort_from && ss.keys && ss.items;
     stage_begin(r, HPT_STAGE_EXTEND, t);
-    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend);
+    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr);
     stage_end(r, t);
-    stage_begin(r, HPT_STAGE_SHADE, t);
-    launch_shade(s, r->info.features, sv, fp, ps, qin, n, qout, sq, sp);
+    stage_begin(r, HPT_STAGE_SHADE, t);  // the sort counts as shading time
+    Queue qshade = qin;
+    if (sort) {
+      launch_shade_sort(s, qin, n, ss);
+      qshade = Queue{ss.items, ss.counts, g.seg_cap};
+    }
+    launch_shade(s, r->info.features, sv, fp, ps, qshade, n, qout, sq, sp);
     stage_end(r, t);
     const bool vmis = (r->info.features & HPT_FEATURE_MEDIUM) && (r->info.features & HPT_FEATURE_VOL_MIS);
     if (!vmis) {
@@ -447,10 +461,12 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, fp);
   key_append(k, path_state(r));
   for (const DevBuf* b : {&r->d_qa, &r->d_qb, &r->d_sh_o, &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill,
-                          &r->d_accum, &r->d_albedo, &r->d_normal, &r->d_final, &r->d_frame})
+                          &r->d_accum, &r->d_albedo, &r->d_normal, &r->d_final, &r->d_frame, &r->d_sort_keys,
+                          &r->d_sort_items})
     key_append(k, b->p);
   key_append(k, frames);
   key_append(k, r->info.features);
+  key_append(k, r->shade_sort_from);
   return k;
 }
 
@@ -515,6 +531,19 @@ uint32_t batch_frames_for(const hpt_renderer* r) {
   return (uint32_t)k;
 }
 
+// Sort keys and sorted entries for `frames` frames per batch while shading
+// sort is on; released when it is off.
+void alloc_sort(hpt_renderer* r, uint32_t frames) {
+  if (r->shade_sort_from == HPT_SHADE_SORT_OFF) {
+    r->d_sort_keys.release();
+    r->d_sort_items.release();
+    return;
+  }
+  const size_t qn = (size_t)kSeg * seg_geom(frames * r->npix).seg_cap;
+  r->d_sort_keys.alloc(qn);
+  r->d_sort_items.alloc(qn * 4);
+}
+
 // Sizes the path state and the queues for `frames` frames per batch.
 void alloc_paths(hpt_renderer* r, uint32_t frames) {
   const size_t n = (size_t)frames * r->npix;
@@ -529,6 +558,7 @@ void alloc_paths(hpt_renderer* r, uint32_t frames) {
   r->d_sh_o.alloc(2 * qn * 16);
   r->d_sh_d.alloc(2 * qn * 16);
   r->d_sh_c.alloc(2 * qn * 16);
+  alloc_sort(r, frames);
   r->batch_cap = frames;
 }
 
@@ -554,7 +584,7 @@ void alloc_frame_state(hpt_renderer* r) {
   r->npix = n;
   alloc_paths(r, batch_frames_for(r));
   const size_t f4 = (size_t)n * 16;
-  r->d_counts.alloc(3 * kSeg * kSegStride * sizeof(uint32_t));
+  r->d_counts.alloc((4 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
   check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
   r->d_accum.alloc(f4);
   r->d_albedo.alloc(f4);
@@ -888,6 +918,13 @@ hpt_status hpt_commit(hpt_renderer* r) {
     r->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     r->sd = sd;  // handles sharing the previous scene keep it
     upload(r->sd->d_tri_shade, r->tri_shade, s);
+    std::vector<uint8_t> tri_key(r->tri_prim.size());
+    for (size_t i = 0; i < tri_key.size(); ++i) {
+      const uint32_t m = r->prims[r->tri_prim[i]].material_index;
+      tri_key[i] = (uint8_t)std::min<uint32_t>(m, kSortKeys - 3);  // HPT_INVALID_INDEX too
+    }
+    if (tri_key.empty()) tri_key.push_back(0);
+    upload(r->sd->d_tri_key, tri_key, s);
     upload(r->sd->d_prims, r->prims, s);
     upload(r->sd->d_materials, r->materials, s);
     upload(r->sd->d_lights, r->lights, s);
@@ -969,12 +1006,13 @@ hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, uint64_t
     uint64_t f = 0, sc = 0;
     for (const DevBuf* b : {&r->d_rng, &r->d_org, &r->d_dir, &r->d_thr, &r->d_rad, &r->d_alb, &r->d_nrm, &r->d_med,
                             &r->d_med2, &r->d_nee_e, &r->d_nee_l, &r->d_hit, &r->d_qa, &r->d_qb, &r->d_sh_o,
-                            &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill, &r->d_stats, &r->d_accum,
+                            &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill, &r->d_stats, &r->d_sort_keys,
+                            &r->d_sort_items, &r->d_accum,
                             &r->d_albedo, &r->d_normal, &r->d_final})
       f += b->bytes;
     if (r->sd)
-      for (const DevBuf* b : {&r->sd->d_nodes, &r->sd->d_ltris, &r->sd->d_tri_shade, &r->sd->d_prims,
-                              &r->sd->d_materials, &r->sd->d_lights, &r->sd->d_cameras, &r->sd->d_env_rgb,
+      for (const DevBuf* b : {&r->sd->d_nodes, &r->sd->d_ltris, &r->sd->d_tri_shade, &r->sd->d_tri_key,
+                              &r->sd->d_prims, &r->sd->d_materials, &r->sd->d_lights, &r->sd->d_cameras, &r->sd->d_env_rgb,
                               &r->sd->d_env_table, &r->sd->d_env_texels, &r->sd->d_tex_texels, &r->sd->d_tex_info,
                               &r->sd->d_srgb_lut, &r->sd->d_blue_noise})
         sc += b->bytes;
@@ -1029,6 +1067,18 @@ hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, doub
     require_committed(r);
     *builder_used = r->bvh_builder_used;
     *build_ms = r->bvh_build_ms;
+  });
+}
+
+hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce) {
+  return guarded([&] {
+    need(r, "null renderer");
+    r->shade_sort_from = first_bounce;
+    if (r->committed && r->batch_cap) {
+      set_device(r);
+      check(hipStreamSynchronize(r->stream), "shade sort");  // buffers may be in use
+      alloc_sort(r, r->batch_cap);
+    }
   });
 }
 

@@ -159,6 +159,7 @@ struct SceneView {
   const Node8* nodes;
   const LeafTri* ltris;
   const TriShade* tri_shade;
+  const uint8_t* tri_key;  // shading sort key (material bucket) of each global triangle
   const PrimInfo* prims;
   const hpt_material* materials;
   const hpt_light* lights;

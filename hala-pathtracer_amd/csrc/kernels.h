@@ -70,6 +70,21 @@ struct ShadowQueue {
   uint32_t seg_cap;  // 2 * the path segment capacity
 };
 
+// Material-sorted shading: extend writes one key per queue slot -- the
+// material bucket of the hit triangle (materials past kSortKeys - 3 share the
+// last bucket), kSortKeys - 2 for an analytic light, kSortKeys - 1 for a miss
+// -- and a counting sort groups each segment's entries by key, so the lanes
+// of a shading wave mostly share one material.  Shading a path does not
+// depend on its queue position: sorted and unsorted launches give the same
+// images bit for bit.
+constexpr uint32_t kSortKeys = 64;
+struct ShadeSort {
+  uint8_t* keys;     // one per queue slot, segment s at keys + s * seg_cap
+  uint32_t* items;   // sorted path ids, same segment layout
+  uint32_t* counts;  // entries of sorted segment s at counts[s * kSegStride]
+  uint32_t* bucket;  // kSeg * kSortKeys key counts (zero between sorts), then kSeg * kSortKeys cursors
+};
+
 struct Spill {
   uint2* buf;      // traversal stack entries beyond the LDS part, may be null
   uint32_t stride; // threads of the launch
@@ -100,8 +115,12 @@ struct StageCounters {
 // qin.items == nullptr: the first bounce, the identity queue over n_max
 // paths.  Extend zeroes the counters of qnext and sq, which the shade launch
 // after it fills.
+// keys: null, or the sort keys of qin's slots to write (see ShadeSort).
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
-                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc);
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys);
+// Groups qin's entries by the keys extend wrote: ss.items and ss.counts form
+// the segmented queue shading then reads (qin may be the identity queue).
+void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss);
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp,
                   const PathState& ps, const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq,
                   Spill sp);

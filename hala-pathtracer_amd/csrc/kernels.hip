@@ -168,9 +168,17 @@ __device__ __forceinline__ LaneRange lane_range(const uint32_t* counts, uint32_t
   return r;
 }
 
-template <bool COUNT>
+// Shading sort key of a closest-hit record (see ShadeSort).
+__device__ __forceinline__ uint8_t shade_key(const SceneView& sv, uint32_t id) {
+  if (id == MISS_ID) return (uint8_t)(kSortKeys - 1);
+  if (id & LIGHT_BIT) return (uint8_t)(kSortKeys - 2);
+  return sv.tri_key[id];
+}
+
+template <bool COUNT, bool SORT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
-                                                   ShadowQueue sq, Spill sp, unsigned long long* stats) {
+                                                   ShadowQueue sq, Spill sp, unsigned long long* stats,
+                                                   uint8_t* keys) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
   if (gid < kSeg) {  // the shade launch after this one appends to these
@@ -182,14 +190,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
   auto entry = [&](uint32_t j) { return items ? items[j] : j; };
   LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
   uint32_t cv = 0, ct = 0;
-  auto store = [&](uint32_t p, const TraceResult& r) {
+  auto store = [&](uint32_t p, uint32_t j, const TraceResult& r) {
     ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+    if (SORT) keys[lr.base + j] = shade_key(sv, r.id);
   };
   if (sv.node_count == 0) {  // lights only
     for (uint32_t i = 0, j = lr.j; i < kRaysPerLane && j < lr.n; ++i, j += 64) {
       const uint32_t p = entry(j);
       const float4 o4 = ps.org[p], d4 = ps.dir[p];
-      store(p, trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st));
+      store(p, j, trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st));
     }
     return;
   }
@@ -217,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
       TraceResult r{0.0f, 0.0f, 0.0f, MISS_ID};
       if (w.found) r = TraceResult{w.best_t, w.bu, w.bv, w.best_id};
       closest_lights(sv, w.r.org, w.r.dir, 0.0f, WORLD_SIZE, w.found, r);
-      store(p, r);
+      store(p, j, r);
       if (COUNT) {
         cv += w.nv;
         ct += w.nt;
@@ -240,6 +249,74 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
     }
   }
   if (COUNT) add_visits(stats, cv, ct);
+}
+
+// ------------------------------------------------------------ shading sort
+// Counting sort of each queue segment by the keys extend wrote: per-block
+// key histograms in LDS summed into per-segment counts, a scan of each
+// segment's kSortKeys counts into cursors, then a scatter that reserves one
+// range per (block, key).  Order within a key is not kept (shading results
+// do not depend on it).
+
+// Entries of segment s of the sort's input: the segment count of a compacted
+// queue, or the slice [s * seg_cap, (s + 1) * seg_cap) of the first bounce's
+// identity queue over [0, n_max).
+__device__ __forceinline__ uint32_t sort_seg_count(const Queue& qin, uint32_t n_max, uint32_t s) {
+  if (qin.items) return qin.counts[s * kSegStride];
+  const uint32_t lo = s * qin.seg_cap;
+  return n_max > lo ? min(qin.seg_cap, n_max - lo) : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_count(Queue qin, uint32_t n_max, ShadeSort ss) {
+  __shared__ uint32_t h[kSortKeys];
+  const uint32_t s = blockIdx.x % kSeg, j0 = (blockIdx.x / kSeg) * kBlock, j = j0 + threadIdx.x;
+  const uint32_t n = sort_seg_count(qin, n_max, s);
+  if (j0 >= n) return;  // the whole block (n and j0 are block-uniform)
+  if (threadIdx.x < kSortKeys) h[threadIdx.x] = 0u;
+  __syncthreads();
+  if (j < n) atomicAdd(&h[ss.keys[s * qin.seg_cap + j]], 1u);
+  __syncthreads();
+  if (threadIdx.x < kSortKeys && h[threadIdx.x])
+    atomicAdd(&ss.bucket[s * kSortKeys + threadIdx.x], h[threadIdx.x]);
+}
+
+// One wave per segment, one lane per key: exclusive prefix of the counts.
+__global__ void __launch_bounds__(kSortKeys) k_sort_scan(ShadeSort ss, uint32_t seg_cap) {
+  static_assert(kSortKeys == 64, "one key per lane of a wave");
+  const uint32_t s = blockIdx.x, k = threadIdx.x;
+  uint32_t* cnt = ss.bucket + s * kSortKeys;
+  uint32_t* cur = ss.bucket + (kSeg + s) * kSortKeys;
+  const uint32_t c = cnt[k];
+  uint32_t x = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if ((int)k >= off) x += y;
+  }
+  cur[k] = s * seg_cap + (x - c);
+  cnt[k] = 0u;  // the next sort counts from zero
+  if (k == kSortKeys - 1) ss.counts[s * kSegStride] = x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(Queue qin, uint32_t n_max, ShadeSort ss) {
+  __shared__ uint32_t h[kSortKeys], base[kSortKeys];
+  const uint32_t s = blockIdx.x % kSeg, j0 = (blockIdx.x / kSeg) * kBlock, j = j0 + threadIdx.x;
+  const uint32_t n = sort_seg_count(qin, n_max, s);
+  if (j0 >= n) return;
+  if (threadIdx.x < kSortKeys) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const bool live = j < n;
+  const uint32_t slot = s * qin.seg_cap + j;
+  uint32_t key = 0u, rank = 0u, item = 0u;
+  if (live) {
+    key = ss.keys[slot];
+    item = qin.items ? qin.items[slot] : slot;
+    rank = atomicAdd(&h[key], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kSortKeys && h[threadIdx.x])
+    base[threadIdx.x] = atomicAdd(&ss.bucket[(kSeg + s) * kSortKeys + threadIdx.x], h[threadIdx.x]);
+  __syncthreads();
+  if (live) ss.items[base[key] + rank] = item;
 }
 
 // ------------------------------------------------------------------ shading
@@ -869,12 +946,28 @@ static inline uint32_t walk_blocks(uint32_t entries) {
 }
 
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
-                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc) {
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys) {
   const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap) : walk_blocks(n_max));
-  if (sc.count && sc.stats)
-    hipLaunchKernelGGL(k_extend<true>, grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats);
-  else
-    hipLaunchKernelGGL(k_extend<false>, grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats);
+  const bool count = sc.count && sc.stats;
+#define HPT_EXTEND(C, S) \
+  hipLaunchKernelGGL((k_extend<C, S>), grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys)
+  if (keys) {
+    if (count) HPT_EXTEND(true, true);
+    else HPT_EXTEND(false, true);
+  } else {
+    if (count) HPT_EXTEND(true, false);
+    else HPT_EXTEND(false, false);
+  }
+#undef HPT_EXTEND
+}
+
+void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss) {
+  // kSeg blocks per chunk of kBlock slots of the segment capacity: the
+  // geometry of a shading launch over a compacted queue.
+  const dim3 grid(kSeg * ((qin.seg_cap + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sort_count, grid, dim3(kBlock), 0, s, qin, n_max, ss);
+  hipLaunchKernelGGL(k_sort_scan, dim3(kSeg), dim3(kSortKeys), 0, s, ss, qin.seg_cap);
+  hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, s, qin, n_max, ss);
 }
 
 template <bool MED, bool VMIS, bool TRANSP>

@@ -157,6 +157,7 @@ pub const HPT_IMAGE_ALBEDO_RGBA32F: c_int = 2;
 pub const HPT_IMAGE_NORMAL_RGBA32F: c_int = 3;
 pub const HPT_LAUNCH_EAGER: u32 = 0;
 pub const HPT_LAUNCH_GRAPH: u32 = 1;
+pub const HPT_SHADE_SORT_OFF: u32 = 0xFFFF_FFFF;
 pub const HPT_BVH_HOST: u32 = 0;
 pub const HPT_BVH_GPU: u32 = 1;
 
@@ -180,6 +181,7 @@ extern "C" {
     pub fn hpt_set_frames_in_flight(r: *mut hpt_renderer, frames: u32) -> hpt_status;
     pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
     pub fn hpt_set_launch_mode(r: *mut hpt_renderer, mode: u32) -> hpt_status;
+    pub fn hpt_set_shade_sort(r: *mut hpt_renderer, first_bounce: u32) -> hpt_status;
     pub fn hpt_set_bvh_builder(r: *mut hpt_renderer, builder: u32) -> hpt_status;
     pub fn hpt_get_image_pointer(r: *mut hpt_renderer, which: c_int, ptr: *mut *mut c_void, bytes: *mut usize)
         -> hpt_status;

@@ -36,6 +36,7 @@ HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
 STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE = range(5)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
+SHADE_SORT_OFF = 0xFFFFFFFF
 BVH_HOST, BVH_GPU = 0, 1
 
 STATUS_NAMES = {
@@ -180,6 +181,7 @@ SIGNATURES = {
     "hpt_get_frames_in_flight": [C.c_void_p, C.POINTER(C.c_uint32)],
     "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
     "hpt_set_launch_mode": [C.c_void_p, C.c_uint32],
+    "hpt_set_shade_sort": [C.c_void_p, C.c_uint32],
     "hpt_set_bvh_builder": [C.c_void_p, C.c_uint32],
     "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
@@ -490,6 +492,11 @@ class HalaRenderer:
         """LAUNCH_GRAPH (the default: full batches replayed as captured HIP
         graphs) or LAUNCH_EAGER (every kernel launched); same images."""
         _check(self._lib.hpt_set_launch_mode(self._h, mode))
+
+    def set_shade_sort(self, first_bounce: int = SHADE_SORT_OFF) -> None:
+        """Group each bounce's paths by hit material before shading, from
+        bounce `first_bounce` on (SHADE_SORT_OFF: never); same images."""
+        _check(self._lib.hpt_set_shade_sort(self._h, first_bounce))
 
     def launch_stats(self) -> tuple:
         """(batches replayed as graphs, graphs captured)."""

@@ -35,6 +35,7 @@ import hpt
 
 PATH_BYTES = 288     # per path in flight: 11 float4 + float2 path fields, 2 queue slots, 2 shadow rays
 IMAGE_BYTES = 52     # per pixel: three rgba32f images + the rgba8 final image
+SORT_BYTES = 5       # per path while shading sort is on: a key byte and a sorted queue entry
 
 
 @dataclass
@@ -83,9 +84,18 @@ def launch_mode(spec: JobSpec, tuning=None, default: int = hpt.LAUNCH_GRAPH) -> 
     return default
 
 
+def shade_sort(spec: JobSpec, tuning=None) -> int:
+    if tuning is not None:
+        t = tuning.get(spec.scene, spec.width, spec.height, 0)
+        if t is not None:
+            return t.shade_sort
+    return hpt.SHADE_SORT_OFF
+
+
 def footprint(spec: JobSpec, max_frames: int = 32, tuning=None) -> int:
     px = spec.width * spec.height
-    return px * (frames_in_flight(spec, max_frames, tuning) * PATH_BYTES + IMAGE_BYTES)
+    per_path = PATH_BYTES + (SORT_BYTES if shade_sort(spec, tuning) != hpt.SHADE_SORT_OFF else 0)
+    return px * (frames_in_flight(spec, max_frames, tuning) * per_path + IMAGE_BYTES)
 
 
 class ResidentScene:
@@ -127,6 +137,7 @@ class Server:
         r.frames_in_flight = frames_in_flight(spec, self.max_frames, self.tuning)
         r.share_scene(rs.template)
         r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
+        r.set_shade_sort(shade_sort(spec, self.tuning))
         return r
 
     def _run_job(self, spec: JobSpec, t0: float, admitted: float, results: list, fp: int):

@@ -338,6 +338,16 @@ HPT_API hpt_status hpt_set_bvh_builder(hpt_renderer* r, uint32_t builder);
 /* The builder the last hpt_commit used and its wall time in milliseconds. */
 HPT_API hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, double* build_ms);
 HPT_API hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode);
+
+/* Material-sorted shading (no counterpart in the reference, whose ray
+ * tracing pipeline dispatches hit shaders itself).  From bounce
+ * `first_bounce` on, the extension kernel writes each hit's material bucket
+ * and the paths of every queue segment are grouped by it before shading, so
+ * a shading wave mostly runs one material's code and parameters.  Images are
+ * the same bit for bit whatever the setting; it changes only speed, and the
+ * autotuner measures it.  HPT_SHADE_SORT_OFF (the default) never sorts. */
+#define HPT_SHADE_SORT_OFF 0xFFFFFFFFu
+HPT_API hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce);
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
 

@@ -155,6 +155,16 @@ def test_autotune_candidates_and_cache(tmp_path):
     assert serve.frames_in_flight(serve.JobSpec(1, "cornell", 256, 256, 10, 8), 16, d) == 10
     assert serve.frames_in_flight(serve.JobSpec(2, "other", 256, 256, 100, 8), 16, d) == 16
     assert serve.footprint(spec, 16, d) == 256 * 256 * (32 * serve.PATH_BYTES + serve.IMAGE_BYTES)
+    assert serve.shade_sort(spec, d) == hpt.SHADE_SORT_OFF
+    # a tuned shading sort costs its key and sorted entry per path; old caches load without it
+    c.put("disney", 64, 64, 0, autotune.Tuning(8, hpt.LAUNCH_GRAPH, 500.0, {"8/graph/s1": 500.0}, 0.5, 1))
+    c.save(path)
+    d = autotune.TuningCache.load(path)
+    sspec = serve.JobSpec(3, "disney", 64, 64, 100, 8)
+    assert serve.shade_sort(sspec, d) == 1
+    assert serve.footprint(sspec, 16, d) == 64 * 64 * (8 * (serve.PATH_BYTES + serve.SORT_BYTES) + serve.IMAGE_BYTES)
+    assert autotune.Tuning(**{"frames_in_flight": 4, "launch": 1, "msamples_per_s": 1.0}).shade_sort == \
+        hpt.SHADE_SORT_OFF
 
 
 @pytest.mark.gpu
@@ -167,6 +177,8 @@ def test_autotune_picks_a_measured_candidate():
     finally:
         rs.close()
     assert set(t.table) >= {"2/graph", "4/graph", "8/graph", "16/graph"}
+    assert sum(1 for k in t.table if "/s" in k) == 3  # sort from bounce 0, 1, 2 at the best batch
+    assert t.shade_sort in (hpt.SHADE_SORT_OFF, 0, 1, 2)
     best = max(t.table.values())
     assert t.msamples_per_s >= 0.97 * best - 1e-9
     assert t.frames_in_flight in (2, 4, 8, 16) and t.launch in (hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER)
