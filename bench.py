@@ -378,7 +378,8 @@ def serving_bench(args):
         "config": {"workload": "serving", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
                    "resident_scenes": {k: v.scene.triangle_count for k, v in resident.items()},
                    "budget_gib": args.budget_gib, "job_frames_in_flight": args.job_frames,
-                   "autotuned": {k: [v.frames_in_flight, "graph" if v.launch == hpt.LAUNCH_GRAPH else "eager"]
+                   "autotuned": {k: [v.frames_in_flight, "graph" if v.launch == hpt.LAUNCH_GRAPH else "eager",
+                                     "sort off" if v.shade_sort == hpt.SHADE_SORT_OFF else f"sort from {v.shade_sort}"]
                                  for k, v in tuning.entries.items()} if tuning else None,
                    "autotune_seconds": round(t_tune, 2), "launch": args.launch,
                    "offered_gsamples": round(offered / 1e9, 3),
