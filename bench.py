@@ -362,11 +362,16 @@ def serving_bench(args):
         t_tune = time.perf_counter() - t0
         log(f"autotuned {len(tuning.entries)} (scene, resolution) classes in {t_tune:.1f} s")
     srv = serve.Server(resident, int(args.budget_gib * 2**30), max_frames=args.job_frames, tuning=tuning,
-                       launch=hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
+                       launch=hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER,
+                       wavefront=args.wavefront, step_paths=int(args.step_mpaths * 2**20))
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
     summ = serve.summarize(res, makespan)
+    if args.wavefront and srv.step_sizes:
+        summ["wavefront_steps"] = len(srv.step_sizes)
+        summ["mean_jobs_per_step"] = round(float(np.mean(srv.step_sizes)), 2)
+        summ["max_jobs_per_step"] = int(max(srv.step_sizes))
     log(f"served: {summ}")
     for rs in resident.values():
         rs.close()
@@ -382,6 +387,8 @@ def serving_bench(args):
                                      "sort off" if v.shade_sort == hpt.SHADE_SORT_OFF else f"sort from {v.shade_sort}"]
                                  for k, v in tuning.entries.items()} if tuning else None,
                    "autotune_seconds": round(t_tune, 2), "launch": args.launch,
+                   "mode": "wavefront per scene" if args.wavefront else "stream per job",
+                   "step_mpaths": args.step_mpaths if args.wavefront else None,
                    "offered_gsamples": round(offered / 1e9, 3),
                    "arrival_span_s": round(jobs[-1].arrival, 3)},
         "serving": summ, "roofline": None, "cpu_baseline": None,
@@ -402,6 +409,10 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=192.0, help="serving: device memory for admitted jobs")
     ap.add_argument("--job-frames", type=int, default=16, help="serving: frames in flight per job (at most)")
+    ap.add_argument("--wavefront", action="store_true",
+                    help="serving: the jobs of each scene share one wavefront per step (hpt_group_render)")
+    ap.add_argument("--step-mpaths", type=float, default=8.0,
+                    help="serving --wavefront: paths (in 2^20) per job and step")
     ap.add_argument("--tune", action="store_true",
                     help="serving: autotune frames in flight and launch mode per (scene, resolution) first")
     ap.add_argument("--steps", type=int, default=None, help="frames (default: the workload's spp)")

@@ -145,11 +145,35 @@ struct hpt_renderer {
   DevBuf d_frame;
   bool frame_synced = false;  // d_frame holds frame_index
   uint64_t graph_launches = 0, graph_captures = 0;
+  // Marks the end of this handle's queued work for a group step to wait on.
+  hipEvent_t ev = nullptr;
   // Profiling.
   int profiling = 0;
   std::vector<StageTimer> timers;
   size_t timers_used = 0;
   hpt_stage_stats stats[HPT_STAGE_COUNT] = {};
+};
+
+// A wavefront shared by several jobs of one scene (hpt_group_render): its own
+// path state, queues and counters, sized for the largest step so far.
+struct hpt_group {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  // End of each of the last two steps: a step waits for the one two before it,
+  // so at most two steps are queued (the host can prepare the next step while
+  // the device runs the current one, and jobs admitted meanwhile join soon).
+  hipEvent_t step_end[2] = {nullptr, nullptr};
+  uint32_t launch_mode = HPT_LAUNCH_GRAPH;
+  uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
+  size_t cap = 0;  // paths the path buffers hold
+  DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
+  DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
+  // The last captured step and the key of the step before this one: a step
+  // repeated with the same jobs, frames and buffers is captured and replayed.
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey, last_key;
+  uint64_t steps = 0, graph_launches = 0, graph_captures = 0;
 };
 
 namespace {
@@ -360,46 +384,63 @@ FrameParams frame_params(const hpt_renderer* r, uint32_t camera_index) {
   return fp;
 }
 
-// Traces `frames` consecutive frames (starting at r->frame_index, or at the
-// device counter `frame_ptr` when given) as one wavefront of
-// frames * width * height paths.
-void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const uint32_t* frame_ptr = nullptr) {
-  const SceneView sv = scene_view(r);
-  FrameParams fp = frame_params(r, camera_index);
-  fp.batch_frames = frames;
-  if (frame_ptr) {
-    fp.frame_index = 0;
-    fp.frame_ptr = frame_ptr;
-  }
-  const PathState ps = path_state(r);
-  const uint32_t n = frames * r->npix;
+// One job's part of a wavefront: its frame parameters (camera, frames,
+// accumulation), its images and its slice [base, base + npaths) of the
+// path state.
+struct WaveJob {
+  FrameParams fp;
+  Images im;
+  uint32_t base, npaths;
+};
+
+// The device buffers one wavefront runs in.
+struct WaveBuffers {
+  PathState ps;
+  uint32_t *qa, *qb, *counts;
+  float4 *sh_o, *sh_d, *sh_c;
+  Spill sp;
+  uint8_t* sort_keys;
+  uint32_t* sort_items;
+};
+
+// Camera paths of every job, the bounce loop over all n paths at once
+// (extend, shade with shade_fp, shadow), then every job's accumulation.
+// `timing`: the handle whose stage timers and counters bracket the
+// launches, or null.
+void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, const FrameParams& shade_fp,
+                     const WaveBuffers& wb, const std::vector<WaveJob>& jobs, uint32_t n, uint32_t fixed_bounces,
+                     uint32_t sort_from, hpt_renderer* timing) {
   const SegGeom g = seg_geom(n);
-  uint32_t* counts = r->d_counts.as<uint32_t>();
+  uint32_t* counts = wb.counts;
   // Segment counters: path queues A and B (ping-pong), the shadow queue, the
   // sorted shading queue, then the sort's key counts and cursors.
-  const Queue qbuf[2] = {{r->d_qa.as<uint32_t>(), counts, g.seg_cap},
-                         {r->d_qb.as<uint32_t>(), counts + kSeg * kSegStride, g.seg_cap}};
-  const ShadowQueue sq{r->d_sh_o.as<float4>(), r->d_sh_d.as<float4>(), r->d_sh_c.as<float4>(),
-                       counts + 2 * kSeg * kSegStride, 2 * g.seg_cap};
-  const Spill sp = ensure_spill(r, 2 * kSeg * g.seg_cap);  // threads of the widest launch (shadow)
-  hipStream_t s = r->stream;
-  StageTimer* t;
-  const ShadeSort ss{r->d_sort_keys.as<uint8_t>(), r->d_sort_items.as<uint32_t>(), counts + 3 * kSeg * kSegStride,
-                     counts + 4 * kSeg * kSegStride};
-  auto* stats = r->profiling ? r->d_stats.as<unsigned long long>() : nullptr;
-  const bool count = (r->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
+  const Queue qbuf[2] = {{wb.qa, counts, g.seg_cap}, {wb.qb, counts + kSeg * kSegStride, g.seg_cap}};
+  const ShadowQueue sq{wb.sh_o, wb.sh_d, wb.sh_c, counts + 2 * kSeg * kSegStride, 2 * g.seg_cap};
+  const ShadeSort ss{wb.sort_keys, wb.sort_items, counts + 3 * kSeg * kSegStride, counts + 4 * kSeg * kSegStride};
+  const PathState& ps = wb.ps;
+  const Spill sp = wb.sp;
+  StageTimer* t = nullptr;
+  auto begin = [&](int stage) {
+    if (timing) stage_begin(timing, stage, t);
+  };
+  auto end = [&] {
+    if (timing) stage_end(timing, t);
+  };
+  auto* stats = timing && timing->profiling ? timing->d_stats.as<unsigned long long>() : nullptr;
+  const bool count = timing && (timing->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
   const StageCounters sc_extend{stats ? stats + kStatsPerStage * HPT_STAGE_EXTEND : nullptr, count};
   const StageCounters sc_shadow{stats ? stats + kStatsPerStage * HPT_STAGE_SHADOW : nullptr, count};
 
-  stage_begin(r, HPT_STAGE_CAMERA, t);
-  launch_camera(s, sv, fp, ps, n);
-  stage_end(r, t);
+  begin(HPT_STAGE_CAMERA);
+  for (size_t j = 0; j < jobs.size(); ++j)
+    launch_camera(s, sv, jobs[j].fp, path_slice(ps, jobs[j].base), jobs[j].npaths, (uint32_t)j);
+  end();
   check(hipGetLastError(), "camera launch");
 
   Queue qin{nullptr, nullptr, g.seg_cap};  // the first bounce walks the camera paths in order
   int out = 0;
-  const bool transparent = (r->info.features & HPT_FEATURE_TRANSPARENT) != 0;
-  const uint32_t fixed_bounces = r->info.max_depth + 1;
+  const bool transparent = (features & HPT_FEATURE_TRANSPARENT) != 0;
+  const bool vmis = (features & HPT_FEATURE_MEDIUM) && (features & HPT_FEATURE_VOL_MIS);
   std::vector<uint32_t> seg_counts;
   for (uint32_t b = 0;; ++b) {
     if (b >= fixed_bounces) {
@@ -415,33 +456,62 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
       if (live == 0 || b > 4096) break;
     }
     const Queue& qout = qbuf[out];
-    const bool sort = b >= r->shade_sort_from && ss.keys && ss.items;
-    stage_begin(r, HPT_STAGE_EXTEND, t);
+    const bool sort = b >= sort_from && ss.keys && ss.items;
+    begin(HPT_STAGE_EXTEND);
     launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr);
-    stage_end(r, t);
-    stage_begin(r, HPT_STAGE_SHADE, t);  // the sort counts as shading time
+    end();
+    begin(HPT_STAGE_SHADE);  // the sort counts as shading time
     Queue qshade = qin;
     if (sort) {
       launch_shade_sort(s, qin, n, ss);
       qshade = Queue{ss.items, ss.counts, g.seg_cap};
     }
-    launch_shade(s, r->info.features, sv, fp, ps, qshade, n, qout, sq, sp);
-    stage_end(r, t);
-    const bool vmis = (r->info.features & HPT_FEATURE_MEDIUM) && (r->info.features & HPT_FEATURE_VOL_MIS);
+    launch_shade(s, features, sv, shade_fp, ps, qshade, n, qout, sq, sp);
+    end();
     if (!vmis) {
-      stage_begin(r, HPT_STAGE_SHADOW, t);
+      begin(HPT_STAGE_SHADOW);
       launch_shadow(s, sv, ps, sq, n, sp, sc_shadow);
-      stage_end(r, t);
+      end();
     }
     check(hipGetLastError(), "bounce launch");
     qin = qout;
     out ^= 1;
   }
-  Images im{r->d_accum.as<float4>(), r->d_albedo.as<float4>(), r->d_normal.as<float4>(), r->d_final.as<uchar4>()};
-  stage_begin(r, HPT_STAGE_ACCUMULATE, t);
-  launch_accumulate(s, fp, ps, im, r->npix);
-  stage_end(r, t);
+  begin(HPT_STAGE_ACCUMULATE);
+  for (const WaveJob& j : jobs) launch_accumulate(s, j.fp, path_slice(ps, j.base), j.im, j.fp.pixels);
+  end();
   check(hipGetLastError(), "accumulate launch");
+}
+
+Images images_of(const hpt_renderer* r) {
+  return Images{r->d_accum.as<float4>(), r->d_albedo.as<float4>(), r->d_normal.as<float4>(), r->d_final.as<uchar4>()};
+}
+
+// Traces `frames` consecutive frames (starting at r->frame_index, or at the
+// device counter `frame_ptr` when given) as one wavefront of
+// frames * width * height paths.
+void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const uint32_t* frame_ptr = nullptr) {
+  FrameParams fp = frame_params(r, camera_index);
+  fp.batch_frames = frames;
+  if (frame_ptr) {
+    fp.frame_index = 0;
+    fp.frame_ptr = frame_ptr;
+  }
+  const uint32_t n = frames * r->npix;
+  WaveBuffers wb;
+  wb.ps = path_state(r);
+  wb.qa = r->d_qa.as<uint32_t>();
+  wb.qb = r->d_qb.as<uint32_t>();
+  wb.counts = r->d_counts.as<uint32_t>();
+  wb.sh_o = r->d_sh_o.as<float4>();
+  wb.sh_d = r->d_sh_d.as<float4>();
+  wb.sh_c = r->d_sh_c.as<float4>();
+  wb.sp = ensure_spill(r, 2 * kSeg * seg_geom(n).seg_cap);  // threads of the widest launch (shadow)
+  wb.sort_keys = r->d_sort_keys.as<uint8_t>();
+  wb.sort_items = r->d_sort_items.as<uint32_t>();
+  const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
+  trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
+                  r->shade_sort_from, r);
 }
 
 template <typename T>
@@ -611,6 +681,71 @@ ExrImage decode_exr_or_throw(const void* data, size_t size, bool pixels) {
   if (res != kExrOk)
     throw Fail(res == kExrUnsupported ? HPT_ERROR_UNSUPPORTED : HPT_ERROR_INVALID_ARGUMENT, "OpenEXR: " + err);
   return img;
+}
+
+}  // namespace
+
+namespace {
+
+// Sizes a group's buffers for n paths (grow only, with a quarter of slack so
+// that steps of slowly growing size do not reallocate each time).
+void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth) {
+  // Buffers are freed or replaced only once the queued steps that use them end.
+  const bool sort = g->shade_sort_from != HPT_SHADE_SORT_OFF;
+  const size_t spill_need = bvh_depth > (uint32_t)kShortStack
+                                ? (size_t)(bvh_depth - kShortStack) * 2 * kSeg *
+                                      seg_geom((uint32_t)std::max<size_t>(n, g->cap)).seg_cap * sizeof(uint2)
+                                : 0;
+  if (n > g->cap || sort != (g->d_sort_keys.p != nullptr) || g->d_spill.bytes < spill_need)
+    check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
+  if (n > g->cap) {
+    const size_t cap = std::max<size_t>(n, std::min<size_t>(g->cap + g->cap / 4, 0x7FFFFFFFu));
+    const size_t f4 = cap * 16;
+    for (DevBuf* b : {&g->d_rng, &g->d_org, &g->d_dir, &g->d_thr, &g->d_rad, &g->d_alb, &g->d_nrm, &g->d_med,
+                      &g->d_nee_e, &g->d_nee_l, &g->d_hit})
+      b->alloc(f4);
+    g->d_med2.alloc(cap * 8);
+    g->d_job.alloc(cap * 2);
+    const size_t qn = (size_t)kSeg * seg_geom((uint32_t)cap).seg_cap;
+    g->d_qa.alloc(qn * 4);
+    g->d_qb.alloc(qn * 4);
+    g->d_sh_o.alloc(2 * qn * 16);
+    g->d_sh_d.alloc(2 * qn * 16);
+    g->d_sh_c.alloc(2 * qn * 16);
+    g->d_sort_keys.release();
+    g->d_sort_items.release();
+    g->cap = cap;
+  }
+  const size_t qn = (size_t)kSeg * seg_geom((uint32_t)g->cap).seg_cap;
+  if (g->shade_sort_from != HPT_SHADE_SORT_OFF) {
+    if (!g->d_sort_keys.p) g->d_sort_keys.alloc(qn);
+    if (!g->d_sort_items.p) g->d_sort_items.alloc(qn * 4);
+  } else {
+    g->d_sort_keys.release();
+    g->d_sort_items.release();
+  }
+  if (bvh_depth > (uint32_t)kShortStack) {
+    const size_t entries = (size_t)(bvh_depth - kShortStack) * 2 * qn;  // threads of a shadow launch
+    if (g->d_spill.bytes < entries * sizeof(uint2)) g->d_spill.alloc(entries * sizeof(uint2));
+  }
+  if (!g->d_counts.p) {
+    g->d_counts.alloc((4 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
+    check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
+  }
+  if (!g->d_limits.p) g->d_limits.alloc(kMaxWaveJobs * sizeof(uint2));
+}
+
+// Environment and light settings shading reads once per launch: equal for
+// every job of a group step.
+bool same_environment(const hpt_renderer* a, const hpt_renderer* b) {
+  return a->env_type == b->env_type && a->env_w == b->env_w && a->env_h == b->env_h &&
+         a->env_total == b->env_total && a->env_rotation == b->env_rotation &&
+         a->env_intensity == b->env_intensity && a->num_lights == b->num_lights &&
+         std::memcmp(a->ground, b->ground, sizeof a->ground) == 0 && std::memcmp(a->sky, b->sky, sizeof a->sky) == 0;
+}
+
+void ensure_event(hipEvent_t& e) {
+  if (!e) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
 }
 
 }  // namespace
@@ -1249,6 +1384,7 @@ void hpt_destroy(hpt_renderer* r) {
   (void)hipSetDevice(r->info.device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+  if (r->ev) (void)hipEventDestroy(r->ev);
   for (auto& t : r->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -1337,6 +1473,235 @@ hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out)
       out->tri_tests = sum[2];
     }
   });
+}
+
+hpt_status hpt_group_create(int32_t device, hpt_group** out) {
+  return guarded([&] {
+    need(out != nullptr, "null argument");
+    *out = nullptr;
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    need(device >= 0 && device < n, "device ordinal out of range");
+    std::unique_ptr<hpt_group> g(new hpt_group());
+    g->device = device;
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
+    ensure_event(g->done);
+    ensure_event(g->step_end[0]);
+    ensure_event(g->step_end[1]);
+    *out = g.release();
+  });
+}
+
+hpt_status hpt_group_set_launch_mode(hpt_group* g, uint32_t mode) {
+  return guarded([&] {
+    need(g, "null group");
+    need(mode == HPT_LAUNCH_EAGER || mode == HPT_LAUNCH_GRAPH, "unknown launch mode");
+    g->launch_mode = mode;
+  });
+}
+
+hpt_status hpt_group_set_shade_sort(hpt_group* g, uint32_t first_bounce) {
+  return guarded([&] {
+    need(g, "null group");
+    g->shade_sort_from = first_bounce;
+  });
+}
+
+hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint32_t* frames,
+                            const uint32_t* camera_indices, uint32_t count) {
+  return guarded([&] {
+    need(g && jobs && frames, "null argument");
+    need(count >= 1 && count <= kMaxWaveJobs, "a group step takes 1 to 256 jobs");
+    hpt_renderer* r0 = jobs[0];
+    need(r0 != nullptr, "null job");
+    for (uint32_t i = 0; i < count; ++i) {
+      hpt_renderer* r = jobs[i];
+      need(r != nullptr, "null job");
+      require_committed(r);
+      need(r->info.device == g->device, "job on another device than the group");
+      need(r->sd == r0->sd, "the jobs of a group step must render one committed scene (hpt_share_scene)");
+      need(r->info.features == r0->info.features, "the jobs of a group step must use one shader permutation");
+      need(same_environment(r, r0), "the jobs of a group step must share environment and light settings");
+      need(!camera_indices || camera_indices[i] < r->camera_count, "camera index out of range");
+      for (uint32_t j = 0; j < i; ++j) need(jobs[j] != r, "a job appears twice in a group step");
+    }
+    check(hipSetDevice(g->device), "hipSetDevice");
+    // Frames per job, capped like hpt_render (max_samples, 32-bit frame indices).
+    std::vector<uint32_t> k(count);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      const hpt_renderer* r = jobs[i];
+      uint64_t left = frames[i];
+      if (r->info.max_samples)
+        left = r->frame_index >= r->info.max_samples ? 0 : std::min<uint64_t>(left, r->info.max_samples - r->frame_index);
+      left = std::min<uint64_t>(left, 0xFFFFFFFFull - r->frame_index);
+      k[i] = (uint32_t)left;
+      total += left * r->npix;
+    }
+    need(2 * total < 0xFFFFFFFFull, "group step too large: queue indices are 32-bit");
+    if (total == 0) return;
+    const uint32_t n = (uint32_t)total;
+    check(hipEventSynchronize(g->step_end[g->steps & 1]), "hipEventSynchronize");  // the step two before
+    group_alloc(g, n, r0->sd->bvh_depth);
+    // The step runs after each job's earlier work on its own stream.
+    for (uint32_t i = 0; i < count; ++i) {
+      ensure_event(jobs[i]->ev);
+      check(hipEventRecord(jobs[i]->ev, jobs[i]->stream), "hipEventRecord");
+      check(hipStreamWaitEvent(g->stream, jobs[i]->ev, 0), "hipStreamWaitEvent");
+    }
+    check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
+
+    const bool graph = g->launch_mode == HPT_LAUNCH_GRAPH && !(r0->info.features & HPT_FEATURE_TRANSPARENT);
+    std::vector<WaveJob> wj;
+    std::vector<hpt_renderer*> live;
+    JobLimits lim{};
+    uint32_t base = 0, max_depth = 0;
+    std::vector<uint8_t> key;
+    for (uint32_t i = 0; i < count; ++i) {
+      hpt_renderer* r = jobs[i];
+      if (!k[i]) continue;
+      if (graph && !r->d_frame.p) r->d_frame.alloc(256);
+      FrameParams fp = frame_params(r, camera_indices ? camera_indices[i] : 0);
+      fp.batch_frames = k[i];
+      FrameParams kfp = fp;
+      kfp.frame_index = 0;
+      kfp.frame_ptr = r->d_frame.as<uint32_t>();
+      key_append(key, kfp);
+      key_append(key, images_of(r));
+      lim.v[wj.size()] = make_uint2(r->info.max_depth, r->info.rr_depth);
+      max_depth = std::max(max_depth, r->info.max_depth);
+      wj.push_back(WaveJob{fp, images_of(r), base, k[i] * r->npix});
+      live.push_back(r);
+      base += k[i] * r->npix;
+    }
+    const SceneView sv = scene_view(r0);
+    FrameParams shade_fp = frame_params(r0, 0);
+    shade_fp.frame_index = 0;  // shading reads no frame index (the path RNG carries it)
+    shade_fp.max_depth = max_depth;
+    shade_fp.job_limits = g->d_limits.as<uint2>();
+    WaveBuffers wb{};
+    wb.ps = PathState{g->d_rng.as<uint4>(),   g->d_org.as<float4>(),   g->d_dir.as<float4>(),
+                      g->d_thr.as<float4>(),  g->d_rad.as<float4>(),   g->d_alb.as<float4>(),
+                      g->d_nrm.as<float4>(),  g->d_med.as<float4>(),   g->d_med2.as<float2>(),
+                      g->d_nee_e.as<float4>(), g->d_nee_l.as<float4>(), g->d_hit.as<float4>(),
+                      g->d_job.as<uint16_t>()};
+    wb.qa = g->d_qa.as<uint32_t>();
+    wb.qb = g->d_qb.as<uint32_t>();
+    wb.counts = g->d_counts.as<uint32_t>();
+    wb.sh_o = g->d_sh_o.as<float4>();
+    wb.sh_d = g->d_sh_d.as<float4>();
+    wb.sh_c = g->d_sh_c.as<float4>();
+    wb.sp = Spill{g->d_spill.as<uint2>(), 2 * kSeg * seg_geom(n).seg_cap};
+    if (!g->d_spill.p) wb.sp = Spill{nullptr, 0};
+    wb.sort_keys = g->d_sort_keys.as<uint8_t>();
+    wb.sort_items = g->d_sort_items.as<uint32_t>();
+    key_append(key, sv);
+    key_append(key, shade_fp);
+    key_append(key, wb.ps);
+    for (const void* q : {(const void*)wb.qa, (const void*)wb.qb, (const void*)wb.counts, (const void*)wb.sh_o,
+                          (const void*)wb.sh_d, (const void*)wb.sh_c, (const void*)wb.sp.buf,
+                          (const void*)wb.sort_keys, (const void*)wb.sort_items})
+      key_append(key, q);
+    key_append(key, wb.sp.stride);
+    key_append(key, n);
+    key_append(key, r0->info.features);
+    key_append(key, g->shade_sort_from);
+    for (const WaveJob& j : wj) key_append(key, j.base);
+    const uint32_t njobs = (uint32_t)wj.size();
+    const uint32_t fixed_bounces = max_depth + 1;
+    const bool replay = graph && key == g->last_key;
+    g->last_key = key;
+    if (replay) {
+      for (hpt_renderer* r : live)
+        if (!r->frame_synced) {
+          launch_frame_counter(g->stream, r->d_frame.as<uint32_t>(), (uint32_t)r->frame_index, false);
+          r->frame_synced = true;
+        }
+      check(hipGetLastError(), "frame counter launch");
+      if (!g->gexec || key != g->gkey) {
+        if (g->gexec) {
+          check(hipStreamSynchronize(g->stream), "graph");
+          check(hipGraphExecDestroy(g->gexec), "hipGraphExecDestroy");
+          g->gexec = nullptr;
+        }
+        std::vector<WaveJob> cj = wj;
+        for (size_t i = 0; i < cj.size(); ++i) {
+          cj[i].fp.frame_index = 0;
+          cj[i].fp.frame_ptr = live[i]->d_frame.as<uint32_t>();
+        }
+        hipGraph_t gr = nullptr;
+        check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+          launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
+          trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, cj, n, fixed_bounces, g->shade_sort_from,
+                          nullptr);
+          for (size_t i = 0; i < cj.size(); ++i)
+            launch_frame_counter(g->stream, live[i]->d_frame.as<uint32_t>(), cj[i].fp.batch_frames, true);
+          check(hipGetLastError(), "group launch");
+        } catch (...) {
+          (void)hipStreamEndCapture(g->stream, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        check(hipStreamEndCapture(g->stream, &gr), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        check(e, "hipGraphInstantiate");
+        g->gkey = key;
+        g->graph_captures++;
+      }
+      check(hipGraphLaunch(g->gexec, g->stream), "hipGraphLaunch");
+      g->graph_launches++;
+    } else {
+      launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
+      check(hipGetLastError(), "job limits launch");
+      trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, wj, n, fixed_bounces, g->shade_sort_from,
+                      nullptr);
+      for (hpt_renderer* r : live) r->frame_synced = false;
+    }
+    for (size_t i = 0; i < live.size(); ++i) live[i]->frame_index += wj[i].fp.batch_frames;
+    // Later work queued on each job's stream (reads, renders) waits for this step.
+    check(hipEventRecord(g->done, g->stream), "hipEventRecord");
+    for (hpt_renderer* r : live) check(hipStreamWaitEvent(r->stream, g->done, 0), "hipStreamWaitEvent");
+    check(hipEventRecord(g->step_end[g->steps & 1], g->stream), "hipEventRecord");
+    g->steps++;
+  });
+}
+
+hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_launches, uint64_t* device_bytes) {
+  return guarded([&] {
+    need(g && steps && graph_launches && device_bytes, "null argument");
+    uint64_t b = 0;
+    for (const DevBuf* d : {&g->d_rng, &g->d_org, &g->d_dir, &g->d_thr, &g->d_rad, &g->d_alb, &g->d_nrm, &g->d_med,
+                            &g->d_med2, &g->d_nee_e, &g->d_nee_l, &g->d_hit, &g->d_job, &g->d_qa, &g->d_qb,
+                            &g->d_sh_o, &g->d_sh_d, &g->d_sh_c, &g->d_counts, &g->d_spill, &g->d_sort_keys,
+                            &g->d_sort_items, &g->d_limits})
+      b += d->bytes;
+    *steps = g->steps;
+    *graph_launches = g->graph_launches;
+    *device_bytes = b;
+  });
+}
+
+hpt_status hpt_group_wait_idle(hpt_group* g) {
+  return guarded([&] {
+    need(g, "null group");
+    check(hipSetDevice(g->device), "hipSetDevice");
+    check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
+  });
+}
+
+void hpt_group_destroy(hpt_group* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->done) (void)hipEventDestroy(g->done);
+  for (hipEvent_t e : g->step_end)
+    if (e) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
 }
 
 }  // extern "C"

@@ -199,6 +199,9 @@ struct FrameParams {
   // When non-null the batch's first frame is read from this device counter
   // instead of frame_index (a captured batch replayed as a graph advances it).
   const uint32_t* frame_ptr;
+  // Shading in a wavefront of several jobs: (max_depth, rr_depth) of job j at
+  // job_limits[j], the job of path p in the path state; null for one job.
+  const uint2* job_limits;
 };
 
 }  // namespace hpt

@@ -20,7 +20,27 @@ struct PathState {
   float4* nee_env;    // sum of unoccluded environment NEE terms
   float4* nee_light;  // sum of unoccluded light NEE terms
   float4* hit;        // closest hit (t, u, v, id bits)
+  uint16_t* job;      // job of each path in a wavefront of several jobs, else null
 };
+
+// Path state arrays from path `base` on: the slice of one job in a wavefront.
+inline PathState path_slice(const PathState& ps, uint32_t base) {
+  PathState s = ps;
+  s.rng += base;
+  s.org += base;
+  s.dir += base;
+  s.thr += base;
+  s.rad += base;
+  s.albedo += base;
+  s.normal += base;
+  s.med += base;
+  s.med2 += base;
+  s.nee_env += base;
+  s.nee_light += base;
+  s.hit += base;
+  if (s.job) s.job += base;
+  return s;
+}
 constexpr uint32_t PATH_ANB = 1u;             // any_non_specular_bounce
 constexpr uint32_t PATH_IN_MEDIUM = 2u;       // is_in_medium
 constexpr uint32_t PATH_SURFACE_SCATTER = 4u; // is_surface_scatter (MEDIUM without VOL_MIS)
@@ -98,8 +118,9 @@ struct Images {
   uchar4* final_rgba8;
 };
 
+// job: the index stored in ps.job (when non-null) for each path launched.
 void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                   uint32_t npaths);
+                   uint32_t npaths, uint32_t job = 0);
 // Device counters of one stage, 64-bit, spread like the queue counters:
 // rays at stats[seg * kStatStride], node visits at stats[(kSeg + seg) *
 // kStatStride], triangle tests at stats[(2 * kSeg + seg) * kStatStride].
@@ -126,6 +147,13 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
                   Spill sp);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc);
+// Depth limits of the jobs of one wavefront (FrameParams::job_limits), passed
+// by value so that a captured launch carries them.
+constexpr uint32_t kMaxWaveJobs = 256;
+struct JobLimits {
+  uint2 v[kMaxWaveJobs];  // (max_depth, rr_depth)
+};
+void launch_job_limits(hipStream_t s, uint2* dst, const JobLimits& lim, uint32_t count);
 // counter[0] = value, or counter[0] += value when `add` (one thread).
 void launch_frame_counter(hipStream_t s, uint32_t* counter, uint32_t value, bool add);
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im,

@@ -75,9 +75,11 @@ __device__ __forceinline__ uint32_t first_frame(const FrameParams& fp) {
 // ------------------------------------------------------------------ camera
 // pathtracer.rgen.glsl:23-54: seed, camera callable, payload initialisation.
 // A batch holds batch_frames consecutive frames, one path per pixel each.
-__global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp, PathState ps, uint32_t npaths) {
+__global__ void __launch_bounds__(kBlock) k_camera(SceneView sv, FrameParams fp, PathState ps, uint32_t npaths,
+                                                   uint32_t job) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= npaths) return;
+  if (ps.job) ps.job[p] = (uint16_t)job;
   const uint32_t j = p / fp.pixels, pix = p - j * fp.pixels;
   const uint32_t x = pix % fp.width, y = pix / fp.width;
   const uint32_t frame = first_frame(fp) + j;
@@ -395,9 +397,9 @@ __device__ __forceinline__ void apply_textures(const SceneView& sv, uint32_t id,
 // inc/lighting.glsl:7-61 (MEDIUM + VOL_MIS): closest-hit steps through
 // transparent or refractive surfaces, Beer-Lambert on exit faces.
 template <bool TRANSP>
-__device__ f3 eval_transmittance(const SceneView& sv, const FrameParams& fp, uint4& rng, f3 o, f3 d, LaneStack& st) {
+__device__ f3 eval_transmittance(const SceneView& sv, uint32_t max_depth, uint4& rng, f3 o, f3 d, LaneStack& st) {
   f3 tr = mk3(1.0f);
-  for (uint32_t depth = 0; depth < fp.max_depth; depth++) {
+  for (uint32_t depth = 0; depth < max_depth; depth++) {
     const TraceResult r = trace_closest(sv, o, d, 0.0f, WORLD_SIZE, st);
     if (r.id == MISS_ID || (r.id & LIGHT_BIT)) break;
     const HitState h = triangle_state(sv, r.id, r.t, r.u, r.v, o, d);
@@ -427,9 +429,10 @@ struct ShadowOut {
 // transmittance walk runs inline (it draws from the path RNG between the
 // environment and light samples) and the sum is returned directly.
 template <bool MED, bool VMIS, bool TRANSP>
-__device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4& rng, const Mat& mat,
-                              const Frame& fr, f3 hit_pos, f3 ffn, f3 V, float aniso, bool is_surface, bool anb,
-                              f3 weight, ShadowOut& s_env, ShadowOut& s_light, LaneStack& st) {
+__device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32_t max_depth, uint4& rng,
+                              const Mat& mat, const Frame& fr, f3 hit_pos, f3 ffn, f3 V, float aniso,
+                              bool is_surface, bool anb, f3 weight, ShadowOut& s_env, ShadowOut& s_light,
+                              LaneStack& st) {
   f3 Ld = mk3(0.0f);
   const f3 scatter_pos = hit_pos + ffn * EPS;
   s_env.valid = false;
@@ -440,7 +443,7 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4&
     const f3 light_dir = sample_env_map(rng, sv, fp, Li, light_pdf);
     if (light_pdf > 0.0f) {
       if (VMIS) {
-        const f3 trm = eval_transmittance<TRANSP>(sv, fp, rng, scatter_pos, light_dir, st);
+        const f3 trm = eval_transmittance<TRANSP>(sv, max_depth, rng, scatter_pos, light_dir, st);
         if (luminance(trm) > 0.0f) {
           Li *= trm;
           float pdf = 0.0f;
@@ -484,7 +487,7 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4&
     f3 Li = ls.emission;
     if (light_pdf > 0.0f && dot(light_dir, ls.normal) < 0.0f) {
       if (VMIS) {
-        const f3 trm = eval_transmittance<TRANSP>(sv, fp, rng, scatter_pos, light_dir, st);
+        const f3 trm = eval_transmittance<TRANSP>(sv, max_depth, rng, scatter_pos, light_dir, st);
         if (luminance(trm) > 0.0f) {
           Li *= trm;
           float pdf = 0.0f;
@@ -533,7 +536,7 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint4&
 #ifndef HPT_SHADE_VMIS_WAVES_PER_EU
 #define HPT_SHADE_VMIS_WAVES_PER_EU 5
 #endif
-template <bool MED, bool VMIS, bool TRANSP>
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(VMIS ? HPT_SHADE_VMIS_WAVES_PER_EU : HPT_SHADE_WAVES_PER_EU)))
     k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
@@ -552,6 +555,14 @@ __global__ void __launch_bounds__(kBlock)
     f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
     float pdf = o4.w;
     uint32_t depth = __float_as_uint(d4.w);
+    // Depth limits: the launch's, or in a wavefront of several jobs (JOBS)
+    // those of the path's job.
+    uint32_t max_depth = fp.max_depth, rr_depth = fp.rr_depth;
+    if (JOBS) {
+      const uint2 lim = fp.job_limits[ps.job[p]];
+      max_depth = lim.x;
+      rr_depth = lim.y;
+    }
     uint32_t flags = __float_as_uint(t4.w);
     bool anb = (flags & PATH_ANB) != 0;
     bool in_medium = (flags & PATH_IN_MEDIUM) != 0;
@@ -626,7 +637,7 @@ __global__ void __launch_bounds__(kBlock)
                 T *= med_color;
                 org = org + dir * scatter_dist;
                 h.pos = org;
-                const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, rng, m, fr, h.pos, h.ffnormal, -dir,
+                const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir,
                                                                   med_aniso, false, anb, T, s_env, s_light, st);
                 radiance += Ld * T;
                 const float r1 = rnd(rng);
@@ -648,7 +659,7 @@ __global__ void __launch_bounds__(kBlock)
           }
           if (!pass) {
             if (kScatterFlag) surface_scatter = true;
-            const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, rng, m, fr, h.pos, h.ffnormal, -dir, med_aniso,
+            const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir, med_aniso,
                                                               true, anb, T, s_env, s_light, st);
             radiance += T * Ld;
             f3 L;
@@ -676,8 +687,8 @@ __global__ void __launch_bounds__(kBlock)
             }
           }
         }
-        if (alive && depth >= fp.max_depth) alive = false;
-        if (alive && depth >= fp.rr_depth && fp.rr_depth > 0) {
+        if (alive && depth >= max_depth) alive = false;
+        if (alive && depth >= rr_depth && rr_depth > 0) {
           const float q = fminf(fmaxf(T.x, fmaxf(T.y, T.z)) + 0.001f, 0.95f);
           if (rnd(rng) > q) alive = false;
           else T /= q;
@@ -929,8 +940,9 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sv, const hpt_r
 // ------------------------------------------------------------------ launchers
 static inline uint32_t grid_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
-void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps, uint32_t npaths) {
-  hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths)), dim3(kBlock), 0, s, sv, fp, ps, npaths);
+void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps, uint32_t npaths,
+                   uint32_t job) {
+  hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths)), dim3(kBlock), 0, s, sv, fp, ps, npaths, job);
 }
 
 // Blocks of a launch over qin: the identity queue's grid, or kSeg blocks per
@@ -973,8 +985,12 @@ void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const Sh
 template <bool MED, bool VMIS, bool TRANSP>
 static void shade_variant(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                           const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp) {
-  hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp, ps,
-                     qin, n_max, qout, sq, sp);
+  if (fp.job_limits)
+    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, true>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp,
+                       ps, qin, n_max, qout, sq, sp);
+  else
+    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, false>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp,
+                       ps, qin, n_max, qout, sq, sp);
 }
 
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
@@ -1011,6 +1027,14 @@ void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, ui
 
 void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps, const Images& im, uint32_t npaths) {
   hipLaunchKernelGGL(k_accumulate, dim3(grid_for(npaths)), dim3(kBlock), 0, s, fp, ps, im, npaths);
+}
+
+__global__ void k_job_limits(uint2* dst, JobLimits lim, uint32_t count) {
+  if (threadIdx.x < count) dst[threadIdx.x] = lim.v[threadIdx.x];
+}
+
+void launch_job_limits(hipStream_t s, uint2* dst, const JobLimits& lim, uint32_t count) {
+  hipLaunchKernelGGL(k_job_limits, dim3(1), dim3(kMaxWaveJobs), 0, s, dst, lim, count);
 }
 
 // The device frame counter of graph replays: set from the host's value, or

@@ -182,6 +182,14 @@ SIGNATURES = {
     "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
     "hpt_set_launch_mode": [C.c_void_p, C.c_uint32],
     "hpt_set_shade_sort": [C.c_void_p, C.c_uint32],
+    "hpt_group_create": [C.c_int32, C.POINTER(C.c_void_p)],
+    "hpt_group_render": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                         C.c_uint32],
+    "hpt_group_set_launch_mode": [C.c_void_p, C.c_uint32],
+    "hpt_group_set_shade_sort": [C.c_void_p, C.c_uint32],
+    "hpt_group_get_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "hpt_group_wait_idle": [C.c_void_p],
+    "hpt_group_destroy": [C.c_void_p],
     "hpt_set_bvh_builder": [C.c_void_p, C.c_uint32],
     "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
@@ -201,7 +209,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        if name == "hpt_destroy":
+        if name in ("hpt_destroy", "hpt_group_destroy"):
             fn.restype = None
         elif name == "hpt_last_error":
             fn.restype = C.c_char_p
@@ -538,6 +546,52 @@ def write_pfm(path: str, rgb: np.ndarray) -> None:
         f.write(b"PF\n%d %d\n-1.0\n" % (a.shape[1], a.shape[0]))
         f.write(a.astype("<f4").tobytes())
 
+
+
+class HalaGroup:
+    """Several jobs of one scene traced as one wavefront per step
+    (hpt_group_*): `render(jobs, frames)` renders frames[i] frames of
+    jobs[i] (HalaRenderer handles sharing one committed scene) with one
+    extension, shading and shadow launch per bounce for all of them; every
+    job ends as if it had rendered alone."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.hpt_group_create(device, C.byref(h)))
+        self._h = h
+
+    def render(self, jobs, frames, cameras=None) -> None:
+        n = len(jobs)
+        hs = (C.c_void_p * n)(*[j._h for j in jobs])
+        fr = (C.c_uint32 * n)(*[int(f) for f in frames])
+        cam = (C.c_uint32 * n)(*[int(c) for c in cameras]) if cameras is not None else None
+        _check(self._lib.hpt_group_render(self._h, hs, fr, cam, n))
+
+    def set_launch_mode(self, mode: int) -> None:
+        _check(self._lib.hpt_group_set_launch_mode(self._h, mode))
+
+    def set_shade_sort(self, first_bounce: int = SHADE_SORT_OFF) -> None:
+        _check(self._lib.hpt_group_set_shade_sort(self._h, first_bounce))
+
+    def stats(self) -> dict:
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self._lib.hpt_group_get_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"steps": a.value, "graph_launches": b.value, "device_bytes": c.value}
+
+    def wait_idle(self) -> None:
+        _check(self._lib.hpt_group_wait_idle(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.hpt_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 def decode_exr(data: bytes) -> np.ndarray:
     """An OpenEXR file's bytes as (H, W, 3) float32 RGB, row 0 at the top

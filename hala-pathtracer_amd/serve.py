@@ -18,6 +18,19 @@ process):
   active jobs (the hardware queues run streams concurrently).  When its last
   chunk ends, the job's image is read back (the latency end point), its
   handle is closed and its memory returns to the budget.
+* Wavefront mode (`wavefront=True`).  The jobs of one resident scene share
+  one wavefront (`hpt_group_render`): a worker per scene renders, step after
+  step, the next chunk of every active job of that scene together -- one
+  camera and accumulation launch per job, one extension, shading and shadow
+  launch per bounce for all of them -- so their bounces interleave in one
+  queue; a job that finishes leaves after its step and jobs admitted
+  meanwhile join the next one.  A job's chunk is sized to about `step_paths`
+  paths (frames = step_paths / pixels, at least 1), so a step costs every job
+  about the same and small jobs are not held back by large ones.  Up to two
+  steps are queued per scene: the next step is queued before the jobs that
+  ended in the previous one are read back.  Workers of different scenes
+  overlap on their own streams.  A step that repeats the previous one (same
+  jobs and chunk sizes) replays a captured HIP graph.
 * Results do not depend on the mix: a job's image is a function of its own
   scene, size, depths and frame range (the RNG is seeded by pixel and frame),
   so it is the same alone and among other jobs, bit for bit.
@@ -117,9 +130,21 @@ class ResidentScene:
         self.template.close()
 
 
+MAX_STEP_JOBS = 256  # jobs per wavefront step (hpt_group_render)
+
+
+class _Active:
+    """A job in a scene's wavefront."""
+
+    def __init__(self, spec, handle, result, fp, k):
+        self.spec, self.handle, self.result, self.fp, self.k = spec, handle, result, fp, k
+        self.done = 0
+
+
 class Server:
     def __init__(self, scenes: dict, budget_bytes: int, max_frames: int = 32, keep_images: bool = False,
-                 device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH):
+                 device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH, wavefront: bool = False,
+                 step_paths: int = 8 << 20):
         self.scenes = scenes                  # name -> ResidentScene
         self.budget = budget_bytes
         self.max_frames = max_frames
@@ -127,14 +152,20 @@ class Server:
         self.launch = launch                  # launch mode of jobs the tuning does not cover
         self.keep_images = keep_images
         self.device = device
+        self.wavefront = wavefront            # jobs of a scene share one wavefront per step
+        self.step_paths = step_paths          # paths per job and wavefront step
         self._lock = threading.Condition()
         self._used = 0
+        self._pending = {}                    # scene -> jobs admitted but not yet in its wavefront
+        self._closed = False
+        self.step_sizes = []                  # jobs per wavefront step, in step order
 
     def _make_job(self, spec: JobSpec):
         rs = self.scenes[spec.scene]
         r = hpt.HalaRenderer(spec.width, spec.height, max_depth=spec.max_depth, rr_depth=spec.rr_depth,
                              enable_tonemap=True, features=rs.features, device=self.device)
-        r.frames_in_flight = frames_in_flight(spec, self.max_frames, self.tuning)
+        # A wavefront job traces in its scene's group; its own batch stays one frame.
+        r.frames_in_flight = 1 if self.wavefront else frames_in_flight(spec, self.max_frames, self.tuning)
         r.share_scene(rs.template)
         r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
         r.set_shade_sort(shade_sort(spec, self.tuning))
@@ -164,13 +195,134 @@ class Server:
             results.append(res)
             self._lock.notify_all()
 
+    def step_frames(self, spec: JobSpec) -> int:
+        """Frames of `spec` per wavefront step: about step_paths paths."""
+        return max(1, min(spec.spp, self.max_frames * 4, round(self.step_paths / (spec.width * spec.height))))
+
+    def job_footprint(self, spec: JobSpec) -> int:
+        """Device bytes admitting `spec` takes: its frame state, or in wavefront
+        mode its share of the scene's wavefront plus its one-frame handle."""
+        if not self.wavefront:
+            return footprint(spec, self.max_frames, self.tuning)
+        px = spec.width * spec.height
+        return px * ((self.step_frames(spec) + 1) * PATH_BYTES + IMAGE_BYTES)
+
+    def _new_group(self, scene: str):
+        g = hpt.HalaGroup(self.device)
+        g.set_launch_mode(self.launch)
+        sorts = [t.shade_sort for key, t in (self.tuning.entries.items() if self.tuning else [])
+                 if key.startswith(scene + "|")]
+        if sorts:  # the choice most classes of this scene were tuned to
+            g.set_shade_sort(max(set(sorts), key=sorts.count))
+        return g
+
+    def _finish(self, a: _Active, t0: float, results: list, error: str | None = None):
+        res = a.result
+        try:
+            if error is None:
+                img = a.handle.read_image(hpt.IMAGE_ACCUM)
+                if self.keep_images:
+                    res.image = img
+        except Exception as e:
+            error = f"{type(e).__name__}: {e}"
+        res.finished = time.perf_counter() - t0
+        res.error = error
+        try:
+            if a.handle is not None:
+                a.handle.close()
+        finally:
+            with self._lock:
+                self._used -= a.fp
+                results.append(res)
+                self._lock.notify_all()
+
+    def _scene_worker(self, scene: str, t0: float, results: list):
+        """The wavefront of one scene: step after step, the next chunk of every
+        active job together; finished jobs leave, newly admitted ones join."""
+        g = None
+        active, finishing = [], []
+        try:
+            while True:
+                with self._lock:
+                    while not self._pending[scene] and not active and not finishing and not self._closed:
+                        self._lock.wait()
+                    active.extend(self._pending[scene])
+                    self._pending[scene].clear()
+                    if not active and not finishing:
+                        return
+                ended = []
+                if active:
+                    if g is None:
+                        g = self._new_group(scene)
+                    step = active[:MAX_STEP_JOBS]
+                    frames = [min(a.k, a.spec.spp - a.done) for a in step]
+                    try:
+                        g.render([a.handle for a in step], frames)  # returns with at most two steps queued
+                    except Exception as e:  # the step's jobs fail; the server goes on
+                        for a in step:
+                            active.remove(a)
+                            self._finish(a, t0, results, f"{type(e).__name__}: {e}")
+                        continue
+                    with self._lock:
+                        self.step_sizes.append(len(step))
+                    for a, n in zip(step, frames):
+                        a.done += n
+                        if a.done >= a.spec.spp:
+                            active.remove(a)
+                            ended.append(a)
+                # Jobs whose last step was queued before this one: their read-back
+                # waits for that step while the device runs the next.
+                for a in finishing:
+                    self._finish(a, t0, results)
+                finishing = ended
+        finally:
+            if g is not None:
+                g.close()
+
+    def _run_wavefront(self, specs: list) -> list:
+        results, workers = [], {}
+        t0 = time.perf_counter()
+        self._closed = False
+        for spec in specs:
+            wait = spec.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            fp = self.job_footprint(spec)
+            with self._lock:
+                while self._used + fp > self.budget:
+                    self._lock.wait()
+                self._used += fp
+            admitted = time.perf_counter() - t0
+            res = JobResult(spec.job_id, spec, admitted, 0.0)
+            k = self.step_frames(spec)
+            try:
+                a = _Active(spec, self._make_job(spec), res, fp, k)
+            except Exception as e:
+                self._finish(_Active(spec, None, res, fp, k), t0, results, f"{type(e).__name__}: {e}")
+                continue
+            with self._lock:
+                self._pending.setdefault(spec.scene, []).append(a)
+                if spec.scene not in workers:
+                    th = threading.Thread(target=self._scene_worker, args=(spec.scene, t0, results), daemon=True)
+                    workers[spec.scene] = th
+                    th.start()
+                self._lock.notify_all()
+        with self._lock:
+            self._closed = True
+            self._lock.notify_all()
+        for th in workers.values():
+            th.join()
+        return results
+
     def run(self, specs: list) -> list:
         """Serves `specs` (arrival times relative to now); returns the results
         in completion order."""
         specs = sorted(specs, key=lambda s: s.arrival)
         for s in specs:
-            if footprint(s, self.max_frames, self.tuning) > self.budget:
-                raise ValueError(f"job {s.job_id} needs {footprint(s, self.max_frames, self.tuning)} B, above the budget")
+            if self.job_footprint(s) > self.budget:
+                raise ValueError(f"job {s.job_id} needs {self.job_footprint(s)} B, above the budget")
+        if self.wavefront:
+            return self._run_wavefront(specs)
         results, threads = [], []
         t0 = time.perf_counter()
         for spec in specs:

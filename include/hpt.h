@@ -348,6 +348,34 @@ HPT_API hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode);
  * autotuner measures it.  HPT_SHADE_SORT_OFF (the default) never sorts. */
 #define HPT_SHADE_SORT_OFF 0xFFFFFFFFu
 HPT_API hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce);
+
+/* Several jobs in one wavefront (no counterpart in the reference, which
+ * renders one scene per process).  A group owns a path state, queues and a
+ * stream of its own; hpt_group_render traces `frames[i]` frames of job
+ * `jobs[i]` (camera `camera_indices[i]`, or 0 when that pointer is null) for
+ * all jobs together: one camera and one accumulation launch per job over its
+ * slice of the wavefront, and one extension, shading and shadow launch per
+ * bounce for the paths of every job, each path shaded with its own job's
+ * depth limits.  The jobs must share one committed scene (hpt_share_scene),
+ * shader permutation and environment; resolution, depths, frame counts and
+ * cameras may differ.  Every job's images and frame counter end exactly as
+ * if it had rendered those frames alone with hpt_render.  The step is queued
+ * on the group's stream after each job's earlier work, and later work on a
+ * job's own stream (hpt_read_image, hpt_wait_idle, hpt_render) waits for it.
+ * With HPT_LAUNCH_GRAPH (the default) a step that repeats the previous one
+ * (same jobs, frames and settings) is captured once and replayed as a HIP
+ * graph.  At most 256 jobs per step. */
+typedef struct hpt_group hpt_group;
+HPT_API hpt_status hpt_group_create(int32_t device, hpt_group** out);
+HPT_API hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint32_t* frames,
+                                    const uint32_t* camera_indices, uint32_t count);
+HPT_API hpt_status hpt_group_set_launch_mode(hpt_group* g, uint32_t mode);
+HPT_API hpt_status hpt_group_set_shade_sort(hpt_group* g, uint32_t first_bounce);
+/* Steps rendered, steps replayed as graphs, device bytes the group holds. */
+HPT_API hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_launches,
+                                       uint64_t* device_bytes);
+HPT_API hpt_status hpt_group_wait_idle(hpt_group* g);
+HPT_API void hpt_group_destroy(hpt_group* g);
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
 

@@ -40,11 +40,34 @@ class _FakeRenderer:
             _FakeRenderer.live -= 1
 
 
+class _FakeGroup:
+    """Stands in for a scene's wavefront: renders its jobs' chunks in turn and
+    checks that every step holds jobs of one scene."""
+
+    def __init__(self, scene):
+        self.scene = scene
+
+    def render(self, jobs, frames):
+        assert all(j.spec.scene == self.scene for j in jobs)
+        for j, n in zip(jobs, frames):
+            assert 1 <= n <= j.spec.spp
+            j.render(n)
+
+    def wait_idle(self):
+        pass
+
+    def close(self):
+        pass
+
+
 class _FakeServer(serve.Server):
     def _make_job(self, spec):
         if spec.scene == "broken":
             raise RuntimeError("device lost")
         return _FakeRenderer(spec)
+
+    def _new_group(self, scene):
+        return _FakeGroup(scene)
 
 
 def _fake_scenes():
@@ -80,6 +103,33 @@ def test_a_failing_job_is_isolated():
     assert len(res) == 5 and len(bad) == 1 and bad[0].job_id == 2 and "device lost" in bad[0].error
 
 
+def test_wavefront_mode_steps_jobs_of_a_scene_together():
+    """Continuous batching: jobs of one scene that overlap in time share
+    wavefront steps; jobs join and leave between steps; the budget holds;
+    every job gets exactly its samples."""
+    jobs = serve.poisson_jobs(40, rate=400.0, scene_names=["a", "b"], seed=2,
+                              resolutions=[(64, 64), (128, 96)], spp_range=(4, 64))
+    srv = _FakeServer(_fake_scenes(), 0, max_frames=8, keep_images=True, wavefront=True, step_paths=64 * 64 * 8)
+    budget = srv.budget = 6 * srv.job_footprint(serve.JobSpec(0, "a", 128, 96, 64, 8))
+    t0 = time.perf_counter()
+    res = srv.run(jobs)
+    assert len(res) == 40 and all(r.error is None for r in res)
+    for r in res:
+        live = [q for q in res if q.admitted <= r.admitted < q.finished]
+        assert sum(srv.job_footprint(q.spec) for q in live) <= budget
+        assert r.image[0, 0, 0] == r.spec.job_id
+    assert max(srv.step_sizes) > 1  # steps shared by several jobs
+    # chunks of about step_paths paths: 8 frames at 64x64, 3 at 128x96
+    assert srv.step_frames(serve.JobSpec(0, "a", 64, 64, 64, 8)) == 8
+    assert srv.step_frames(serve.JobSpec(0, "a", 128, 96, 64, 8)) == 3
+    assert srv.step_frames(serve.JobSpec(0, "a", 128, 96, 2, 8)) == 2
+    assert len(srv.step_sizes) >= max(-(-j.spp // srv.step_frames(j)) for j in jobs)
+    assert serve.summarize(res, time.perf_counter() - t0)["failed"] == 0
+    bad = _FakeServer(_fake_scenes(), 1 << 40, wavefront=True).run(
+        [serve.JobSpec(i, "broken" if i == 1 else "a", 32, 32, 8, 8) for i in range(3)])
+    assert sorted(r.job_id for r in bad if r.error) == [1]
+
+
 def test_job_larger_than_the_budget_is_refused():
     with pytest.raises(ValueError, match="above the budget"):
         _FakeServer(_fake_scenes(), 1000).run([serve.JobSpec(0, "a", 64, 64, 8, 8)])
@@ -97,7 +147,8 @@ def test_poisson_job_distribution():
 
 
 @pytest.mark.gpu
-def test_results_identical_alone_and_in_the_mix():
+@pytest.mark.parametrize("wavefront", [False, True], ids=["streams", "wavefront"])
+def test_results_identical_alone_and_in_the_mix(wavefront):
     scenes = {
         "cornell": serve.ResidentScene("cornell", hpt_scenes.cornell_box(), 0),
         "grid": serve.ResidentScene("grid", hpt_scenes.disney_grid(env_width=128, env_height=64, segments=16,
@@ -108,12 +159,13 @@ def test_results_identical_alone_and_in_the_mix():
     specs = [serve.JobSpec(0, "cornell", 64, 48, 20, 8), serve.JobSpec(1, "grid", 96, 54, 12, 10),
              serve.JobSpec(2, "medium", 80, 45, 8, 12), serve.JobSpec(3, "cornell", 40, 40, 33, 9),
              serve.JobSpec(4, "grid", 64, 64, 5, 8), serve.JobSpec(5, "medium", 48, 32, 17, 10)]
-    print("[gpu] serving 6 jobs together", flush=True)
-    mixed = {r.job_id: r for r in serve.Server(scenes, 1 << 34, keep_images=True).run(specs)}
+    print(f"[gpu] serving 6 jobs together (wavefront {wavefront})", flush=True)
+    srv = serve.Server(scenes, 1 << 34, max_frames=8, keep_images=True, wavefront=wavefront)
+    mixed = {r.job_id: r for r in srv.run(specs)}
     assert all(r.error is None for r in mixed.values()), [r.error for r in mixed.values()]
     for s in specs:
         print(f"[gpu] job {s.job_id} alone", flush=True)
-        alone = serve.Server(scenes, 1 << 34, keep_images=True).run([s])[0]
+        alone = serve.Server(scenes, 1 << 34, keep_images=True).run([s])[0]  # streams, 32-frame batches
         assert np.array_equal(alone.image, mixed[s.job_id].image)
     # one of them against a handle that owns its scene (no sharing at all)
     s = specs[1]
