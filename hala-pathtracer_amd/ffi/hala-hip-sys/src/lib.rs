@@ -151,6 +151,11 @@ pub struct hpt_renderer {
     _private: [u8; 0],
 }
 
+#[repr(C)]
+pub struct hpt_group {
+    _private: [u8; 0],
+}
+
 pub const HPT_IMAGE_FINAL_RGBA8: c_int = 0;
 pub const HPT_IMAGE_ACCUM_RGBA32F: c_int = 1;
 pub const HPT_IMAGE_ALBEDO_RGBA32F: c_int = 2;
@@ -182,6 +187,15 @@ extern "C" {
     pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
     pub fn hpt_set_launch_mode(r: *mut hpt_renderer, mode: u32) -> hpt_status;
     pub fn hpt_set_shade_sort(r: *mut hpt_renderer, first_bounce: u32) -> hpt_status;
+    pub fn hpt_group_create(device: i32, out: *mut *mut hpt_group) -> hpt_status;
+    pub fn hpt_group_render(g: *mut hpt_group, jobs: *const *mut hpt_renderer, frames: *const u32,
+        camera_indices: *const u32, count: u32) -> hpt_status;
+    pub fn hpt_group_set_launch_mode(g: *mut hpt_group, mode: u32) -> hpt_status;
+    pub fn hpt_group_set_shade_sort(g: *mut hpt_group, first_bounce: u32) -> hpt_status;
+    pub fn hpt_group_get_stats(g: *mut hpt_group, steps: *mut u64, graph_launches: *mut u64,
+        device_bytes: *mut u64) -> hpt_status;
+    pub fn hpt_group_wait_idle(g: *mut hpt_group) -> hpt_status;
+    pub fn hpt_group_destroy(g: *mut hpt_group);
     pub fn hpt_set_bvh_builder(r: *mut hpt_renderer, builder: u32) -> hpt_status;
     pub fn hpt_get_image_pointer(r: *mut hpt_renderer, which: c_int, ptr: *mut *mut c_void, bytes: *mut usize)
         -> hpt_status;
