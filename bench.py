@@ -221,6 +221,35 @@ def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
             "curve_spp_render_s_denoise_ms_plain_denoised": curve}
 
 
+def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort):
+    """The same target through the serving path: one job per variant
+    submitted to serve.Server -- plain at the plain tracer's threshold sample
+    count, denoised (the server denoises finished jobs on the device) at the
+    denoised threshold count -- timed from submission to the image in host
+    memory (admission, tracing, denoising, read-back), then scored against
+    the reference."""
+    import autotune
+    import serve
+
+    rs = serve.ResidentScene("scene", scene, features)
+    tuning = autotune.TuningCache()
+    tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort))
+    out = {}
+    try:
+        for label, spp, den in (("plain", spp_plain, None), ("denoised", spp_denoised, net)):
+            srv = serve.Server({"scene": rs}, 1 << 38, max_frames=32, keep_images=True, tuning=tuning, denoiser=den)
+            srv.run([serve.JobSpec(0, "scene", w, h, min(spp, 8), depth, rr, 0.0)])  # warm-up: allocations, capture
+            res = srv.run([serve.JobSpec(0, "scene", w, h, spp, depth, rr, 0.0)])[0]
+            if res.error:
+                raise RuntimeError(res.error)
+            out[label] = {"spp": spp, "latency_s": round(res.latency, 4), "relmse": relmse(res.image, ref)}
+    finally:
+        rs.close()
+    out["speedup"] = round(out["plain"]["latency_s"] / out["denoised"]["latency_s"], 2)
+    log(f"served: plain {out['plain']}, denoised {out['denoised']}, {out['speedup']}x sooner")
+    return out
+
+
 def train_step_time(denoiser, images, W, H, steps):
     """Seconds per training step (batch 8 of 128x128 crops of `images`,
     forward + L1 loss + backward + Adam) and its convolution FLOP."""
@@ -547,6 +576,8 @@ def main():
             den["weights"] = os.path.relpath(path, ROOT)
             if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
                 den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
+                conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, denoiser.UNet.load(path),
+                                               conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort)
             conv["denoised"] = den
     base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES, threads=args.cpu_threads)
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,

@@ -144,7 +144,7 @@ class _Active:
 class Server:
     def __init__(self, scenes: dict, budget_bytes: int, max_frames: int = 32, keep_images: bool = False,
                  device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH, wavefront: bool = False,
-                 step_paths: int = 8 << 20):
+                 step_paths: int = 8 << 20, denoiser=None):
         self.scenes = scenes                  # name -> ResidentScene
         self.budget = budget_bytes
         self.max_frames = max_frames
@@ -154,6 +154,8 @@ class Server:
         self.device = device
         self.wavefront = wavefront            # jobs of a scene share one wavefront per step
         self.step_paths = step_paths          # paths per job and wavefront step
+        self.denoiser = denoiser              # denoiser.UNet: a finished job's image is denoised before read-back
+        self._net_lock = threading.Lock()     # one network, one set of activation buffers
         self._lock = threading.Condition()
         self._used = 0
         self._pending = {}                    # scene -> jobs admitted but not yet in its wavefront
@@ -182,7 +184,7 @@ class Server:
                 r.render(n)
                 r.wait_idle()
                 done += n
-            img = r.read_image(hpt.IMAGE_ACCUM)
+            img = self._result_image(r)
             res.finished = time.perf_counter() - t0
             if self.keep_images:
                 res.image = img
@@ -207,6 +209,19 @@ class Server:
         px = spec.width * spec.height
         return px * ((self.step_frames(spec) + 1) * PATH_BYTES + IMAGE_BYTES)
 
+    def _result_image(self, r) -> np.ndarray:
+        """A finished job's image in host memory: its accumulated radiance, or
+        that image denoised on the device from the job's resident colour,
+        albedo and normal images and its sample count."""
+        if self.denoiser is None:
+            return r.read_image(hpt.IMAGE_ACCUM)
+        import denoiser
+
+        with self._net_lock:
+            out = self.denoiser.denoise_renderer(r)
+            denoiser.synchronize()
+            return out.numpy()
+
     def _new_group(self, scene: str):
         g = hpt.HalaGroup(self.device)
         g.set_launch_mode(self.launch)
@@ -220,7 +235,7 @@ class Server:
         res = a.result
         try:
             if error is None:
-                img = a.handle.read_image(hpt.IMAGE_ACCUM)
+                img = self._result_image(a.handle)
                 if self.keep_images:
                     res.image = img
         except Exception as e:

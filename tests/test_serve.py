@@ -234,3 +234,26 @@ def test_autotune_picks_a_measured_candidate():
     best = max(t.table.values())
     assert t.msamples_per_s >= 0.97 * best - 1e-9
     assert t.frames_in_flight in (2, 4, 8, 16) and t.launch in (hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER)
+
+
+@pytest.mark.gpu
+def test_server_denoises_finished_jobs():
+    """With a denoiser the server returns each finished job's image denoised
+    on the device: the same bytes as denoising a solo render of the job."""
+    import denoiser
+
+    net = denoiser.UNet.load(denoiser.weights_path())
+    rs = serve.ResidentScene("cornell", hpt_scenes.cornell_box(), 0)
+    spec = serve.JobSpec(0, "cornell", 64, 48, 8, 8)
+    print("[gpu] served job with the denoiser", flush=True)
+    for wavefront in (False, True):
+        res = serve.Server({"cornell": rs}, 1 << 34, keep_images=True, denoiser=net, wavefront=wavefront).run([spec])[0]
+        assert res.error is None, res.error
+        r = hpt.HalaRenderer(64, 48, max_depth=8, rr_depth=3, enable_tonemap=True)
+        r.share_scene(rs.template)
+        r.render(8)
+        want = net.denoise_renderer(r).numpy()
+        denoiser.synchronize()
+        r.close()
+        assert res.image.shape == (48, 64, 4) and np.array_equal(res.image, want)
+    rs.close()
