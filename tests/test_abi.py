@@ -111,3 +111,18 @@ def test_denoiser_entry_points_reject_bad_shapes_without_a_device():
     assert lib.hpt_nn_adam(None, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), 4, 0.1, 0.9,
                            0.999, 1e-8, 0) == 1  # step 0
     assert lib.hpt_nn_wgrad_scratch_floats(1, 32, 32, 16, 32, 3) == 1 * 32 * 9 * 16 + 32
+
+
+def test_group_and_exr_entry_points_reject_bad_arguments_without_a_device():
+    lib = hpt.load_library()
+    assert lib.hpt_group_render(None, None, None, None, 0) == 1
+    assert b"null" in lib.hpt_last_error()
+    assert lib.hpt_group_set_launch_mode(None, 0) == 1
+    assert lib.hpt_group_set_shade_sort(None, 0) == 1
+    assert lib.hpt_group_wait_idle(None) == 1
+    lib.hpt_group_destroy(None)
+    assert lib.hpt_set_shade_sort(None, 0) == 1
+    assert lib.hpt_exr_info(None, 0, None, None) == 1
+    w = C.c_uint32()
+    assert lib.hpt_exr_info(b"not an exr file", 15, C.byref(w), C.byref(w)) == 1
+    assert b"magic" in lib.hpt_last_error()
