@@ -468,8 +468,9 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding of frames is "
-              f"described in DESIGN.md and not measured here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): frames shard across GPUs by frame "
+              f"range with one sum-reduce of the accumulators at the end, which is not built or measured here.",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "serving":

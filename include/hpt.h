@@ -42,7 +42,7 @@ enum {
   HPT_ERROR_OUT_OF_MEMORY = 2,    /* device or host allocation failed      */
   HPT_ERROR_DEVICE = 3,           /* a HIP call failed                     */
   HPT_ERROR_STATE = 4,            /* call made in the wrong order          */
-  HPT_ERROR_UNSUPPORTED = 5       /* feature not available in this build   */
+  HPT_ERROR_UNSUPPORTED = 5       /* feature or file variant not supported */
 };
 
 /* ------------------------------------------------------------------------ */

@@ -2,7 +2,7 @@
 reference application (src/config/*.rs) and the environment readers.
 
 The glTF conversion lives in the reference's unvendored renderer, so these
-tests pin this build's stated rules: exact round trips through the writer,
+tests pin the loader's stated rules: exact round trips through the writer,
 the structure of the reference's own assets (tests/golden/gltf), and
 closed-form checks of the conversions (light radiance from the exported
 power, camera axes from node rotations).
