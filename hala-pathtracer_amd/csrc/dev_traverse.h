@@ -81,11 +81,36 @@ __device__ __forceinline__ bool tri_test(const RayPre& r, f3 A0, f3 B0, f3 C0, f
 }
 
 // ------------------------------------------------------------ analytic lights
+// Evaluated with every product and sum rounded on its own (no fused
+// multiply-add), as in the host restatement: the sphere discriminant
+// b^2 - |op|^2 + r^2 cancels for a distant light, where one ulp of
+// contraction moves t by ~1e-5 relative.  The vector helpers carry the
+// pragma themselves: it covers only the code written inside a function.
+namespace nofma {
+__device__ __forceinline__ float dot(f3 a, f3 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+#pragma clang fp contract(off)
+  return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ f3 axpy(f3 o, f3 d, float t) {  // o + d * t
+#pragma clang fp contract(off)
+  return f3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+}
+__device__ __forceinline__ f3 normalize(f3 a) {
+  const float l = sqrtf(dot(a, a));
+  return f3{a.x / l, a.y / l, a.z / l};
+}
+}  // namespace nofma
+
 // common/intersection.glsl:6-33
 __device__ __forceinline__ float sphere_isect(float rad, f3 pos, f3 o, f3 d) {
+#pragma clang fp contract(off)
   const f3 op = pos - o;
-  const float b = dot(op, d);
-  float det = b * b - dot(op, op) + rad * rad;
+  const float b = nofma::dot(op, d);
+  float det = b * b - nofma::dot(op, op) + rad * rad;
   if (det < 0.0f) return INF;
   det = sqrtf(det);
   const float t1 = b - det;
@@ -97,21 +122,23 @@ __device__ __forceinline__ float sphere_isect(float rad, f3 pos, f3 o, f3 d) {
 
 // inc/light/quad.glsl:42-60 with common/intersection.glsl:43-72.
 __device__ __forceinline__ float quad_light_isect(const hpt_light& L, f3 o, f3 d, float& pdf) {
+#pragma clang fp contract(off)
   const f3 lu = ld3(L.u), lv = ld3(L.v), lp = ld3(L.position);
-  const f3 normal = normalize(cross(lv, lu));
-  const float cos_theta = dot(-d, normal);
+  const f3 normal = nofma::normalize(nofma::cross(lv, lu));
+  const float cos_theta = nofma::dot(-d, normal);
   if (cos_theta < 0.0f) return INF;
-  const float pw = dot(normal, lp);
-  const f3 u = lu / dot(lu, lu);
-  const f3 v = lv / dot(lv, lv);
-  const float dt = dot(d, normal);
-  const float t = (pw - dot(normal, o)) / dt;
+  const float pw = nofma::dot(normal, lp);
+  const float uu = nofma::dot(lu, lu), vv = nofma::dot(lv, lv);
+  const f3 u = f3{lu.x / uu, lu.y / uu, lu.z / uu};
+  const f3 v = f3{lv.x / vv, lv.y / vv, lv.z / vv};
+  const float dt = nofma::dot(d, normal);
+  const float t = (pw - nofma::dot(normal, o)) / dt;
   if (t > EPS) {
-    const f3 p = o + d * t;
+    const f3 p = nofma::axpy(o, d, t);
     const f3 vi = p - lp;
-    const float a1 = dot(u, vi);
+    const float a1 = nofma::dot(u, vi);
     if (a1 >= 0.0f && a1 <= 1.0f) {
-      const float a2 = dot(v, vi);
+      const float a2 = nofma::dot(v, vi);
       if (a2 >= 0.0f && a2 <= 1.0f) {
         pdf = (t * t) / (L.area * cos_theta);
         return t;
@@ -123,13 +150,15 @@ __device__ __forceinline__ float quad_light_isect(const hpt_light& L, f3 o, f3 d
 
 // inc/light/sphere.glsl:50-66.
 __device__ __forceinline__ float sphere_light_isect(const hpt_light& L, f3 o, f3 d, float& pdf) {
+#pragma clang fp contract(off)
   const f3 lp = ld3(L.position);
-  const float dist = length(lp - o);
+  const f3 lo = lp - o;
+  const float dist = sqrtf(nofma::dot(lo, lo));
   if (dist < L.radius) return INF;
   const float t = sphere_isect(L.radius, lp, o, d);
   if (t < 0.0f) return INF;
-  const f3 hit_pt = o + t * d;
-  const float cos_theta = dot(-d, normalize(hit_pt - lp));
+  const f3 hit_pt = nofma::axpy(o, d, t);
+  const float cos_theta = nofma::dot(-d, nofma::normalize(hit_pt - lp));
   pdf = (t * t) / (L.area * cos_theta * 0.5f);
   return t;
 }

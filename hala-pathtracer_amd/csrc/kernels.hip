@@ -204,11 +204,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
     }
     return;
   }
+  uint32_t j0 = lr.j, p0 = 0u;
+  bool act0 = j0 < lr.n;
+  if (act0) p0 = entry(j0);
   // Pipeline: current ray (path p), next ray (path pn, origin/direction
   // loaded), the one after (path pnn, id loaded).
-  uint32_t i = 0, j = lr.j;
-  bool active = j < lr.n;
-  uint32_t p = active ? entry(j) : 0u;
+  uint32_t i = 0, j = j0;
+  bool active = act0;
+  uint32_t p = p0;
   bool has_n = active && kRaysPerLane > 1 && j + 64 < lr.n;
   uint32_t pn = has_n ? entry(j + 64) : 0u;
   bool has_nn = has_n && kRaysPerLane > 2 && j + 128 < lr.n;
