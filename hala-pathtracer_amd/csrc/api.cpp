@@ -853,6 +853,7 @@ hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
           std::memcpy(ts.uv[k], v.tex_coord, 8);
         }
         ts.prim = p;
+        ts.material = pd.material_index;
         tri_shade.push_back(ts);
         tri_prim.push_back(p);
       }

@@ -113,15 +113,17 @@ struct LeafTri {
 static_assert(sizeof(LeafTri) == 48, "LeafTri is 48 bytes");
 
 // Shading record of one triangle (128 B, one cache line): its vertices'
-// normals and tangents copied from the vertex buffer, its primitive and its
-// texture coordinates.  A hit reads one line here instead of an index
-// triple and three scattered vertices.
+// normals and tangents copied from the vertex buffer, its primitive, its
+// texture coordinates and its primitive's material.  A hit reads one line
+// here instead of an index triple and three scattered vertices, and the
+// material and primitive records can then be fetched side by side.
 struct TriShade {
   float n[3][3];   // vertex normals (object space)
   float t[3][3];   // vertex tangents (object space)
   uint32_t prim;
   float uv[3][2];
-  uint32_t pad[7];
+  uint32_t material;  // the primitive's material index (PrimInfo::material_index)
+  uint32_t pad[6];
 };
 static_assert(sizeof(TriShade) == 128, "TriShade is 128 bytes");
 

@@ -335,7 +335,7 @@ __device__ __forceinline__ HitState triangle_state(const SceneView& sv, uint32_t
                                                    f3 org, f3 dir) {
   HitState h;
   const float4* rec = reinterpret_cast<const float4*>(sv.tri_shade + id);
-  const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3], q4 = rec[4];
+  const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3], q4 = rec[4], q6 = rec[6];
   const PrimInfo& pi = sv.prims[__float_as_uint(q4.z)];
   const f3 na = mk3(q0.x, q0.y, q0.z), nb = mk3(q0.w, q1.x, q1.y), nc = mk3(q1.z, q1.w, q2.x);
   const f3 ta = mk3(q2.y, q2.z, q2.w), tb = mk3(q3.x, q3.y, q3.z), tc = mk3(q3.w, q4.x, q4.y);
@@ -351,7 +351,7 @@ __device__ __forceinline__ HitState triangle_state(const SceneView& sv, uint32_t
   h.bitangent = cross(n, tg);
   h.pos = org + dir * t;
   h.ffnormal = dot(n, dir) <= 0.0f ? n : -n;
-  h.material = pi.material_index;
+  h.material = __float_as_uint(q6.y);  // TriShade::material
   return h;
 }
 
