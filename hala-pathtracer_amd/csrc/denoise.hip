@@ -176,6 +176,18 @@ int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float
   const int tiles = nout / 16;
   int groups = (tiles + 7) / 8;
   while (tiles % groups) ++groups;
+  // Low-resolution layers: split the output channels into more grid rows
+  // until the grid gives every CU two workgroups (256 CUs).  Each output
+  // still sums K in the same order, so the result does not change.
+  auto blocks = [&](int gr) {
+    const int mt = tiles / gr <= 4 ? 4 : 2;
+    const long long waves = (P + 16LL * mt - 1) / (16LL * mt);
+    return (long long)xcd_grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock)) * gr;
+  };
+  while (groups < tiles && blocks(groups) < 512) {
+    do ++groups;
+    while (tiles % groups);
+  }
   switch (tiles / groups) {
     case 1: return go_conv<InT, KS, 4, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
     case 2: return go_conv<InT, KS, 4, 2, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
