@@ -31,6 +31,10 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 constexpr int kWgradChunk = 1024;  // pixels per weight-gradient partial sum (a multiple of 32)
 constexpr unsigned kXcds = 8;      // MI355X: workgroups are dealt round-robin over 8 XCDs, each with its own L2
+#ifndef HPT_CONV_TILE
+#define HPT_CONV_TILE 1
+#endif
+constexpr bool kConvTile = HPT_CONV_TILE;  // forward 3x3 convolutions stage their input tile in LDS
 
 // Block b runs on XCD b % 8; this maps it to tile (b % 8) * (grid / 8) + b / 8
 // so that each XCD walks a contiguous range of pixel tiles and a tile's
@@ -157,6 +161,143 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
         }
       }
   }
+}
+
+// Forward 3x3 convolution over bf16 NHWC with the block's input staged in
+// LDS: a block owns a 16 x 16 pixel tile of one image and first copies the
+// tile's 18 x 18 pixels (a one-pixel halo, zeros outside the image) of all
+// input channels into LDS; wave q then computes tile rows 4q .. 4q + 3 (16
+// pixels each) for 16*NT output channels, reading every tap's A fragment
+// from LDS instead of through L1.  The K order per output is that of k_conv,
+// so the results are the same bits.
+constexpr int kTile = 16;
+constexpr int kHalo = kTile + 2;
+
+template <int NT>
+__global__ __launch_bounds__(kThreads) void k_conv3_tile(const uint16_t* __restrict__ in,
+                                                         const uint16_t* __restrict__ wgt,
+                                                         const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                         float* __restrict__ yf, int n_img, int h, int w, int cin,
+                                                         int nstore, int relu) {
+  extern __shared__ __align__(16) uint16_t tile[];  // kHalo * kHalo pixels x cin
+  const int tiles_x = (w + kTile - 1) / kTile, tiles_y = (h + kTile - 1) / kTile;
+  const unsigned b = xcd_block();
+  if (b >= (unsigned)(n_img * tiles_x * tiles_y)) return;  // the whole block, before any barrier
+  const int img = (int)b / (tiles_x * tiles_y);
+  const int rem = (int)b - img * tiles_x * tiles_y;
+  const int ty0 = (rem / tiles_x) * kTile, tx0 = (rem - (rem / tiles_x) * tiles_x) * kTile;
+  const int C8 = cin >> 3;
+  for (int idx = threadIdx.x; idx < kHalo * kHalo * C8; idx += kThreads) {
+    const int pix = idx / C8, c8 = idx - pix * C8;
+    const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + (pix - (pix / kHalo) * kHalo);
+    u32x4 v = u32x4{0u, 0u, 0u, 0u};
+    if ((unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w)
+      v = *reinterpret_cast<const u32x4*>(in + ((size_t)(img * h + yy) * w + xx) * cin + c8 * 8);
+    *reinterpret_cast<u32x4*>(tile + (size_t)pix * cin + c8 * 8) = v;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, q = threadIdx.x >> 6;
+  const int n0 = blockIdx.y * (16 * NT);
+  const int S = 9 * C8, K = 9 * cin;
+  const uint16_t* wrow[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) wrow[j] = wgt + (size_t)(n0 + 16 * j + m) * K;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  int s = g, tap = g / C8, c8 = g - (g / C8) * C8;
+  const int steps = (S + 3) >> 2;
+  for (int step = 0; step < steps; ++step) {
+    const bool sv = s < S;
+    const int ty = tap / 3, tx = tap - ty * 3;
+    bf16x8 a[4], bb[NT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      a[i] = sv ? load8(tile + ((size_t)((4 * q + i + ty) * kHalo + m + tx)) * cin + c8 * 8) : zero8();
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bb[j] = sv ? load8(wrow[j] + s * 8) : zero8();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+    s += 4;
+    c8 += 4;
+    while (c8 >= C8) {
+      c8 -= C8;
+      ++tap;
+    }
+  }
+
+  // Lane (m, g) holds D[4g + r][m]: tile row 4q + i, column 4g + r, channel n0 + 16j + m.
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = n0 + 16 * j + m;
+    if (co >= nstore) continue;
+    const float bv = bias[co];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int yy = ty0 + 4 * q + i;
+      if (yy >= h) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int xx = tx0 + 4 * g + r;
+        if (xx >= w) continue;
+        const size_t o = ((size_t)(img * h + yy) * w + xx) * nstore + co;
+        float v = acc[i][j][r] + bv;
+        if (relu) v = fmaxf(v, 0.f);
+        if (y) y[o] = (uint16_t)bf16_bits(v);
+        if (yf) yf[o] = v;
+      }
+    }
+  }
+}
+
+// LDS a tiled convolution may take: two blocks per CU (160 KB of LDS).
+constexpr size_t kTileLdsMax = 80 * 1024;
+
+template <int NT>
+bool allow_tile_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_tile<NT>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLdsMax) ==
+                         hipSuccess;
+  return ok;
+}
+
+// The tiled forward convolution when the 18 x 18 tile of all input channels
+// fits in kTileLdsMax, the layer has at least 256 tiles (smaller images keep
+// the CUs busier with the pixel-linear kernel and its channel split), and
+// the output channels split into tiles of at most 4 x 16; returns -1 when
+// it does not apply.
+int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y, float* yf,
+                      int n_img, int h, int wd, int cin, int nout, int nstore, int relu) {
+  const size_t lds = (size_t)kHalo * kHalo * cin * sizeof(uint16_t);
+  const unsigned blocks = (unsigned)(n_img * ((h + kTile - 1) / kTile) * ((wd + kTile - 1) / kTile));
+  if (lds > kTileLdsMax || blocks < 256) return -1;
+  const int tiles = nout / 16;
+  int groups = (tiles + 3) / 4;
+  while (tiles % groups) ++groups;
+  while (groups < tiles && (long long)xcd_grid(blocks) * groups < 512) {
+    do ++groups;
+    while (tiles % groups);
+  }
+  const dim3 grid(xcd_grid(blocks), groups);
+  const int nt = tiles / groups;
+  if (lds > 65536) {
+    const bool ok = nt == 1 ? allow_tile_lds<1>() : nt == 2 ? allow_tile_lds<2>() : nt == 3 ? allow_tile_lds<3>()
+                                                                                          : allow_tile_lds<4>();
+    if (!ok) return -1;
+  }
+  switch (nt) {
+    case 1: hipLaunchKernelGGL(k_conv3_tile<1>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
+    case 2: hipLaunchKernelGGL(k_conv3_tile<2>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
+    case 3: hipLaunchKernelGGL(k_conv3_tile<3>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
+    default: hipLaunchKernelGGL(k_conv3_tile<4>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
+  }
+  return (int)hipGetLastError();
 }
 
 template <typename InT, int KS, int MT, int NT, bool FORWARD>
@@ -591,6 +732,10 @@ int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16_t* w, cons
                         int n, int h, int wd, int cin, int cout, int cout_pad, int k, int relu) {
   if (!conv_shape_ok(n, h, wd, cin, cout_pad, k) || cout <= 0 || cout > cout_pad || !x || !w || !b || (!y && !y_f32))
     return bad();
+  if (k == 3 && kConvTile) {
+    const int e = launch_conv3_tile((hipStream_t)stream, x, w, b, y, y_f32, n, h, wd, cin, cout_pad, cout, relu);
+    if (e >= 0) return e;
+  }
   return launch_conv<uint16_t, true>((hipStream_t)stream, x, w, b, y, y_f32, n * h * wd, h, wd, cin, cout_pad, cout,
                                      relu, k);
 }
