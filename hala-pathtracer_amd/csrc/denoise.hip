@@ -68,14 +68,24 @@ __device__ __forceinline__ bf16x8 load8(const float* p) {
       bf16x8, u32x4{pack_bf16(a.x, a.y), pack_bf16(a.z, a.w), pack_bf16(b.x, b.y), pack_bf16(b.z, b.w)});
 }
 
+// A convolution input given as concat(upsample2x(low), skip): channels
+// [0, c_low) are those of the half-resolution tensor `low` at (y / 2, x / 2)
+// (nearest upsampling), channels [c_low, cin) those of the full-resolution
+// tensor passed as the input.  The operands are the same values as in the
+// materialised concatenation, so the results are the same bits.
+struct UpCat {
+  const uint16_t* low;  // null: a plain input
+  int c_low;
+};
+
 // ------------------------------------------------------------------ convolution
 // FORWARD: y = act(out + bias) as bf16 and/or float32, `nstore` channels per pixel.
 // Otherwise (data gradient): yf = out as float32, `nstore` (= the GEMM's N) channels.
-template <typename InT, int KS, int MT, int NT, bool FORWARD>
+template <typename InT, int KS, int MT, int NT, bool FORWARD, bool UPCAT = false>
 __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, const uint16_t* __restrict__ wgt,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ y,
                                                    float* __restrict__ yf, int P, int h, int w, int cin, int nstore,
-                                                   int relu) {
+                                                   int relu, UpCat uc) {
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
   const long long p0l = ((long long)xcd_block() * kWavesPerBlock + (threadIdx.x >> 6)) * (16 * MT);
   if (p0l >= P) return;  // uniform over the wave; no barriers below
@@ -83,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
   const int n0 = blockIdx.y * (16 * NT);
   const int C8 = cin >> 3, S = KS * KS * C8, K = KS * KS * cin;
 
-  int px[MT], py[MT];
+  int px[MT], py[MT], pimg[MT];
   bool pv[MT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
@@ -93,7 +103,9 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
     const int row = pc / w;
     px[i] = pc - row * w;
     py[i] = row % h;
+    pimg[i] = row / h;
   }
+  const int C8low = UPCAT ? uc.c_low >> 3 : 0, cskip = UPCAT ? cin - uc.c_low : 0;
   const uint16_t* wrow[NT];
 #pragma unroll
   for (int j = 0; j < NT; ++j) wrow[j] = wgt + (size_t)(n0 + 16 * j + m) * K;
@@ -114,10 +126,15 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
       const bool ok = sv && pv[i] && (unsigned)(py[i] + oy) < (unsigned)h && (unsigned)(px[i] + ox) < (unsigned)w;
-      if (ok)
-        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * cin + c8 * 8);
-      else
+      if (!ok)
         a[i] = zero8();
+      else if (!UPCAT)
+        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * cin + c8 * 8);
+      else if (c8 < C8low)
+        a[i] = load8(uc.low + ((size_t)(pimg[i] * (h >> 1) + ((py[i] + oy) >> 1)) * (w >> 1) + ((px[i] + ox) >> 1)) *
+                                  uc.c_low + c8 * 8);
+      else
+        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * cskip + (c8 - C8low) * 8);
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -173,12 +190,12 @@ __global__ __launch_bounds__(kThreads) void k_conv(const InT* __restrict__ in, c
 constexpr int kTile = 16;
 constexpr int kHalo = kTile + 2;
 
-template <int NT>
+template <int NT, bool UPCAT = false>
 __global__ __launch_bounds__(kThreads) void k_conv3_tile(const uint16_t* __restrict__ in,
                                                          const uint16_t* __restrict__ wgt,
                                                          const float* __restrict__ bias, uint16_t* __restrict__ y,
                                                          float* __restrict__ yf, int n_img, int h, int w, int cin,
-                                                         int nstore, int relu) {
+                                                         int nstore, int relu, UpCat uc) {
   extern __shared__ __align__(16) uint16_t tile[];  // kHalo * kHalo pixels x cin
   const int tiles_x = (w + kTile - 1) / kTile, tiles_y = (h + kTile - 1) / kTile;
   const unsigned b = xcd_block();
@@ -191,8 +208,16 @@ __global__ __launch_bounds__(kThreads) void k_conv3_tile(const uint16_t* __restr
     const int pix = idx / C8, c8 = idx - pix * C8;
     const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + (pix - (pix / kHalo) * kHalo);
     u32x4 v = u32x4{0u, 0u, 0u, 0u};
-    if ((unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w)
-      v = *reinterpret_cast<const u32x4*>(in + ((size_t)(img * h + yy) * w + xx) * cin + c8 * 8);
+    if ((unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w) {
+      const uint16_t* src;
+      if (!UPCAT)
+        src = in + ((size_t)(img * h + yy) * w + xx) * cin + c8 * 8;
+      else if (c8 < (uc.c_low >> 3))
+        src = uc.low + ((size_t)(img * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * uc.c_low + c8 * 8;
+      else
+        src = in + ((size_t)(img * h + yy) * w + xx) * (cin - uc.c_low) + (c8 - (uc.c_low >> 3)) * 8;
+      v = *reinterpret_cast<const u32x4*>(src);
+    }
     *reinterpret_cast<u32x4*>(tile + (size_t)pix * cin + c8 * 8) = v;
   }
   __syncthreads();
@@ -259,9 +284,9 @@ __global__ __launch_bounds__(kThreads) void k_conv3_tile(const uint16_t* __restr
 // LDS a tiled convolution may take: two blocks per CU (160 KB of LDS).
 constexpr size_t kTileLdsMax = 80 * 1024;
 
-template <int NT>
+template <int NT, bool UPCAT>
 bool allow_tile_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
-  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_tile<NT>),
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_tile<NT, UPCAT>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLdsMax) ==
                          hipSuccess;
   return ok;
@@ -272,8 +297,9 @@ bool allow_tile_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
 // the CUs busier with the pixel-linear kernel and its channel split), and
 // the output channels split into tiles of at most 4 x 16; returns -1 when
 // it does not apply.
+template <bool UPCAT>
 int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y, float* yf,
-                      int n_img, int h, int wd, int cin, int nout, int nstore, int relu) {
+                      int n_img, int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc) {
   const size_t lds = (size_t)kHalo * kHalo * cin * sizeof(uint16_t);
   const unsigned blocks = (unsigned)(n_img * ((h + kTile - 1) / kTile) * ((wd + kTile - 1) / kTile));
   if (lds > kTileLdsMax || blocks < 256) return -1;
@@ -287,33 +313,42 @@ int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, con
   const dim3 grid(xcd_grid(blocks), groups);
   const int nt = tiles / groups;
   if (lds > 65536) {
-    const bool ok = nt == 1 ? allow_tile_lds<1>() : nt == 2 ? allow_tile_lds<2>() : nt == 3 ? allow_tile_lds<3>()
-                                                                                          : allow_tile_lds<4>();
+    const bool ok = nt == 1   ? allow_tile_lds<1, UPCAT>()
+                    : nt == 2 ? allow_tile_lds<2, UPCAT>()
+                    : nt == 3 ? allow_tile_lds<3, UPCAT>()
+                              : allow_tile_lds<4, UPCAT>();
     if (!ok) return -1;
   }
   switch (nt) {
-    case 1: hipLaunchKernelGGL(k_conv3_tile<1>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
-    case 2: hipLaunchKernelGGL(k_conv3_tile<2>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
-    case 3: hipLaunchKernelGGL(k_conv3_tile<3>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
-    default: hipLaunchKernelGGL(k_conv3_tile<4>, grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu); break;
+    case 1: hipLaunchKernelGGL((k_conv3_tile<1, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc); break;
+    case 2: hipLaunchKernelGGL((k_conv3_tile<2, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc); break;
+    case 3: hipLaunchKernelGGL((k_conv3_tile<3, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc); break;
+    default: hipLaunchKernelGGL((k_conv3_tile<4, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc); break;
   }
   return (int)hipGetLastError();
 }
 
 template <typename InT, int KS, int MT, int NT, bool FORWARD>
 int go_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P, int h,
-            int wd, int cin, int nstore, int relu, int groups) {
+            int wd, int cin, int nstore, int relu, int groups, UpCat uc) {
   const int tiles = (P + 16 * MT - 1) / (16 * MT);
   const dim3 grid(xcd_grid((tiles + kWavesPerBlock - 1) / kWavesPerBlock), groups);
+  if constexpr (KS == 3 && FORWARD && sizeof(InT) == 2) {
+    if (uc.low) {
+      hipLaunchKernelGGL((k_conv<InT, KS, MT, NT, FORWARD, true>), grid, dim3(kThreads), 0, st, in, w, b, y, yf, P, h,
+                         wd, cin, nstore, relu, uc);
+      return (int)hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_conv<InT, KS, MT, NT, FORWARD>), grid, dim3(kThreads), 0, st, in, w, b, y, yf, P, h, wd, cin,
-                     nstore, relu);
+                     nstore, relu, uc);
   return (int)hipGetLastError();
 }
 
 // N = nout channels (a multiple of 16) split into `groups` grid rows of NT <= 8 tiles each.
 template <typename InT, int KS, bool FORWARD>
 int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
-                   int h, int wd, int cin, int nout, int nstore, int relu) {
+                   int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc) {
   const int tiles = nout / 16;
   int groups = (tiles + 7) / 8;
   while (tiles % groups) ++groups;
@@ -330,22 +365,22 @@ int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float
     while (tiles % groups);
   }
   switch (tiles / groups) {
-    case 1: return go_conv<InT, KS, 4, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 2: return go_conv<InT, KS, 4, 2, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 3: return go_conv<InT, KS, 4, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 4: return go_conv<InT, KS, 4, 4, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 5: return go_conv<InT, KS, 2, 5, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 6: return go_conv<InT, KS, 2, 6, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    case 7: return go_conv<InT, KS, 2, 7, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
-    default: return go_conv<InT, KS, 2, 8, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups);
+    case 1: return go_conv<InT, KS, 4, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 2: return go_conv<InT, KS, 4, 2, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 3: return go_conv<InT, KS, 4, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 4: return go_conv<InT, KS, 4, 4, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 5: return go_conv<InT, KS, 2, 5, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 6: return go_conv<InT, KS, 2, 6, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    case 7: return go_conv<InT, KS, 2, 7, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
+    default: return go_conv<InT, KS, 2, 8, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nstore, relu, groups, uc);
   }
 }
 
 template <typename InT, bool FORWARD>
 int launch_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
-                int h, int wd, int cin, int nout, int nstore, int relu, int k) {
-  if (k == 1) return launch_conv_ks<InT, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu);
-  return launch_conv_ks<InT, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu);
+                int h, int wd, int cin, int nout, int nstore, int relu, int k, UpCat uc = UpCat{nullptr, 0}) {
+  if (k == 1) return launch_conv_ks<InT, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc);
+  return launch_conv_ks<InT, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc);
 }
 
 // ------------------------------------------------------------------ weight gradient
@@ -733,11 +768,29 @@ int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16_t* w, cons
   if (!conv_shape_ok(n, h, wd, cin, cout_pad, k) || cout <= 0 || cout > cout_pad || !x || !w || !b || (!y && !y_f32))
     return bad();
   if (k == 3 && kConvTile) {
-    const int e = launch_conv3_tile((hipStream_t)stream, x, w, b, y, y_f32, n, h, wd, cin, cout_pad, cout, relu);
+    const int e = launch_conv3_tile<false>((hipStream_t)stream, x, w, b, y, y_f32, n, h, wd, cin, cout_pad, cout, relu,
+                                           UpCat{nullptr, 0});
     if (e >= 0) return e;
   }
   return launch_conv<uint16_t, true>((hipStream_t)stream, x, w, b, y, y_f32, n * h * wd, h, wd, cin, cout_pad, cout,
                                      relu, k);
+}
+
+int hpt_nn_conv_forward_upcat(void* stream, const uint16_t* low, const uint16_t* skip, const uint16_t* w,
+                              const float* b, uint16_t* y, float* y_f32, int n, int h, int wd, int c_low, int c_skip,
+                              int cout, int cout_pad, int relu) {
+  const int cin = c_low + c_skip;
+  if (!conv_shape_ok(n, h, wd, cin, cout_pad, 3) || h % 2 || wd % 2 || c_low <= 0 || c_low % 8 || c_skip <= 0 ||
+      c_skip % 8 || cout <= 0 || cout > cout_pad || !low || !skip || !w || !b || (!y && !y_f32))
+    return bad();
+  const UpCat uc{low, c_low};
+  if (kConvTile) {
+    const int e =
+        launch_conv3_tile<true>((hipStream_t)stream, skip, w, b, y, y_f32, n, h, wd, cin, cout_pad, cout, relu, uc);
+    if (e >= 0) return e;
+  }
+  return launch_conv<uint16_t, true>((hipStream_t)stream, skip, w, b, y, y_f32, n * h * wd, h, wd, cin, cout_pad,
+                                     cout, relu, 3, uc);
 }
 
 int hpt_nn_conv_backward_data(void* stream, const float* dy, const uint16_t* wt, float* dx, int n, int h, int wd,

@@ -40,6 +40,7 @@ _vp, _i, _sz, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 # Entry points of include/hpt_nn.h with their argument types.
 NN_SIGNATURES = {
     "hpt_nn_conv_forward": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i],
+    "hpt_nn_conv_forward_upcat": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_conv_backward_data": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_conv_backward_weights": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_wgrad_scratch_floats": [_i, _i, _i, _i, _i, _i],
@@ -348,6 +349,11 @@ class UNet:
         _call("hpt_nn_conv_forward", STREAM, _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_f32), n, h, w,
               L.cin, L.cout, L.cout_pad, L.k, int(L.relu))
 
+    def _conv_upcat(self, L, low, skip, y, n, h, w, c_low, c_skip):
+        """L over concat(upsample2x(low), skip) without materialising it."""
+        _call("hpt_nn_conv_forward_upcat", STREAM, _p(low), _p(skip), _p(L.w), _p(self.params.addr(L.b_off)), _p(y),
+              None, n, h, w, c_low, c_skip, L.cout, L.cout_pad, int(L.relu))
+
     def forward_input(self, x: DeviceArray, n: int, h: int, w: int, train: bool = False) -> DeviceArray:
         """Runs the network on the bf16 input x (n*h*w*16 bit patterns, NHWC;
         copied into the network's input buffer unless it is that buffer);
@@ -369,10 +375,14 @@ class UNet:
             cur = b[f"y_enc{l}b"]
         for l in range(Lv - 2, -1, -1):
             hl, wl = h >> l, w >> l
-            _call("hpt_nn_upsample_forward", st, _p(cur), _p(b[f"up{l}"]), n, hl // 2, wl // 2, self.levels[l + 1])
-            _call("hpt_nn_concat_forward", st, _p(b[f"up{l}"]), _p(b[f"y_enc{l}b"]), _p(b[f"cat{l}"]),
-                  n * hl * wl, self.levels[l + 1], self.levels[l])
-            self._conv(self.layers[f"dec{l}a"], b[f"cat{l}"], b[f"y_dec{l}a"], None, n, hl, wl)
+            if train:  # the backward pass reads the concatenated input
+                _call("hpt_nn_upsample_forward", st, _p(cur), _p(b[f"up{l}"]), n, hl // 2, wl // 2, self.levels[l + 1])
+                _call("hpt_nn_concat_forward", st, _p(b[f"up{l}"]), _p(b[f"y_enc{l}b"]), _p(b[f"cat{l}"]),
+                      n * hl * wl, self.levels[l + 1], self.levels[l])
+                self._conv(self.layers[f"dec{l}a"], b[f"cat{l}"], b[f"y_dec{l}a"], None, n, hl, wl)
+            else:
+                self._conv_upcat(self.layers[f"dec{l}a"], cur, b[f"y_enc{l}b"], b[f"y_dec{l}a"], n, hl, wl,
+                                 self.levels[l + 1], self.levels[l])
             self._conv(self.layers[f"dec{l}b"], b[f"y_dec{l}a"], b[f"y_dec{l}b"], None, n, hl, wl)
             cur = b[f"y_dec{l}b"]
         self._conv(self.layers["out"], cur, None, b["out"], n, h, w)
@@ -499,21 +509,22 @@ class UNet:
         kernels run on; the input buffer is used as it is."""
         b = self._buffers(n, h, w, False)
         ev = {name: [] for name in self.layers}
-        orig = self._conv
 
-        def timed(L, x, y, y_f32, nn, hh, ww):
-            s, e = Event(), Event()
-            s.record()
-            orig(L, x, y, y_f32, nn, hh, ww)
-            e.record()
-            ev[L.name].append((s, e))
+        def timed(orig):
+            def run(L, *args):
+                s, e = Event(), Event()
+                s.record()
+                orig(L, *args)
+                e.record()
+                ev[L.name].append((s, e))
+            return run
 
-        self._conv = timed
+        self._conv, self._conv_upcat = timed(self._conv), timed(self._conv_upcat)
         try:
             for _ in range(reps):
                 self.forward_input(b["x"], n, h, w)
         finally:
-            del self._conv
+            del self._conv, self._conv_upcat
         synchronize()
         out = {}
         for name, L in self.layers.items():

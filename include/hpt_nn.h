@@ -31,6 +31,18 @@ HPT_NN_API int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16
                                    uint16_t* y, float* y_f32, int n, int h, int wd, int cin, int cout,
                                    int cout_pad, int k, int relu);
 
+/* hpt_nn_conv_forward with k = 3 on the input concat(upsample2x(low), skip)
+ * without materialising it: `low` bf16 [n][h/2][wd/2][c_low] read with
+ * nearest upsampling, `skip` bf16 [n][h][wd][c_skip] (h and wd even, channel
+ * counts multiples of 8); the weights see input channels [0, c_low) from
+ * `low` and [c_low, c_low + c_skip) from `skip` -- the channel order of
+ * hpt_nn_upsample_forward followed by hpt_nn_concat_forward(up, skip), whose
+ * result this equals bit for bit.  The U-Net decoder's first convolution of
+ * each level at inference. */
+HPT_NN_API int hpt_nn_conv_forward_upcat(void* stream, const uint16_t* low, const uint16_t* skip,
+                                         const uint16_t* w, const float* b, uint16_t* y, float* y_f32, int n,
+                                         int h, int wd, int c_low, int c_skip, int cout, int cout_pad, int relu);
+
 /* dx = conv_k^T(dy, w): dy float32 [n][h][w][cout_pad] (channels >= cout
  * zero, already multiplied by the activation's derivative), wt bf16
  * [cin][k*k][cout_pad] = w transposed and spatially flipped

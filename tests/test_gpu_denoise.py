@@ -118,6 +118,46 @@ def test_conv_forward_matches_oracle(dn, shape):
     assert np.array_equal(_host_bf16(y, ref.shape), U.bf16(got))
 
 
+# Images of at least 256 16x16 tiles take the LDS-tiled forward kernel (the
+# 112-channel case needs 72 KB of LDS); the shapes above take the pixel-linear one.
+TILED_SHAPES = [(4, 128, 128, 16, 32, 3), (4, 128, 128, 112, 48, 3), (1, 256, 272, 80, 32, 3)]
+
+
+@pytest.mark.parametrize("shape", TILED_SHAPES)
+def test_conv_forward_tiled_matches_oracle(dn, shape):
+    test_conv_forward_matches_oracle(dn, shape)
+
+
+UPCAT_SHAPES = [  # n, h, w, c_low, c_skip, cout: pixel-linear, tiled, tiled with 72 KB, linear (160 channels)
+    (2, 16, 24, 32, 16, 32), (4, 128, 128, 48, 32, 32), (4, 128, 128, 64, 48, 48), (1, 64, 64, 96, 64, 64)]
+
+
+@pytest.mark.parametrize("shape", UPCAT_SHAPES)
+def test_conv_forward_upcat_equals_upsample_concat_conv(dn, shape):
+    """The decoder's fused input (nearest 2x upsampling of `low` concatenated
+    before `skip`) gives the bits of the three separate steps."""
+    n, h, wd, cl, cs, cout = shape
+    print(f"conv upcat {shape}", flush=True)
+    rng = np.random.default_rng(7)
+    low = _dev_bf16(rng.normal(0, 1, (n, h // 2, wd // 2, cl)).astype(np.float32))
+    skip = _dev_bf16(rng.normal(0, 1, (n, h, wd, cs)).astype(np.float32))
+    _, w, _, cp = _weights(dn, rng, cl + cs, cout, 3)
+    b = _dev(rng.normal(0, 0.1, cout).astype(np.float32))
+    up = _zeros(n * h * wd * cl, np.uint16)
+    cat = _zeros(n * h * wd * (cl + cs), np.uint16)
+    y_ref, y = _zeros(n * h * wd * cout, np.uint16), _zeros(n * h * wd * cout, np.uint16)
+    _call(dn, "hpt_nn_upsample_forward", None, _p(low), _p(up), n, h // 2, wd // 2, cl)
+    _call(dn, "hpt_nn_concat_forward", None, _p(up), _p(skip), _p(cat), n * h * wd, cl, cs)
+    _call(dn, "hpt_nn_conv_forward", None, _p(cat), _p(w), _p(b), _p(y_ref), None, n, h, wd, cl + cs, cout, cp, 3, 1)
+    _call(dn, "hpt_nn_conv_forward_upcat", None, _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h, wd, cl, cs, cout,
+          cp, 1)
+    dn.synchronize()
+    assert np.array_equal(y.numpy(), y_ref.numpy())
+    bad = dn.nn_library().hpt_nn_conv_forward_upcat(None, _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h + 1,
+                                                      wd, cl, cs, cout, cp, 1)
+    assert bad != 0  # odd height: no half-resolution tensor to upsample
+
+
 @pytest.mark.parametrize("shape", CONV_SHAPES)
 def test_conv_backward_data_matches_oracle(dn, shape):
     n, h, wd, cin, cout, k = shape
