@@ -212,6 +212,9 @@ extern "C" {
     pub fn hpt_nn_conv_forward(stream: *mut c_void, x: *const u16, w: *const u16, b: *const f32, y: *mut u16,
         y_f32: *mut f32, n: c_int, h: c_int, wd: c_int, cin: c_int, cout: c_int, cout_pad: c_int, k: c_int,
         relu: c_int) -> c_int;
+    pub fn hpt_nn_conv_forward_upcat(stream: *mut c_void, low: *const u16, skip: *const u16, w: *const u16,
+        b: *const f32, y: *mut u16, y_f32: *mut f32, n: c_int, h: c_int, wd: c_int, c_low: c_int, c_skip: c_int,
+        cout: c_int, cout_pad: c_int, relu: c_int) -> c_int;
     pub fn hpt_nn_conv_backward_data(stream: *mut c_void, dy: *const f32, wt: *const u16, dx: *mut f32, n: c_int,
         h: c_int, wd: c_int, cin: c_int, cout: c_int, cout_pad: c_int, k: c_int) -> c_int;
     pub fn hpt_nn_conv_backward_weights(stream: *mut c_void, dy: *const f32, x: *const u16, dw: *mut f32,
