@@ -224,6 +224,63 @@ def test_final_image_matches_oracle_tonemap():
     r.close()
 
 
+@pytest.mark.parametrize("tonemap,aces,simple,ev", [(False, False, False, 0.0), (True, False, False, 0.0),
+                                                    (True, True, True, 0.0), (True, True, False, 1.5),
+                                                    (True, False, False, -1.0)],
+                         ids=["srgb_only", "reinhard", "simple_aces", "fitted_aces_ev1.5", "reinhard_ev-1"])
+def test_final_image_tonemap_modes(tonemap, aces, simple, ev):
+    # pathtracer.rgen.glsl:259-304 and common/tonemap.glsl:1-47: exposure 2^EV,
+    # then fitted ACES, Narkowicz ACES or luminance Reinhard (or none), then
+    # the sRGB curve, per the renderer's flags.
+    scene = hpt_scenes.cornell_box()
+    w = h = 32
+    r = _renderer(scene, w, h, enable_tonemap=tonemap, enable_aces=aces, use_simple_aces=simple, exposure_value=ev)
+    r.render(2)
+    acc = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    fin = r.read_image(hpt.IMAGE_FINAL_RGBA8)
+    r.close()
+
+    def oracle_final(st):
+        ref = np.array([pyoracle.final_color(st, c) for c in acc.reshape(-1, 3)]).reshape(h, w, 3)
+        return np.floor(np.clip(ref, 0, 1) * 255 + 0.5)
+
+    ref8 = oracle_final(pyoracle.settings(w, h, 8, 3, tonemap=tonemap, aces=aces, simple_aces=simple, exposure=ev))
+    assert np.abs(fin[..., :3].astype(np.int32) - ref8).max() <= 1
+    assert (fin[..., 3] == 255).all()
+    # the mode matters: the default (fitted ACES, EV 0) maps this image differently
+    assert not np.array_equal(ref8, oracle_final(pyoracle.settings(w, h, 8, 3)))
+
+
+def test_orthographic_camera_per_sample_and_converged():
+    # orthographic_ray.rcall.glsl:12-45: parallel rays over the camera's
+    # xmag x ymag window, blue-noise tent jitter as for the pinhole.
+    scene = hpt_scenes.cornell_box()
+    persp_scene = hpt_scenes.cornell_box()
+    scene.cameras[0]["type"] = hpt.CAMERA_ORTHOGRAPHIC
+    # a window wider than the box: its white back wall and boxes, and black
+    # sky around it (the side walls lie parallel to the rays)
+    scene.cameras[0]["focal_distance_or_xmag"] = 1.3
+    scene.cameras[0]["aperture_or_ymag"] = 1.3
+    w = h = 64
+    r = _renderer(scene, w, h)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    gpu_alb = r.read_image(hpt.IMAGE_ALBEDO)[..., :3]
+    r.close()
+    o = pyoracle.OracleScene(scene)
+    ref, alb, nrm = o.trace_paths(pyoracle.settings(w, h, 8, 3), 0)
+    agree = _per_sample_agreement(gpu, ref)
+    print(f"[gpu] orthographic per-sample agreement {agree:.4f}", flush=True)
+    assert agree > 0.97
+    assert np.array_equal(gpu_alb, alb)
+    # not the pinhole's view of the same box
+    p = _renderer(persp_scene, w, h)
+    p.render(1)
+    assert not np.array_equal(p.read_image(hpt.IMAGE_ALBEDO)[..., :3], gpu_alb)
+    p.close()
+    _stat_test(scene, 48, 48, 32)
+
+
 def test_max_samples_caps_accumulation():
     scene = hpt_scenes.cornell_box()
     r = _renderer(scene, 16, 16, max_samples=3)
