@@ -532,7 +532,8 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32
 // SIMD for the plain permutations, 5 for VOL_MIS, whose inline transmittance
 // walks are latency-bound traversals (measured on the 1080p icosphere scene
 // with a medium: 3 -> 577, 4 -> 626, 5 -> 633, 6 -> 618, 8 -> 556
-// Msamples/s).
+// Msamples/s unsorted; sorted by material from bounce 1: 4 -> 722, 5 -> 727,
+// 6 -> 723).
 #ifndef HPT_SHADE_WAVES_PER_EU
 #define HPT_SHADE_WAVES_PER_EU 3
 #endif
