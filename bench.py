@@ -356,27 +356,12 @@ def denoise_bench(args):
     print(json.dumps(out_json), flush=True)
 
 
-def serving_scenes():
-    """The 8 resident scenes of the batched/serving workload: the Cornell box,
-    the Disney grid, the large-mesh scene with lights and a medium, and five
-    random triangle soups of 1e5..2e6 triangles (log-uniform, seeded)."""
-    rng = np.random.default_rng(0)
-    scenes = [("cornell", hpt_scenes.cornell_box(), 0),
-              ("disney_grid", hpt_scenes.disney_grid(seed=0), 0),
-              ("icosphere_lights_medium", hpt_scenes.displaced_icosphere_lights_medium(level=8),
-               hpt_scenes.MEDIUM_SCENE_FEATURES)]
-    for i in range(5):
-        n = int(round(math.exp(rng.uniform(math.log(1e5), math.log(2e6)))))
-        scenes.append((f"soup_{n}", hpt_scenes.triangle_soup(n, seed=100 + i), 0))
-    return scenes
-
-
 def serving_bench(args):
     import serve
 
     log("building and committing the 8 resident scenes")
     resident = {}
-    for name, sc, feat in serving_scenes():
+    for name, sc, feat in hpt_scenes.batched_job_scenes():
         rs = serve.ResidentScene(name, sc, feat)
         resident[name] = rs
         log(f"  {name}: {sc.triangle_count} triangles, commit {rs.commit_seconds:.2f} s, "

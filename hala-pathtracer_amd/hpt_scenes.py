@@ -268,11 +268,14 @@ def sky_env_map(width=2048, height=1024, seed=0, sun_elevation_deg=35.0, sun_azi
     return img.astype(np.float32)
 
 
-def disney_grid(seed=0, width=1920, height=1080, env_width=2048, env_height=1024, segments=64, rings=32):
+def disney_grid(seed=0, width=1920, height=1080, env_width=2048, env_height=1024, segments=64, rings=32,
+                subsurface=False):
     """7x7 Disney UV spheres (64x32 segments) on a Lambert ground plane:
     metallic = col/6, roughness = 0.05 + 0.95*row/6, ior 1.5; sheen,
     clearcoat, specular tint and anisotropy U(0,1) from the seed; sky
-    environment map with a sun, sampled by the alias table."""
+    environment map with a sun, sampled by the alias table.  `subsurface`:
+    each sphere's subsurface weight is 0.25 + 0.75 * row/6 (the fake
+    subsurface lobe of inc/bxdf/disney.glsl:93-112)."""
     rng = np.random.default_rng(seed)
     n = 7
     mats = hpt.new_materials(n * n + 1)
@@ -284,6 +287,8 @@ def disney_grid(seed=0, width=1920, height=1080, env_width=2048, env_height=1024
             disney_material(mats[k], tuple(rng.uniform(0.2, 0.9, 3)), metallic=col / 6.0,
                             roughness=0.05 + 0.95 * row / 6.0, anisotropic=rng.uniform(),
                             sheen=rng.uniform(), clearcoat=rng.uniform(), specular_tint=rng.uniform())
+            if subsurface:
+                mats[k]["subsurface"] = 0.25 + 0.75 * row / 6.0
             c = ((col - 3) * spacing, (row - 3) * spacing, radius)
             v, ix = uv_sphere_mesh(c, radius, segments, rings)
             prims.append(hpt.Primitive(v, ix, k))
@@ -379,20 +384,29 @@ def smooth_mesh(pos, faces):
     return v, faces.astype(np.uint32)
 
 
-def displaced_icosphere_lights_medium(level=8, seed=0, amplitude=0.1, width=1920, height=1080):
+def displaced_icosphere_lights_medium(level=8, seed=0, amplitude=0.1, width=1920, height=1080,
+                                      boundary="transparent"):
     """Large mesh + mixed lights + medium: an icosphere (20*4^level
     triangles; level 8 = 1,310,720) displaced by fBm (6 octaves, lacunarity
     2, gain 0.5, amplitude 0.1) on a ground quad; one point, spot (inner 20,
     outer 30 degrees), directional, rect and sphere (r = 0.2) light; a
     homogeneous scattering sphere (sigma_s 0.5, sigma_a 0.05, HG g 0.3) behind
     an opacity-0 boundary; thin-lens camera (f = 50 mm on a 24 mm sensor,
-    aperture 0.05) focused on the mesh.  Needs MEDIUM|VOL_MIS|TRANSPARENT."""
+    aperture 0.05) focused on the mesh.  Needs MEDIUM|VOL_MIS|TRANSPARENT.
+    `boundary` "glass" or "rough_glass" makes the medium's boundary a Disney
+    dielectric (specular transmission 1, ior 1.5, roughness 0.02 or 0.35)
+    instead: paths refract into the medium and the volume-MIS shadow walk
+    passes the refractive surface (inc/lighting.glsl:45-47)."""
     mats = hpt.new_materials(4)
     disney_material(mats[0], (0.8, 0.55, 0.35), metallic=0.2, roughness=0.45, clearcoat=0.3, sheen=0.2)
     diffuse_material(mats[1], (0.6, 0.6, 0.6), roughness=0.3)
-    # medium boundary: invisible surface carrying the scattering medium
-    diffuse_material(mats[2], (1.0, 1.0, 1.0))
-    mats[2]["opacity"] = 0.0
+    if boundary == "transparent":
+        # medium boundary: invisible surface carrying the scattering medium
+        diffuse_material(mats[2], (1.0, 1.0, 1.0))
+        mats[2]["opacity"] = 0.0
+    else:
+        disney_material(mats[2], (1.0, 1.0, 1.0), roughness={"glass": 0.02, "rough_glass": 0.35}[boundary],
+                        specular_transmission=1.0, ior=1.5)
     mats[2]["medium_type"] = hpt.MEDIUM_SCATTER
     mats[2]["medium_density"] = 0.55
     mats[2]["medium_color"] = 0.5 / 0.55
@@ -466,6 +480,21 @@ def triangle_soup(count, seed=0):
     diffuse_material(mats[0], (0.5, 0.5, 0.5))
     cam = look_at_camera((0.5, -1.5, 0.5), (0.5, 0.5, 0.5), yfov_deg=50.0)
     return hpt.Scene(cam, np.zeros(0, hpt.LIGHT_DTYPE), mats, [hpt.Primitive(v, ix, 0)])
+
+
+def batched_job_scenes(seed=0):
+    """The 8 resident scenes of the batched and served workloads, as
+    (name, scene, shader permutation): the Cornell box, the Disney grid, the
+    large mesh with lights and a medium, and five random triangle soups of
+    1e5..2e6 triangles (log-uniform, seeded)."""
+    rng = np.random.default_rng(seed)
+    scenes = [("cornell", cornell_box(), 0),
+              ("disney_grid", disney_grid(seed=0), 0),
+              ("icosphere_lights_medium", displaced_icosphere_lights_medium(level=8), MEDIUM_SCENE_FEATURES)]
+    for i in range(5):
+        n = int(round(math.exp(rng.uniform(math.log(1e5), math.log(2e6)))))
+        scenes.append((f"soup_{n}", triangle_soup(n, seed=100 + i), 0))
+    return scenes
 
 
 def nested_triangles(count=200, ratio=0.7):

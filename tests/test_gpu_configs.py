@@ -1,0 +1,124 @@
+"""The benchmark configurations at full size on the GPU.
+
+* C1 and C2 (Disney grid with the 2048x1024 sky map; the 1.31 M-triangle
+  displaced icosphere with five lights and a medium), one 1920x1080 frame
+  each, path by path against the oracle (pathtracer.rgen.glsl:22-305).
+* C3: 64 jobs drawn from the batched-job distribution over the 8 resident
+  scenes (the three above and five triangle soups of 1e5-2e6 triangles),
+  served together, one wavefront per scene and bounce step or a stream per
+  job; every job's image equals its solo render bit for bit.
+* C4: the same distribution arriving as a Poisson process at 20 jobs/s,
+  admitted by device-memory footprint, each finished job denoised on the
+  device; every served image equals denoising the job's solo render.
+Sample counts are capped (C3: 4 spp, C4: 8 spp) to keep the step short; the
+job draw, resolutions, depths and scenes are the configuration's.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import hpt
+import hpt_scenes
+import pyoracle
+import serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _agreement(gpu, ref, tol=1e-3):
+    err = np.abs(gpu - ref) / (np.abs(ref) + 1e-2)
+    return float((err.max(axis=-1) < tol).mean())
+
+
+@pytest.mark.parametrize("case", ["disney_grid_env_1080p", "icosphere_lights_medium_1080p",
+                                  "icosphere_lights_medium_1080p_sorted"])
+def test_full_size_scene_one_frame_per_sample(case):
+    if case.startswith("disney"):
+        scene, depth, features, sort = hpt_scenes.disney_grid(seed=0), 10, 0, hpt.SHADE_SORT_OFF
+    else:
+        scene, depth, features = hpt_scenes.displaced_icosphere_lights_medium(level=8), 12, \
+            hpt_scenes.MEDIUM_SCENE_FEATURES
+        sort = 1 if case.endswith("sorted") else hpt.SHADE_SORT_OFF
+    w, h = 1920, 1080
+    print(f"[gpu] {case}: {scene.triangle_count} triangles, {w}x{h}, depth {depth}", flush=True)
+    r = hpt.HalaRenderer(w, h, max_depth=depth, rr_depth=3, features=features)
+    r.set_scene(scene)
+    r.commit()
+    r.set_shade_sort(sort)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    alb = r.read_image(hpt.IMAGE_ALBEDO)[..., :3]
+    r.close()
+    o = pyoracle.OracleScene(scene)
+    ref, ref_alb, _ = o.trace_paths(pyoracle.settings(w, h, depth, 3, features), 0)
+    o.close()
+    agree = _agreement(gpu, ref)
+    same_alb = float((alb == ref_alb).all(axis=-1).mean())
+    print(f"[gpu] {case}: per-sample agreement {agree:.5f}, first-hit albedo identical {same_alb:.5f}", flush=True)
+    assert agree >= 0.995
+    assert same_alb >= 0.9999
+
+
+@pytest.fixture(scope="module")
+def resident():
+    print("[gpu] committing the 8 resident scenes of the batched-job workload", flush=True)
+    scenes = {name: serve.ResidentScene(name, sc, feat) for name, sc, feat in hpt_scenes.batched_job_scenes()}
+    soups = [rs.scene.triangle_count for name, rs in scenes.items() if name.startswith("soup_")]
+    assert len(scenes) == 8 and len(soups) == 5 and all(1e5 <= n <= 2e6 for n in soups)
+    yield scenes
+    for rs in scenes.values():
+        rs.close()
+
+
+def _solo(rs, spec):
+    r = hpt.HalaRenderer(spec.width, spec.height, max_depth=spec.max_depth, rr_depth=spec.rr_depth,
+                         enable_tonemap=True, features=rs.features)
+    r.share_scene(rs.template)
+    r.render(spec.spp)
+    return r
+
+
+@pytest.mark.parametrize("wavefront", [True, False], ids=["wavefront_per_scene", "stream_per_job"])
+def test_c3_batched_uneven_jobs_equal_solo_renders(resident, wavefront):
+    jobs = serve.poisson_jobs(64, 0.0, sorted(resident), seed=0)  # all at t = 0
+    for j in jobs:
+        j.spp = min(j.spp, 4)
+    assert {(j.width, j.height) for j in jobs} == {(256, 256), (512, 512), (1024, 768), (1920, 1080)}
+    assert len({j.scene for j in jobs}) == 8 and {j.max_depth for j in jobs} == set(range(8, 13))
+    print(f"[gpu] C3: 64 jobs, wavefront {wavefront}", flush=True)
+    srv = serve.Server(resident, 64 << 30, max_frames=16, keep_images=True, wavefront=wavefront,
+                       step_paths=4 << 20)
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert all(r.error is None for r in res.values()), [r.error for r in res.values() if r.error]
+    if wavefront:
+        print(f"[gpu] C3: {len(srv.step_sizes)} wavefront steps, up to {max(srv.step_sizes)} jobs per step",
+              flush=True)
+        assert max(srv.step_sizes) > 1  # jobs of one scene did share steps
+    for j in jobs:
+        r = _solo(resident[j.scene], j)
+        assert np.array_equal(r.read_image(hpt.IMAGE_ACCUM), res[j.job_id].image), j
+        r.close()
+
+
+def test_c4_poisson_serving_with_the_denoiser(resident):
+    import denoiser
+
+    net = denoiser.UNet.load(denoiser.weights_path())
+    jobs = serve.poisson_jobs(64, 20.0, sorted(resident), seed=1)
+    for j in jobs:
+        j.spp = min(j.spp, 8)
+    print(f"[gpu] C4: 64 Poisson jobs at 20/s over {jobs[-1].arrival:.2f} s, denoised", flush=True)
+    srv = serve.Server(resident, 64 << 30, max_frames=16, keep_images=True, denoiser=net)
+    t = time.perf_counter()
+    results = srv.run(jobs)
+    summ = serve.summarize(results, time.perf_counter() - t)
+    print(f"[gpu] C4: {summ}", flush=True)
+    res = {r.job_id: r for r in results}
+    assert summ["failed"] == 0, [r.error for r in results if r.error]
+    for j in jobs:
+        r = _solo(resident[j.scene], j)
+        want = net.denoise_renderer(r).numpy()
+        denoiser.synchronize()
+        r.close()
+        assert np.array_equal(want, res[j.job_id].image), j

@@ -20,6 +20,16 @@ import pyoracle
 pytestmark = pytest.mark.gpu
 
 
+# Acceptance bars.  Per-sample: the fraction of pixels whose frame-0 radiance
+# agrees with the oracle's within 1e-3 relative (observed 0.9996-1.0000 over
+# every scene here; the rest differ by a transcendental's last bits that
+# change a later branch).  Independent frames: the fraction of pixels within
+# 3 sigma (observed 0.9937-0.9983 at 32-64 frames; per-pixel estimates are
+# heavy-tailed, so exact normal coverage is not expected).
+PER_SAMPLE_MIN = 0.995
+THREE_SIGMA_MIN = 0.99
+
+
 def _renderer(scene, w, h, max_depth=8, rr_depth=3, features=0, **kw):
     print(f"[gpu] render {w}x{h} depth {max_depth} rr {rr_depth} features {features}", flush=True)
     r = hpt.HalaRenderer(w, h, max_depth=max_depth, rr_depth=rr_depth, features=features, **kw)
@@ -46,7 +56,7 @@ def _oracle_frames(o, st, f0, f1, h, w):
     return mean, var
 
 
-def _stat_test(scene, w, h, frames, features=0, max_depth=8, rr_depth=3, min_frac=0.95):
+def _stat_test(scene, w, h, frames, features=0, max_depth=8, rr_depth=3, min_frac=THREE_SIGMA_MIN):
     """GPU frames [0, N) against the oracle's frames [0, N) (the same sample
     streams: per-pixel means must agree far inside 3 sigma) and against the
     oracle's frames [N, 2N) (independent samples: the image mean within 4
@@ -86,7 +96,7 @@ def test_cornell_per_sample():
     ref, alb, nrm = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] cornell per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
     # first-hit AOVs depend only on the primary ray
     assert np.array_equal(r.read_image(hpt.IMAGE_ALBEDO)[..., :3], alb)
     assert np.allclose(r.read_image(hpt.IMAGE_NORMAL)[..., :3], nrm, atol=1e-6)
@@ -140,7 +150,7 @@ def test_disney_env_per_sample_and_converged():
     ref, _, _ = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] disney per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.9
+    assert agree >= PER_SAMPLE_MIN
     r.close()
     _stat_test(scene, w, h, 48, max_depth=10, rr_depth=3)
 
@@ -165,9 +175,9 @@ def test_lights_medium_permutations(features):
     ref, _, _ = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] features {features} per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.9
+    assert agree >= PER_SAMPLE_MIN
     r.close()
-    _stat_test(scene, w, h, 32, features=features, max_depth=12, rr_depth=3, min_frac=0.95)
+    _stat_test(scene, w, h, 32, features=features, max_depth=12, rr_depth=3)
 
 
 @pytest.mark.parametrize("medium_type", [hpt.MEDIUM_ABSORB, hpt.MEDIUM_EMISSIVE], ids=["absorb", "emissive"])
@@ -204,9 +214,71 @@ def test_absorbing_and_emissive_media(medium_type, features):
     plain.close()
     print(f"[gpu] medium {medium_type} features {features} per-sample agreement {agree:.4f}, "
           f"pixels changed by the medium {changed:.3f}", flush=True)
-    assert agree > 0.9
+    assert agree >= PER_SAMPLE_MIN
     assert changed > 0.02
-    _stat_test(sc, w, h, 32, features=features, max_depth=12, rr_depth=3, min_frac=0.95)
+    _stat_test(sc, w, h, 32, features=features, max_depth=12, rr_depth=3)
+
+
+@pytest.mark.parametrize("features", [
+    hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS | hpt.FEATURE_TRANSPARENT,
+    hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS,
+    hpt.FEATURE_MEDIUM | hpt.FEATURE_TRANSPARENT,
+    hpt.FEATURE_MEDIUM,
+], ids=["vol_mis_transparent", "vol_mis", "transparent", "medium"])
+@pytest.mark.parametrize("boundary", ["glass", "rough_glass"])
+def test_glass_boundary_medium_permutations(boundary, features):
+    """The lights scene with its scattering medium behind a Disney dielectric
+    (specular transmission 1, ior 1.5, roughness 0.02 / 0.35): paths enter
+    the medium by refraction in every MEDIUM permutation
+    (inc/bxdf/disney.glsl:179-219 eval, :493-512 refraction sampling,
+    pathtracer.rgen.glsl:233-243 medium entry), and the volume-MIS walk passes
+    the refractive surface (inc/lighting.glsl:45-47)."""
+    def scene(mtype):
+        sc = hpt_scenes.displaced_icosphere_lights_medium(level=3, boundary=boundary)
+        sc.materials[2]["medium_type"] = mtype
+        return sc
+
+    w, h = 64, 36
+    sc = scene(hpt.MEDIUM_SCATTER)
+    r = _renderer(sc, w, h, max_depth=12, rr_depth=3, features=features)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    r.close()
+    o = pyoracle.OracleScene(sc)
+    ref, _, _ = o.trace_paths(pyoracle.settings(w, h, 12, 3, features), 0)
+    agree = _per_sample_agreement(gpu, ref)
+    # the medium lies on the paths of 10-14 % of the pixels (the oracle's
+    # figure): without it they differ, so every permutation enters it
+    plain = _renderer(scene(hpt.MEDIUM_NONE), w, h, max_depth=12, rr_depth=3, features=features)
+    plain.render(1)
+    changed = float((np.abs(gpu - plain.read_image(hpt.IMAGE_ACCUM)[..., :3]).max(axis=-1) > 1e-4).mean())
+    plain.close()
+    print(f"[gpu] {boundary} boundary features {features} per-sample agreement {agree:.4f}, "
+          f"pixels changed by the medium {changed:.3f}", flush=True)
+    assert agree >= PER_SAMPLE_MIN
+    assert changed > 0.05
+    _stat_test(sc, w, h, 32, features=features, max_depth=12, rr_depth=3)
+
+
+def test_subsurface_per_sample_and_converged():
+    """Disney spheres with the fake subsurface lobe (inc/bxdf/disney.glsl:93-112)
+    weighted 0.25-1.0 by row, under the sky map."""
+    scene = hpt_scenes.disney_grid(seed=1, env_width=256, env_height=128, segments=16, rings=8, subsurface=True)
+    w, h = 96, 54
+    r = _renderer(scene, w, h, max_depth=10, rr_depth=3)
+    o = pyoracle.OracleScene(scene)
+    r.render(1)
+    gpu = r.read_image(hpt.IMAGE_ACCUM)[..., :3]
+    r.close()
+    ref, _, _ = o.trace_paths(pyoracle.settings(w, h, 10, 3), 0)
+    agree = _per_sample_agreement(gpu, ref)
+    plain = hpt_scenes.disney_grid(seed=1, env_width=256, env_height=128, segments=16, rings=8)
+    ref0, _, _ = pyoracle.OracleScene(plain).trace_paths(pyoracle.settings(w, h, 10, 3), 0)
+    changed = float((np.abs(ref - ref0).max(axis=-1) > 1e-4).mean())
+    print(f"[gpu] subsurface per-sample agreement {agree:.4f}, pixels changed by the lobe {changed:.3f}", flush=True)
+    assert agree >= PER_SAMPLE_MIN
+    assert changed > 0.1
+    _stat_test(scene, w, h, 48, max_depth=10, rr_depth=3)
 
 
 def test_final_image_matches_oracle_tonemap():
@@ -271,7 +343,7 @@ def test_orthographic_camera_per_sample_and_converged():
     ref, alb, nrm = o.trace_paths(pyoracle.settings(w, h, 8, 3), 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] orthographic per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
     assert np.array_equal(gpu_alb, alb)
     # not the pinhole's view of the same box
     p = _renderer(persp_scene, w, h)
@@ -337,7 +409,7 @@ def test_deep_tree_render_per_sample():
     ref, _, _ = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] deep tree per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
     r.close()
 
 
@@ -354,7 +426,7 @@ def test_textured_materials_per_sample_and_converged():
     ref, alb, nrm = o.trace_paths(st, 0)
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] textured per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
     # first-hit AOVs after the maps: the texture coordinate is interpolated
     # with fused multiply-adds on the GPU, so allow a few ulps of weight.
     assert np.allclose(r.read_image(hpt.IMAGE_ALBEDO)[..., :3], alb, atol=1e-4)
@@ -385,7 +457,7 @@ def test_reference_cornell_asset_per_sample():
     r.render(1)
     agree = _per_sample_agreement(r.read_image(hpt.IMAGE_ACCUM)[..., :3], o.trace_paths(st, 0)[0])
     print(f"[gpu] reference cornell asset per-sample agreement {agree:.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
     r.close()
 
 
@@ -425,7 +497,7 @@ def test_app_with_an_exr_hdri_per_sample(tmp_path):
     ref = o.trace_paths(pyoracle.settings(w, h, 6, 3, app.scene_features(scene)), 0)[0]
     agree = _per_sample_agreement(gpu, ref)
     print(f"[gpu] EXR hdri per-sample agreement {agree:.4f}, mean {gpu.mean():.4f} vs {ref.mean():.4f}", flush=True)
-    assert agree > 0.97
+    assert agree >= PER_SAMPLE_MIN
 
 
 @pytest.mark.gpu
