@@ -1,20 +1,27 @@
-"""Throughput benchmark of the path-tracing hot path on one MI355X.
+"""Benchmark of the path-tracing hot path on one MI355X.
 
-Default workload: the Disney material grid with an importance-sampled sky
-map -- 7x7 UV spheres (64x32 segments, 194,434 triangles with the ground) at
-1920x1080, max depth 10, Russian roulette from depth 3.  A step is one frame:
-one camera path per pixel through the full bounce loop plus accumulation; the
-default --steps is the configuration's 256 samples per pixel.  --workload
-selects the Cornell box (512x512, 64 spp, depth 8) or the 1.3 M-triangle
-displaced icosphere with five light types and a scattering medium (1080p,
-128 spp, depth 12); --relmse adds the plain tracer's time to relMSE <= 1e-3
-against a 4096-spp reference.
-
-Reports Msamples/s (pixels x frames / wall seconds, inputs resident in HBM),
-the roofline of the dominant kernel (BVH traversal of the extension rays,
-HBM-bound) from HIP events recorded on the renderer's stream around every
-launch, and the CPU oracle's throughput on a bounded sample of the same
-workload on this host.
+Default workload (the configuration the north-star target is quoted on):
+the 1.31 M-triangle fBm-displaced icosphere with a point, spot,
+directional, rect and sphere light and a homogeneous scattering medium,
+thin-lens camera, 1920x1080, max depth 12, Russian roulette from depth 3,
+the MEDIUM|VOL_MIS|TRANSPARENT permutation.  A step is one frame: one
+camera path per pixel through the full bounce loop plus accumulation; the
+default --steps is the configuration's 128 samples per pixel.  The line
+carries
+  * value: Msamples/s (pixels x frames / wall seconds, inputs resident in HBM);
+  * roofline: the dominant kernel's (the shading kernel with its volume-MIS
+    transmittance walks), with the extension walk's as roofline_extend,
+    each from HIP events recorded on the renderer's stream around every
+    launch and an algorithmic byte model per unit;
+  * time_to_relmse: the plain tracer, the tracer + learned denoiser and the
+    served (submission to image in host memory) times to relMSE <= 1e-3
+    against a 4096-spp render of the same scene;
+  * serving: p50/p99 job latency and Msamples/s of 64 Poisson jobs at 20/s
+    over the 8 resident scenes of the batched-job workload;
+  * cpu_baseline: the CPU oracle on a bounded sample of the same workload.
+--quick skips the relMSE and serving runs.  --workload selects the Disney
+grid (1080p, 256 spp, depth 10), the Cornell box (512x512, 64 spp, depth 8),
+the serving run alone, or the denoiser alone.
 """
 from __future__ import annotations
 
@@ -36,6 +43,13 @@ import hpt_scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 NODE_BYTES, TRI_BYTES, RAY_BYTES = 80, 48, 48  # per visited node, tested triangle, ray (in + out)
+# Per shading event: the survey's 384 B of scene records (material 144,
+# primitive 96, indices 12, three vertices 132) plus the path state the
+# shading kernel reads (origin, direction, throughput, hit, RNG, radiance,
+# medium: 120 B, and its queue entry) and writes back (104 B, and one queue
+# slot).
+SHADE_EVENT_BYTES = 384 + 232
+HEADLINE = "icosphere_lights_medium_1080p"
 
 
 def log(msg):
@@ -77,18 +91,19 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s
             "sample": f"{what} ({paths} paths, {dt:.1f} s)"}
 
 
-def profiled_traffic(workload, frames_in_flight):
-    """HBM bytes per k_extend launch from the committed rocprofv3 summary of
-    the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
-    reading of a 1 GiB copy), when it was taken at the same batch size; else
-    None."""
-    path = os.path.join(ROOT, "profiles", f"k_extend_{workload}.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        m = re.match(r"traffic_bytes_per_launch k_extend \(frames_in_flight (\d+)\) = (\d+)", line)
-        if m and int(m.group(1)) == frames_in_flight:
-            return int(m.group(2))
+def profiled_traffic(kernel, workload, batch_frames):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    of the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    the reading of a 1 GiB copy), when it was taken at the same batch size
+    (frames traced together); else None."""
+    d = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        if not name.endswith(f"_{workload}.md"):
+            continue
+        for line in open(os.path.join(d, name)):
+            m = re.match(r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+)\) = (\d+)", line)
+            if m and m.group(1) == kernel and int(m.group(2)) == batch_frames:
+                return int(m.group(3))
     return None
 
 
@@ -122,8 +137,8 @@ def copy_calibration(mib=1024, reps=4):
 
 
 # The workloads (scene, image, depth, Russian-roulette start, shader permutation,
-# samples per pixel): the Disney grid is the headline; the Cornell box and the
-# large-mesh scene with five light types and a medium are run with --workload.
+# samples per pixel): the large-mesh scene with five light types and a medium
+# is the headline; the Disney grid and the Cornell box run with --workload.
 WORKLOADS = {
     "disney_grid_env_1080p": dict(scene=lambda: hpt_scenes.disney_grid(seed=0), width=1920, height=1080,
                                   depth=10, rr=3, features=0, spp=256),
@@ -356,7 +371,11 @@ def denoise_bench(args):
     print(json.dumps(out_json), flush=True)
 
 
-def serving_bench(args):
+def run_serving(args, jobs_seed=0):
+    """The batched/served workload: 8 resident scenes (hpt_scenes.batched_job_scenes),
+    args.jobs Poisson arrivals at args.rate from the job distribution, served
+    by serve.Server.  Returns (summary, resident scene sizes, jobs, makespan,
+    tuning, tuning seconds)."""
     import serve
 
     log("building and committing the 8 resident scenes")
@@ -366,7 +385,7 @@ def serving_bench(args):
         resident[name] = rs
         log(f"  {name}: {sc.triangle_count} triangles, commit {rs.commit_seconds:.2f} s, "
             f"{rs.scene_bytes / 2**20:.0f} MiB")
-    jobs = serve.poisson_jobs(args.jobs, args.rate, sorted(resident), seed=0)
+    jobs = serve.poisson_jobs(args.jobs, args.rate, sorted(resident), seed=jobs_seed)
     offered = sum(j.width * j.height * j.spp for j in jobs)
     log(f"{len(jobs)} jobs, {offered / 1e9:.2f} Gsamples offered over {jobs[-1].arrival:.2f} s of arrivals")
     tuning, t_tune = None, 0.0
@@ -382,40 +401,65 @@ def serving_bench(args):
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
     summ = serve.summarize(res, makespan)
+    summ["offered_gsamples"] = round(offered / 1e9, 3)
+    summ["arrival_span_s"] = round(jobs[-1].arrival, 3)
+    summ["arrival_rate_per_s"] = args.rate
+    summ["mode"] = "wavefront per scene" if args.wavefront else "stream per job"
     if args.wavefront and srv.step_sizes:
         summ["wavefront_steps"] = len(srv.step_sizes)
         summ["mean_jobs_per_step"] = round(float(np.mean(srv.step_sizes)), 2)
         summ["max_jobs_per_step"] = int(max(srv.step_sizes))
     log(f"served: {summ}")
+    sizes = {k: v.scene.triangle_count for k, v in resident.items()}
     for rs in resident.values():
         rs.close()
+    return summ, sizes, jobs, makespan, tuning, t_tune
+
+
+def serving_bench(args):
+    summ, sizes, jobs, makespan, tuning, t_tune = run_serving(args)
     out = {
         "metric": "Msamples/s", "value": summ["msamples_per_s"], "unit": "Msamples/s", "n_gpus": 1,
         "steps": len(jobs), "warmup": 0, "ms_per_step": round(makespan / len(jobs) * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "serving", "jobs": len(jobs), "arrival_rate_per_s": args.rate,
-                   "resident_scenes": {k: v.scene.triangle_count for k, v in resident.items()},
+                   "resident_scenes": sizes,
                    "budget_gib": args.budget_gib, "job_frames_in_flight": args.job_frames,
                    "autotuned": {k: [v.frames_in_flight, "graph" if v.launch == hpt.LAUNCH_GRAPH else "eager",
                                      "sort off" if v.shade_sort == hpt.SHADE_SORT_OFF else f"sort from {v.shade_sort}"]
                                  for k, v in tuning.entries.items()} if tuning else None,
                    "autotune_seconds": round(t_tune, 2), "launch": args.launch,
-                   "mode": "wavefront per scene" if args.wavefront else "stream per job",
-                   "step_mpaths": args.step_mpaths if args.wavefront else None,
-                   "offered_gsamples": round(offered / 1e9, 3),
-                   "arrival_span_s": round(jobs[-1].arrival, 3)},
+                   "mode": summ["mode"],
+                   "step_mpaths": args.step_mpaths if args.wavefront else None},
         "serving": summ, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
 
+def stage_roofline(kernel, stats_timed, per_unit_bytes, units_key="rays", traffic=None, extra=None):
+    """Roofline of one stage's kernel: algorithmic bytes per launch (units
+    per launch in the timed run x bytes per unit) over its mean launch time
+    from the HIP events bracketing every launch on the renderer's stream."""
+    launches = max(1, stats_timed["launches"])
+    ms = stats_timed["total_ms"] / launches
+    bytes_per_launch = stats_timed[units_key] / launches * per_unit_bytes
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+           "avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch),
+           "launches": stats_timed["launches"]}
+    if extra:
+        out.update(extra)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "denoise"], default="disney_grid_env_1080p")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "denoise"], default=HEADLINE)
     ap.add_argument("--denoise", action="store_true",
-                    help="with --relmse: also the time to the threshold of the denoised estimate")
+                    help="with --relmse: also the time to the threshold of the denoised estimate (and served)")
     ap.add_argument("--weights", default="", help="denoiser weights (default: the committed network)")
     ap.add_argument("--train-steps", type=int, default=50,
                     help="denoise: timed training steps (0 skips the training measurement)")
@@ -443,7 +487,13 @@ def main():
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced together per batch (0: sized by the library)")
     ap.add_argument("--relmse", action="store_true",
-                    help="also measure the plain tracer's time to relMSE <= --threshold against a reference")
+                    help="also measure the plain tracer's time to relMSE <= --threshold against a reference "
+                         "(on by default for the headline workload)")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the served workload (p50/p99 job latency at --rate; on by default for the "
+                         "headline workload)")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline workload without the relMSE and serving measurements")
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--threshold", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -464,6 +514,8 @@ def main():
     if args.workload == "denoise":
         denoise_bench(args)
         return
+    if args.workload == HEADLINE and not args.quick:
+        args.relmse = args.denoise = args.serving = True
     wl = WORKLOADS[args.workload]
     W = args.width or wl["width"]
     H = args.height or wl["height"]
@@ -502,13 +554,20 @@ def main():
         log(f"shade sort autotuned in {time.perf_counter() - t0:.1f} s: "
             f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'} ({sort_table})")
 
-    # Visit counts per ray from a counted warm-up (separate kernel variant).
+    # Visit counts per ray (extend) and per shading event (the volume-MIS
+    # walks inside shading) from a counted warm-up (separate kernel variants).
     r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
     r.render(max(1, args.warmup))
     ext = r.stage_stats(hpt.STAGE_EXTEND)
+    sh = r.stage_stats(hpt.STAGE_SHADE)
     nodes_per_ray = ext["node_visits"] / max(1, ext["rays"])
     tris_per_ray = ext["tri_tests"] / max(1, ext["rays"])
-    log(f"warm-up: extend {ext['rays']} rays, {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray")
+    walk_q = sh["queries"] / max(1, sh["rays"])
+    walk_nv = sh["node_visits"] / max(1, sh["rays"])
+    walk_nt = sh["tri_tests"] / max(1, sh["rays"])
+    log(f"warm-up: extend {ext['rays']} rays, {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray; "
+        f"shade {sh['rays']} events, {walk_q:.3f} walk queries, {walk_nv:.2f} walk nodes, {walk_nt:.2f} walk tris "
+        f"per event")
 
     # Timed region: `steps` frames from a fresh accumulation, submitted the way
     # the renderer submits them (full batches as graph replays by default).
@@ -532,21 +591,38 @@ def main():
     r.render(steps)
     r.wait_idle()
     profiled_wall = time.perf_counter() - t0
-    stats = {s: r.stage_stats(s) for s in range(5)}
-    ext = stats[hpt.STAGE_EXTEND]
-    shadow = stats[hpt.STAGE_SHADOW]
+    stats = {s: r.stage_stats(s) for s in range(hpt.STAGE_SORT + 1)}
+    ext_t, shade_t, shadow_t = stats[hpt.STAGE_EXTEND], stats[hpt.STAGE_SHADE], stats[hpt.STAGE_SHADOW]
     samples = W * H * steps
     msps = samples / wall / 1e6
-    ext_ms = ext["total_ms"] / max(1, ext["launches"])
-    bytes_per_launch = (ext["rays"] / max(1, ext["launches"])) * (
-        NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES)
-    achieved = bytes_per_launch / (ext_ms * 1e-3) / 1e9
-    rays_total = ext["rays"] + shadow["rays"]
+    batch_frames = min(steps, fif)
+    rays_total = ext_t["rays"] + shadow_t["rays"] + int(round(shade_t["rays"] * walk_q))
+    names = ["camera", "extend", "shade", "shadow", "accum", "sort"]
     log("stage totals: " + ", ".join(f"{n} {stats[s]['total_ms']:.1f} ms/{stats[s]['launches']}"
-                                       for s, n in enumerate(["camera", "extend", "shade", "shadow", "accum"])))
+                                       for s, n in enumerate(names)))
     log(f"wall {wall:.3f} s ({args.launch}, {graph_launches} graph replays), {msps:.1f} Msamples/s, "
         f"{rays_total / wall / 1e6:.1f} Mrays/s; eager with stage events {profiled_wall:.3f} s")
     r.close()
+
+    # Rooflines: the extension walk (80 B per node, 48 B per triangle, 48 B
+    # per ray) and the shading kernel (616 B per shading event -- material,
+    # primitive, indices and vertices as the survey's 384 B plus 232 B of path
+    # state read and written -- plus its volume-MIS walks at the extension's
+    # rate per closest-hit query).
+    roof_ext = stage_roofline("k_extend", ext_t, NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
+                              traffic=profiled_traffic("k_extend", args.workload, batch_frames),
+                              extra={"nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)})
+    shade_bytes = SHADE_EVENT_BYTES + NODE_BYTES * walk_nv + TRI_BYTES * walk_nt + RAY_BYTES * walk_q
+    roof_shade = stage_roofline("k_shade", shade_t, shade_bytes,
+                                traffic=profiled_traffic("k_shade", args.workload, batch_frames),
+                                extra={"bytes_per_event": round(shade_bytes, 1),
+                                       "walk_queries_per_event": round(walk_q, 4),
+                                       "walk_nodes_per_event": round(walk_nv, 3),
+                                       "walk_tris_per_event": round(walk_nt, 3)})
+    dominant = max((("extend", ext_t), ("shade", shade_t), ("shadow", shadow_t)), key=lambda kv: kv[1]["total_ms"])[0]
+    roofline = roof_shade if dominant == "shade" else roof_ext
+    log(f"dominant stage: {dominant}; k_shade {roof_shade['achieved']:.0f} GB/s ({roof_shade['frac']:.3f}), "
+        f"k_extend {roof_ext['achieved']:.0f} GB/s ({roof_ext['frac']:.3f})")
 
     conv = None
     if args.relmse:
@@ -565,13 +641,18 @@ def main():
                 conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, denoiser.UNet.load(path),
                                                conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort)
             conv["denoised"] = den
+    serving = None
+    if args.serving:
+        serving, _, _, _, _, _ = run_serving(args)
     base = None if args.no_cpu_baseline else cpu_baseline(scene, W, H, DEPTH, RR, FEATURES, threads=args.cpu_threads)
     config = {"workload": args.workload, "width": W, "height": H, "spp": steps, "max_depth": DEPTH,
               "rr_depth": RR, "features": FEATURES, "triangles": scene.triangle_count,
-              "lights": int(len(scene.lights)), "frames_in_flight": fif, "launch": args.launch,
+              "lights": int(len(scene.lights)), "frames_in_flight": fif, "batch_frames": batch_frames,
+              "launch": args.launch,
               "shade_sort": "off" if shade_sort == hpt.SHADE_SORT_OFF else shade_sort,
               "shade_sort_tuning": sort_table,
               "eager_profiled_wall_s": round(profiled_wall, 4),
+              "stage_ms": {n: round(stats[s]["total_ms"], 3) for s, n in enumerate(names)},
               "mrays_per_s": round(rays_total / wall / 1e6, 2)}
     if scene.env_rgb is not None:
         config["env_map"] = [int(scene.env_rgb.shape[1]), int(scene.env_rgb.shape[0])]
@@ -589,15 +670,26 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": config,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(args.workload, fif),
-                     "kernel": "k_extend", "avg_launch_ms": round(ext_ms, 4),
-                     "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
+        "roofline": roofline,
+        "roofline_extend": roof_ext,
+        "roofline_shade": roof_shade,
         "cpu_baseline": base,
     }
     if conv is not None:
+        tt = {"threshold": args.threshold, "ref_spp": args.ref_spp, "plain_s": conv["time_to_threshold_s"],
+              "plain_spp": conv["spp_at_threshold"]}
+        den = conv.get("denoised") or {}
+        tt["denoised_s"] = den.get("time_to_threshold_s")
+        tt["denoised_spp"] = den.get("spp_at_threshold")
+        sv = conv.get("served") or {}
+        if sv:
+            tt["served_plain_s"] = sv["plain"]["latency_s"]
+            tt["served_denoised_s"] = sv["denoised"]["latency_s"]
+            tt["served_speedup_vs_plain"] = round(conv["time_to_threshold_s"] / sv["denoised"]["latency_s"], 2)
+        out["time_to_relmse"] = tt
         out["relmse"] = conv
+    if serving is not None:
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
 
 

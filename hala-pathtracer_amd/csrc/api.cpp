@@ -430,6 +430,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   const bool count = timing && (timing->profiling & HPT_PROFILE_COUNT_VISITS) != 0;
   const StageCounters sc_extend{stats ? stats + kStatsPerStage * HPT_STAGE_EXTEND : nullptr, count};
   const StageCounters sc_shadow{stats ? stats + kStatsPerStage * HPT_STAGE_SHADOW : nullptr, count};
+  const StageCounters sc_shade{stats ? stats + kStatsPerStage * HPT_STAGE_SHADE : nullptr, count};
 
   begin(HPT_STAGE_CAMERA);
   for (size_t j = 0; j < jobs.size(); ++j)
@@ -460,13 +461,15 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     begin(HPT_STAGE_EXTEND);
     launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr);
     end();
-    begin(HPT_STAGE_SHADE);  // the sort counts as shading time
     Queue qshade = qin;
     if (sort) {
+      begin(HPT_STAGE_SORT);
       launch_shade_sort(s, qin, n, ss);
+      end();
       qshade = Queue{ss.items, ss.counts, g.seg_cap};
     }
-    launch_shade(s, features, sv, shade_fp, ps, qshade, n, qout, sq, sp);
+    begin(HPT_STAGE_SHADE);
+    launch_shade(s, features, sv, shade_fp, ps, qshade, n, qout, sq, sp, sc_shade);
     end();
     if (!vmis) {
       begin(HPT_STAGE_SHADOW);
@@ -1466,12 +1469,13 @@ hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out)
       check(hipMemcpy(v.data(), r->d_stats.as<unsigned long long>() + (size_t)kStatsPerStage * stage,
                       v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost),
             "stats");
-      uint64_t sum[3] = {0, 0, 0};
-      for (uint32_t j = 0; j < 3; ++j)
+      uint64_t sum[kStatCounters] = {};
+      for (uint32_t j = 0; j < kStatCounters; ++j)
         for (uint32_t sg = 0; sg < kSeg; ++sg) sum[j] += v[(j * kSeg + sg) * kStatStride];
       out->rays = sum[0];
       out->node_visits = sum[1];
       out->tri_tests = sum[2];
+      out->queries = sum[3];
     }
   });
 }

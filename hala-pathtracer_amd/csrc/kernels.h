@@ -123,11 +123,15 @@ void launch_camera(hipStream_t s, const SceneView& sv, const FrameParams& fp, co
                    uint32_t npaths, uint32_t job = 0);
 // Device counters of one stage, 64-bit, spread like the queue counters:
 // rays at stats[seg * kStatStride], node visits at stats[(kSeg + seg) *
-// kStatStride], triangle tests at stats[(2 * kSeg + seg) * kStatStride].
-// Rays are counted whenever stats is non-null; visits only by the counting
-// kernel variant (count == true).
+// kStatStride], triangle tests at stats[(2 * kSeg + seg) * kStatStride],
+// closest-hit queries at stats[(3 * kSeg + seg) * kStatStride].  Rays (for
+// shading: shading events) are counted whenever stats is non-null; visits,
+// tests and queries only by the counting kernel variants (count == true):
+// the traversal kernels' own walks, and the volume-MIS transmittance walks
+// of the shading kernel.
 constexpr uint32_t kStatStride = 32;
-constexpr uint32_t kStatsPerStage = 3 * kSeg * kStatStride;
+constexpr uint32_t kStatCounters = 4;
+constexpr uint32_t kStatsPerStage = kStatCounters * kSeg * kStatStride;
 struct StageCounters {
   unsigned long long* stats;
   bool count;
@@ -144,7 +148,7 @@ void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, cons
 void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss);
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp,
                   const PathState& ps, const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq,
-                  Spill sp);
+                  Spill sp, StageCounters sc);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc);
 // Depth limits of the jobs of one wavefront (FrameParams::job_limits), passed

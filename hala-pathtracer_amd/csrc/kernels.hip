@@ -399,11 +399,41 @@ __device__ __forceinline__ void apply_textures(const SceneView& sv, uint32_t id,
 
 // inc/lighting.glsl:7-61 (MEDIUM + VOL_MIS): closest-hit steps through
 // transparent or refractive surfaces, Beer-Lambert on exit faces.
-template <bool TRANSP>
-__device__ f3 eval_transmittance(const SceneView& sv, uint32_t max_depth, uint4& rng, f3 o, f3 d, LaneStack& st) {
+// Closest-hit queries, node visits and triangle tests of one lane's walks
+// (counted by the profiling variant of the shading kernel).
+struct WalkCounts {
+  uint32_t q = 0, nv = 0, nt = 0;
+};
+
+// Adds per-lane walk counts to the stage counters (node visits, triangle
+// tests, queries), one atomic per wave and counter.
+__device__ __forceinline__ void add_walks(unsigned long long* stats, const WalkCounts& wc) {
+  unsigned long long a = wc.nv, b = wc.nt, c = wc.q;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+    c += __shfl_xor(c, off);
+  }
+  if (lane_id() == 0) {
+    const uint32_t s = blockIdx.x % kSeg;
+    atomicAdd(stats + (kSeg + s) * kStatStride, a);
+    atomicAdd(stats + (2 * kSeg + s) * kStatStride, b);
+    atomicAdd(stats + (3 * kSeg + s) * kStatStride, c);
+  }
+}
+
+template <bool TRANSP, bool COUNT>
+__device__ f3 eval_transmittance(const SceneView& sv, uint32_t max_depth, uint4& rng, f3 o, f3 d, LaneStack& st,
+                                 WalkCounts& wc) {
   f3 tr = mk3(1.0f);
   for (uint32_t depth = 0; depth < max_depth; depth++) {
-    const TraceResult r = trace_closest(sv, o, d, 0.0f, WORLD_SIZE, st);
+    uint32_t nv = 0, nt = 0;
+    const TraceResult r = trace_closest(sv, o, d, 0.0f, WORLD_SIZE, st, COUNT ? &nv : nullptr, COUNT ? &nt : nullptr);
+    if (COUNT) {
+      wc.q++;
+      wc.nv += nv;
+      wc.nt += nt;
+    }
     if (r.id == MISS_ID || (r.id & LIGHT_BIT)) break;
     const HitState h = triangle_state(sv, r.id, r.t, r.u, r.v, o, d);
     if (h.material == INVALID) return mk3(0.0f);
@@ -431,11 +461,11 @@ struct ShadowOut {
 // each unoccluded-candidate term is returned with its ray.  With VOL_MIS the
 // transmittance walk runs inline (it draws from the path RNG between the
 // environment and light samples) and the sum is returned directly.
-template <bool MED, bool VMIS, bool TRANSP>
+template <bool MED, bool VMIS, bool TRANSP, bool COUNT>
 __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32_t max_depth, uint4& rng,
                               const Mat& mat, const Frame& fr, f3 hit_pos, f3 ffn, f3 V, float aniso,
                               bool is_surface, bool anb, f3 weight, ShadowOut& s_env, ShadowOut& s_light,
-                              LaneStack& st) {
+                              LaneStack& st, WalkCounts& wc) {
   f3 Ld = mk3(0.0f);
   const f3 scatter_pos = hit_pos + ffn * EPS;
   s_env.valid = false;
@@ -446,7 +476,7 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32
     const f3 light_dir = sample_env_map(rng, sv, fp, Li, light_pdf);
     if (light_pdf > 0.0f) {
       if (VMIS) {
-        const f3 trm = eval_transmittance<TRANSP>(sv, max_depth, rng, scatter_pos, light_dir, st);
+        const f3 trm = eval_transmittance<TRANSP, COUNT>(sv, max_depth, rng, scatter_pos, light_dir, st, wc);
         if (luminance(trm) > 0.0f) {
           Li *= trm;
           float pdf = 0.0f;
@@ -490,7 +520,7 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32
     f3 Li = ls.emission;
     if (light_pdf > 0.0f && dot(light_dir, ls.normal) < 0.0f) {
       if (VMIS) {
-        const f3 trm = eval_transmittance<TRANSP>(sv, max_depth, rng, scatter_pos, light_dir, st);
+        const f3 trm = eval_transmittance<TRANSP, COUNT>(sv, max_depth, rng, scatter_pos, light_dir, st, wc);
         if (luminance(trm) > 0.0f) {
           Li *= trm;
           float pdf = 0.0f;
@@ -540,15 +570,18 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32
 #ifndef HPT_SHADE_VMIS_WAVES_PER_EU
 #define HPT_SHADE_VMIS_WAVES_PER_EU 5
 #endif
-template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
+// stats: null, or the shading stage's counters (shading events as rays; with
+// COUNT the volume-MIS walks' queries, node visits and triangle tests).
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS, bool COUNT>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(VMIS ? HPT_SHADE_VMIS_WAVES_PER_EU : HPT_SHADE_WAVES_PER_EU)))
     k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
-            Spill sp) {
+            Spill sp, unsigned long long* stats) {
   __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
   uint32_t p;
-  const bool active = queue_entry(qin, n_max, nullptr, p);
+  const bool active = queue_entry(qin, n_max, stats, p);
+  WalkCounts wc;
   bool cont = false;
   ShadowOut s_env, s_light;
   s_env.valid = false;
@@ -641,8 +674,8 @@ __global__ void __launch_bounds__(kBlock)
                 T *= med_color;
                 org = org + dir * scatter_dist;
                 h.pos = org;
-                const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir,
-                                                                  med_aniso, false, anb, T, s_env, s_light, st);
+                const f3 Ld = direct_lighting<MED, VMIS, TRANSP, COUNT>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir,
+                                                                  med_aniso, false, anb, T, s_env, s_light, st, wc);
                 radiance += Ld * T;
                 const float r1 = rnd(rng);
                 const float r2 = rnd(rng);
@@ -663,8 +696,8 @@ __global__ void __launch_bounds__(kBlock)
           }
           if (!pass) {
             if (kScatterFlag) surface_scatter = true;
-            const f3 Ld = direct_lighting<MED, VMIS, TRANSP>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir, med_aniso,
-                                                              true, anb, T, s_env, s_light, st);
+            const f3 Ld = direct_lighting<MED, VMIS, TRANSP, COUNT>(sv, fp, max_depth, rng, m, fr, h.pos, h.ffnormal, -dir, med_aniso,
+                                                              true, anb, T, s_env, s_light, st, wc);
             radiance += T * Ld;
             f3 L;
             float spdf;
@@ -715,6 +748,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
   }
+  if (COUNT) add_walks(stats, wc);
   // Compact the survivors and the shadow rays.
   seg_append(cont, qout, p);
   if (!VMIS) {
@@ -986,23 +1020,31 @@ void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const Sh
   hipLaunchKernelGGL(k_sort_scatter, grid, dim3(kBlock), 0, s, qin, n_max, ss);
 }
 
+// The walk-counting variant exists for the volume-MIS permutations of a
+// single-job launch (the only shading kernels that walk).
 template <bool MED, bool VMIS, bool TRANSP>
 static void shade_variant(hipStream_t s, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                          const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp) {
+                          const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp,
+                          StageCounters sc) {
+  const dim3 grid(queue_grid(qin, n_max));
   if (fp.job_limits)
-    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, true>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp,
-                       ps, qin, n_max, qout, sq, sp);
+    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, true, false>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, n_max,
+                       qout, sq, sp, sc.stats);
+  else if (VMIS && sc.count && sc.stats)
+    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, false, VMIS>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, n_max,
+                       qout, sq, sp, sc.stats);
   else
-    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, false>), dim3(queue_grid(qin, n_max)), dim3(kBlock), 0, s, sv, fp,
-                       ps, qin, n_max, qout, sq, sp);
+    hipLaunchKernelGGL((k_shade<MED, VMIS, TRANSP, false, false>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, n_max,
+                       qout, sq, sp, sc.stats);
 }
 
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                  const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp) {
+                  const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq, Spill sp,
+                  StageCounters sc) {
   const bool med = features & HPT_FEATURE_MEDIUM;
   const bool vmis = med && (features & HPT_FEATURE_VOL_MIS);
   const bool tr = features & HPT_FEATURE_TRANSPARENT;
-#define HPT_SHADE(M, V, T) shade_variant<M, V, T>(s, sv, fp, ps, qin, n_max, qout, sq, sp)
+#define HPT_SHADE(M, V, T) shade_variant<M, V, T>(s, sv, fp, ps, qin, n_max, qout, sq, sp, sc)
   if (!med) {
     if (tr) HPT_SHADE(false, false, true);
     else HPT_SHADE(false, false, false);

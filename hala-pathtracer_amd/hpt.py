@@ -33,7 +33,7 @@ MATERIAL_DIFFUSE, MATERIAL_DISNEY = 0, 1
 FEATURE_MEDIUM, FEATURE_VOL_MIS, FEATURE_TRANSPARENT = 1, 2, 4
 IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
 HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
-STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE = range(5)
+STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT = range(6)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 SHADE_SORT_OFF = 0xFFFFFFFF
@@ -141,7 +141,7 @@ class _BvhInfo(C.Structure):
 
 class _StageStats(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double), ("rays", C.c_uint64),
-                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("queries", C.c_uint64)]
 
 
 # Every exported entry point with its argument types (the declarations of
@@ -519,7 +519,7 @@ class HalaRenderer:
         s = _StageStats()
         _check(self._lib.hpt_get_stage_stats(self._h, stage, C.byref(s)))
         return {"launches": s.launches, "total_ms": s.total_ms, "rays": s.rays, "node_visits": s.node_visits,
-                "tri_tests": s.tri_tests}
+                "tri_tests": s.tri_tests, "queries": s.queries}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -461,14 +461,20 @@ enum {
   HPT_STAGE_SHADE = 2,   /* surface/medium interaction, NEE, BSDF sampling */
   HPT_STAGE_SHADOW = 3,  /* any-hit traversal of the shadow queue */
   HPT_STAGE_ACCUMULATE = 4,
-  HPT_STAGE_COUNT = 5
+  HPT_STAGE_SORT = 5,    /* material sort of the shading queue (hpt_set_shade_sort) */
+  HPT_STAGE_COUNT = 6
 };
 typedef struct hpt_stage_stats {
   uint64_t launches;
   double total_ms;
-  uint64_t rays;        /* rays the stage consumed (device-side counts) */
-  uint64_t node_visits; /* with HPT_PROFILE_COUNT_VISITS */
-  uint64_t tri_tests;   /* with HPT_PROFILE_COUNT_VISITS */
+  uint64_t rays;        /* rays the stage consumed (device-side counts); for
+                           HPT_STAGE_SHADE the shading events */
+  uint64_t node_visits; /* with HPT_PROFILE_COUNT_VISITS; for HPT_STAGE_SHADE
+                           those of the volume-MIS transmittance walks */
+  uint64_t tri_tests;   /* with HPT_PROFILE_COUNT_VISITS (likewise) */
+  uint64_t queries;     /* with HPT_PROFILE_COUNT_VISITS: closest-hit queries
+                           of HPT_STAGE_SHADE's transmittance walks
+                           (inc/lighting.glsl:19-32); 0 for other stages */
 } hpt_stage_stats;
 
 HPT_API hpt_status hpt_set_profiling(hpt_renderer* r, int enable);
