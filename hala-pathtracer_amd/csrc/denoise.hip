@@ -19,6 +19,8 @@
 // sum, and a second pass adds the chunks in order (no atomics: the result is deterministic).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -950,18 +952,45 @@ int hpt_nn_write_output(void* stream, const float* out, float* dst, int dst_w, i
   return last();
 }
 
+// Zeroing runs on a non-blocking stream of the library per device, not on
+// the legacy null stream: work there would depend on every blocking stream,
+// and HIP refuses it while another thread captures a graph.
+static hipError_t zeroing_stream(hipStream_t* out) {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!streams[dev]) {
+    e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+  }
+  *out = streams[dev];
+  return hipSuccess;
+}
+
 int hpt_nn_alloc(size_t bytes, void** ptr) {
   if (!ptr) return bad();
   *ptr = nullptr;
   if (!bytes) return 0;
   hipError_t e = hipMalloc(ptr, bytes);
   if (e != hipSuccess) return (int)e;
-  e = hipMemset(*ptr, 0, bytes);
+  hipStream_t s = nullptr;
+  e = zeroing_stream(&s);
+  if (e == hipSuccess) e = hipMemsetAsync(*ptr, 0, bytes, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(*ptr);
     *ptr = nullptr;
   }
   return (int)e;
+}
+
+int hpt_nn_stream_create(void** stream) {
+  if (!stream) return bad();
+  return (int)hipStreamCreateWithFlags(reinterpret_cast<hipStream_t*>(stream), hipStreamNonBlocking);
 }
 
 int hpt_nn_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }

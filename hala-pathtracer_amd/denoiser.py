@@ -18,7 +18,10 @@ and activations on the MFMA; float32 sums.  Images are NHWC on the device.
 
 Memory.  Every buffer is device memory of libhpt.so's own HIP runtime
 (DeviceArray over hpt_nn_alloc), and every operation is a call into the
-library on the null stream; there is no CPU path.
+library on one non-blocking stream of the process (stream()); there is no
+CPU path.  Nothing runs on the legacy null stream, whose work would depend
+on every blocking stream: a renderer capturing a HIP graph on another thread
+(the server's jobs) must not see the denoiser's work join its capture.
 """
 from __future__ import annotations
 
@@ -34,7 +37,7 @@ LEVELS = (32, 48, 64, 96, 128)
 IN_CHANNELS = 16
 OUT_CHANNELS = 3
 OUT_PAD = 16
-STREAM = None  # the null stream (renderer streams are non-blocking: wait for a renderer before reading its images)
+_stream = None  # the denoiser's stream (non-blocking, like the renderers': wait for a renderer before reading its images)
 
 _vp, _i, _sz, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 # Entry points of include/hpt_nn.h with their argument types.
@@ -64,6 +67,7 @@ NN_SIGNATURES = {
     "hpt_nn_copy": [_vp, _vp, _vp, _sz],
     "hpt_nn_fill": [_vp, _vp, _f, _sz],
     "hpt_nn_stream_synchronize": [_vp],
+    "hpt_nn_stream_create": [C.POINTER(_vp)],
     "hpt_nn_event_create": [C.POINTER(_vp)],
     "hpt_nn_event_destroy": [_vp],
     "hpt_nn_event_record": [_vp, _vp],
@@ -92,9 +96,19 @@ def _call(name, *args):
         raise RuntimeError(f"{name} failed with HIP error {rc}")
 
 
+def stream():
+    """The process's denoiser stream, created on first use."""
+    global _stream
+    if _stream is None:
+        p = C.c_void_p()
+        _call("hpt_nn_stream_create", C.byref(p))
+        _stream = p
+    return _stream
+
+
 def synchronize() -> None:
-    """Waits for the denoiser's work (the null stream)."""
-    _call("hpt_nn_stream_synchronize", STREAM)
+    """Waits for the denoiser's work (its stream)."""
+    _call("hpt_nn_stream_synchronize", stream())
 
 
 class DeviceArray:
@@ -127,26 +141,26 @@ class DeviceArray:
         a = np.ascontiguousarray(a, self.dtype)
         if offset + a.size > self.count:
             raise ValueError("upload past the end of the buffer")
-        _call("hpt_nn_copy", STREAM, C.c_void_p(self.addr(offset)), C.c_void_p(a.ctypes.data), a.nbytes)
+        _call("hpt_nn_copy", stream(), C.c_void_p(self.addr(offset)), C.c_void_p(a.ctypes.data), a.nbytes)
         synchronize()
 
     def copy_from_device(self, src: int, nbytes: int, offset: int = 0) -> None:
         """Device-to-device copy from address `src` (queued on the stream)."""
         if offset * self.dtype.itemsize + nbytes > self.nbytes:
             raise ValueError("copy past the end of the buffer")
-        _call("hpt_nn_copy", STREAM, C.c_void_p(self.addr(offset)), C.c_void_p(src), nbytes)
+        _call("hpt_nn_copy", stream(), C.c_void_p(self.addr(offset)), C.c_void_p(src), nbytes)
 
     def numpy(self) -> np.ndarray:
         """A host copy, after the queued work has finished."""
         out = np.empty(self.count, self.dtype)
-        _call("hpt_nn_copy", STREAM, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), self.nbytes)
+        _call("hpt_nn_copy", stream(), C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), self.nbytes)
         synchronize()
         return out.reshape(self.shape)
 
     def fill(self, value: float) -> None:
         if self.dtype != np.float32:
             raise TypeError("fill is for float32 buffers")
-        _call("hpt_nn_fill", STREAM, C.c_void_p(self.ptr), value, self.count)
+        _call("hpt_nn_fill", stream(), C.c_void_p(self.ptr), value, self.count)
 
     def addr(self, offset: int = 0) -> int:
         """Device address of element `offset`."""
@@ -184,8 +198,9 @@ class Event:
         if getattr(self, "e", None) and self.e.value and _nn is not None:
             _nn.hpt_nn_event_destroy(self.e)
 
-    def record(self, stream=STREAM) -> None:
-        _call("hpt_nn_event_record", self.e, stream)
+    def record(self, on=None) -> None:
+        """Records the event on stream `on` (default: the denoiser's)."""
+        _call("hpt_nn_event_record", self.e, on if on is not None else stream())
 
     def elapsed_ms(self, stop: "Event") -> float:
         ms = C.c_float()
@@ -282,7 +297,7 @@ class UNet:
     def prepare_weights(self) -> None:
         """Master float32 weights -> the bf16 copies the convolutions read."""
         for L in self.layers.values():
-            _call("hpt_nn_prepare_weights", STREAM, _p(self.params.addr(L.w_off)), _p(L.w), _p(L.wt), L.cin, L.cout,
+            _call("hpt_nn_prepare_weights", stream(), _p(self.params.addr(L.w_off)), _p(L.w), _p(L.wt), L.cin, L.cout,
                   L.cout_pad, L.k)
 
     def save(self, path: str) -> None:
@@ -346,12 +361,12 @@ class UNet:
 
     # -------------------------------------------------------- passes
     def _conv(self, L, x, y, y_f32, n, h, w):
-        _call("hpt_nn_conv_forward", STREAM, _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_f32), n, h, w,
+        _call("hpt_nn_conv_forward", stream(), _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_f32), n, h, w,
               L.cin, L.cout, L.cout_pad, L.k, int(L.relu))
 
     def _conv_upcat(self, L, low, skip, y, n, h, w, c_low, c_skip):
         """L over concat(upsample2x(low), skip) without materialising it."""
-        _call("hpt_nn_conv_forward_upcat", STREAM, _p(low), _p(skip), _p(L.w), _p(self.params.addr(L.b_off)), _p(y),
+        _call("hpt_nn_conv_forward_upcat", stream(), _p(low), _p(skip), _p(L.w), _p(self.params.addr(L.b_off)), _p(y),
               None, n, h, w, c_low, c_skip, L.cout, L.cout_pad, int(L.relu))
 
     def forward_input(self, x: DeviceArray, n: int, h: int, w: int, train: bool = False) -> DeviceArray:
@@ -361,7 +376,7 @@ class UNet:
         b = self._buffers(n, h, w, train)
         if x.ptr != b["x"].ptr:
             b["x"].copy_from_device(x.ptr, b["x"].nbytes)
-        st = STREAM
+        st = stream()
         Lv = len(self.levels)
         cur = b["x"]
         for l in range(Lv):
@@ -394,7 +409,7 @@ class UNet:
         """dy: gradient w.r.t. the layer's output (modified in place by the
         ReLU's derivative); weight and bias gradients into self.grads; the
         input gradient into dx when dx is given."""
-        st = STREAM
+        st = stream()
         if L.relu:
             _call("hpt_nn_relu_backward", st, _p(dy), _p(y), n * h * w * L.cout)
         _call("hpt_nn_conv_backward_weights", st, _p(dy), _p(x), _p(self.grads.addr(L.w_off)),
@@ -406,7 +421,7 @@ class UNet:
         """Gradients of the loss whose output gradient is in the "dout"
         buffer (float32, 16 channels per pixel) into self.grads."""
         b = self._buffers(n, h, w, True)
-        st = STREAM
+        st = stream()
         Lv = len(self.levels)
         lay = self.layers
         self._conv_back(lay["out"], b["dout"], b["y_dec0b"] if Lv > 1 else b["y_enc0b"], None, b["dx_out"], n, h, w)
@@ -437,11 +452,11 @@ class UNet:
         """L1 loss of `out` against `target` (log colours): its mean is
         added to loss_sum[0] and its gradient written to the "dout" buffer."""
         b = self._buffers(n, h, w, True)
-        _call("hpt_nn_l1_loss", STREAM, _p(out), _p(target), _p(b["dout"]), _p(loss_sum), n * h * w, OUT_PAD)
+        _call("hpt_nn_l1_loss", stream(), _p(out), _p(target), _p(b["dout"]), _p(loss_sum), n * h * w, OUT_PAD)
 
     def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         self.step_count += 1
-        _call("hpt_nn_adam", STREAM, _p(self.params), _p(self.grads), _p(self.adam_m), _p(self.adam_v),
+        _call("hpt_nn_adam", stream(), _p(self.params), _p(self.grads), _p(self.adam_m), _p(self.adam_v),
               self.param_count, lr, beta1, beta2, eps, self.step_count)
         self.prepare_weights()
 
@@ -471,14 +486,14 @@ class UNet:
         device images of src_w x src_h averaged over `spp` samples, given as
         DeviceArrays or addresses."""
         color, albedo, normal = images
-        _call("hpt_nn_make_input", STREAM, _p(color), _p(albedo), _p(normal), src_w, src_h, x0, y0, _p(out), h, w,
+        _call("hpt_nn_make_input", stream(), _p(color), _p(albedo), _p(normal), src_w, src_h, x0, y0, _p(out), h, w,
               mirror, spp_feature(spp))
         return out
 
     def make_target(self, color, src_w: int, src_h: int, x0: int, y0: int, h: int, w: int, out, mirror: int = 0):
         """log(1 + colour) of the same window of a reference image into `out`
         (h*w*3 float32)."""
-        _call("hpt_nn_make_target", STREAM, _p(color), src_w, src_h, x0, y0, _p(out), h, w, mirror)
+        _call("hpt_nn_make_target", stream(), _p(color), src_w, src_h, x0, y0, _p(out), h, w, mirror)
         return out
 
     def denoise(self, images, width: int, height: int, spp: int, out: DeviceArray | None = None) -> DeviceArray:
@@ -493,7 +508,7 @@ class UNet:
         res = self.forward_input(b["x"], 1, h, w)
         if out is None:
             out = DeviceArray(height * width * 4, np.float32, (height, width, 4))
-        _call("hpt_nn_write_output", STREAM, _p(res), _p(out), width, height, 0, 0, h, w)
+        _call("hpt_nn_write_output", stream(), _p(res), _p(out), width, height, 0, 0, h, w)
         return out
 
     def denoise_renderer(self, r: "hpt.HalaRenderer", out: DeviceArray | None = None) -> DeviceArray:

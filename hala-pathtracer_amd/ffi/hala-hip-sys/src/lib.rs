@@ -255,6 +255,8 @@ extern "C" {
     pub fn hpt_nn_copy(stream: *mut c_void, dst: *mut c_void, src: *const c_void, bytes: usize) -> c_int;
     pub fn hpt_nn_fill(stream: *mut c_void, p: *mut f32, value: f32, count: usize) -> c_int;
     pub fn hpt_nn_stream_synchronize(stream: *mut c_void) -> c_int;
+    /// A non-blocking stream for the denoiser's work (never the null stream).
+    pub fn hpt_nn_stream_create(stream: *mut *mut c_void) -> c_int;
     pub fn hpt_nn_event_create(event: *mut *mut c_void) -> c_int;
     pub fn hpt_nn_event_destroy(event: *mut c_void) -> c_int;
     pub fn hpt_nn_event_record(event: *mut c_void, stream: *mut c_void) -> c_int;

@@ -134,6 +134,11 @@ HPT_NN_API int hpt_nn_copy(void* stream, void* dst, const void* src, size_t byte
 /* p[i] = value for count floats. */
 HPT_NN_API int hpt_nn_fill(void* stream, float* p, float value, size_t count);
 HPT_NN_API int hpt_nn_stream_synchronize(void* stream);
+/* A non-blocking stream for the denoiser's work (never the legacy null
+ * stream, whose work would wait for every blocking stream and which HIP
+ * refuses while another thread captures a graph).  Lives until the process
+ * ends. */
+HPT_NN_API int hpt_nn_stream_create(void** stream);
 HPT_NN_API int hpt_nn_event_create(void** event);
 HPT_NN_API int hpt_nn_event_destroy(void* event);
 HPT_NN_API int hpt_nn_event_record(void* event, void* stream);

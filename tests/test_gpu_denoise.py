@@ -74,7 +74,7 @@ def _weights(dn, rng, cin, cout, k):
     master = rng.normal(0, (2.0 / (k * k * cin)) ** 0.5, (cout, k * k, cin)).astype(np.float32)
     w = _zeros(cp * k * k * cin, np.uint16)
     wt = _zeros(cin * k * k * cp, np.uint16)
-    _call(dn, "hpt_nn_prepare_weights", None, _p(_dev(master)), _p(w), _p(wt), cin, cout, cp, k)
+    _call(dn, "hpt_nn_prepare_weights", dn.stream(), _p(_dev(master)), _p(w), _p(wt), cin, cout, cp, k)
     dn.synchronize()
     return master, w, wt, cp
 
@@ -104,7 +104,7 @@ def test_conv_forward_matches_oracle(dn, shape):
     y = _zeros(n * h * wd * cout, np.uint16)
     yf = _zeros(n * h * wd * cout)
     relu = int(cout > 3)
-    _call(dn, "hpt_nn_conv_forward", None, _p(_dev_bf16(x)), _p(w), _p(_dev(b)), _p(y), _p(yf), n, h, wd, cin, cout,
+    _call(dn, "hpt_nn_conv_forward", dn.stream(), _p(_dev_bf16(x)), _p(w), _p(_dev(b)), _p(y), _p(yf), n, h, wd, cin, cout,
           cp, k, relu)
     dn.synchronize()
     wb = U.bf16(master).astype(np.float64)
@@ -146,14 +146,14 @@ def test_conv_forward_upcat_equals_upsample_concat_conv(dn, shape):
     up = _zeros(n * h * wd * cl, np.uint16)
     cat = _zeros(n * h * wd * (cl + cs), np.uint16)
     y_ref, y = _zeros(n * h * wd * cout, np.uint16), _zeros(n * h * wd * cout, np.uint16)
-    _call(dn, "hpt_nn_upsample_forward", None, _p(low), _p(up), n, h // 2, wd // 2, cl)
-    _call(dn, "hpt_nn_concat_forward", None, _p(up), _p(skip), _p(cat), n * h * wd, cl, cs)
-    _call(dn, "hpt_nn_conv_forward", None, _p(cat), _p(w), _p(b), _p(y_ref), None, n, h, wd, cl + cs, cout, cp, 3, 1)
-    _call(dn, "hpt_nn_conv_forward_upcat", None, _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h, wd, cl, cs, cout,
+    _call(dn, "hpt_nn_upsample_forward", dn.stream(), _p(low), _p(up), n, h // 2, wd // 2, cl)
+    _call(dn, "hpt_nn_concat_forward", dn.stream(), _p(up), _p(skip), _p(cat), n * h * wd, cl, cs)
+    _call(dn, "hpt_nn_conv_forward", dn.stream(), _p(cat), _p(w), _p(b), _p(y_ref), None, n, h, wd, cl + cs, cout, cp, 3, 1)
+    _call(dn, "hpt_nn_conv_forward_upcat", dn.stream(), _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h, wd, cl, cs, cout,
           cp, 1)
     dn.synchronize()
     assert np.array_equal(y.numpy(), y_ref.numpy())
-    bad = dn.nn_library().hpt_nn_conv_forward_upcat(None, _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h + 1,
+    bad = dn.nn_library().hpt_nn_conv_forward_upcat(dn.stream(), _p(low), _p(skip), _p(w), _p(b), _p(y), None, n, h + 1,
                                                       wd, cl, cs, cout, cp, 1)
     assert bad != 0  # odd height: no half-resolution tensor to upsample
 
@@ -167,7 +167,7 @@ def test_conv_backward_data_matches_oracle(dn, shape):
     dy = np.zeros((n, h, wd, cp), np.float32)
     dy[..., :cout] = rng.normal(0, 1, (n, h, wd, cout))
     dx = _zeros(n * h * wd * cin)
-    _call(dn, "hpt_nn_conv_backward_data", None, _p(_dev(dy)), _p(wt), _p(dx), n, h, wd, cin, cout, cp, k)
+    _call(dn, "hpt_nn_conv_backward_data", dn.stream(), _p(_dev(dy)), _p(wt), _p(dx), n, h, wd, cin, cout, cp, k)
     dn.synchronize()
     dyb = U.bf16(dy[..., :cout]).astype(np.float64)
     wb = U.bf16(master).astype(np.float64)
@@ -190,7 +190,7 @@ def test_conv_backward_weights_matches_oracle(dn, shape):
     scratch = _zeros(lib.hpt_nn_wgrad_scratch_floats(n, h, wd, cin, cp, k))
     dw = _zeros(cout * k * k * cin)
     db = _zeros(cout)
-    _call(dn, "hpt_nn_conv_backward_weights", None, _p(_dev(dy)), _p(_dev_bf16(x)), _p(dw), _p(db), _p(scratch), n, h,
+    _call(dn, "hpt_nn_conv_backward_weights", dn.stream(), _p(_dev(dy)), _p(_dev_bf16(x)), _p(dw), _p(db), _p(scratch), n, h,
           wd, cin, cout, cp, k)
     dn.synchronize()
     dyb = U.bf16(dy[..., :cout]).astype(np.float64)
@@ -264,7 +264,7 @@ def test_make_input_loss_adam_and_output(dn):
     normal = rng.uniform(-1, 1, (H, W, 4)).astype(np.float32)
     h, w, x0, y0 = 8, 16, 2, 3  # the window sticks out of the image on the right and bottom
     x = _zeros(h * w * 16, np.uint16)
-    _call(dn, "hpt_nn_make_input", None, _p(_dev(color)), _p(_dev(albedo)), _p(_dev(normal)), W, H, x0, y0, _p(x), h,
+    _call(dn, "hpt_nn_make_input", dn.stream(), _p(_dev(color)), _p(_dev(albedo)), _p(_dev(normal)), W, H, x0, y0, _p(x), h,
           w, 0, 0.625)
     got = _host_bf16(x, (h, w, 16))
     full = U.make_input(color, albedo, normal, 0.625)
@@ -276,13 +276,13 @@ def test_make_input_loss_adam_and_output(dn):
     # mirrored windows read the same pixels in reverse order; the target is log(1 + colour)
     for m in (1, 2, 3):
         xm = _zeros(h * w * 16, np.uint16)
-        _call(dn, "hpt_nn_make_input", None, _p(_dev(color)), _p(_dev(albedo)), _p(_dev(normal)), W, H, x0, y0,
+        _call(dn, "hpt_nn_make_input", dn.stream(), _p(_dev(color)), _p(_dev(albedo)), _p(_dev(normal)), W, H, x0, y0,
               _p(xm), h, w, m, 0.625)
         gm = _host_bf16(xm, (h, w, 16))
         exp = got[:, ::-1] if m == 1 else (got[::-1] if m == 2 else got[::-1, ::-1])
         assert np.array_equal(gm, exp)
     tg = _zeros(h * w * 3)
-    _call(dn, "hpt_nn_make_target", None, _p(_dev(color)), W, H, x0, y0, _p(tg), h, w, 0)
+    _call(dn, "hpt_nn_make_target", dn.stream(), _p(_dev(color)), W, H, x0, y0, _p(tg), h, w, 0)
     ref_t = np.zeros((h, w, 3), np.float32)
     ref_t[:ys.stop - y0, :xs.stop - x0] = np.log1p(np.maximum(color[ys, xs, :3], 0))
     assert np.allclose(tg.numpy().reshape(h, w, 3), ref_t, rtol=5e-7, atol=1e-7)
@@ -293,7 +293,7 @@ def test_make_input_loss_adam_and_output(dn):
     dy = _zeros(P * 16)
     dy.fill(7.0)
     loss = _zeros(1)
-    _call(dn, "hpt_nn_l1_loss", None, _p(_dev(out)), _p(_dev(tgt)), _p(dy), _p(loss), P, 16)
+    _call(dn, "hpt_nn_l1_loss", dn.stream(), _p(_dev(out)), _p(_dev(tgt)), _p(dy), _p(loss), P, 16)
     ref_loss, ref_g = U.l1_loss(out.astype(np.float64), tgt.astype(np.float64))
     gd = dy.numpy().reshape(P, 16)
     assert np.array_equal(gd[:, 3:], np.zeros((P, 13), np.float32))
@@ -306,14 +306,14 @@ def test_make_input_loss_adam_and_output(dn):
     rp, rm, rv = p.astype(np.float64), np.zeros(500), np.zeros(500)
     for t in (1, 2, 3):
         gt = g * t
-        _call(dn, "hpt_nn_adam", None, _p(pd), _p(_dev(gt)), _p(md), _p(vd), 500, 1e-2, 0.9, 0.999, 1e-8, t)
+        _call(dn, "hpt_nn_adam", dn.stream(), _p(pd), _p(_dev(gt)), _p(md), _p(vd), 500, 1e-2, 0.9, 0.999, 1e-8, t)
         rp, rm, rv = U.adam(rp, gt.astype(np.float64), rm, rv, 1e-2, 0.9, 0.999, 1e-8, t)
     assert np.allclose(pd.numpy(), rp, rtol=1e-5, atol=1e-6)
 
     o = rng.normal(0, 1, (h * w, 3)).astype(np.float32)
     dst = _zeros(H * W * 4)
     dst.fill(-3.0)
-    _call(dn, "hpt_nn_write_output", None, _p(_dev(o)), _p(dst), W, H, x0, y0, h, w)
+    _call(dn, "hpt_nn_write_output", dn.stream(), _p(_dev(o)), _p(dst), W, H, x0, y0, h, w)
     img = dst.numpy().reshape(H, W, 4)
     o3 = o.reshape(h, w, 3)
     win = img[ys, xs]
@@ -338,7 +338,7 @@ def test_unet_forward_backward_match_oracle(dn):
     b = net._buffers(n, h, w, True)
     b["target"].upload(tgt.reshape(-1))
     loss = _zeros(1)
-    _call(dn, "hpt_nn_l1_loss", None, _p(out), _p(b["target"]), _p(b["dout"]), _p(loss), n * h * w, 16)
+    _call(dn, "hpt_nn_l1_loss", dn.stream(), _p(out), _p(b["target"]), _p(b["dout"]), _p(loss), n * h * w, 16)
     net.backward(n, h, w)
     got_out = out.numpy().reshape(n, h, w, 3)
     grads = net.get_grads()
