@@ -14,6 +14,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1372,6 +1373,59 @@ hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t n, uint3
       std::memcpy(&v, &out[i].w, 4);
       occluded[i] = v;
     }
+  });
+}
+
+// ------------------------------------------------ allocation-free ray queries
+// hpt_trace_closest_device keeps the traversal stack entries that do not fit
+// a lane's LDS short stack (trees deeper than kShortStack levels) in this
+// array instead of a buffer of the handle.  Each GPU holds its own copy until
+// the process ends; host code uses the address hipGetSymbolAddress reports
+// for the current device, never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchEntries = kScratchBytes / sizeof(uint2);
+static __device__ __align__(256) uint2 scratch[kScratchEntries];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+uint2* g_scratch[kScratchDevices] = {};
+
+// The scratch array's address on the current device `device`, looked up
+// once per device.
+uint2* scratch_on(int device) {
+  need(device >= 0 && device < kScratchDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = static_cast<uint2*>(ptr);
+  }
+  return g_scratch[device];
+}
+
+}  // namespace
+
+hpt_status hpt_trace_closest_device(hpt_renderer* r, const hpt_ray* rays, uint32_t n, hpt_hit* hits) {
+  return guarded([&] {
+    need(r && (n == 0 || (rays && hits)), "null argument");
+    set_device(r);
+    require_committed(r);
+    if (n == 0) return;
+    Spill sp{nullptr, 0};
+    if (r->sd->bvh_depth > (uint32_t)kShortStack) {
+      const size_t threads = round_threads(n);
+      const size_t entries = (size_t)(r->sd->bvh_depth - kShortStack) * threads;
+      need(entries <= kScratchEntries,
+           "hpt_trace_closest_device: " + std::to_string(n) + " rays over a " + std::to_string(r->sd->bvh_depth) +
+               "-level tree need " + std::to_string(entries) + " stack entries beyond the LDS stack, above the " +
+               std::to_string(kScratchEntries) + " the device scratch array holds; split the call");
+      sp.buf = scratch_on(r->info.device);
+      sp.stride = (uint32_t)threads;
+    }
+    launch_trace_hits(r->stream, scene_view(r), rays, n, hits, sp);
+    check(hipGetLastError(), "trace launch");
   });
 }
 

@@ -167,6 +167,9 @@ void launch_accumulate(hipStream_t s, const FrameParams& fp, const PathState& ps
 void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, uint32_t w, uint32_t h,
                        EnvTexel* out);
 
+// Closest hits of device-resident rays as hpt_hit records.
+void launch_trace_hits(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, hpt_hit* hits, Spill sp);
+
 // Ray-query entry points: out = (t, u, v, id bits) per ray; counters optional.
 void launch_trace_rays(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, int any_hit,
                        float4* out, uint32_t* node_visits, uint32_t* tri_tests, Spill sp);

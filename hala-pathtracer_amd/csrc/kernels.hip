@@ -975,6 +975,29 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(SceneView sv, const hpt_r
   if (ntri) ntri[k] = b;
 }
 
+// Closest hits written as hpt_hit records (the conversion hpt_trace_closest
+// does on the host): triangles as (primitive, triangle within it), analytic
+// lights as their index, misses as HPT_HIT_NONE.
+__global__ void __launch_bounds__(kBlock) k_trace_hits(SceneView sv, const hpt_ray* rays, uint32_t n, hpt_hit* hits,
+                                                       Spill sp) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const hpt_ray r = rays[k];
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + k : nullptr, (int)sp.stride, 0};
+  const TraceResult t = trace_closest(sv, ld3(r.origin), ld3(r.direction), r.tmin, r.tmax, st);
+  hpt_hit h;
+  if (t.id == MISS_ID) {
+    h = hpt_hit{0.0f, 0.0f, 0.0f, HPT_HIT_NONE, HPT_INVALID_INDEX, HPT_INVALID_INDEX};
+  } else if (t.id & LIGHT_BIT) {
+    h = hpt_hit{t.t, 0.0f, 0.0f, HPT_HIT_LIGHT, t.id & ~LIGHT_BIT, HPT_INVALID_INDEX};
+  } else {
+    const uint32_t prim = __float_as_uint(reinterpret_cast<const float4*>(sv.tri_shade + t.id)[4].z);
+    h = hpt_hit{t.t, t.u, t.v, HPT_HIT_TRIANGLE, prim, t.id - sv.prims[prim].local_base};
+  }
+  hits[k] = h;
+}
+
 // ------------------------------------------------------------------ launchers
 static inline uint32_t grid_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
@@ -1091,6 +1114,10 @@ __global__ void k_frame_counter(uint32_t* counter, uint32_t value, int add) {
 
 void launch_frame_counter(hipStream_t s, uint32_t* counter, uint32_t value, bool add) {
   hipLaunchKernelGGL(k_frame_counter, dim3(1), dim3(64), 0, s, counter, value, add ? 1 : 0);
+}
+
+void launch_trace_hits(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, hpt_hit* hits, Spill sp) {
+  hipLaunchKernelGGL(k_trace_hits, dim3(grid_for(n)), dim3(kBlock), 0, s, sv, rays, n, hits, sp);
 }
 
 void launch_trace_rays(hipStream_t s, const SceneView& sv, const hpt_ray* rays, uint32_t n, int any_hit, float4* out,

@@ -165,6 +165,7 @@ SIGNATURES = {
     "hpt_read_image": [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t],
     "hpt_trace_closest": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
     "hpt_trace_any": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "hpt_trace_closest_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
     "hpt_wait_idle": [C.c_void_p],
     "hpt_destroy": [C.c_void_p],
     "hpt_last_error": [],
@@ -447,6 +448,12 @@ class HalaRenderer:
         hits = np.zeros(len(rays), HIT_DTYPE)
         _check(self._lib.hpt_trace_closest(self._h, _ptr(rays), len(rays), _ptr(hits)))
         return hits
+
+    def trace_closest_device(self, rays_ptr: int, count: int, hits_ptr: int) -> None:
+        """hpt_trace_closest_device: `count` rays at device address rays_ptr
+        (RAY_DTYPE) into hpt_hit records at hits_ptr, queued on the handle's
+        stream with no allocation (wait_idle before reading)."""
+        _check(self._lib.hpt_trace_closest_device(self._h, C.c_void_p(rays_ptr), count, C.c_void_p(hits_ptr)))
 
     def trace_any(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

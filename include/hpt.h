@@ -411,6 +411,21 @@ HPT_API hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint3
 HPT_API hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t count,
                                  uint32_t* occluded);
 
+/* Closest-hit queries that allocate nothing (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * `rays` and `hits` are device addresses on the handle's device, `count`
+ * entries each; the hits equal hpt_trace_closest's.  Queued on the handle's
+ * stream: the call returns before the work ends (hpt_wait_idle).  For trees
+ * deeper than the traversal's 8-entry LDS stack, each ray's further stack
+ * entries live in a 64 MiB device array the library holds per GPU (found at
+ * the first call on that GPU and kept until the process ends):
+ * count * (tree levels - 8) must not exceed 8,388,608 entries, else
+ * HPT_ERROR_INVALID_ARGUMENT.  All calls on one GPU share that array: calls
+ * that overlap in time on one GPU (several handles or streams) race, so the
+ * caller serialises them. */
+HPT_API hpt_status hpt_trace_closest_device(hpt_renderer* r, const hpt_ray* rays, uint32_t count, hpt_hit* hits);
+
 /* HalaRenderer::wait_idle (hala-pathtracer.rs:228). */
 HPT_API hpt_status hpt_wait_idle(hpt_renderer* r);
 
