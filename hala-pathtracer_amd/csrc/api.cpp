@@ -1389,12 +1389,12 @@ static __device__ __align__(256) uint2 scratch[kScratchEntries];
 namespace {
 
 constexpr int kScratchDevices = 64;
-std::mutex g_scratch_mutex;
-uint2* g_scratch[kScratchDevices] = {};
+static std::mutex g_scratch_mutex;
+static uint2* g_scratch[kScratchDevices] = {};
 
 // The scratch array's address on the current device `device`, looked up
 // once per device.
-uint2* scratch_on(int device) {
+static uint2* scratch_on(int device) {
   need(device >= 0 && device < kScratchDevices, "device ordinal out of range");
   std::lock_guard<std::mutex> lock(g_scratch_mutex);
   if (!g_scratch[device]) {
@@ -1449,6 +1449,159 @@ void hpt_destroy(hpt_renderer* r) {
   }
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
+}
+
+// ---------------------------------------------------------------- presentation
+// A ring of pinned host images that the renderer's final rgba8 image is
+// copied into behind its frames: the window loop takes the newest finished
+// one without waiting for the tracer (the role of the swapchain hand-off in
+// the reference's update()/render() loop, hala-pathtracer.rs:236-289).
+struct hpt_presenter {
+  hpt_renderer* r = nullptr;
+  int32_t device = 0;
+  hipStream_t copy_stream = nullptr;
+  size_t bytes = 0;
+  DevBuf staging;  // one device copy of the final image per slot
+  std::vector<uint8_t*> host;
+  std::vector<hipEvent_t> staged, ready;
+  std::vector<uint64_t> frame;
+  // Slot states: free, copy in flight, ready, held by the caller.
+  enum { kFree = 0, kInFlight = 1, kReady = 2, kHeld = 3 };
+  std::vector<int> state;
+  uint32_t next = 0;
+  uint64_t presented = 0, dropped = 0;
+};
+
+namespace {
+
+static void present_poll(hpt_presenter* p) {
+  for (size_t i = 0; i < p->state.size(); ++i) {
+    if (p->state[i] != hpt_presenter::kInFlight) continue;
+    const hipError_t e = hipEventQuery(p->ready[i]);
+    if (e == hipSuccess) p->state[i] = hpt_presenter::kReady;
+    else if (e != hipErrorNotReady) check(e, "presentation copy");
+  }
+}
+
+static void present_free(hpt_presenter* p) {
+  (void)hipSetDevice(p->device);
+  if (p->copy_stream) (void)hipStreamSynchronize(p->copy_stream);
+  for (uint8_t* h : p->host)
+    if (h) (void)hipHostFree(h);
+  for (hipEvent_t e : p->staged)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->ready)
+    if (e) (void)hipEventDestroy(e);
+  if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
+  delete p;
+}
+
+}  // namespace
+
+hpt_status hpt_present_create(hpt_renderer* r, uint32_t slots, hpt_presenter** out) {
+  hpt_presenter* p = nullptr;
+  const hpt_status st = guarded([&] {
+    need(r && out, "null argument");
+    *out = nullptr;
+    need(slots >= 2 && slots <= 16, "slots must be 2..16");
+    require_committed(r);
+    set_device(r);
+    p = new hpt_presenter;
+    p->r = r;
+    p->device = r->info.device;
+    p->bytes = r->d_final.bytes;
+    check(hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    p->staging.alloc(p->bytes * slots);
+    p->host.assign(slots, nullptr);
+    p->staged.assign(slots, nullptr);
+    p->ready.assign(slots, nullptr);
+    p->frame.assign(slots, 0);
+    p->state.assign(slots, hpt_presenter::kFree);
+    for (uint32_t i = 0; i < slots; ++i) {
+      check(hipHostMalloc(reinterpret_cast<void**>(&p->host[i]), p->bytes, hipHostMallocDefault), "hipHostMalloc");
+      check(hipEventCreateWithFlags(&p->staged[i], hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&p->ready[i], hipEventDisableTiming), "hipEventCreate");
+    }
+    *out = p;
+  });
+  if (st != HPT_OK && p) present_free(p);
+  return st;
+}
+
+hpt_status hpt_present_frame(hpt_presenter* p) {
+  return guarded([&] {
+    need(p, "null presenter");
+    hpt_renderer* r = p->r;
+    set_device(r);
+    need(r->d_final.bytes == p->bytes, "the renderer's image size changed");
+    present_poll(p);
+    // Next slot in ring order that is not in flight or held; a ready frame
+    // that was never taken is overwritten (newer frames replace it).
+    const uint32_t n = (uint32_t)p->state.size();
+    int slot = -1;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t i = (p->next + k) % n;
+      if (p->state[i] == hpt_presenter::kFree || p->state[i] == hpt_presenter::kReady) {
+        slot = (int)i;
+        break;
+      }
+    }
+    if (slot < 0) {  // every slot busy: drop this frame rather than wait
+      p->dropped++;
+      return;
+    }
+    uint8_t* dev = p->staging.as<uint8_t>() + (size_t)slot * p->bytes;
+    // The device copy is ordered on the renderer's stream, before any later
+    // frame rewrites the final image; the host copy then runs on a stream of
+    // its own and the tracer never waits for it.
+    check(hipMemcpyAsync(dev, r->d_final.p, p->bytes, hipMemcpyDeviceToDevice, r->stream), "present stage");
+    check(hipEventRecord(p->staged[slot], r->stream), "hipEventRecord");
+    check(hipStreamWaitEvent(p->copy_stream, p->staged[slot], 0), "hipStreamWaitEvent");
+    check(hipMemcpyAsync(p->host[slot], dev, p->bytes, hipMemcpyDeviceToHost, p->copy_stream), "present copy");
+    check(hipEventRecord(p->ready[slot], p->copy_stream), "hipEventRecord");
+    p->frame[slot] = r->frame_index;
+    p->state[slot] = hpt_presenter::kInFlight;
+    p->next = (uint32_t)(slot + 1) % n;
+    p->presented++;
+  });
+}
+
+hpt_status hpt_present_acquire(hpt_presenter* p, const void** pixels, uint64_t* frame_index) {
+  return guarded([&] {
+    need(p && pixels && frame_index, "null argument");
+    *pixels = nullptr;
+    *frame_index = 0;
+    set_device(p->r);
+    present_poll(p);
+    int best = -1;
+    for (size_t i = 0; i < p->state.size(); ++i) {
+      const int s = p->state[i];
+      if ((s == hpt_presenter::kReady || s == hpt_presenter::kHeld) && (best < 0 || p->frame[i] > p->frame[best]))
+        best = (int)i;
+    }
+    if (best < 0) return;  // nothing finished yet
+    for (size_t i = 0; i < p->state.size(); ++i) {
+      const int s = p->state[i];
+      // frames older than the one handed out are stale: their slots free up
+      if ((int)i != best && (s == hpt_presenter::kHeld || (s == hpt_presenter::kReady && p->frame[i] < p->frame[best])))
+        p->state[i] = hpt_presenter::kFree;
+    }
+    p->state[best] = hpt_presenter::kHeld;
+    *pixels = p->host[best];
+    *frame_index = p->frame[best];
+  });
+}
+
+hpt_status hpt_present_get_stats(hpt_presenter* p, uint64_t* presented, uint64_t* dropped) {
+  return guarded([&] {
+    need(p && presented && dropped, "null argument");
+    *presented = p->presented;
+    *dropped = p->dropped;
+  });
+}
+
+void hpt_present_destroy(hpt_presenter* p) {
+  if (p) present_free(p);
 }
 
 const char* hpt_last_error(void) { return g_last_error.c_str(); }

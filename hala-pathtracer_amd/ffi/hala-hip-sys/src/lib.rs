@@ -203,6 +203,62 @@ extern "C" {
     pub fn hpt_wait_idle(r: *mut hpt_renderer) -> hpt_status;
     pub fn hpt_destroy(r: *mut hpt_renderer);
     pub fn hpt_last_error() -> *const c_char;
+    pub fn hpt_trace_closest_device(r: *mut hpt_renderer, rays: *const c_void, count: u32, hits: *mut c_void)
+        -> hpt_status;
+    pub fn hpt_present_create(r: *mut hpt_renderer, slots: u32, out: *mut *mut hpt_presenter) -> hpt_status;
+    pub fn hpt_present_frame(p: *mut hpt_presenter) -> hpt_status;
+    pub fn hpt_present_acquire(p: *mut hpt_presenter, pixels: *mut *const c_void, frame_index: *mut u64)
+        -> hpt_status;
+    pub fn hpt_present_get_stats(p: *mut hpt_presenter, presented: *mut u64, dropped: *mut u64) -> hpt_status;
+    pub fn hpt_present_destroy(p: *mut hpt_presenter);
+}
+
+/// Opaque presenter handle (hpt_present_*).
+#[repr(C)]
+pub struct hpt_presenter {
+    _private: [u8; 0],
+}
+
+/// The frame hand-off of the reference's window loop (update()/render(),
+/// hala-pathtracer.rs:236-289): after each `render`, `present` queues a copy
+/// of the final rgba8 image into a ring of pinned host buffers; `newest`
+/// returns the latest finished one without waiting for the tracer, for the
+/// window's texture upload.  Drop it before its renderer.
+pub struct Presenter {
+    raw: *mut hpt_presenter,
+    width: u32,
+    height: u32,
+}
+
+impl Presenter {
+    pub fn new(renderer: &mut HalaRenderer, slots: u32) -> anyhow::Result<Self> {
+        let mut raw = std::ptr::null_mut();
+        check(unsafe { hpt_present_create(renderer.raw, slots, &mut raw) })?;
+        Ok(Presenter { raw, width: renderer.width, height: renderer.height })
+    }
+
+    pub fn present(&mut self) -> anyhow::Result<()> {
+        check(unsafe { hpt_present_frame(self.raw) })
+    }
+
+    /// (frame count, rgba8 pixels) of the newest finished copy, valid until
+    /// the next call; None before the first copy has finished.
+    pub fn newest(&mut self) -> anyhow::Result<Option<(u64, &[u8])>> {
+        let mut px: *const c_void = std::ptr::null();
+        let mut frame = 0u64;
+        check(unsafe { hpt_present_acquire(self.raw, &mut px, &mut frame) })?;
+        if px.is_null() {
+            return Ok(None);
+        }
+        let n = (self.width * self.height * 4) as usize;
+        Ok(Some((frame, unsafe { std::slice::from_raw_parts(px as *const u8, n) })))
+    }
+}
+
+impl Drop for Presenter {
+    fn drop(&mut self) {
+        unsafe { hpt_present_destroy(self.raw) }
+    }
 }
 
 // The learned denoiser's kernels (include/hpt_nn.h), the replacement of the

@@ -194,6 +194,11 @@ SIGNATURES = {
     "hpt_set_bvh_builder": [C.c_void_p, C.c_uint32],
     "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "hpt_present_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
+    "hpt_present_frame": [C.c_void_p],
+    "hpt_present_acquire": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)],
+    "hpt_present_get_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "hpt_present_destroy": [C.c_void_p],
 }
 
 _lib = None
@@ -210,7 +215,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        if name in ("hpt_destroy", "hpt_group_destroy"):
+        if name in ("hpt_destroy", "hpt_group_destroy", "hpt_present_destroy"):
             fn.restype = None
         elif name == "hpt_last_error":
             fn.restype = C.c_char_p
@@ -599,6 +604,50 @@ class HalaGroup:
             self.close()
         except Exception:
             pass
+
+class Presenter:
+    """hpt_present_*: a ring of pinned host copies of a renderer's final
+    rgba8 image for a window loop (the reference presents through its
+    swapchain, hala-pathtracer.rs:236-289).  present() after render() queues
+    a copy without waiting; acquire() returns the newest finished copy and
+    its frame count, or None."""
+
+    def __init__(self, renderer: "HalaRenderer", slots: int = 3):
+        self._lib = load_library()
+        self.renderer = renderer
+        h = C.c_void_p()
+        _check(self._lib.hpt_present_create(renderer._h, slots, C.byref(h)))
+        self._h = h
+
+    def present(self) -> None:
+        _check(self._lib.hpt_present_frame(self._h))
+
+    def acquire(self):
+        """(frame count, (H, W, 4) uint8 copy of the slot) or None."""
+        p, f = C.c_void_p(), C.c_uint64()
+        _check(self._lib.hpt_present_acquire(self._h, C.byref(p), C.byref(f)))
+        if not p.value:
+            return None
+        r = self.renderer
+        buf = (C.c_uint8 * (r.width * r.height * 4)).from_address(p.value)
+        return f.value, np.frombuffer(buf, np.uint8).reshape(r.height, r.width, 4).copy()
+
+    def stats(self) -> tuple:
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(self._lib.hpt_present_get_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.hpt_present_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 def decode_exr(data: bytes) -> np.ndarray:
     """An OpenEXR file's bytes as (H, W, 3) float32 RGB, row 0 at the top

@@ -426,6 +426,27 @@ HPT_API hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t 
  * caller serialises them. */
 HPT_API hpt_status hpt_trace_closest_device(hpt_renderer* r, const hpt_ray* rays, uint32_t count, hpt_hit* hits);
 
+/* Presentation: the hand-off of finished frames to a window loop (the
+ * reference's update()/render() pair presents the final image through its
+ * swapchain, hala-pathtracer.rs:236-289).  A presenter keeps `slots` (2..16)
+ * pinned host copies of the handle's final rgba8 image (width*height*4
+ * bytes each).  hpt_present_frame, called after hpt_render, queues a copy of
+ * the image as it stands after the frames queued so far: a device copy on
+ * the handle's stream, then the host copy on a stream of the presenter's
+ * own, so the tracer never waits for it; when every slot is in flight or
+ * held the frame is dropped (counted) instead.  hpt_present_acquire returns,
+ * without waiting, the newest finished copy and the frame count it was
+ * taken at (pixels = NULL when none has finished); that slot stays valid and
+ * unchanged until the next hpt_present_acquire.  Destroy a presenter before
+ * its renderer. */
+typedef struct hpt_presenter hpt_presenter;
+HPT_API hpt_status hpt_present_create(hpt_renderer* r, uint32_t slots, hpt_presenter** out);
+HPT_API hpt_status hpt_present_frame(hpt_presenter* p);
+HPT_API hpt_status hpt_present_acquire(hpt_presenter* p, const void** pixels, uint64_t* frame_index);
+/* Frames queued for presentation and frames dropped because no slot was free. */
+HPT_API hpt_status hpt_present_get_stats(hpt_presenter* p, uint64_t* presented, uint64_t* dropped);
+HPT_API void hpt_present_destroy(hpt_presenter* p);
+
 /* HalaRenderer::wait_idle (hala-pathtracer.rs:228). */
 HPT_API hpt_status hpt_wait_idle(hpt_renderer* r);
 
