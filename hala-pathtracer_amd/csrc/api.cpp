@@ -1606,6 +1606,14 @@ void hpt_present_destroy(hpt_presenter* p) {
 
 const char* hpt_last_error(void) { return g_last_error.c_str(); }
 
+hpt_status hpt_get_image_size(hpt_renderer* r, uint32_t* width, uint32_t* height) {
+  return guarded([&] {
+    need(r && width && height, "null argument");
+    *width = r->info.width;
+    *height = r->info.height;
+  });
+}
+
 hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info) {
   return guarded([&] {
     need(r && info, "null argument");
@@ -1917,3 +1925,9 @@ void hpt_group_destroy(hpt_group* g) {
 }
 
 }  // extern "C"
+
+namespace hpt {
+// The calling thread's message for hpt_last_error(), for entry points
+// implemented in other files of the library (the denoiser's).
+void set_last_error(const std::string& message) { g_last_error = message; }
+}  // namespace hpt

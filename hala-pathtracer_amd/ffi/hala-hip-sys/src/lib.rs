@@ -211,6 +211,47 @@ extern "C" {
         -> hpt_status;
     pub fn hpt_present_get_stats(p: *mut hpt_presenter, presented: *mut u64, dropped: *mut u64) -> hpt_status;
     pub fn hpt_present_destroy(p: *mut hpt_presenter);
+    pub fn hpt_get_image_size(r: *mut hpt_renderer, width: *mut u32, height: *mut u32) -> hpt_status;
+    pub fn hpt_denoiser_create(npz: *const u8, size: usize, device: i32, out: *mut *mut hpt_denoiser)
+        -> hpt_status;
+    pub fn hpt_denoise(d: *mut hpt_denoiser, r: *mut hpt_renderer, dst: *mut f32, bytes: usize,
+        dst_on_device: c_int) -> hpt_status;
+    pub fn hpt_denoiser_destroy(d: *mut hpt_denoiser);
+}
+
+/// Opaque denoiser handle (hpt_denoiser_*).
+#[repr(C)]
+pub struct hpt_denoiser {
+    _private: [u8; 0],
+}
+
+/// The learned denoiser in place of denoise.py's oidnDenoise step
+/// (denoise.py:26-30), run on the renderer's resident colour, albedo and
+/// normal images.
+pub struct Denoiser {
+    raw: *mut hpt_denoiser,
+}
+
+impl Denoiser {
+    /// `npz`: the bytes of the trained network's .npz file.
+    pub fn new(npz: &[u8], device: i32) -> anyhow::Result<Self> {
+        let mut raw = std::ptr::null_mut();
+        check(unsafe { hpt_denoiser_create(npz.as_ptr(), npz.len(), device, &mut raw) })?;
+        Ok(Denoiser { raw })
+    }
+
+    /// Denoised colour, width*height RGBA f32 (alpha 1).
+    pub fn denoise(&mut self, renderer: &mut HalaRenderer) -> anyhow::Result<Vec<f32>> {
+        let mut v = vec![0f32; (renderer.width * renderer.height * 4) as usize];
+        check(unsafe { hpt_denoise(self.raw, renderer.raw, v.as_mut_ptr(), v.len() * 4, 0) })?;
+        Ok(v)
+    }
+}
+
+impl Drop for Denoiser {
+    fn drop(&mut self) {
+        unsafe { hpt_denoiser_destroy(self.raw) }
+    }
 }
 
 /// Opaque presenter handle (hpt_present_*).

@@ -199,6 +199,10 @@ SIGNATURES = {
     "hpt_present_acquire": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)],
     "hpt_present_get_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_present_destroy": [C.c_void_p],
+    "hpt_get_image_size": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "hpt_denoiser_create": [C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(C.c_void_p)],
+    "hpt_denoise": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int],
+    "hpt_denoiser_destroy": [C.c_void_p],
 }
 
 _lib = None
@@ -215,7 +219,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        if name in ("hpt_destroy", "hpt_group_destroy", "hpt_present_destroy"):
+        if name in ("hpt_destroy", "hpt_group_destroy", "hpt_present_destroy", "hpt_denoiser_destroy"):
             fn.restype = None
         elif name == "hpt_last_error":
             fn.restype = C.c_char_p
@@ -604,6 +608,42 @@ class HalaGroup:
             self.close()
         except Exception:
             pass
+
+class Denoiser:
+    """hpt_denoiser_*: the trained U-Net loaded from its .npz bytes behind the
+    C ABI, denoising a renderer's resident images (the oidnDenoise step of
+    denoise.py:26-30 that the reference's Save button feeds)."""
+
+    def __init__(self, npz: bytes | str, device: int = 0):
+        self._lib = load_library()
+        if isinstance(npz, str):
+            with open(npz, "rb") as f:
+                npz = f.read()
+        h = C.c_void_p()
+        _check(self._lib.hpt_denoiser_create(npz, len(npz), device, C.byref(h)))
+        self._h = h
+
+    def denoise(self, r: "HalaRenderer") -> np.ndarray:
+        """(H, W, 4) float32 denoised colour of the renderer's images."""
+        out = np.empty((r.height, r.width, 4), np.float32)
+        _check(self._lib.hpt_denoise(self._h, r._h, _ptr(out), out.nbytes, 0))
+        return out
+
+    def denoise_into(self, r: "HalaRenderer", device_ptr: int) -> None:
+        """The same into a device buffer of width*height*16 bytes."""
+        _check(self._lib.hpt_denoise(self._h, r._h, C.c_void_p(device_ptr), r.width * r.height * 16, 1))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.hpt_denoiser_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class Presenter:
     """hpt_present_*: a ring of pinned host copies of a renderer's final

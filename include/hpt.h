@@ -447,6 +447,26 @@ HPT_API hpt_status hpt_present_acquire(hpt_presenter* p, const void** pixels, ui
 HPT_API hpt_status hpt_present_get_stats(hpt_presenter* p, uint64_t* presented, uint64_t* dropped);
 HPT_API void hpt_present_destroy(hpt_presenter* p);
 
+/* Width and height the handle was created with. */
+HPT_API hpt_status hpt_get_image_size(hpt_renderer* r, uint32_t* width, uint32_t* height);
+
+/* The learned denoiser (replaces the OIDN step of denoise.py:26-30 that the
+ * Save button's images go through, hala-pathtracer.rs:246-252).
+ * hpt_denoiser_create loads a trained network from the bytes of the .npz
+ * file the training run writes (arrays `levels`, `params` float32,
+ * `residual`; weights/unet_denoiser.npz ships with the library) onto
+ * `device`.  hpt_denoise waits for the renderer's queued frames, runs the
+ * network on its resident colour, albedo and normal images with its frame
+ * count as the sample count, and writes the denoised colour as
+ * width*height float4 (RGB, alpha 1) into `dst`: host memory, or device
+ * memory on the denoiser's device when dst_on_device is non-zero; `bytes`
+ * must be width*height*16.  Synchronous.  One handle is externally
+ * synchronised; its activation buffers follow the last image size. */
+typedef struct hpt_denoiser hpt_denoiser;
+HPT_API hpt_status hpt_denoiser_create(const void* npz, size_t size, int32_t device, hpt_denoiser** out);
+HPT_API hpt_status hpt_denoise(hpt_denoiser* d, hpt_renderer* r, float* dst, size_t bytes, int dst_on_device);
+HPT_API void hpt_denoiser_destroy(hpt_denoiser* d);
+
 /* HalaRenderer::wait_idle (hala-pathtracer.rs:228). */
 HPT_API hpt_status hpt_wait_idle(hpt_renderer* r);
 

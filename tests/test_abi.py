@@ -126,3 +126,41 @@ def test_group_and_exr_entry_points_reject_bad_arguments_without_a_device():
     w = C.c_uint32()
     assert lib.hpt_exr_info(b"not an exr file", 15, C.byref(w), C.byref(w)) == 1
     assert b"magic" in lib.hpt_last_error()
+
+
+def _npz_bytes(tmp_path, compressed=False, **arrays):
+    p = tmp_path / ("w.npz")
+    (np.savez_compressed if compressed else np.savez)(p, **arrays)
+    return p.read_bytes()
+
+
+def test_denoiser_weights_are_checked_before_the_device(tmp_path):
+    """hpt_denoiser_create reads the .npz bytes (stored or deflated zip
+    entries, .npy arrays) and checks the layer layout before it touches a
+    device: malformed files fail with HPT_ERROR_INVALID_ARGUMENT here, a
+    well-formed one gets as far as the device check."""
+    lib = hpt.load_library()
+    w = os.path.join(ROOT, "hala-pathtracer_amd", "weights", "unet_denoiser.npz")
+    good = open(w, "rb").read()
+    with np.load(w, allow_pickle=False) as f:
+        levels, params, step, residual = f["levels"], f["params"], f["step"], f["residual"]
+    h = C.c_void_p()
+
+    def create(data):
+        rc = lib.hpt_denoiser_create(data, len(data), 0, C.byref(h))
+        return rc, lib.hpt_last_error().decode()
+
+    for data, what in ((b"not a zip at all, just bytes", "zip"), (good[: len(good) // 2], "zip"),
+                       (_npz_bytes(tmp_path, levels=levels, params=params[:-1], step=step, residual=residual),
+                        "parameters"),
+                       (_npz_bytes(tmp_path, levels=levels, step=step, residual=residual), "params"),
+                       (_npz_bytes(tmp_path, levels=levels, params=params.astype(np.float64), step=step,
+                                   residual=residual), "dtype")):
+        rc, msg = create(data)
+        assert rc != 0 and what in msg, (rc, msg)
+    for data in (good, _npz_bytes(tmp_path, True, levels=levels, params=params, step=step, residual=residual)):
+        rc, msg = create(data)
+        if rc != 0:  # no GPU here: the file passed and the device check failed
+            assert "weights" not in msg, msg
+        else:
+            lib.hpt_denoiser_destroy(h)
