@@ -484,6 +484,8 @@ def main():
                          "choice among off and bounces 0-2 at this batch size, measured before the timed region)")
     ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
+    ap.add_argument("--vmis-phases", choices=["on", "off"], default="on",
+                    help="volume-MIS shading as phase kernels with queued walks (on) or one kernel with inline walks")
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced together per batch (0: sized by the library)")
     ap.add_argument("--relmse", action="store_true",
@@ -532,6 +534,7 @@ def main():
         rr.commit()
         rr.set_launch_mode(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
         rr.set_shade_sort(shade_sort)
+        rr.set_vmis_phases(args.vmis_phases == "on")
         return rr
 
     shade_sort = hpt.SHADE_SORT_OFF if args.shade_sort in ("off", "auto") else int(args.shade_sort)
@@ -554,20 +557,24 @@ def main():
         log(f"shade sort autotuned in {time.perf_counter() - t0:.1f} s: "
             f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'} ({sort_table})")
 
-    # Visit counts per ray (extend) and per shading event (the volume-MIS
-    # walks inside shading) from a counted warm-up (separate kernel variants).
+    # Visit counts per ray (extend), per shading event, and of the volume-MIS
+    # transmittance walks (inline in shading, or the walk kernel's when
+    # shading is phased) from a counted warm-up (separate kernel variants).
     r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
     r.render(max(1, args.warmup))
     ext = r.stage_stats(hpt.STAGE_EXTEND)
     sh = r.stage_stats(hpt.STAGE_SHADE)
+    wk = r.stage_stats(hpt.STAGE_WALK)
     nodes_per_ray = ext["node_visits"] / max(1, ext["rays"])
     tris_per_ray = ext["tri_tests"] / max(1, ext["rays"])
-    walk_q = sh["queries"] / max(1, sh["rays"])
-    walk_nv = sh["node_visits"] / max(1, sh["rays"])
-    walk_nt = sh["tri_tests"] / max(1, sh["rays"])
+    walks = wk if wk["queries"] else sh  # the stage whose kernels ran the walks
+    walk_queries = walks["queries"] / max(1, sh["rays"])  # per shading event
+    walk_bytes_per_query = (NODE_BYTES * walks["node_visits"] + TRI_BYTES * walks["tri_tests"] +
+                            RAY_BYTES * walks["queries"]) / max(1, walks["queries"])
+    phased = wk["launches"] > 0
     log(f"warm-up: extend {ext['rays']} rays, {nodes_per_ray:.2f} nodes/ray, {tris_per_ray:.2f} tris/ray; "
-        f"shade {sh['rays']} events, {walk_q:.3f} walk queries, {walk_nv:.2f} walk nodes, {walk_nt:.2f} walk tris "
-        f"per event")
+        f"shade {sh['rays']} events, {walk_queries:.3f} walk queries per event, {walk_bytes_per_query:.0f} B per "
+        f"walk query ({'walk kernel' if phased else 'inline'})")
 
     # Timed region: `steps` frames from a fresh accumulation, submitted the way
     # the renderer submits them (full batches as graph replays by default).
@@ -591,13 +598,14 @@ def main():
     r.render(steps)
     r.wait_idle()
     profiled_wall = time.perf_counter() - t0
-    stats = {s: r.stage_stats(s) for s in range(hpt.STAGE_SORT + 1)}
+    stats = {s: r.stage_stats(s) for s in range(hpt.STAGE_WALK + 1)}
     ext_t, shade_t, shadow_t = stats[hpt.STAGE_EXTEND], stats[hpt.STAGE_SHADE], stats[hpt.STAGE_SHADOW]
     samples = W * H * steps
     msps = samples / wall / 1e6
     batch_frames = min(steps, fif)
-    rays_total = ext_t["rays"] + shadow_t["rays"] + int(round(shade_t["rays"] * walk_q))
-    names = ["camera", "extend", "shade", "shadow", "accum", "sort"]
+    walk_t = stats[hpt.STAGE_WALK]
+    rays_total = ext_t["rays"] + shadow_t["rays"] + int(round(shade_t["rays"] * walk_queries))
+    names = ["camera", "extend", "shade", "shadow", "accum", "sort", "walk"]
     log("stage totals: " + ", ".join(f"{n} {stats[s]['total_ms']:.1f} ms/{stats[s]['launches']}"
                                        for s, n in enumerate(names)))
     log(f"wall {wall:.3f} s ({args.launch}, {graph_launches} graph replays), {msps:.1f} Msamples/s, "
@@ -605,24 +613,33 @@ def main():
     r.close()
 
     # Rooflines: the extension walk (80 B per node, 48 B per triangle, 48 B
-    # per ray) and the shading kernel (616 B per shading event -- material,
-    # primitive, indices and vertices as the survey's 384 B plus 232 B of path
-    # state read and written -- plus its volume-MIS walks at the extension's
-    # rate per closest-hit query).
+    # per ray); shading (616 B per shading event -- material, primitive,
+    # indices and vertices as the survey's 384 B plus 232 B of path state read
+    # and written -- plus, when the walks run inline, its volume-MIS walks at
+    # the extension's rate per closest-hit query); the walk kernel (that rate
+    # per query) when shading is phased.
     roof_ext = stage_roofline("k_extend", ext_t, NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
                               traffic=profiled_traffic("k_extend", args.workload, batch_frames),
                               extra={"nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)})
-    shade_bytes = SHADE_EVENT_BYTES + NODE_BYTES * walk_nv + TRI_BYTES * walk_nt + RAY_BYTES * walk_q
-    roof_shade = stage_roofline("k_shade", shade_t, shade_bytes,
-                                traffic=profiled_traffic("k_shade", args.workload, batch_frames),
+    shade_bytes = SHADE_EVENT_BYTES + (0.0 if phased else walk_queries * walk_bytes_per_query)
+    shade_name = "k_vmis_a/b/c" if phased else "k_shade"
+    roof_shade = stage_roofline(shade_name, shade_t, shade_bytes,
+                                traffic=None if phased else profiled_traffic("k_shade", args.workload, batch_frames),
                                 extra={"bytes_per_event": round(shade_bytes, 1),
-                                       "walk_queries_per_event": round(walk_q, 4),
-                                       "walk_nodes_per_event": round(walk_nv, 3),
-                                       "walk_tris_per_event": round(walk_nt, 3)})
-    dominant = max((("extend", ext_t), ("shade", shade_t), ("shadow", shadow_t)), key=lambda kv: kv[1]["total_ms"])[0]
-    roofline = roof_shade if dominant == "shade" else roof_ext
-    log(f"dominant stage: {dominant}; k_shade {roof_shade['achieved']:.0f} GB/s ({roof_shade['frac']:.3f}), "
-        f"k_extend {roof_ext['achieved']:.0f} GB/s ({roof_ext['frac']:.3f})")
+                                       "walk_queries_per_event": round(walk_queries, 4)})
+    rooflines = {"extend": roof_ext, "shade": roof_shade}
+    if phased:
+        # the walk kernel's units are its closest-hit queries: the timed
+        # run's shading events times the warm-up's queries per event
+        walk_t = dict(walk_t, rays=int(round(shade_t["rays"] * walk_queries)))
+        rooflines["walk"] = stage_roofline("k_walk", walk_t, walk_bytes_per_query,
+                                           traffic=profiled_traffic("k_walk", args.workload, batch_frames),
+                                           extra={"bytes_per_query": round(walk_bytes_per_query, 1)})
+    stage_of = {"extend": ext_t, "shade": shade_t, "shadow": shadow_t, "walk": walk_t}
+    dominant = max((k for k in rooflines), key=lambda k: stage_of[k]["total_ms"])
+    roofline = rooflines[dominant]
+    log(f"dominant stage: {dominant}; " + ", ".join(f"{v['kernel']} {v['achieved']:.0f} GB/s ({v['frac']:.3f})"
+                                                    for v in rooflines.values()))
 
     conv = None
     if args.relmse:
@@ -651,6 +668,7 @@ def main():
               "launch": args.launch,
               "shade_sort": "off" if shade_sort == hpt.SHADE_SORT_OFF else shade_sort,
               "shade_sort_tuning": sort_table,
+              "vmis_phases": args.vmis_phases if FEATURES & hpt.FEATURE_VOL_MIS else None,
               "eager_profiled_wall_s": round(profiled_wall, 4),
               "stage_ms": {n: round(stats[s]["total_ms"], 3) for s, n in enumerate(names)},
               "mrays_per_s": round(rays_total / wall / 1e6, 2)}
@@ -673,6 +691,7 @@ def main():
         "roofline": roofline,
         "roofline_extend": roof_ext,
         "roofline_shade": roof_shade,
+        "roofline_walk": rooflines.get("walk"),
         "cpu_baseline": base,
     }
     if conv is not None:

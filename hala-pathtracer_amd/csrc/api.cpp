@@ -125,6 +125,11 @@ struct hpt_renderer {
   std::shared_ptr<SceneDevice> sd;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
+  // Phased volume-MIS shading (MEDIUM|VOL_MIS): the per-path NEE state
+  // between its kernels and the queue of its scattering events; off = the
+  // single shading kernel with inline walks.
+  uint32_t vmis_phases = 1;
+  DevBuf d_nee_dir, d_nee_li, d_nee_ld, d_nee_tr, d_qc;
   // Material-sorted shading from bounce shade_sort_from on (HPT_SHADE_SORT_OFF:
   // never): one key byte and one sorted entry per queue slot.
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
@@ -170,6 +175,7 @@ struct hpt_group {
   size_t cap = 0;  // paths the path buffers hold
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
+  DevBuf d_nee_dir, d_nee_li, d_nee_ld, d_nee_tr, d_qc;  // phased volume-MIS shading, sized like the paths
   // The last captured step and the key of the step before this one: a step
   // repeated with the same jobs, frames and buffers is captured and replayed.
   hipGraphExec_t gexec = nullptr;
@@ -306,6 +312,10 @@ PathState path_state(const hpt_renderer* r) {
   ps.nee_env = r->d_nee_e.as<float4>();
   ps.nee_light = r->d_nee_l.as<float4>();
   ps.hit = r->d_hit.as<float4>();
+  ps.nee_dir = r->d_nee_dir.as<float4>();
+  ps.nee_li = r->d_nee_li.as<float4>();
+  ps.nee_ld = r->d_nee_ld.as<float4>();
+  ps.nee_tr = r->d_nee_tr.as<float4>();
   return ps;
 }
 
@@ -398,6 +408,7 @@ struct WaveJob {
 struct WaveBuffers {
   PathState ps;
   uint32_t *qa, *qb, *counts;
+  uint32_t* qc;       // the phased volume-MIS shading's event queue (null: not phased)
   float4 *sh_o, *sh_d, *sh_c;
   Spill sp;
   uint8_t* sort_keys;
@@ -432,6 +443,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   const StageCounters sc_extend{stats ? stats + kStatsPerStage * HPT_STAGE_EXTEND : nullptr, count};
   const StageCounters sc_shadow{stats ? stats + kStatsPerStage * HPT_STAGE_SHADOW : nullptr, count};
   const StageCounters sc_shade{stats ? stats + kStatsPerStage * HPT_STAGE_SHADE : nullptr, count};
+  const StageCounters sc_walk{stats ? stats + kStatsPerStage * HPT_STAGE_WALK : nullptr, count};
 
   begin(HPT_STAGE_CAMERA);
   for (size_t j = 0; j < jobs.size(); ++j)
@@ -443,6 +455,14 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   int out = 0;
   const bool transparent = (features & HPT_FEATURE_TRANSPARENT) != 0;
   const bool vmis = (features & HPT_FEATURE_MEDIUM) && (features & HPT_FEATURE_VOL_MIS);
+  // Phased volume-MIS shading: scattering events queue (qnee), the first
+  // walk's queue in the shadow queue's first half of each segment (sq's
+  // counters), the second walk's in the other half.
+  const bool phased = vmis && wb.qc && ps.nee_dir;
+  uint32_t* phase_counts = phased ? counts + 4 * kSeg * kSegStride + 2 * kSeg * kSortKeys : nullptr;
+  const Queue qnee{wb.qc, phase_counts, g.seg_cap};
+  const ShadowQueue wq1{wb.sh_o + g.seg_cap, wb.sh_d + g.seg_cap, wb.sh_c + g.seg_cap,
+                        phased ? phase_counts + kSeg * kSegStride : nullptr, 2 * g.seg_cap};
   std::vector<uint32_t> seg_counts;
   for (uint32_t b = 0;; ++b) {
     if (b >= fixed_bounces) {
@@ -460,7 +480,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     const Queue& qout = qbuf[out];
     const bool sort = b >= sort_from && ss.keys && ss.items;
     begin(HPT_STAGE_EXTEND);
-    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr);
+    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr, phase_counts);
     end();
     Queue qshade = qin;
     if (sort) {
@@ -469,9 +489,29 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
       end();
       qshade = Queue{ss.items, ss.counts, g.seg_cap};
     }
-    begin(HPT_STAGE_SHADE);
-    launch_shade(s, features, sv, shade_fp, ps, qshade, n, qout, sq, sp, sc_shade);
-    end();
+    if (phased) {
+      begin(HPT_STAGE_SHADE);
+      launch_vmis_a(s, transparent, sv, shade_fp, ps, qshade, n, qout, qnee, sq, sc_shade);
+      end();
+      begin(HPT_STAGE_WALK);
+      launch_walk(s, transparent, sv, shade_fp, ps, sq, g.seg_cap, sp, sc_walk);
+      end();
+      if (shade_fp.env_type != HPT_ENV_SKY) {
+        begin(HPT_STAGE_SHADE);
+        launch_vmis_b(s, transparent, sv, shade_fp, ps, qnee, wq1);
+        end();
+        begin(HPT_STAGE_WALK);
+        launch_walk(s, transparent, sv, shade_fp, ps, wq1, g.seg_cap, sp, sc_walk);
+        end();
+      }
+      begin(HPT_STAGE_SHADE);
+      launch_vmis_c(s, transparent, sv, shade_fp, ps, qnee, qout);
+      end();
+    } else {
+      begin(HPT_STAGE_SHADE);
+      launch_shade(s, features, sv, shade_fp, ps, qshade, n, qout, sq, sp, sc_shade);
+      end();
+    }
     if (!vmis) {
       begin(HPT_STAGE_SHADOW);
       launch_shadow(s, sv, ps, sq, n, sp, sc_shadow);
@@ -513,6 +553,7 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   wb.sp = ensure_spill(r, 2 * kSeg * seg_geom(n).seg_cap);  // threads of the widest launch (shadow)
   wb.sort_keys = r->d_sort_keys.as<uint8_t>();
   wb.sort_items = r->d_sort_items.as<uint32_t>();
+  wb.qc = r->vmis_phases ? r->d_qc.as<uint32_t>() : nullptr;
   const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
   trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
                   r->shade_sort_from, r);
@@ -536,8 +577,9 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, path_state(r));
   for (const DevBuf* b : {&r->d_qa, &r->d_qb, &r->d_sh_o, &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill,
                           &r->d_accum, &r->d_albedo, &r->d_normal, &r->d_final, &r->d_frame, &r->d_sort_keys,
-                          &r->d_sort_items})
+                          &r->d_sort_items, &r->d_qc})
     key_append(k, b->p);
+  key_append(k, r->vmis_phases);
   key_append(k, frames);
   key_append(k, r->info.features);
   key_append(k, r->shade_sort_from);
@@ -595,9 +637,16 @@ constexpr uint32_t kAutoBatchFrames = 32;
 // slots and two shadow-queue entries of 3 float4.
 constexpr size_t kPathBytes = 11 * 16 + 8 + 2 * 4 + 2 * 48;  // (+ segment slack)
 
+// Phased volume-MIS shading: four float4 of NEE state and an event-queue slot.
+constexpr size_t kPhaseBytes = 4 * 16 + 4;
+
+bool vmis_features(uint32_t features) {
+  return (features & HPT_FEATURE_MEDIUM) && (features & HPT_FEATURE_VOL_MIS);
+}
+
 uint32_t batch_frames_for(const hpt_renderer* r) {
   if (r->frames_in_flight) return r->frames_in_flight;
-  const size_t per_frame = (size_t)r->npix * kPathBytes;
+  const size_t per_frame = (size_t)r->npix * (kPathBytes + (vmis_features(r->info.features) ? kPhaseBytes : 0));
   size_t k = kPathBudget / (per_frame ? per_frame : 1);
   if (k > kAutoBatchFrames) k = kAutoBatchFrames;
   if (k < 1) k = 1;
@@ -632,6 +681,12 @@ void alloc_paths(hpt_renderer* r, uint32_t frames) {
   r->d_sh_o.alloc(2 * qn * 16);
   r->d_sh_d.alloc(2 * qn * 16);
   r->d_sh_c.alloc(2 * qn * 16);
+  if (vmis_features(r->info.features)) {
+    for (DevBuf* b : {&r->d_nee_dir, &r->d_nee_li, &r->d_nee_ld, &r->d_nee_tr}) b->alloc(f4);
+    r->d_qc.alloc(qn * 4);
+  } else {
+    for (DevBuf* b : {&r->d_nee_dir, &r->d_nee_li, &r->d_nee_ld, &r->d_nee_tr, &r->d_qc}) b->release();
+  }
   alloc_sort(r, frames);
   r->batch_cap = frames;
 }
@@ -658,7 +713,7 @@ void alloc_frame_state(hpt_renderer* r) {
   r->npix = n;
   alloc_paths(r, batch_frames_for(r));
   const size_t f4 = (size_t)n * 16;
-  r->d_counts.alloc((4 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
+  r->d_counts.alloc((6 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
   check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
   r->d_accum.alloc(f4);
   r->d_albedo.alloc(f4);
@@ -693,14 +748,15 @@ namespace {
 
 // Sizes a group's buffers for n paths (grow only, with a quarter of slack so
 // that steps of slowly growing size do not reallocate each time).
-void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth) {
+void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth, bool vmis) {
   // Buffers are freed or replaced only once the queued steps that use them end.
   const bool sort = g->shade_sort_from != HPT_SHADE_SORT_OFF;
   const size_t spill_need = bvh_depth > (uint32_t)kShortStack
                                 ? (size_t)(bvh_depth - kShortStack) * 2 * kSeg *
                                       seg_geom((uint32_t)std::max<size_t>(n, g->cap)).seg_cap * sizeof(uint2)
                                 : 0;
-  if (n > g->cap || sort != (g->d_sort_keys.p != nullptr) || g->d_spill.bytes < spill_need)
+  if (n > g->cap || sort != (g->d_sort_keys.p != nullptr) || g->d_spill.bytes < spill_need ||
+      vmis != (g->d_qc.p != nullptr))
     check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
   if (n > g->cap) {
     const size_t cap = std::max<size_t>(n, std::min<size_t>(g->cap + g->cap / 4, 0x7FFFFFFFu));
@@ -718,7 +774,14 @@ void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth) {
     g->d_sh_c.alloc(2 * qn * 16);
     g->d_sort_keys.release();
     g->d_sort_items.release();
+    for (DevBuf* b : {&g->d_nee_dir, &g->d_nee_li, &g->d_nee_ld, &g->d_nee_tr, &g->d_qc}) b->release();
     g->cap = cap;
+  }
+  if (vmis && !g->d_qc.p) {
+    for (DevBuf* b : {&g->d_nee_dir, &g->d_nee_li, &g->d_nee_ld, &g->d_nee_tr}) b->alloc(g->cap * 16);
+    g->d_qc.alloc((size_t)kSeg * seg_geom((uint32_t)g->cap).seg_cap * 4);
+  } else if (!vmis && g->d_qc.p) {
+    for (DevBuf* b : {&g->d_nee_dir, &g->d_nee_li, &g->d_nee_ld, &g->d_nee_tr, &g->d_qc}) b->release();
   }
   const size_t qn = (size_t)kSeg * seg_geom((uint32_t)g->cap).seg_cap;
   if (g->shade_sort_from != HPT_SHADE_SORT_OFF) {
@@ -733,7 +796,7 @@ void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth) {
     if (g->d_spill.bytes < entries * sizeof(uint2)) g->d_spill.alloc(entries * sizeof(uint2));
   }
   if (!g->d_counts.p) {
-    g->d_counts.alloc((4 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
+    g->d_counts.alloc((6 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
     check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
   }
   if (!g->d_limits.p) g->d_limits.alloc(kMaxWaveJobs * sizeof(uint2));
@@ -1219,6 +1282,14 @@ hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce) {
       check(hipStreamSynchronize(r->stream), "shade sort");  // buffers may be in use
       alloc_sort(r, r->batch_cap);
     }
+  });
+}
+
+hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(on <= 1, "on must be 0 or 1");
+    r->vmis_phases = on;
   });
 }
 
@@ -1763,7 +1834,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     if (total == 0) return;
     const uint32_t n = (uint32_t)total;
     check(hipEventSynchronize(g->step_end[g->steps & 1]), "hipEventSynchronize");  // the step two before
-    group_alloc(g, n, r0->sd->bvh_depth);
+    group_alloc(g, n, r0->sd->bvh_depth, vmis_features(r0->info.features));
     // The step runs after each job's earlier work on its own stream.
     for (uint32_t i = 0; i < count; ++i) {
       ensure_event(jobs[i]->ev);
@@ -1805,7 +1876,8 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
                       g->d_thr.as<float4>(),  g->d_rad.as<float4>(),   g->d_alb.as<float4>(),
                       g->d_nrm.as<float4>(),  g->d_med.as<float4>(),   g->d_med2.as<float2>(),
                       g->d_nee_e.as<float4>(), g->d_nee_l.as<float4>(), g->d_hit.as<float4>(),
-                      g->d_job.as<uint16_t>()};
+                      g->d_job.as<uint16_t>(), g->d_nee_dir.as<float4>(), g->d_nee_li.as<float4>(),
+                      g->d_nee_ld.as<float4>(), g->d_nee_tr.as<float4>()};
     wb.qa = g->d_qa.as<uint32_t>();
     wb.qb = g->d_qb.as<uint32_t>();
     wb.counts = g->d_counts.as<uint32_t>();
@@ -1816,12 +1888,13 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     if (!g->d_spill.p) wb.sp = Spill{nullptr, 0};
     wb.sort_keys = g->d_sort_keys.as<uint8_t>();
     wb.sort_items = g->d_sort_items.as<uint32_t>();
+    wb.qc = g->d_qc.as<uint32_t>();
     key_append(key, sv);
     key_append(key, shade_fp);
     key_append(key, wb.ps);
     for (const void* q : {(const void*)wb.qa, (const void*)wb.qb, (const void*)wb.counts, (const void*)wb.sh_o,
                           (const void*)wb.sh_d, (const void*)wb.sh_c, (const void*)wb.sp.buf,
-                          (const void*)wb.sort_keys, (const void*)wb.sort_items})
+                          (const void*)wb.sort_keys, (const void*)wb.sort_items, (const void*)wb.qc})
       key_append(key, q);
     key_append(key, wb.sp.stride);
     key_append(key, n);

@@ -21,6 +21,11 @@ struct PathState {
   float4* nee_light;  // sum of unoccluded light NEE terms
   float4* hit;        // closest hit (t, u, v, id bits)
   uint16_t* job;      // job of each path in a wavefront of several jobs, else null
+  // Between the phases of the volume-MIS shading (null when it is not phased):
+  float4* nee_dir;    // NEE sample direction, .w = its pdf
+  float4* nee_li;     // NEE sample radiance, .w = flag bits (walk queued, light with an area)
+  float4* nee_ld;     // the event's NEE sum so far, .w = flag bits (surface event)
+  float4* nee_tr;     // transmittance of the last walk
 };
 
 // Path state arrays from path `base` on: the slice of one job in a wavefront.
@@ -39,6 +44,12 @@ inline PathState path_slice(const PathState& ps, uint32_t base) {
   s.nee_light += base;
   s.hit += base;
   if (s.job) s.job += base;
+  if (s.nee_dir) {
+    s.nee_dir += base;
+    s.nee_li += base;
+    s.nee_ld += base;
+    s.nee_tr += base;
+  }
   return s;
 }
 constexpr uint32_t PATH_ANB = 1u;             // any_non_specular_bounce
@@ -141,8 +152,11 @@ struct StageCounters {
 // paths.  Extend zeroes the counters of qnext and sq, which the shade launch
 // after it fills.
 // keys: null, or the sort keys of qin's slots to write (see ShadeSort).
+// phase_counts: null, or two counter blocks (kSeg * kSegStride words each)
+// of the phased volume-MIS shading's queues, zeroed with qnext and sq.
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
-                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys);
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
+                   uint32_t* phase_counts);
 // Groups qin's entries by the keys extend wrote: ss.items and ss.counts form
 // the segmented queue shading then reads (qin may be the identity queue).
 void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss);
@@ -151,6 +165,19 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
                   Spill sp, StageCounters sc);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc);
+// Volume-MIS shading in phases (MEDIUM|VOL_MIS): A over qin (finished and
+// passing paths to qout, scattering events to qnee, their first walk to wq),
+// the walks of a walk queue, B over qnee (environment maps: the second walk
+// to wq), C over qnee (survivors to qout).  Same images as launch_shade.
+void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qin, uint32_t n_max, const Queue& qout, const Queue& qnee, const ShadowQueue& wq,
+                   StageCounters sc);
+void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc);
+void launch_vmis_b(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qnee, const ShadowQueue& wq);
+void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qnee, const Queue& qout);
 // Depth limits of the jobs of one wavefront (FrameParams::job_limits), passed
 // by value so that a captured launch carries them.
 constexpr uint32_t kMaxWaveJobs = 256;

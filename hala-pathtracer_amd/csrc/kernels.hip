@@ -180,12 +180,16 @@ __device__ __forceinline__ uint8_t shade_key(const SceneView& sv, uint32_t id) {
 template <bool COUNT, bool SORT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
                                                    ShadowQueue sq, Spill sp, unsigned long long* stats,
-                                                   uint8_t* keys) {
+                                                   uint8_t* keys, uint32_t* phase_counts) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
   const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
   if (gid < kSeg) {  // the shade launch after this one appends to these
     qnext.counts[gid * kSegStride] = 0;
     sq.counts[gid * kSegStride] = 0;
+    if (phase_counts) {  // the phased volume-MIS shading's queues
+      phase_counts[gid * kSegStride] = 0;
+      phase_counts[(kSeg + gid) * kSegStride] = 0;
+    }
   }
   const LaneRange lr = lane_range(qin.counts, qin.seg_cap, qin.items == nullptr, n_max, stats);
   const uint32_t* items = qin.items ? qin.items + lr.base : nullptr;
@@ -768,6 +772,429 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ------------------------------------------ volume-MIS shading in phases
+// The volume-MIS shading kernel above runs the transmittance walks of
+// inc/lighting.glsl:7-61 inline: a traversal loop and the whole shading state
+// are live together (registers spill), and lanes whose walks run long stall
+// the rest of their wave.  The phased form cuts one bounce's shading at the
+// walks, keeping the order in which a path draws from its RNG stream:
+//   A  (k_vmis_a) up to the first walk: emitters and misses, material,
+//      medium, transparency; at a scattering event the environment sample
+//      (environment maps) or else the light pick and sample, and the walk's
+//      request;
+//   W  (k_walk)   the queued walks, one lane per walk: closest hits through
+//      transparent or refractive surfaces with Beer-Lambert on exit faces;
+//   B  (k_vmis_b, environment maps only) the environment term with its
+//      walk's transmittance, then the light pick and sample and its walk's
+//      request; W again;
+//   C  (k_vmis_c) the light term, then phase-function or BSDF sampling, the
+//      continuation and Russian roulette.
+// Paths that end or pass through a surface in A continue from A.  A path's
+// state between the phases lives in PathState::nee_* (below); the surface's
+// hit state and material are recomputed from the hit record in each phase.
+// Every value is computed by the same operations in the same order as the
+// monolithic kernel, so the images are the same bit for bit.
+constexpr uint32_t NEE_WALK = 1u;     // nee_li.w: a walk was queued for this sample
+constexpr uint32_t NEE_AREA = 2u;     // nee_li.w: the sampled light has an area (MIS applies)
+constexpr uint32_t NEE_SURFACE = 4u;  // nee_ld.w: a surface interaction (else a medium scattering event)
+
+// Appends a walk request (origin, direction, path) to this block's segment
+// of wq; want == false lanes take part in the wave's reservation only.
+__device__ __forceinline__ void walk_request(bool want, const ShadowQueue& wq, f3 o, f3 d, uint32_t p) {
+  const uint32_t k = seg_reserve(want, wq);
+  if (want) {
+    wq.o[k] = make_float4(o.x, o.y, o.z, 0.0f);
+    wq.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+  }
+}
+
+// The light pick and sample of inc/lighting.glsl:154-175 at hit_pos into
+// nee_dir / nee_li; returns whether the sample needs its walk.
+__device__ __forceinline__ bool light_sample_request(const SceneView& sv, const FrameParams& fp, uint4& rng,
+                                                     f3 hit_pos, const PathState& ps, uint32_t p) {
+  bool want = false;
+  if (fp.num_lights > 0) {
+    const int li = (int)(rnd(rng) * (float)fp.num_lights);
+    const hpt_light& L = sv.lights[li];
+    LightSample ls;
+    light_sample(rng, fp.num_lights, hit_pos, L, ls);
+    want = ls.pdf > 0.0f && dot(ls.direction, ls.normal) < 0.0f;
+    if (want) {
+      ps.nee_dir[p] = make_float4(ls.direction.x, ls.direction.y, ls.direction.z, ls.pdf);
+      ps.nee_li[p] = make_float4(ls.emission.x, ls.emission.y, ls.emission.z,
+                                 __uint_as_float(NEE_WALK | (L.area > 0.0f ? NEE_AREA : 0u)));
+    }
+  }
+  if (!want) ps.nee_li[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+  return want;
+}
+
+// The term of one NEE sample after its walk (inc/lighting.glsl:120-150 and
+// :202-238 with VOL_MIS): Ld += mis * Li * f / light_pdf.
+template <bool ENV>
+__device__ __forceinline__ void nee_term(const PathState& ps, uint32_t p, bool is_surface, bool anb, const Frame& fr,
+                                         const Mat& m, f3 V, f3 ffn, float aniso, f3& Ld) {
+  const float4 li4 = ps.nee_li[p];
+  if (!(__float_as_uint(li4.w) & NEE_WALK)) return;
+  const float4 dir4 = ps.nee_dir[p], tr4 = ps.nee_tr[p];
+  const f3 trm = mk3(tr4.x, tr4.y, tr4.z), light_dir = mk3(dir4.x, dir4.y, dir4.z);
+  const float light_pdf = dir4.w;
+  if (luminance(trm) > 0.0f) {
+    f3 Li = mk3(li4.x, li4.y, li4.z);
+    Li *= trm;
+    float pdf = 0.0f;
+    f3 f;
+    if (is_surface) {
+      f = bxdf_eval(anb, fr, m, V, ffn, light_dir, pdf);
+    } else {
+      pdf = phase_hg(dot(V, light_dir), aniso);
+      f = mk3(pdf);
+    }
+    if (pdf > 0.0f) {
+      float mis = 1.0f;
+      if (ENV || (__float_as_uint(li4.w) & NEE_AREA)) mis = power_heuristic(light_pdf, pdf);
+      if (mis > 0.0f) Ld += mis * Li * f / light_pdf;
+    }
+  }
+}
+
+// A surface interaction's state as the monolithic kernel builds it: hit
+// state, material (textures applied), shading frame.
+struct SurfaceState {
+  HitState h;
+  Mat m;
+  Frame fr;
+};
+// s.h holds the hit state of a triangle with a valid material.
+__device__ __forceinline__ void surface_state(const SceneView& sv, const float4& h4, f3 dir, uint32_t depth,
+                                              SurfaceState& s) {
+  const uint32_t id = __float_as_uint(h4.w);
+  s.m = load_material(sv.materials + s.h.material);
+  s.fr.eta = dot(dir, s.h.normal) < 0.0f ? (1.0f / s.m.ior) : s.m.ior;  // before the normal map
+  if ((s.m.base_color_map & s.m.metallic_roughness_map & s.m.emission_map & s.m.normal_map) != INVALID)
+    apply_textures(sv, id, h4.y, h4.z, dir, s.m, s.h);
+  s.fr.tangent = s.h.tangent;
+  s.fr.bitangent = s.h.bitangent;
+  s.fr.depth = depth;
+}
+
+// End of a bounce for a path that lives on (pathtracer.rgen.glsl:245-256):
+// depth limit, Russian roulette, the next segment's state.
+__device__ __forceinline__ bool continue_path(const PathState& ps, uint32_t p, uint4& rng, bool alive, uint32_t depth,
+                                              uint32_t max_depth, uint32_t rr_depth, f3 org, f3 dir, float pdf, f3 T,
+                                              bool anb, bool in_medium, f3 med_color, float med_density,
+                                              float med_aniso, uint32_t med_type) {
+  if (alive && depth >= max_depth) alive = false;
+  if (alive && depth >= rr_depth && rr_depth > 0) {
+    const float q = fminf(fmaxf(T.x, fmaxf(T.y, T.z)) + 0.001f, 0.95f);
+    if (rnd(rng) > q) alive = false;
+    else T /= q;
+  }
+  if (alive) {
+    depth++;
+    const uint32_t nf = (anb ? PATH_ANB : 0u) | (in_medium ? PATH_IN_MEDIUM : 0u);
+    ps.org[p] = make_float4(org.x, org.y, org.z, pdf);
+    ps.dir[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(depth));
+    ps.thr[p] = make_float4(T.x, T.y, T.z, __uint_as_float(nf));
+    ps.med[p] = make_float4(med_color.x, med_color.y, med_color.z, med_density);
+    ps.med2[p] = make_float2(med_aniso, __uint_as_float(med_type));
+  }
+  ps.rng[p] = rng;
+  return alive;
+}
+
+#ifndef HPT_VMIS_PHASE_WAVES_PER_EU
+#define HPT_VMIS_PHASE_WAVES_PER_EU 4
+#endif
+
+template <bool TRANSP, bool JOBS>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_VMIS_PHASE_WAVES_PER_EU)))
+    k_vmis_a(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, Queue qnee,
+             ShadowQueue wq, unsigned long long* stats) {
+  uint32_t p;
+  const bool active = queue_entry(qin, n_max, stats, p);
+  bool cont = false, nee = false, walk = false;
+  f3 wo = mk3(0.0f), wd = mk3(0.0f);
+  if (active) {
+    const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p], h4 = ps.hit[p];
+    f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
+    float pdf = o4.w;
+    uint32_t depth = __float_as_uint(d4.w);
+    uint32_t max_depth = fp.max_depth, rr_depth = fp.rr_depth;
+    if (JOBS) {
+      const uint2 lim = fp.job_limits[ps.job[p]];
+      max_depth = lim.x;
+      rr_depth = lim.y;
+    }
+    const uint32_t flags = __float_as_uint(t4.w);
+    const bool anb = (flags & PATH_ANB) != 0;
+    bool in_medium = (flags & PATH_IN_MEDIUM) != 0;
+    uint4 rng = ps.rng[p];
+    const float4 r4 = ps.rad[p];
+    f3 radiance = mk3(r4.x, r4.y, r4.z);
+    const float4 m4 = ps.med[p];
+    const float2 m2 = ps.med2[p];
+    f3 med_color = mk3(m4.x, m4.y, m4.z);
+    float med_density = m4.w, med_aniso = m2.x;
+    uint32_t med_type = __float_as_uint(m2.y);
+    const uint32_t id = __float_as_uint(h4.w);
+    const float t_hit = h4.x;
+
+    if (id == MISS_ID) {
+      const f3 env = env_eval(sv, fp, depth, pdf, dir, false);
+      radiance += T * env;
+      if (depth == 0) {
+        ps.albedo[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        ps.normal[p] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
+      }
+    } else if (id & LIGHT_BIT) {
+      const hpt_light& L = sv.lights[id & ~LIGHT_BIT];
+      float light_pdf = 1.0f;
+      light_isect(L, org, dir, light_pdf);
+      float mis = 1.0f;
+      if (depth > 0) mis = power_heuristic(pdf, light_pdf);
+      radiance += mis * ld3(L.intensity) * T;
+    } else {
+      SurfaceState s;
+      s.h = triangle_state(sv, id, t_hit, h4.y, h4.z, org, dir);
+      if (s.h.material == INVALID) {
+        radiance = mk3(0.7f, 0.0f, 0.7f);
+        ps.nee_env[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.nee_light[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        goto done;
+      }
+      surface_state(sv, h4, dir, depth, s);
+      if (depth == 0) {
+        ps.albedo[p] = make_float4(s.m.base_color.x, s.m.base_color.y, s.m.base_color.z, 0.0f);
+        ps.normal[p] = make_float4(s.h.normal.x, s.h.normal.y, s.h.normal.z, 0.0f);
+      }
+      radiance += T * s.m.emission;
+      bool medium_sampled = false;
+      if (in_medium) {
+        if (med_type == HPT_MEDIUM_ABSORB) {
+          const f3 e = -(1.0f - med_color) * t_hit * med_density;
+          T *= mk3(expf(e.x), expf(e.y), expf(e.z));
+        } else if (med_type == HPT_MEDIUM_EMISSIVE) {
+          radiance += med_color * t_hit * med_density * T;
+        } else {
+          const float scatter_dist = fminf(-logf(rnd(rng)) / med_density, t_hit);
+          medium_sampled = scatter_dist < t_hit;
+          if (medium_sampled) {
+            T *= med_color;
+            org = org + dir * scatter_dist;
+            s.h.pos = org;
+          }
+        }
+      }
+      bool pass = false;
+      if (!medium_sampled && TRANSP) {
+        if (s.m.opacity == 0.0f || rnd(rng) > s.m.opacity) {
+          depth--;  // uint: wraps at depth 0 and the max_depth test ends the path
+          pass = true;
+        }
+      }
+      if (pass) {
+        // A pass-through: the continuation without next-event estimation.
+        org = s.h.pos + dir * EPS;
+        if (dot(dir, s.h.normal) < 0.0f && s.m.medium_type != HPT_MEDIUM_NONE) {
+          in_medium = true;
+          med_color = s.m.medium_color;
+          med_density = s.m.medium_density;
+          med_aniso = s.m.medium_anisotropy;
+          med_type = s.m.medium_type;
+        } else {
+          in_medium = false;
+          med_type = HPT_MEDIUM_NONE;
+        }
+        cont = continue_path(ps, p, rng, true, depth, max_depth, rr_depth, org, dir, pdf, T, anb, in_medium,
+                             med_color, med_density, med_aniso, med_type);
+      } else {
+        // A scattering event (surface or medium): next-event estimation
+        // starts (inc/lighting.glsl:69-75).
+        nee = true;
+        const f3 scatter_pos = s.h.pos + s.h.ffnormal * EPS;
+        if (fp.env_type != HPT_ENV_SKY) {
+          f3 Li;
+          float light_pdf;
+          const f3 light_dir = sample_env_map(rng, sv, fp, Li, light_pdf);
+          walk = light_pdf > 0.0f;
+          ps.nee_dir[p] = make_float4(light_dir.x, light_dir.y, light_dir.z, light_pdf);
+          ps.nee_li[p] = make_float4(Li.x, Li.y, Li.z, __uint_as_float(walk ? NEE_WALK : 0u));
+          wd = light_dir;
+        } else {
+          walk = light_sample_request(sv, fp, rng, s.h.pos, ps, p);
+          if (walk) {
+            const float4 dd = ps.nee_dir[p];
+            wd = mk3(dd.x, dd.y, dd.z);
+          }
+        }
+        wo = scatter_pos;
+        ps.nee_ld[p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(medium_sampled ? 0u : NEE_SURFACE));
+        ps.org[p] = make_float4(org.x, org.y, org.z, pdf);
+        ps.thr[p] = make_float4(T.x, T.y, T.z, __uint_as_float(flags));
+        ps.rng[p] = rng;
+      }
+    }
+    ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+  }
+done:
+  seg_append(cont, qout, p);
+  seg_append(nee, qnee, p);
+  walk_request(walk, wq, wo, wd, p);
+}
+
+// The queued transmittance walks (inc/lighting.glsl:7-61), one lane per walk:
+// the path's RNG stream advances for partly transparent surfaces, as inline.
+template <bool TRANSP, bool JOBS, bool COUNT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
+    k_walk(SceneView sv, FrameParams fp, PathState ps, ShadowQueue wq, Spill sp, unsigned long long* stats) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const SegSlot t = seg_slot(wq.counts);
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
+  WalkCounts wc;
+  if (t.j < t.n) {
+    const uint32_t k = t.seg * wq.seg_cap + t.j;
+    const float4 o4 = wq.o[k], d4 = wq.d[k];
+    const uint32_t p = __float_as_uint(d4.w);
+    const uint32_t max_depth = JOBS ? fp.job_limits[ps.job[p]].x : fp.max_depth;
+    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+    uint4 rng = ps.rng[p];
+    const f3 tr = eval_transmittance<TRANSP, COUNT>(sv, max_depth, rng, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                                     st, wc);
+    ps.nee_tr[p] = make_float4(tr.x, tr.y, tr.z, 0.0f);
+    ps.rng[p] = rng;
+  }
+  if (COUNT) add_walks(stats, wc);
+}
+
+// A scattering event as A left it: the hit state of the surface that ended
+// the segment (its position replaced by the path's origin, the scattering
+// point, for a medium event) and, for a surface event, its material and
+// frame.
+struct NeeEvent {
+  bool is_surface, anb;
+  SurfaceState s;
+  f3 V;
+  float aniso;
+};
+__device__ __forceinline__ void nee_event(const SceneView& sv, const PathState& ps, uint32_t p, f3 org, f3 dir,
+                                          uint32_t depth, uint32_t flags, NeeEvent& e) {
+  e.is_surface = (__float_as_uint(ps.nee_ld[p].w) & NEE_SURFACE) != 0;
+  e.anb = (flags & PATH_ANB) != 0;
+  e.V = -dir;
+  e.aniso = ps.med2[p].x;
+  const float4 h4 = ps.hit[p];
+  e.s.h = triangle_state(sv, __float_as_uint(h4.w), h4.x, h4.y, h4.z, org, dir);
+  if (e.is_surface) surface_state(sv, h4, dir, depth, e.s);
+  else e.s.h.pos = org;
+}
+
+// Environment maps: the environment term, then the light pick and sample.
+template <bool TRANSP, bool JOBS>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_VMIS_PHASE_WAVES_PER_EU)))
+    k_vmis_b(SceneView sv, FrameParams fp, PathState ps, Queue qnee, ShadowQueue wq) {
+  const SegSlot t = seg_slot(qnee.counts);
+  bool walk = false;
+  f3 wo = mk3(0.0f), wd = mk3(0.0f);
+  uint32_t p = 0;
+  if (t.j < t.n) {
+    p = qnee.items[t.seg * qnee.seg_cap + t.j];
+    const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p];
+    const f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z);
+    NeeEvent e;
+    nee_event(sv, ps, p, org, dir, __float_as_uint(d4.w), __float_as_uint(t4.w), e);
+    f3 Ld = mk3(0.0f);
+    nee_term<true>(ps, p, e.is_surface, e.anb, e.s.fr, e.s.m, e.V, e.s.h.ffnormal, e.aniso, Ld);
+    uint4 rng = ps.rng[p];
+    const f3 scatter_pos = e.s.h.pos + e.s.h.ffnormal * EPS;
+    walk = light_sample_request(sv, fp, rng, e.s.h.pos, ps, p);
+    if (walk) {
+      const float4 dd = ps.nee_dir[p];
+      wd = mk3(dd.x, dd.y, dd.z);
+    }
+    wo = scatter_pos;
+    ps.nee_ld[p] = make_float4(Ld.x, Ld.y, Ld.z, ps.nee_ld[p].w);
+    ps.rng[p] = rng;
+  }
+  walk_request(walk, wq, wo, wd, p);
+}
+
+// The light term, then the scattered direction (phase function or BSDF),
+// the continuation and Russian roulette.
+template <bool TRANSP, bool JOBS>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_SHADE_WAVES_PER_EU)))
+    k_vmis_c(SceneView sv, FrameParams fp, PathState ps, Queue qnee, Queue qout) {
+  const SegSlot t = seg_slot(qnee.counts);
+  bool cont = false;
+  uint32_t p = 0;
+  if (t.j < t.n) {
+    p = qnee.items[t.seg * qnee.seg_cap + t.j];
+    const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p];
+    f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
+    float pdf = o4.w;
+    const uint32_t depth = __float_as_uint(d4.w);
+    uint32_t max_depth = fp.max_depth, rr_depth = fp.rr_depth;
+    if (JOBS) {
+      const uint2 lim = fp.job_limits[ps.job[p]];
+      max_depth = lim.x;
+      rr_depth = lim.y;
+    }
+    const uint32_t flags = __float_as_uint(t4.w);
+    bool in_medium = (flags & PATH_IN_MEDIUM) != 0;
+    const float4 m4 = ps.med[p];
+    const float2 m2 = ps.med2[p];
+    f3 med_color = mk3(m4.x, m4.y, m4.z);
+    float med_density = m4.w, med_aniso = m2.x;
+    uint32_t med_type = __float_as_uint(m2.y);
+    NeeEvent e;
+    nee_event(sv, ps, p, org, dir, depth, flags, e);
+    const float4 ld4 = ps.nee_ld[p];
+    f3 Ld = mk3(ld4.x, ld4.y, ld4.z);
+    nee_term<false>(ps, p, e.is_surface, e.anb, e.s.fr, e.s.m, e.V, e.s.h.ffnormal, e.aniso, Ld);
+    uint4 rng = ps.rng[p];
+    const float4 r4 = ps.rad[p];
+    f3 radiance = mk3(r4.x, r4.y, r4.z);
+    bool alive = true, anb = e.anb;
+    if (!e.is_surface) {
+      radiance += Ld * T;
+      const float r1 = rnd(rng);
+      const float r2 = rnd(rng);
+      const f3 sdir = sample_hg(-dir, med_aniso, r1, r2);
+      pdf = phase_hg(dot(-dir, sdir), med_aniso);
+      dir = sdir;
+    } else {
+      radiance += T * Ld;
+      f3 L;
+      float spdf;
+      uint32_t sflags = 0;
+      const f3 f = bxdf_sample(rng, anb, e.s.fr, e.s.m, -dir, e.s.h.ffnormal, L, spdf, sflags);
+      dir = L;
+      pdf = spdf;
+      if (pdf > 0.0f) T *= f / pdf;
+      else alive = false;
+      if (alive) {
+        org = e.s.h.pos + dir * EPS;
+        if (dot(dir, e.s.h.normal) < 0.0f && e.s.m.medium_type != HPT_MEDIUM_NONE) {
+          in_medium = true;
+          med_color = e.s.m.medium_color;
+          med_density = e.s.m.medium_density;
+          med_aniso = e.s.m.medium_anisotropy;
+          med_type = e.s.m.medium_type;
+        } else {
+          in_medium = false;
+          med_type = HPT_MEDIUM_NONE;
+        }
+      }
+    }
+    cont = continue_path(ps, p, rng, alive, depth, max_depth, rr_depth, org, dir, pdf, T, anb, in_medium, med_color,
+                         med_density, med_aniso, med_type);
+    ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+  }
+  seg_append(cont, qout, p);
+}
+
 // ------------------------------------------------------------------ shadow
 // Any-hit per shadow ray; an unoccluded ray adds its contribution to the
 // path's environment (slot 0) or light (slot 1) NEE sum.  Each path has at
@@ -1019,11 +1446,13 @@ static inline uint32_t walk_blocks(uint32_t entries) {
 }
 
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
-                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys) {
+                   const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
+                   uint32_t* phase_counts) {
   const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap) : walk_blocks(n_max));
   const bool count = sc.count && sc.stats;
-#define HPT_EXTEND(C, S) \
-  hipLaunchKernelGGL((k_extend<C, S>), grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys)
+#define HPT_EXTEND(C, S)                                                                                       \
+  hipLaunchKernelGGL((k_extend<C, S>), grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys, \
+                     phase_counts)
   if (keys) {
     if (count) HPT_EXTEND(true, true);
     else HPT_EXTEND(false, true);
@@ -1079,6 +1508,68 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
     else HPT_SHADE(true, true, false);
   }
 #undef HPT_SHADE
+}
+
+void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qin, uint32_t n_max, const Queue& qout, const Queue& qnee, const ShadowQueue& wq,
+                   StageCounters sc) {
+  const dim3 grid(queue_grid(qin, n_max));
+#define HPT_A(T, J) hipLaunchKernelGGL((k_vmis_a<T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, n_max, qout, qnee, wq, \
+                                       sc.stats)
+  if (fp.job_limits) {
+    if (transp) HPT_A(true, true);
+    else HPT_A(false, true);
+  } else {
+    if (transp) HPT_A(true, false);
+    else HPT_A(false, false);
+  }
+#undef HPT_A
+}
+
+void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc) {
+  const dim3 grid(kSeg * ((seg_cap + kBlock - 1) / kBlock));
+  const bool count = sc.count && sc.stats;
+#define HPT_W(T, J, C) hipLaunchKernelGGL((k_walk<T, J, C>), grid, dim3(kBlock), 0, s, sv, fp, ps, wq, sp, sc.stats)
+  if (fp.job_limits) {
+    if (transp) HPT_W(true, true, false);
+    else HPT_W(false, true, false);
+  } else if (count) {
+    if (transp) HPT_W(true, false, true);
+    else HPT_W(false, false, true);
+  } else {
+    if (transp) HPT_W(true, false, false);
+    else HPT_W(false, false, false);
+  }
+#undef HPT_W
+}
+
+void launch_vmis_b(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qnee, const ShadowQueue& wq) {
+  const dim3 grid(kSeg * ((qnee.seg_cap + kBlock - 1) / kBlock));
+#define HPT_B(T, J) hipLaunchKernelGGL((k_vmis_b<T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qnee, wq)
+  if (fp.job_limits) {
+    if (transp) HPT_B(true, true);
+    else HPT_B(false, true);
+  } else {
+    if (transp) HPT_B(true, false);
+    else HPT_B(false, false);
+  }
+#undef HPT_B
+}
+
+void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                   const Queue& qnee, const Queue& qout) {
+  const dim3 grid(kSeg * ((qnee.seg_cap + kBlock - 1) / kBlock));
+#define HPT_C(T, J) hipLaunchKernelGGL((k_vmis_c<T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qnee, qout)
+  if (fp.job_limits) {
+    if (transp) HPT_C(true, true);
+    else HPT_C(false, true);
+  } else {
+    if (transp) HPT_C(true, false);
+    else HPT_C(false, false);
+  }
+#undef HPT_C
 }
 
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,

@@ -33,7 +33,7 @@ MATERIAL_DIFFUSE, MATERIAL_DISNEY = 0, 1
 FEATURE_MEDIUM, FEATURE_VOL_MIS, FEATURE_TRANSPARENT = 1, 2, 4
 IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
 HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
-STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT = range(6)
+STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT, STAGE_WALK = range(7)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 SHADE_SORT_OFF = 0xFFFFFFFF
@@ -183,6 +183,7 @@ SIGNATURES = {
     "hpt_get_image_pointer": [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)],
     "hpt_set_launch_mode": [C.c_void_p, C.c_uint32],
     "hpt_set_shade_sort": [C.c_void_p, C.c_uint32],
+    "hpt_set_vmis_phases": [C.c_void_p, C.c_uint32],
     "hpt_group_create": [C.c_int32, C.POINTER(C.c_void_p)],
     "hpt_group_render": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                          C.c_uint32],
@@ -517,6 +518,10 @@ class HalaRenderer:
         graphs) or LAUNCH_EAGER (every kernel launched); same images."""
         _check(self._lib.hpt_set_launch_mode(self._h, mode))
 
+    def set_vmis_phases(self, on: bool = True) -> None:
+        """Phased (default) or single-kernel volume-MIS shading; same images."""
+        _check(self._lib.hpt_set_vmis_phases(self._h, int(bool(on))))
+
     def set_shade_sort(self, first_bounce: int = SHADE_SORT_OFF) -> None:
         """Group each bounce's paths by hit material before shading, from
         bounce `first_bounce` on (SHADE_SORT_OFF: never); same images."""
@@ -586,6 +591,10 @@ class HalaGroup:
 
     def set_launch_mode(self, mode: int) -> None:
         _check(self._lib.hpt_group_set_launch_mode(self._h, mode))
+
+    def set_vmis_phases(self, on: bool = True) -> None:
+        """Phased (default) or single-kernel volume-MIS shading; same images."""
+        _check(self._lib.hpt_set_vmis_phases(self._h, int(bool(on))))
 
     def set_shade_sort(self, first_bounce: int = SHADE_SORT_OFF) -> None:
         _check(self._lib.hpt_group_set_shade_sort(self._h, first_bounce))

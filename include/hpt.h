@@ -349,6 +349,16 @@ HPT_API hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode);
 #define HPT_SHADE_SORT_OFF 0xFFFFFFFFu
 HPT_API hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce);
 
+/* Volume-MIS shading (MEDIUM|VOL_MIS, inc/lighting.glsl:7-61) in phases (no
+ * counterpart in the reference, where the walks are traceRayEXT calls inside
+ * the raygen shader).  on = 1, the default: each bounce's shading runs as a
+ * kernel up to the first transmittance walk, a kernel of the queued walks,
+ * (for environment maps a second shading kernel and walk kernel,) and a
+ * kernel that finishes the bounce, so no shading state is live during a
+ * traversal; on = 0: one shading kernel with the walks inline.  The images
+ * are the same bit for bit.  Other permutations ignore it. */
+HPT_API hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on);
+
 /* Several jobs in one wavefront (no counterpart in the reference, which
  * renders one scene per process).  A group owns a path state, queues and a
  * stream of its own; hpt_group_render traces `frames[i]` frames of job
@@ -518,7 +528,8 @@ enum {
   HPT_STAGE_SHADOW = 3,  /* any-hit traversal of the shadow queue */
   HPT_STAGE_ACCUMULATE = 4,
   HPT_STAGE_SORT = 5,    /* material sort of the shading queue (hpt_set_shade_sort) */
-  HPT_STAGE_COUNT = 6
+  HPT_STAGE_WALK = 6,    /* transmittance walks of the phased volume-MIS shading */
+  HPT_STAGE_COUNT = 7
 };
 typedef struct hpt_stage_stats {
   uint64_t launches;
@@ -529,8 +540,10 @@ typedef struct hpt_stage_stats {
                            those of the volume-MIS transmittance walks */
   uint64_t tri_tests;   /* with HPT_PROFILE_COUNT_VISITS (likewise) */
   uint64_t queries;     /* with HPT_PROFILE_COUNT_VISITS: closest-hit queries
-                           of HPT_STAGE_SHADE's transmittance walks
-                           (inc/lighting.glsl:19-32); 0 for other stages */
+                           of the transmittance walks (inc/lighting.glsl:19-32)
+                           -- HPT_STAGE_SHADE's inline walks, or HPT_STAGE_WALK's
+                           when shading is phased; 0 for other stages.  For
+                           HPT_STAGE_WALK, rays = walks */
 } hpt_stage_stats;
 
 HPT_API hpt_status hpt_set_profiling(hpt_renderer* r, int enable);

@@ -594,3 +594,41 @@ def test_shade_sort_gives_the_same_images(case):
         out = run(sort_from, mode)
         for k, (a, b) in enumerate(zip(base, out)):
             assert np.array_equal(a, b), (sort_from, mode, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["transparent_sky", "glass_sky", "transparent_env_map", "absorb_env_map"])
+@pytest.mark.parametrize("features", [hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS | hpt.FEATURE_TRANSPARENT,
+                                      hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS], ids=["transparent", "opaque"])
+def test_vmis_phases_give_the_same_images(case, features):
+    """Volume-MIS shading in phases (shading kernels cut at the queued
+    transmittance walks, hpt_set_vmis_phases) gives the single kernel's
+    images bit for bit: sky (light walks only) and environment maps (an
+    environment walk, then a light walk), scattering and absorbing media,
+    transparent and glass boundaries, eager and graph, sorted and not."""
+    boundary = "glass" if case.startswith("glass") else "transparent"
+    sc = hpt_scenes.displaced_icosphere_lights_medium(level=3, width=96, height=64, boundary=boundary)
+    if case.startswith("absorb"):
+        sc.materials[2]["medium_type"] = hpt.MEDIUM_ABSORB
+        sc.materials[2]["medium_color"] = (0.2, 0.5, 0.8)
+    if case.endswith("env_map"):
+        sc.env_rgb = hpt_scenes.sky_env_map(64, 32, seed=2, sun_radiance=200.0, sun_diameter_deg=8.0)
+    print(f"[gpu] vmis phases {case} features {features}", flush=True)
+
+    def run(phases, mode, sort):
+        r = hpt.HalaRenderer(96, 64, max_depth=8, rr_depth=3, features=features)
+        r.set_scene(sc)
+        r.frames_in_flight = 4
+        r.commit()
+        r.set_launch_mode(mode)
+        r.set_shade_sort(sort)
+        r.set_vmis_phases(phases)
+        r.render(8)
+        out = [r.read_image(im) for im in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_NORMAL)]
+        r.close()
+        return out
+
+    base = run(False, hpt.LAUNCH_EAGER, hpt.SHADE_SORT_OFF)
+    for mode, sort in ((hpt.LAUNCH_EAGER, hpt.SHADE_SORT_OFF), (hpt.LAUNCH_GRAPH, 1)):
+        for k, (a, b) in enumerate(zip(base, run(True, mode, sort))):
+            assert np.array_equal(a, b), (mode, sort, k)
