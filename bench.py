@@ -598,14 +598,17 @@ def main():
     r.render(steps)
     r.wait_idle()
     profiled_wall = time.perf_counter() - t0
-    stats = {s: r.stage_stats(s) for s in range(hpt.STAGE_WALK + 1)}
+    stats = {s: r.stage_stats(s) for s in range(hpt.STAGE_TAIL + 1)}
     ext_t, shade_t, shadow_t = stats[hpt.STAGE_EXTEND], stats[hpt.STAGE_SHADE], stats[hpt.STAGE_SHADOW]
     samples = W * H * steps
     msps = samples / wall / 1e6
     batch_frames = min(steps, fif)
     walk_t = stats[hpt.STAGE_WALK]
-    rays_total = ext_t["rays"] + shadow_t["rays"] + int(round(shade_t["rays"] * walk_queries))
-    names = ["camera", "extend", "shade", "shadow", "accum", "sort", "walk"]
+    # rays: closest-hit and shadow queries, the walks' closest-hit queries,
+    # and the tail's closest-hit queries (one per bounce it traced)
+    rays_total = (ext_t["rays"] + shadow_t["rays"] + int(round(shade_t["rays"] * walk_queries)) +
+                  stats[hpt.STAGE_TAIL]["queries"])
+    names = ["camera", "extend", "shade", "shadow", "accum", "sort", "walk", "tail"]
     log("stage totals: " + ", ".join(f"{n} {stats[s]['total_ms']:.1f} ms/{stats[s]['launches']}"
                                        for s, n in enumerate(names)))
     log(f"wall {wall:.3f} s ({args.launch}, {graph_launches} graph replays), {msps:.1f} Msamples/s, "

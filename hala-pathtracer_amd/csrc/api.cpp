@@ -125,6 +125,7 @@ struct hpt_renderer {
   std::shared_ptr<SceneDevice> sd;
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_stats;
+  DevBuf d_tail_cut;  // paths cut at kMaxBounces since hpt_create (one device word)
   // Phased volume-MIS shading (MEDIUM|VOL_MIS): the per-path NEE state
   // between its kernels and the queue of its scattering events; off = the
   // single shading kernel with inline walks.
@@ -176,6 +177,7 @@ struct hpt_group {
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
   DevBuf d_nee_dir, d_nee_li, d_nee_ld, d_nee_tr, d_qc;  // phased volume-MIS shading, sized like the paths
+  DevBuf d_tail_cut;  // paths cut at kMaxBounces since hpt_group_create (one device word)
   // The last captured step and the key of the step before this one: a step
   // repeated with the same jobs, frames and buffers is captured and replayed.
   hipGraphExec_t gexec = nullptr;
@@ -413,7 +415,12 @@ struct WaveBuffers {
   Spill sp;
   uint8_t* sort_keys;
   uint32_t* sort_items;
+  unsigned long long* tail_cut;  // paths the tail cut at kMaxBounces (one device word)
 };
+
+// Bounces (closest-hit queries) a path may take; the tail cuts a path that
+// passes through transparent surfaces beyond this and counts it.
+constexpr uint32_t kMaxBounces = 4096;
 
 // Camera paths of every job, the bounce loop over all n paths at once
 // (extend, shade with shade_fp, shadow), then every job's accumulation.
@@ -444,6 +451,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   const StageCounters sc_shadow{stats ? stats + kStatsPerStage * HPT_STAGE_SHADOW : nullptr, count};
   const StageCounters sc_shade{stats ? stats + kStatsPerStage * HPT_STAGE_SHADE : nullptr, count};
   const StageCounters sc_walk{stats ? stats + kStatsPerStage * HPT_STAGE_WALK : nullptr, count};
+  const StageCounters sc_tail{stats ? stats + kStatsPerStage * HPT_STAGE_TAIL : nullptr, count};
 
   begin(HPT_STAGE_CAMERA);
   for (size_t j = 0; j < jobs.size(); ++j)
@@ -463,20 +471,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   const Queue qnee{wb.qc, phase_counts, g.seg_cap};
   const ShadowQueue wq1{wb.sh_o + g.seg_cap, wb.sh_d + g.seg_cap, wb.sh_c + g.seg_cap,
                         phased ? phase_counts + kSeg * kSegStride : nullptr, 2 * g.seg_cap};
-  std::vector<uint32_t> seg_counts;
-  for (uint32_t b = 0;; ++b) {
-    if (b >= fixed_bounces) {
-      // Pass-through of transparent surfaces does not count toward depth
-      // (pathtracer.rgen.glsl:199): keep going while paths remain.
-      if (!transparent) break;
-      seg_counts.resize(kSeg * kSegStride);
-      check(hipMemcpyAsync(seg_counts.data(), qin.counts, seg_counts.size() * 4, hipMemcpyDeviceToHost, s),
-            "queue count");
-      check(hipStreamSynchronize(s), "sync");
-      uint64_t live = 0;
-      for (uint32_t i = 0; i < kSeg; ++i) live += seg_counts[i * kSegStride];
-      if (live == 0 || b > 4096) break;
-    }
+  for (uint32_t b = 0; b < fixed_bounces; ++b) {
     const Queue& qout = qbuf[out];
     const bool sort = b >= sort_from && ss.keys && ss.items;
     begin(HPT_STAGE_EXTEND);
@@ -521,6 +516,16 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     qin = qout;
     out ^= 1;
   }
+  if (transparent && fixed_bounces > 0) {
+    // Pass-throughs of transparent surfaces do not count toward depth
+    // (pathtracer.rgen.glsl:196-200): the paths still queued run to their
+    // end in the tail kernel, on the device, with no host round trip.
+    begin(HPT_STAGE_TAIL);
+    launch_tail(s, features, sv, shade_fp, ps, qin, sp, fixed_bounces < kMaxBounces ? kMaxBounces - fixed_bounces : 0u,
+                wb.tail_cut, sc_tail);
+    end();
+    check(hipGetLastError(), "tail launch");
+  }
   begin(HPT_STAGE_ACCUMULATE);
   for (const WaveJob& j : jobs) launch_accumulate(s, j.fp, path_slice(ps, j.base), j.im, j.fp.pixels);
   end();
@@ -553,6 +558,7 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   wb.sp = ensure_spill(r, 2 * kSeg * seg_geom(n).seg_cap);  // threads of the widest launch (shadow)
   wb.sort_keys = r->d_sort_keys.as<uint8_t>();
   wb.sort_items = r->d_sort_items.as<uint32_t>();
+  wb.tail_cut = r->d_tail_cut.as<unsigned long long>();
   wb.qc = r->vmis_phases ? r->d_qc.as<uint32_t>() : nullptr;
   const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
   trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
@@ -577,7 +583,7 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, path_state(r));
   for (const DevBuf* b : {&r->d_qa, &r->d_qb, &r->d_sh_o, &r->d_sh_d, &r->d_sh_c, &r->d_counts, &r->d_spill,
                           &r->d_accum, &r->d_albedo, &r->d_normal, &r->d_final, &r->d_frame, &r->d_sort_keys,
-                          &r->d_sort_items, &r->d_qc})
+                          &r->d_sort_items, &r->d_qc, &r->d_tail_cut})
     key_append(k, b->p);
   key_append(k, r->vmis_phases);
   key_append(k, frames);
@@ -688,6 +694,10 @@ void alloc_paths(hpt_renderer* r, uint32_t frames) {
     for (DevBuf* b : {&r->d_nee_dir, &r->d_nee_li, &r->d_nee_ld, &r->d_nee_tr, &r->d_qc}) b->release();
   }
   alloc_sort(r, frames);
+  if (!r->d_tail_cut.p) {
+    r->d_tail_cut.alloc(256);
+    check(hipMemsetAsync(r->d_tail_cut.p, 0, r->d_tail_cut.bytes, r->stream), "hipMemsetAsync");
+  }
   r->batch_cap = frames;
 }
 
@@ -800,6 +810,10 @@ void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth, bool vmis) {
     check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
   }
   if (!g->d_limits.p) g->d_limits.alloc(kMaxWaveJobs * sizeof(uint2));
+  if (!g->d_tail_cut.p) {
+    g->d_tail_cut.alloc(256);
+    check(hipMemsetAsync(g->d_tail_cut.p, 0, g->d_tail_cut.bytes, g->stream), "hipMemsetAsync");
+  }
 }
 
 // Environment and light settings shading reads once per launch: equal for
@@ -1239,8 +1253,7 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
     if (r->info.max_samples)
       left = r->frame_index >= r->info.max_samples ? 0 : std::min<uint64_t>(left, r->info.max_samples - r->frame_index);
     left = std::min<uint64_t>(left, 0xFFFFFFFFull - r->frame_index);  // frame indices stay 32-bit
-    const bool graph = r->launch_mode == HPT_LAUNCH_GRAPH && !r->profiling &&
-                       !(r->info.features & HPT_FEATURE_TRANSPARENT);
+    const bool graph = r->launch_mode == HPT_LAUNCH_GRAPH && !r->profiling;
     while (left) {
       const uint32_t k = (uint32_t)std::min<uint64_t>(left, cap);
       if (graph && k == cap) {
@@ -1253,6 +1266,19 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
       left -= k;
     }
     if (r->profiling & 1) collect_timers(r);
+  });
+}
+
+hpt_status hpt_get_truncated_paths(hpt_renderer* r, uint64_t* paths) {
+  return guarded([&] {
+    need(r && paths, "null argument");
+    *paths = 0;
+    if (!r->d_tail_cut.p) return;
+    set_device(r);
+    unsigned long long v = 0;
+    check(hipMemcpyAsync(&v, r->d_tail_cut.p, sizeof(v), hipMemcpyDeviceToHost, r->stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+    *paths = v;
   });
 }
 
@@ -1843,7 +1869,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     }
     check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
 
-    const bool graph = g->launch_mode == HPT_LAUNCH_GRAPH && !(r0->info.features & HPT_FEATURE_TRANSPARENT);
+    const bool graph = g->launch_mode == HPT_LAUNCH_GRAPH;
     std::vector<WaveJob> wj;
     std::vector<hpt_renderer*> live;
     JobLimits lim{};
@@ -1888,13 +1914,15 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     if (!g->d_spill.p) wb.sp = Spill{nullptr, 0};
     wb.sort_keys = g->d_sort_keys.as<uint8_t>();
     wb.sort_items = g->d_sort_items.as<uint32_t>();
+    wb.tail_cut = g->d_tail_cut.as<unsigned long long>();
     wb.qc = g->d_qc.as<uint32_t>();
     key_append(key, sv);
     key_append(key, shade_fp);
     key_append(key, wb.ps);
     for (const void* q : {(const void*)wb.qa, (const void*)wb.qb, (const void*)wb.counts, (const void*)wb.sh_o,
                           (const void*)wb.sh_d, (const void*)wb.sh_c, (const void*)wb.sp.buf,
-                          (const void*)wb.sort_keys, (const void*)wb.sort_items, (const void*)wb.qc})
+                          (const void*)wb.sort_keys, (const void*)wb.sort_items, (const void*)wb.qc,
+                          (const void*)wb.tail_cut})
       key_append(key, q);
     key_append(key, wb.sp.stride);
     key_append(key, n);
@@ -1969,11 +1997,25 @@ hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_la
     for (const DevBuf* d : {&g->d_rng, &g->d_org, &g->d_dir, &g->d_thr, &g->d_rad, &g->d_alb, &g->d_nrm, &g->d_med,
                             &g->d_med2, &g->d_nee_e, &g->d_nee_l, &g->d_hit, &g->d_job, &g->d_qa, &g->d_qb,
                             &g->d_sh_o, &g->d_sh_d, &g->d_sh_c, &g->d_counts, &g->d_spill, &g->d_sort_keys,
-                            &g->d_sort_items, &g->d_limits})
+                            &g->d_sort_items, &g->d_limits, &g->d_nee_dir, &g->d_nee_li, &g->d_nee_ld,
+                            &g->d_nee_tr, &g->d_qc, &g->d_tail_cut})
       b += d->bytes;
     *steps = g->steps;
     *graph_launches = g->graph_launches;
     *device_bytes = b;
+  });
+}
+
+hpt_status hpt_group_get_truncated_paths(hpt_group* g, uint64_t* paths) {
+  return guarded([&] {
+    need(g && paths, "null argument");
+    *paths = 0;
+    if (!g->d_tail_cut.p) return;
+    check(hipSetDevice(g->device), "hipSetDevice");
+    unsigned long long v = 0;
+    check(hipMemcpyAsync(&v, g->d_tail_cut.p, sizeof(v), hipMemcpyDeviceToHost, g->stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
+    *paths = v;
   });
 }
 

@@ -163,6 +163,12 @@ void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const Sh
 void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp,
                   const PathState& ps, const Queue& qin, uint32_t n_max, const Queue& qout, const ShadowQueue& sq,
                   Spill sp, StageCounters sc);
+// The paths left in qin after the wavefront's bounces, each traced to its
+// end by one lane (k_tail): at most kTailChunks blocks per segment, a path cut
+// after max_steps bounces and counted in *cut.
+constexpr uint32_t kTailChunks = 16;
+void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc);
 // Volume-MIS shading in phases (MEDIUM|VOL_MIS): A over qin (finished and

@@ -574,24 +574,20 @@ __device__ f3 direct_lighting(const SceneView& sv, const FrameParams& fp, uint32
 #ifndef HPT_SHADE_VMIS_WAVES_PER_EU
 #define HPT_SHADE_VMIS_WAVES_PER_EU 5
 #endif
-// stats: null, or the shading stage's counters (shading events as rays; with
-// COUNT the volume-MIS walks' queries, node visits and triangle tests).
+// The bounce of path p from its closest-hit record ps.hit[p]: radiance,
+// AOVs and the next segment written to ps; returns whether the path lives
+// on.  Without VOL_MIS the shadow rays come back in s_env / s_light (valid
+// set when one was made) for the caller to trace; with VOL_MIS the
+// transmittance walks run here (st: the walks' traversal stack, wc: their
+// counts with COUNT).
 template <bool MED, bool VMIS, bool TRANSP, bool JOBS, bool COUNT>
-__global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(VMIS ? HPT_SHADE_VMIS_WAVES_PER_EU : HPT_SHADE_WAVES_PER_EU)))
-    k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
-            Spill sp, unsigned long long* stats) {
-  __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t p;
-  const bool active = queue_entry(qin, n_max, stats, p);
-  WalkCounts wc;
+__device__ __forceinline__ bool shade_path(const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                                           uint32_t p, LaneStack& st, WalkCounts& wc, ShadowOut& s_env,
+                                           ShadowOut& s_light) {
   bool cont = false;
-  ShadowOut s_env, s_light;
   s_env.valid = false;
   s_light.valid = false;
-  if (active) {
-    LaneStack st{lds_stack + (VMIS ? threadIdx.x : 0), kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  {
     const float4 o4 = ps.org[p], d4 = ps.dir[p], t4 = ps.thr[p], h4 = ps.hit[p];
     f3 org = mk3(o4.x, o4.y, o4.z), dir = mk3(d4.x, d4.y, d4.z), T = mk3(t4.x, t4.y, t4.z);
     float pdf = o4.w;
@@ -752,6 +748,29 @@ __global__ void __launch_bounds__(kBlock)
     }
     ps.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
   }
+  return cont;
+}
+
+// stats: null, or the shading stage's counters (shading events as rays; with
+// COUNT the volume-MIS walks' queries, node visits and triangle tests).
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS, bool COUNT>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(VMIS ? HPT_SHADE_VMIS_WAVES_PER_EU : HPT_SHADE_WAVES_PER_EU)))
+    k_shade(SceneView sv, FrameParams fp, PathState ps, Queue qin, uint32_t n_max, Queue qout, ShadowQueue sq,
+            Spill sp, unsigned long long* stats) {
+  __shared__ uint2 lds_stack[VMIS ? kShortStack * kBlock : 1];
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t p;
+  const bool active = queue_entry(qin, n_max, stats, p);
+  WalkCounts wc;
+  bool cont = false;
+  ShadowOut s_env, s_light;
+  s_env.valid = false;
+  s_light.valid = false;
+  if (active) {
+    LaneStack st{lds_stack + (VMIS ? threadIdx.x : 0), kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+    cont = shade_path<MED, VMIS, TRANSP, JOBS, COUNT>(sv, fp, ps, p, st, wc, s_env, s_light);
+  }
   if (COUNT) add_walks(stats, wc);
   // Compact the survivors and the shadow rays.
   seg_append(cont, qout, p);
@@ -769,6 +788,69 @@ __global__ void __launch_bounds__(kBlock)
       sq.d[i1] = make_float4(s_light.d.x, s_light.d.y, s_light.d.z, __uint_as_float((p << 1) | 1u));
       sq.c[i1] = make_float4(s_light.c.x, s_light.c.y, s_light.c.z, 0.0f);
     }
+  }
+}
+
+// ------------------------------------------------------------------- tail
+// Pass-throughs of transparent surfaces do not count toward a path's depth
+// (pathtracer.rgen.glsl:196-200), so with TRANSPARENT paths can outlive the
+// max_depth + 1 bounces of the wavefront.  The tail takes the paths still in
+// the queue after those bounces and runs each to its end on one lane --
+// closest hit (as extend), the bounce (shade_path), the shadow rays (as
+// shadow) -- with the same operations in the same order as the wavefront,
+// so the images are the same; no host round trip decides how many bounces
+// are left, which lets the whole batch be captured in one graph.  The grid
+// strides over the queue: block b takes chunks b / kSeg, b / kSeg + G, ...
+// (G = gridDim.x / kSeg) of segment b % kSeg.  A path is cut after
+// max_steps tail bounces; cut paths keep the radiance gathered so far and
+// are counted in *cut.
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
+__global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, PathState ps, Queue qin, Spill sp,
+                                                 uint32_t max_steps, unsigned long long* cut,
+                                                 unsigned long long* stats) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t seg = blockIdx.x % kSeg, n = qin.counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)n);
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  const uint32_t stride = (gridDim.x / kSeg) * kBlock;
+  WalkCounts wc;
+  for (uint32_t j = (blockIdx.x / kSeg) * kBlock + threadIdx.x; j < n; j += stride) {
+    const uint32_t p = qin.items[seg * qin.seg_cap + j];
+    for (uint32_t step = 0;; ++step) {
+      if (step == max_steps) {
+        atomicAdd(cut, 1ull);
+        break;
+      }
+      const float4 o4 = ps.org[p], d4 = ps.dir[p];
+      const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st);
+      ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+      wc.q++;
+      ShadowOut s_env, s_light;
+      const bool cont = shade_path<MED, VMIS, TRANSP, JOBS, false>(sv, fp, ps, p, st, wc, s_env, s_light);
+      if (!VMIS) {
+        if (s_env.valid && !trace_any(sv, s_env.o, s_env.d, 0.0f, s_env.tmax, st)) {
+          float4 a = ps.nee_env[p];
+          a.x += s_env.c.x;
+          a.y += s_env.c.y;
+          a.z += s_env.c.z;
+          ps.nee_env[p] = a;
+        }
+        if (s_light.valid && !trace_any(sv, s_light.o, s_light.d, 0.0f, s_light.tmax, st)) {
+          float4 a = ps.nee_light[p];
+          a.x += s_light.c.x;
+          a.y += s_light.c.y;
+          a.z += s_light.c.z;
+          ps.nee_light[p] = a;
+        }
+      }
+      if (!cont) break;
+    }
+  }
+  if (stats) {  // bounces traced: the stage's query count
+    unsigned long long q = wc.q;
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
+    if (lane_id() == 0) atomicAdd(stats + (3 * kSeg + seg) * kStatStride, q);
   }
 }
 
@@ -1508,6 +1590,32 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
     else HPT_SHADE(true, true, false);
   }
 #undef HPT_SHADE
+}
+
+void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc) {
+  const uint32_t chunks = (qin.seg_cap + kBlock - 1) / kBlock;
+  const dim3 grid(kSeg * (chunks < kTailChunks ? chunks : kTailChunks));
+  const bool med = features & HPT_FEATURE_MEDIUM;
+  const bool vmis = med && (features & HPT_FEATURE_VOL_MIS);
+  const bool tr = features & HPT_FEATURE_TRANSPARENT;
+#define HPT_T(M, V, T, J) \
+  hipLaunchKernelGGL((k_tail<M, V, T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, sp, max_steps, cut, sc.stats)
+#define HPT_TJ(M, V, T)            \
+  if (fp.job_limits) HPT_T(M, V, T, true); \
+  else HPT_T(M, V, T, false)
+  if (!med) {
+    if (tr) HPT_TJ(false, false, true);
+    else HPT_TJ(false, false, false);
+  } else if (!vmis) {
+    if (tr) HPT_TJ(true, false, true);
+    else HPT_TJ(true, false, false);
+  } else {
+    if (tr) HPT_TJ(true, true, true);
+    else HPT_TJ(true, true, false);
+  }
+#undef HPT_TJ
+#undef HPT_T
 }
 
 void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,

@@ -187,6 +187,8 @@ extern "C" {
     pub fn hpt_get_frames_in_flight(r: *mut hpt_renderer, frames: *mut u32) -> hpt_status;
     pub fn hpt_set_launch_mode(r: *mut hpt_renderer, mode: u32) -> hpt_status;
     pub fn hpt_set_shade_sort(r: *mut hpt_renderer, first_bounce: u32) -> hpt_status;
+    pub fn hpt_set_vmis_phases(r: *mut hpt_renderer, on: u32) -> hpt_status;
+    pub fn hpt_get_truncated_paths(r: *mut hpt_renderer, paths: *mut u64) -> hpt_status;
     pub fn hpt_group_create(device: i32, out: *mut *mut hpt_group) -> hpt_status;
     pub fn hpt_group_render(g: *mut hpt_group, jobs: *const *mut hpt_renderer, frames: *const u32,
         camera_indices: *const u32, count: u32) -> hpt_status;
@@ -194,6 +196,7 @@ extern "C" {
     pub fn hpt_group_set_shade_sort(g: *mut hpt_group, first_bounce: u32) -> hpt_status;
     pub fn hpt_group_get_stats(g: *mut hpt_group, steps: *mut u64, graph_launches: *mut u64,
         device_bytes: *mut u64) -> hpt_status;
+    pub fn hpt_group_get_truncated_paths(g: *mut hpt_group, paths: *mut u64) -> hpt_status;
     pub fn hpt_group_wait_idle(g: *mut hpt_group) -> hpt_status;
     pub fn hpt_group_destroy(g: *mut hpt_group);
     pub fn hpt_set_bvh_builder(r: *mut hpt_renderer, builder: u32) -> hpt_status;

@@ -33,7 +33,7 @@ MATERIAL_DIFFUSE, MATERIAL_DISNEY = 0, 1
 FEATURE_MEDIUM, FEATURE_VOL_MIS, FEATURE_TRANSPARENT = 1, 2, 4
 IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
 HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
-STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT, STAGE_WALK = range(7)
+STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT, STAGE_WALK, STAGE_TAIL = range(8)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 SHADE_SORT_OFF = 0xFFFFFFFF
@@ -195,6 +195,8 @@ SIGNATURES = {
     "hpt_set_bvh_builder": [C.c_void_p, C.c_uint32],
     "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "hpt_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "hpt_group_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_present_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
     "hpt_present_frame": [C.c_void_p],
     "hpt_present_acquire": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)],
@@ -533,6 +535,13 @@ class HalaRenderer:
         _check(self._lib.hpt_get_launch_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def truncated_paths(self) -> int:
+        """Paths stopped at the 4096-bounce limit since the handle was made
+        (TRANSPARENT pass-throughs do not count toward max_depth)."""
+        v = C.c_uint64()
+        _check(self._lib.hpt_get_truncated_paths(self._h, C.byref(v)))
+        return v.value
+
     def set_profiling(self, flags: int) -> None:
         _check(self._lib.hpt_set_profiling(self._h, flags))
 
@@ -603,6 +612,12 @@ class HalaGroup:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(self._lib.hpt_group_get_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return {"steps": a.value, "graph_launches": b.value, "device_bytes": c.value}
+
+    def truncated_paths(self) -> int:
+        """Paths of this group's steps stopped at the 4096-bounce limit."""
+        v = C.c_uint64()
+        _check(self._lib.hpt_group_get_truncated_paths(self._h, C.byref(v)))
+        return v.value
 
     def wait_idle(self) -> None:
         _check(self._lib.hpt_group_wait_idle(self._h))

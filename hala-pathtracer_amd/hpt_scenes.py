@@ -456,6 +456,42 @@ def displaced_icosphere_lights_medium(level=8, seed=0, amplitude=0.1, width=1920
 MEDIUM_SCENE_FEATURES = hpt.FEATURE_MEDIUM | hpt.FEATURE_VOL_MIS | hpt.FEATURE_TRANSPARENT
 
 
+def transparent_layers(layers=8):
+    """`layers` parallel quads above a Lambert floor (albedo 0.7), alternately
+    opacity 0 and 0.5, the odd ones carrying an absorbing medium (density
+    0.8) that paths enter on their way up (normals face the floor); a point
+    light above them, a directional light, sky.  The camera looks under the
+    layers at the floor: a camera ray that hits a layer first ends there
+    (its pass-through wraps depth 0, pathtracer.rgen.glsl:199,247), but paths
+    that bounce off the floor pass through up to `layers` quads, and with
+    TRANSPARENT each pass is a bounce that does not count toward depth
+    (pathtracer.rgen.glsl:196-200)."""
+    mats = hpt.new_materials(3)
+    diffuse_material(mats[0], (0.7, 0.7, 0.7))
+    diffuse_material(mats[1], (0.9, 0.4, 0.3))
+    mats[1]["opacity"] = 0.0
+    diffuse_material(mats[2], (0.3, 0.6, 0.9))
+    mats[2]["opacity"] = 0.5
+    mats[2]["medium_type"] = hpt.MEDIUM_ABSORB
+    mats[2]["medium_density"] = 0.8
+    mats[2]["medium_color"] = (0.2, 0.5, 0.7)
+    q, qi = quad_mesh((-4, -4, 0), (8, 0, 0), (0, 8, 0))
+    prims = [hpt.Primitive(q, qi, 0)]
+    for i in range(layers):
+        q, qi = quad_mesh((-1.5, -1.5, 0.3 + 0.25 * i), (0, 3, 0), (3, 0, 0))
+        prims.append(hpt.Primitive(q, qi, 1 + (i & 1)))
+    lights = np.zeros(2, hpt.LIGHT_DTYPE)
+    lights[0]["type"] = hpt.LIGHT_POINT
+    lights[0]["position"] = (0.6, -0.4, 4.5)
+    lights[0]["intensity"] = (9.0, 8.0, 7.0)
+    lights[1]["type"] = hpt.LIGHT_DIRECTIONAL
+    dd = np.array([0.2, 0.3, -0.93])
+    lights[1]["u"] = dd / np.linalg.norm(dd)
+    lights[1]["intensity"] = (0.8, 0.8, 0.9)
+    cam = look_at_camera((0.0, -5.0, 0.2), (0.0, 0.0, 0.0), yfov_deg=40.0)
+    return hpt.Scene(cam, lights, mats, prims, ground_color=(0.1, 0.1, 0.1), sky_color=(0.3, 0.35, 0.45))
+
+
 def triangle_soup(count, seed=0):
     """`count` triangles: v0 uniform in the unit cube, v1 and v2 at
     v0 + lognormal(-4, 0.5) * random unit vector; Lambert 0.5."""

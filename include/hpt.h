@@ -389,6 +389,16 @@ HPT_API void hpt_group_destroy(hpt_group* g);
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
 
+/* Paths stopped at the bounce limit since the handle was made.  With
+ * TRANSPARENT a pass-through does not count toward max_depth
+ * (pathtracer.rgen.glsl:196-200), so the number of bounces a path takes has
+ * no fixed bound; the library stops a path after 4096 bounces (closest-hit
+ * queries), keeps the radiance it gathered and counts it here.  A non-zero
+ * count means the images of those frames are biased.  Waits for the
+ * handle's (the group's) queued work. */
+HPT_API hpt_status hpt_get_truncated_paths(hpt_renderer* r, uint64_t* paths);
+HPT_API hpt_status hpt_group_get_truncated_paths(hpt_group* g, uint64_t* paths);
+
 /* Sets the frame counter (the UBO frame_index) the next frame renders with.
  * The RNG is a pure function of (pixel, frame) (common/wltcRS.glsl:9-19), so
  * a job resumes from its accumulated images plus this counter. */
@@ -529,7 +539,10 @@ enum {
   HPT_STAGE_ACCUMULATE = 4,
   HPT_STAGE_SORT = 5,    /* material sort of the shading queue (hpt_set_shade_sort) */
   HPT_STAGE_WALK = 6,    /* transmittance walks of the phased volume-MIS shading */
-  HPT_STAGE_COUNT = 7
+  HPT_STAGE_TAIL = 7,    /* TRANSPARENT: the paths left after max_depth + 1 bounces,
+                            each run to its end by one lane (rays = such paths,
+                            queries = the bounces they took) */
+  HPT_STAGE_COUNT = 8
 };
 typedef struct hpt_stage_stats {
   uint64_t launches;

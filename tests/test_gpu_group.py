@@ -67,8 +67,9 @@ def test_group_steps_equal_solo_renders(case, mode, sort):
     stats = g.stats()
     print(f"[gpu] group stats {stats}", flush=True)
     assert stats["steps"] == len(PLAN)
-    if mode == hpt.LAUNCH_GRAPH and not features & hpt.FEATURE_TRANSPARENT:
+    if mode == hpt.LAUNCH_GRAPH:  # TRANSPARENT too: its tail runs on the device
         assert stats["graph_launches"] == 3  # steps 1, 4 and 5 repeat their predecessor
+    assert g.truncated_paths() == 0
     for i, spec in enumerate(SPECS):
         assert jobs[i].frame_index == totals[i]
         solo = _job(t, spec, features)
