@@ -161,6 +161,8 @@ def time_to_relmse(make_renderer, ref, t_ref, ref_spp, threshold, chunk, max_spp
     frames against the reference `ref`; the render time until the estimate's
     relMSE first meets `threshold` (relMSE evaluation excluded)."""
     r = make_renderer()
+    r.render(chunk)  # warm-up: graph capture
+    r.reset_accumulation()
     curve, t_render, spp, hit = [], 0.0, 0, None
     while spp < max_spp:
         r.wait_idle()
@@ -199,13 +201,14 @@ def render_reference(make_renderer, ref_spp):
 
 def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
     """The denoised estimate at 2, 4, 8, ... spp (progressive accumulation
-    from frame 0): time = render time up to that sample count + one denoiser
-    pass (on the renderer's resident images, same GPU); the first sample
-    count whose denoised image meets `threshold` (relMSE evaluation and the
-    passes of earlier sample counts excluded)."""
+    from frame 0, 2-frame batches): time = render time up to that sample
+    count + one denoiser pass (on the renderer's resident images, same GPU);
+    the first sample count whose denoised image meets `threshold` (relMSE
+    evaluation and the passes of earlier sample counts excluded)."""
     import denoiser
 
-    r = make_renderer()
+    r = make_renderer(2)
+    r.render(2)  # warm-up: graph capture
     curve, t_render, spp, hit = [], 0.0, 0, None
     out = denoiser.DeviceArray(r.height * r.width * 4, np.float32, (r.height, r.width, 4))
     net.denoise_renderer(r, out)  # allocate the network's buffers outside the timed region
@@ -236,7 +239,7 @@ def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
             "curve_spp_render_s_denoise_ms_plain_denoised": curve}
 
 
-def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort):
+def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort, tails):
     """The same target through the serving path: one job per variant
     submitted to serve.Server -- plain at the plain tracer's threshold sample
     count, denoised (the server denoises finished jobs on the device) at the
@@ -247,11 +250,11 @@ def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_den
     import serve
 
     rs = serve.ResidentScene("scene", scene, features)
-    tuning = autotune.TuningCache()
-    tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort))
     out = {}
     try:
         for label, spp, den in (("plain", spp_plain, None), ("denoised", spp_denoised, net)):
+            tuning = autotune.TuningCache()
+            tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort, tails[label]))
             srv = serve.Server({"scene": rs}, 1 << 38, max_frames=32, keep_images=True, tuning=tuning, denoiser=den)
             srv.run([serve.JobSpec(0, "scene", w, h, min(spp, 8), depth, rr, 0.0)])  # warm-up: allocations, capture
             res = srv.run([serve.JobSpec(0, "scene", w, h, spp, depth, rr, 0.0)])[0]
@@ -484,6 +487,8 @@ def main():
                          "choice among off and bounces 0-2 at this batch size, measured before the timed region)")
     ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
+    ap.add_argument("--tail-from", default="auto",
+                    help="first bounce run by the tail kernel: auto (autotuned), none, or a bounce index")
     ap.add_argument("--vmis-phases", choices=["on", "off"], default="on",
                     help="volume-MIS shading as phase kernels with queued walks (on) or one kernel with inline walks")
     ap.add_argument("--frames-in-flight", type=int, default=0,
@@ -527,17 +532,22 @@ def main():
     scene = wl["scene"]()
     log(f"{scene.triangle_count} triangles; creating renderer")
 
-    def make_renderer():
+    def make_renderer(frames_in_flight=None):
+        """The headline renderer, or one with `frames_in_flight`-frame
+        batches and the tail bounce tuned for that size."""
         rr = hpt.HalaRenderer(W, H, max_depth=DEPTH, rr_depth=RR, features=FEATURES)
         rr.set_scene(scene)
-        rr.frames_in_flight = args.frames_in_flight
+        rr.frames_in_flight = frames_in_flight or args.frames_in_flight
         rr.commit()
         rr.set_launch_mode(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER)
         rr.set_shade_sort(shade_sort)
         rr.set_vmis_phases(args.vmis_phases == "on")
+        rr.set_tail_from(tails["denoised"] if frames_in_flight else tails["plain"])
         return rr
 
     shade_sort = hpt.SHADE_SORT_OFF if args.shade_sort in ("off", "auto") else int(args.shade_sort)
+    fixed_tail = hpt.TAIL_FROM_NONE if args.tail_from in ("none", "auto") else int(args.tail_from)
+    tails = {"plain": fixed_tail, "denoised": fixed_tail}  # big batches / 2-frame batches
     t0 = time.perf_counter()
     r = make_renderer()
     log(f"create + commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
@@ -554,8 +564,12 @@ def main():
         shade_sort = tuned.shade_sort
         sort_table = {k: round(v, 1) for k, v in tuned.table.items()}
         r.set_shade_sort(shade_sort)
-        log(f"shade sort autotuned in {time.perf_counter() - t0:.1f} s: "
-            f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'} ({sort_table})")
+        if args.tail_from == "auto":
+            tails["plain"] = tuned.tail_from
+            r.set_tail_from(tuned.tail_from)
+        log(f"shade sort and tail autotuned in {time.perf_counter() - t0:.1f} s: "
+            f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'}, tail from "
+            f"{'none' if tails['plain'] == hpt.TAIL_FROM_NONE else tails['plain']} ({sort_table})")
 
     # Visit counts per ray (extend), per shading event, and of the volume-MIS
     # transmittance walks (inline in shading, or the walk kernel's when
@@ -653,13 +667,23 @@ def main():
             import denoiser
 
             path = args.weights or denoiser.weights_path()
+            if args.tail_from == "auto":  # the tail bounce for 2-frame batches
+                import autotune
+
+                tmpl = make_renderer()
+                t2 = autotune.tune(tmpl, W, H, DEPTH, RR, FEATURES, samples=8 * W * H, candidates=(2,),
+                                   modes=(hpt.LAUNCH_GRAPH,), sorts=())
+                tmpl.close()
+                tails["denoised"] = t2.tail_from
+                log(f"tail for 2-frame batches: {'none' if t2.tail_from == hpt.TAIL_FROM_NONE else t2.tail_from} "
+                    f"({ {k: round(v, 1) for k, v in t2.table.items()} })")
             den = time_to_relmse_denoised(make_renderer, ref, denoiser.UNet.load(path), args.threshold,
                                           args.ref_spp // 2)
             den["weights"] = os.path.relpath(path, ROOT)
             if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
                 den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
                 conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, denoiser.UNet.load(path),
-                                               conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort)
+                                               conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort, tails)
             conv["denoised"] = den
     serving = None
     if args.serving:
@@ -671,6 +695,7 @@ def main():
               "launch": args.launch,
               "shade_sort": "off" if shade_sort == hpt.SHADE_SORT_OFF else shade_sort,
               "shade_sort_tuning": sort_table,
+              "tail_from": {k: (None if v == hpt.TAIL_FROM_NONE else v) for k, v in tails.items()},
               "vmis_phases": args.vmis_phases if FEATURES & hpt.FEATURE_VOL_MIS else None,
               "eager_profiled_wall_s": round(profiled_wall, 4),
               "stage_ms": {n: round(stats[s]["total_ms"], 3) for s, n in enumerate(names)},

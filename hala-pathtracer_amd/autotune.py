@@ -3,16 +3,18 @@ reference, whose frame loop has no tunables).
 
 What it tunes: the frames traced together per batch (the wavefront and
 queue size, `hpt_set_frames_in_flight`), the launch mode (graph replay or
-eager launches) and the shading sort threshold (the first bounce whose paths
-are grouped by material before shading, `hpt_set_shade_sort`, or never) for
-one (scene, resolution, depth) class.  Larger batches
+eager launches), the shading sort threshold (the first bounce whose paths
+are grouped by material before shading, `hpt_set_shade_sort`, or never) and
+the bounce from which the live paths run to their end in the tail kernel
+(`hpt_set_tail_from`, or none) for one (scene, resolution, depth) class.  Larger batches
 fill the GPU with more paths per launch and amortise the short tail bounces;
 they cost k * W * H * 288 bytes of device memory, which the serving loop
 spends from its admission budget.  The tuner measures, it does not model:
 each candidate renders a warm-up batch and then a timed run on the job's own
 scene, and the smallest batch within `tolerance` of the best throughput
 wins (less memory and a finer progress granularity for the same speed),
-then graph replay, then no sort (it costs 5 bytes per path).
+then graph replay, then no sort (it costs 5 bytes per path); the tail
+bounce, which costs no memory, is the fastest at that choice.
 
 Results are kept in a TuningCache keyed by (scene name, width, height,
 max_depth) and can be saved as JSON.
@@ -30,6 +32,7 @@ PATH_BYTES = 288
 SORT_BYTES = 5  # key byte and sorted entry per queue slot while shading sort is on
 CANDIDATES = (2, 4, 8, 16, 32, 64)
 SORT_CANDIDATES = (0, 1, 2)
+TAIL_CANDIDATES = (3, 4, 5, 6, 8)
 
 
 @dataclass
@@ -40,6 +43,7 @@ class Tuning:
     table: dict = field(default_factory=dict)  # "k/mode[/sN]" -> Msamples/s
     seconds: float = 0.0                       # time the tuning took
     shade_sort: int = hpt.SHADE_SORT_OFF       # first sorted bounce
+    tail_from: int = hpt.TAIL_FROM_NONE        # first bounce run by the tail kernel
 
 
 def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: int | None = None,
@@ -59,20 +63,22 @@ def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: 
 def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, rr_depth: int = 3, features: int = 0,
          samples: float = 32e6, max_frames: int = 64, memory_bytes: int | None = None, tolerance: float = 0.03,
          modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), sorts=SORT_CANDIDATES, candidates=CANDIDATES,
-         device: int = 0) -> Tuning:
+         tails=TAIL_CANDIDATES, device: int = 0) -> Tuning:
     """Measures every candidate batch size (then launch mode, then shading
-    sort threshold) on a handle that shares `template`'s committed scene,
-    each over about `samples` paths (whole batches); returns the choice."""
+    sort threshold, then tail bounce) on a handle that shares `template`'s
+    committed scene, each over about `samples` paths (whole batches); returns
+    the choice."""
     t_start = time.perf_counter()
     r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=rr_depth, features=features, device=device)
     table = {}
     try:
         r.share_scene(template)
 
-        def measure(k, mode, sort=hpt.SHADE_SORT_OFF):
+        def measure(k, mode, sort=hpt.SHADE_SORT_OFF, tail=hpt.TAIL_FROM_NONE):
             r.frames_in_flight = k
             r.set_launch_mode(mode)
             r.set_shade_sort(sort)
+            r.set_tail_from(tail)
             n = max(1, -(-int(samples) // (width * height * k))) * k
             r.reset_accumulation()
             r.render(k)  # warm-up: allocation and graph capture
@@ -83,6 +89,8 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
             key = f"{k}/{'graph' if mode == hpt.LAUNCH_GRAPH else 'eager'}"
             if sort != hpt.SHADE_SORT_OFF:
                 key += f"/s{sort}"
+            if tail != hpt.TAIL_FROM_NONE:
+                key += f"/t{tail}"
             table[key] = width * height * n / (time.perf_counter() - t0) / 1e6
 
         # Batch sizes with the first mode, then the other modes at the best size.
@@ -94,18 +102,28 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
         k_best, mode_best = max(table, key=table.get).split("/")[:2]
         for sort in sorts:
             measure(int(k_best), hpt.LAUNCH_GRAPH if mode_best == "graph" else hpt.LAUNCH_EAGER, sort)
+        best = max(table.values())
+        ok = []
+        for key, v in table.items():
+            parts = key.split("/")
+            if v >= (1 - tolerance) * best and len(parts) <= 3:
+                sort = int(parts[2][1:]) if len(parts) > 2 else hpt.SHADE_SORT_OFF
+                ok.append((int(parts[0]), parts[1], sort, v))
+        k, mode, sort, v = min(ok, key=lambda t: (t[0], t[1] != "graph", t[2] != hpt.SHADE_SORT_OFF, -t[3]))
+        mode = hpt.LAUNCH_GRAPH if mode == "graph" else hpt.LAUNCH_EAGER
+        # The tail bounce at that choice: the fastest, none included.
+        tail = hpt.TAIL_FROM_NONE
+        for tf in tails:
+            if tf > max_depth:
+                continue
+            before = set(table)
+            measure(k, mode, sort, tf)
+            (key,) = set(table) - before
+            if table[key] > v:
+                tail, v = tf, table[key]
     finally:
         r.close()
-    best = max(table.values())
-    ok = []
-    for key, v in table.items():
-        if v >= (1 - tolerance) * best:
-            parts = key.split("/")
-            sort = int(parts[2][1:]) if len(parts) > 2 else hpt.SHADE_SORT_OFF
-            ok.append((int(parts[0]), parts[1], sort, v))
-    k, mode, sort, v = min(ok, key=lambda t: (t[0], t[1] != "graph", t[2] != hpt.SHADE_SORT_OFF, -t[2]))
-    return Tuning(k, hpt.LAUNCH_GRAPH if mode == "graph" else hpt.LAUNCH_EAGER, v, table,
-                  time.perf_counter() - t_start, sort)
+    return Tuning(k, mode, v, table, time.perf_counter() - t_start, sort, tail)
 
 
 class TuningCache:

@@ -135,6 +135,9 @@ struct hpt_renderer {
   // never): one key byte and one sorted entry per queue slot.
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
   DevBuf d_sort_keys, d_sort_items;
+  // Bounces from tail_from on run in the tail kernel (HPT_TAIL_FROM_NONE:
+  // only the paths that outlive max_depth + 1 bounces with TRANSPARENT).
+  uint32_t tail_from = HPT_TAIL_FROM_NONE;
   DevBuf d_accum, d_albedo, d_normal, d_final;
   uint32_t npix = 0;          // width * height
   uint32_t batch_cap = 0;     // frames the path buffers hold
@@ -173,6 +176,7 @@ struct hpt_group {
   hipEvent_t step_end[2] = {nullptr, nullptr};
   uint32_t launch_mode = HPT_LAUNCH_GRAPH;
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
+  uint32_t tail_from = HPT_TAIL_FROM_NONE;
   size_t cap = 0;  // paths the path buffers hold
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
@@ -428,7 +432,7 @@ constexpr uint32_t kMaxBounces = 4096;
 // launches, or null.
 void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, const FrameParams& shade_fp,
                      const WaveBuffers& wb, const std::vector<WaveJob>& jobs, uint32_t n, uint32_t fixed_bounces,
-                     uint32_t sort_from, hpt_renderer* timing) {
+                     uint32_t sort_from, uint32_t tail_from, hpt_renderer* timing) {
   const SegGeom g = seg_geom(n);
   uint32_t* counts = wb.counts;
   // Segment counters: path queues A and B (ping-pong), the shadow queue, the
@@ -471,7 +475,10 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
   const Queue qnee{wb.qc, phase_counts, g.seg_cap};
   const ShadowQueue wq1{wb.sh_o + g.seg_cap, wb.sh_d + g.seg_cap, wb.sh_c + g.seg_cap,
                         phased ? phase_counts + kSeg * kSegStride : nullptr, 2 * g.seg_cap};
-  for (uint32_t b = 0; b < fixed_bounces; ++b) {
+  // Bounces [0, wave_bounces) run as wavefront launches; the paths left then
+  // go to the tail kernel.
+  const uint32_t wave_bounces = std::max<uint32_t>(1u, std::min(fixed_bounces, tail_from));
+  for (uint32_t b = 0; b < wave_bounces; ++b) {
     const Queue& qout = qbuf[out];
     const bool sort = b >= sort_from && ss.keys && ss.items;
     begin(HPT_STAGE_EXTEND);
@@ -516,12 +523,13 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     qin = qout;
     out ^= 1;
   }
-  if (transparent && fixed_bounces > 0) {
-    // Pass-throughs of transparent surfaces do not count toward depth
-    // (pathtracer.rgen.glsl:196-200): the paths still queued run to their
-    // end in the tail kernel, on the device, with no host round trip.
+  if (transparent || wave_bounces < fixed_bounces) {
+    // The paths still queued run to their end in the tail kernel, on the
+    // device, with no host round trip: those past a tail_from bounce, and
+    // with TRANSPARENT those that outlive max_depth + 1 bounces
+    // (pass-throughs do not count toward depth, pathtracer.rgen.glsl:196-200).
     begin(HPT_STAGE_TAIL);
-    launch_tail(s, features, sv, shade_fp, ps, qin, sp, fixed_bounces < kMaxBounces ? kMaxBounces - fixed_bounces : 0u,
+    launch_tail(s, features, sv, shade_fp, ps, qin, sp, wave_bounces < kMaxBounces ? kMaxBounces - wave_bounces : 0u,
                 wb.tail_cut, sc_tail);
     end();
     check(hipGetLastError(), "tail launch");
@@ -562,7 +570,7 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   wb.qc = r->vmis_phases ? r->d_qc.as<uint32_t>() : nullptr;
   const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
   trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
-                  r->shade_sort_from, r);
+                  r->shade_sort_from, r->tail_from, r);
 }
 
 template <typename T>
@@ -589,6 +597,7 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, frames);
   key_append(k, r->info.features);
   key_append(k, r->shade_sort_from);
+  key_append(k, r->tail_from);
   return k;
 }
 
@@ -1311,6 +1320,22 @@ hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce) {
   });
 }
 
+hpt_status hpt_set_tail_from(hpt_renderer* r, uint32_t first_bounce) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(first_bounce >= 1, "the first bounce always runs as a wavefront");
+    r->tail_from = first_bounce;
+  });
+}
+
+hpt_status hpt_group_set_tail_from(hpt_group* g, uint32_t first_bounce) {
+  return guarded([&] {
+    need(g, "null group");
+    need(first_bounce >= 1, "the first bounce always runs as a wavefront");
+    g->tail_from = first_bounce;
+  });
+}
+
 hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on) {
   return guarded([&] {
     need(r, "null renderer");
@@ -1928,6 +1953,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     key_append(key, n);
     key_append(key, r0->info.features);
     key_append(key, g->shade_sort_from);
+    key_append(key, g->tail_from);
     for (const WaveJob& j : wj) key_append(key, j.base);
     const uint32_t njobs = (uint32_t)wj.size();
     const uint32_t fixed_bounces = max_depth + 1;
@@ -1955,7 +1981,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
         check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
           launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
-          trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, cj, n, fixed_bounces, g->shade_sort_from,
+          trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, cj, n, fixed_bounces, g->shade_sort_from, g->tail_from,
                           nullptr);
           for (size_t i = 0; i < cj.size(); ++i)
             launch_frame_counter(g->stream, live[i]->d_frame.as<uint32_t>(), cj[i].fp.batch_frames, true);
@@ -1977,7 +2003,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     } else {
       launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
       check(hipGetLastError(), "job limits launch");
-      trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, wj, n, fixed_bounces, g->shade_sort_from,
+      trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, wj, n, fixed_bounces, g->shade_sort_from, g->tail_from,
                       nullptr);
       for (hpt_renderer* r : live) r->frame_synced = false;
     }

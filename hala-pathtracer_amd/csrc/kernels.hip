@@ -799,11 +799,16 @@ __global__ void __launch_bounds__(kBlock)
 // closest hit (as extend), the bounce (shade_path), the shadow rays (as
 // shadow) -- with the same operations in the same order as the wavefront,
 // so the images are the same; no host round trip decides how many bounces
-// are left, which lets the whole batch be captured in one graph.  The grid
-// strides over the queue: block b takes chunks b / kSeg, b / kSeg + G, ...
-// (G = gridDim.x / kSeg) of segment b % kSeg.  A path is cut after
-// max_steps tail bounces; cut paths keep the radiance gathered so far and
-// are counted in *cut.
+// are left, which lets the whole batch be captured in one graph.  The tail
+// can also start earlier, at any bounce: late bounces hold few paths, and a
+// wavefront launch over them lasts as long as its slowest path's chain of
+// dependent loads, where here each lane pays only its own path's chain.
+// The grid strides over the queue: block b takes chunks b / kSeg,
+// b / kSeg + G, ... (G = gridDim.x / kSeg) of segment b % kSeg.  (Handing
+// out paths per lane through per-segment cursors, so a lane whose path ends
+// takes the next at once, measured slower: 18.7 against 16.5 ms for two
+// 2-frame 1080p batches.)  A path is cut after max_steps tail bounces; cut
+// paths keep the radiance gathered so far and are counted in *cut.
 template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
 __global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, PathState ps, Queue qin, Spill sp,
                                                  uint32_t max_steps, unsigned long long* cut,

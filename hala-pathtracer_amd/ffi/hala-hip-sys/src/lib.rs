@@ -163,6 +163,7 @@ pub const HPT_IMAGE_NORMAL_RGBA32F: c_int = 3;
 pub const HPT_LAUNCH_EAGER: u32 = 0;
 pub const HPT_LAUNCH_GRAPH: u32 = 1;
 pub const HPT_SHADE_SORT_OFF: u32 = 0xFFFF_FFFF;
+pub const HPT_TAIL_FROM_NONE: u32 = 0xFFFF_FFFF;
 pub const HPT_BVH_HOST: u32 = 0;
 pub const HPT_BVH_GPU: u32 = 1;
 
@@ -189,6 +190,8 @@ extern "C" {
     pub fn hpt_set_shade_sort(r: *mut hpt_renderer, first_bounce: u32) -> hpt_status;
     pub fn hpt_set_vmis_phases(r: *mut hpt_renderer, on: u32) -> hpt_status;
     pub fn hpt_get_truncated_paths(r: *mut hpt_renderer, paths: *mut u64) -> hpt_status;
+    pub fn hpt_set_tail_from(r: *mut hpt_renderer, first_bounce: u32) -> hpt_status;
+    pub fn hpt_group_set_tail_from(g: *mut hpt_group, first_bounce: u32) -> hpt_status;
     pub fn hpt_group_create(device: i32, out: *mut *mut hpt_group) -> hpt_status;
     pub fn hpt_group_render(g: *mut hpt_group, jobs: *const *mut hpt_renderer, frames: *const u32,
         camera_indices: *const u32, count: u32) -> hpt_status;

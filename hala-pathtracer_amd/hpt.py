@@ -35,6 +35,7 @@ IMAGE_FINAL_RGBA8, IMAGE_ACCUM, IMAGE_ALBEDO, IMAGE_NORMAL = range(4)
 HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
 STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT, STAGE_WALK, STAGE_TAIL = range(8)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
+TAIL_FROM_NONE = 0xFFFFFFFF
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 SHADE_SORT_OFF = 0xFFFFFFFF
 BVH_HOST, BVH_GPU = 0, 1
@@ -196,6 +197,8 @@ SIGNATURES = {
     "hpt_get_bvh_build_stats": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "hpt_set_tail_from": [C.c_void_p, C.c_uint32],
+    "hpt_group_set_tail_from": [C.c_void_p, C.c_uint32],
     "hpt_group_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_present_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
     "hpt_present_frame": [C.c_void_p],
@@ -535,6 +538,12 @@ class HalaRenderer:
         _check(self._lib.hpt_get_launch_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_tail_from(self, first_bounce: int = TAIL_FROM_NONE) -> None:
+        """Run the paths still live at bounce `first_bounce` to their end in
+        one kernel, a lane per path (TAIL_FROM_NONE: every bounce up to
+        max_depth + 1 in the wavefront); same images."""
+        _check(self._lib.hpt_set_tail_from(self._h, first_bounce))
+
     def truncated_paths(self) -> int:
         """Paths stopped at the 4096-bounce limit since the handle was made
         (TRANSPARENT pass-throughs do not count toward max_depth)."""
@@ -612,6 +621,9 @@ class HalaGroup:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(self._lib.hpt_group_get_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return {"steps": a.value, "graph_launches": b.value, "device_bytes": c.value}
+
+    def set_tail_from(self, first_bounce: int = TAIL_FROM_NONE) -> None:
+        _check(self._lib.hpt_group_set_tail_from(self._h, first_bounce))
 
     def truncated_paths(self) -> int:
         """Paths of this group's steps stopped at the 4096-bounce limit."""

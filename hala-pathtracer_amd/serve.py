@@ -105,6 +105,14 @@ def shade_sort(spec: JobSpec, tuning=None) -> int:
     return hpt.SHADE_SORT_OFF
 
 
+def tail_from(spec: JobSpec, tuning=None) -> int:
+    if tuning is not None:
+        t = tuning.get(spec.scene, spec.width, spec.height, 0)
+        if t is not None:
+            return t.tail_from
+    return hpt.TAIL_FROM_NONE
+
+
 def footprint(spec: JobSpec, max_frames: int = 32, tuning=None) -> int:
     px = spec.width * spec.height
     per_path = PATH_BYTES + (SORT_BYTES if shade_sort(spec, tuning) != hpt.SHADE_SORT_OFF else 0)
@@ -171,6 +179,7 @@ class Server:
         r.share_scene(rs.template)
         r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
         r.set_shade_sort(shade_sort(spec, self.tuning))
+        r.set_tail_from(tail_from(spec, self.tuning))
         return r
 
     def _run_job(self, spec: JobSpec, t0: float, admitted: float, results: list, fp: int):
@@ -229,6 +238,10 @@ class Server:
                  if key.startswith(scene + "|")]
         if sorts:  # the choice most classes of this scene were tuned to
             g.set_shade_sort(max(set(sorts), key=sorts.count))
+        tails = [t.tail_from for key, t in (self.tuning.entries.items() if self.tuning else [])
+                 if key.startswith(scene + "|")]
+        if tails:
+            g.set_tail_from(max(set(tails), key=tails.count))
         return g
 
     def _finish(self, a: _Active, t0: float, results: list, error: str | None = None):

@@ -386,6 +386,22 @@ HPT_API hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* 
                                        uint64_t* device_bytes);
 HPT_API hpt_status hpt_group_wait_idle(hpt_group* g);
 HPT_API void hpt_group_destroy(hpt_group* g);
+/* The bounce from which the paths still live run to their end in one kernel,
+ * each path on one lane (closest hit, shading and shadow rays in the order
+ * of the wavefront: same images), instead of one wavefront launch per
+ * bounce (no counterpart in the reference, whose rays run that way).  Late
+ * bounces hold few paths, and each wavefront launch over them costs the
+ * dependent-load latency of its slowest path; the tail pays one such chain
+ * per path instead of one per bounce.  first_bounce >= 1;
+ * HPT_TAIL_FROM_NONE (the default) keeps every bounce up to max_depth + 1
+ * in the wavefront, and only TRANSPARENT paths that outlive those (a
+ * pass-through does not count toward depth, pathtracer.rgen.glsl:196-200)
+ * go to the tail.  The best value depends on scene and batch size
+ * (autotune.py picks it). */
+#define HPT_TAIL_FROM_NONE 0xFFFFFFFFu
+HPT_API hpt_status hpt_set_tail_from(hpt_renderer* r, uint32_t first_bounce);
+HPT_API hpt_status hpt_group_set_tail_from(hpt_group* g, uint32_t first_bounce);
+
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
 

@@ -102,3 +102,70 @@ def test_tail_converges_to_the_oracle():
     diff = gpu.mean() - mean.mean()
     print(f"[gpu] tail converged: image mean {gpu.mean():.5f} vs oracle {mean.mean():.5f} (se {se:.5f})", flush=True)
     assert abs(diff) < 4.0 * se + 1e-6
+
+
+@pytest.mark.parametrize("case", ["disney", "icosphere_medium"])
+def test_earlier_tail_gives_the_same_images(case):
+    """hpt_set_tail_from: the paths live at bounce 1, 2 or 4 run to their end
+    in the tail kernel -- the same images as the wavefront, in eager and
+    graph launches; the tail took paths."""
+    if case == "disney":
+        sc = hpt_scenes.disney_grid(seed=3, width=96, height=64, env_width=64, env_height=32, segments=16, rings=8)
+        features, w, h = 0, 96, 64
+    else:
+        sc = hpt_scenes.displaced_icosphere_lights_medium(level=3, width=96, height=54)
+        features, w, h = hpt_scenes.MEDIUM_SCENE_FEATURES, 96, 54
+    base = _renderer(sc, w, h, 8, 3, features, frames_in_flight=2)
+    base.render(4)
+    want = [base.read_image(im) for im in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_FINAL_RGBA8)]
+    base.close()
+    for tf in (1, 2, 4):
+        for mode in (hpt.LAUNCH_EAGER, hpt.LAUNCH_GRAPH):
+            r = _renderer(sc, w, h, 8, 3, features, mode=mode, frames_in_flight=2)
+            r.set_tail_from(tf)
+            r.render(4)
+            got = [r.read_image(im) for im in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_FINAL_RGBA8)]
+            r.set_profiling(hpt.PROFILE_TIMING)
+            r.render(2)
+            tail = r.stage_stats(hpt.STAGE_TAIL)
+            r.close()
+            print(f"[gpu] {case} tail from {tf} mode {mode}: tail paths {tail['rays']}", flush=True)
+            for a, b in zip(want, got):
+                assert np.array_equal(a, b), (case, tf, mode)
+            assert tail["launches"] == 1 and tail["rays"] > 0
+    with pytest.raises(hpt.HptError):
+        r = _renderer(sc, w, h, 8, 3, features)
+        try:
+            r.set_tail_from(0)
+        finally:
+            r.close()
+
+
+def test_group_with_an_earlier_tail_equals_solo():
+    sc = hpt_scenes.displaced_icosphere_lights_medium(level=3, width=96, height=54)
+    features = hpt_scenes.MEDIUM_SCENE_FEATURES
+    t = _renderer(sc, 8, 8, 1, 0, features)
+    specs = [(96, 54, 8, 3), (64, 40, 6, 2), (48, 32, 12, 3)]
+
+    def job(spec):
+        w, h, d, rr = spec
+        r = hpt.HalaRenderer(w, h, max_depth=d, rr_depth=rr, features=features)
+        r.frames_in_flight = 2
+        r.share_scene(t)
+        return r
+
+    jobs = [job(s) for s in specs]
+    g = hpt.HalaGroup()
+    g.set_tail_from(3)
+    for _ in range(3):  # the second and third steps replay the captured graph
+        g.render(jobs, [2, 2, 2])
+    assert g.stats()["graph_launches"] == 2
+    for r, s in zip(jobs, specs):
+        solo = job(s)
+        solo.render(6)
+        assert np.array_equal(r.read_image(hpt.IMAGE_ACCUM), solo.read_image(hpt.IMAGE_ACCUM))
+        solo.close()
+        r.close()
+    assert g.truncated_paths() == 0
+    g.close()
+    t.close()
