@@ -91,19 +91,20 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s
             "sample": f"{what} ({paths} paths, {dt:.1f} s)"}
 
 
-def profiled_traffic(kernel, workload, batch_frames):
+def profiled_traffic(kernel, workload, batch_frames, tail_from):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     of the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
     the reading of a 1 GiB copy), when it was taken at the same batch size
-    (frames traced together); else None."""
+    (frames traced together) and tail bounce; else None."""
     d = os.path.join(ROOT, "profiles")
+    tail = "none" if tail_from == hpt.TAIL_FROM_NONE else str(tail_from)
     for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
         if not name.endswith(f"_{workload}.md"):
             continue
         for line in open(os.path.join(d, name)):
-            m = re.match(r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+)\) = (\d+)", line)
-            if m and m.group(1) == kernel and int(m.group(2)) == batch_frames:
-                return int(m.group(3))
+            m = re.match(r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+), tail_from (\w+)\) = (\d+)", line)
+            if m and m.group(1) == kernel and int(m.group(2)) == batch_frames and m.group(3) == tail:
+                return int(m.group(4))
     return None
 
 
@@ -636,12 +637,13 @@ def main():
     # the extension's rate per closest-hit query); the walk kernel (that rate
     # per query) when shading is phased.
     roof_ext = stage_roofline("k_extend", ext_t, NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
-                              traffic=profiled_traffic("k_extend", args.workload, batch_frames),
+                              traffic=profiled_traffic("k_extend", args.workload, batch_frames, tails["plain"]),
                               extra={"nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)})
     shade_bytes = SHADE_EVENT_BYTES + (0.0 if phased else walk_queries * walk_bytes_per_query)
     shade_name = "k_vmis_a/b/c" if phased else "k_shade"
     roof_shade = stage_roofline(shade_name, shade_t, shade_bytes,
-                                traffic=None if phased else profiled_traffic("k_shade", args.workload, batch_frames),
+                                traffic=profiled_traffic("k_vmis" if phased else "k_shade", args.workload,
+                                                         batch_frames, tails["plain"]),
                                 extra={"bytes_per_event": round(shade_bytes, 1),
                                        "walk_queries_per_event": round(walk_queries, 4)})
     rooflines = {"extend": roof_ext, "shade": roof_shade}
@@ -650,7 +652,7 @@ def main():
         # run's shading events times the warm-up's queries per event
         walk_t = dict(walk_t, rays=int(round(shade_t["rays"] * walk_queries)))
         rooflines["walk"] = stage_roofline("k_walk", walk_t, walk_bytes_per_query,
-                                           traffic=profiled_traffic("k_walk", args.workload, batch_frames),
+                                           traffic=profiled_traffic("k_walk", args.workload, batch_frames, tails["plain"]),
                                            extra={"bytes_per_query": round(walk_bytes_per_query, 1)})
     stage_of = {"extend": ext_t, "shade": shade_t, "shadow": shadow_t, "walk": walk_t}
     dominant = max((k for k in rooflines), key=lambda k: stage_of[k]["total_ms"])
