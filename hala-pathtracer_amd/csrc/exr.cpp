@@ -351,75 +351,73 @@ void huf_unpack(const uint8_t* src, size_t n, uint16_t* out, size_t out_n) {
 }
 
 // ------------------------------------------------------------- PIZ wavelet
-// Inverse lifting steps: 14-bit (values below 2^14 after the table) as
-// signed 16-bit arithmetic, otherwise modulo 2^16 with an offset of 2^15.
-inline void wdec14(uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-  const int ls = (int16_t)l, hi = (int16_t)h;
-  const int ai = ls + (hi & 1) + (hi >> 1);
-  a = (uint16_t)(int16_t)ai;
-  b = (uint16_t)(int16_t)(ai - hi);
-}
+// PIZ stores each channel plane as a 2D Haar transform built from one 1D
+// lifting pair per two words: the mean m and difference d of (a, b).  Two
+// variants exist and the file's largest word rank picks one:
+//   ranks below 2^14: signed 16-bit, m = floor((a + b) / 2), d = a - b, so
+//     a = m + ceil(d / 2) and b = a - d;
+//   otherwise modular 16-bit: a is offset by 2^15, m = floor((a' + b) / 2),
+//     d = a' - b, and m moves by 2^15 when d < 0, so b = m - (d >> 1) and
+//     a = d + b - 2^15 (all mod 2^16).
+// The forward transform runs its levels from the finest pair distance
+// (1) to the coarsest (half the largest power of two <= min(nx, ny)); at a
+// level with pair distance s it combines the words at (x, y), (x + s, y),
+// (x, y + s), (x + s, y + s) for x, y on the grid of spacing 2s, rows
+// first, then columns; a column (row) left over when nx (ny) has bit s set
+// takes the one-dimensional pair.  Decoding undoes the levels from the
+// coarsest, columns before rows.  Written from that definition; files
+// decode bit for bit only with exactly these steps and this order.
+struct LiftPair {
+  bool small;  // the signed 14-bit variant
+  void undo(uint16_t m, uint16_t d, uint16_t& a, uint16_t& b) const {
+    if (small) {
+      const int mi = (int16_t)m, di = (int16_t)d;
+      const int ai = mi + ((di + (di & 1)) >> 1);
+      a = (uint16_t)ai;
+      b = (uint16_t)(ai - di);
+    } else {
+      const uint32_t bb = ((uint32_t)m - (uint32_t)(d >> 1)) & 0xFFFFu;
+      b = (uint16_t)bb;
+      a = (uint16_t)(((uint32_t)d + bb + 0x8000u) & 0xFFFFu);  // + 2^15 == - 2^15 (mod 2^16)
+    }
+  }
+};
 
-inline void wdec16(uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-  const int m = l, d = h;
-  const int bb = (m - (d >> 1)) & 0xFFFF;
-  const int aa = (d + bb - 0x8000) & 0xFFFF;
-  b = (uint16_t)bb;
-  a = (uint16_t)aa;
-}
-
-// Undoes the 2D transform of one plane: nx x ny words, x stride ox, y
-// stride oy.  Levels run from the coarsest (largest power of two p2 <=
-// min(nx, ny)) to the finest; an odd column or row at a level takes a 1D step.
-void wav2_unpack(uint16_t* in, int nx, int ox, int ny, int oy, uint16_t mx) {
-  const bool w14 = mx < (1 << 14);
-  auto dec = [w14](uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-    if (w14)
-      wdec14(l, h, a, b);
-    else
-      wdec16(l, h, a, b);
-  };
+// Undoes the transform of one plane of nx x ny words; word (x, y) is at
+// plane[y * row + x * col].  max_rank: the largest word rank of the chunk.
+void piz_unwavelet(uint16_t* plane, int nx, int col, int ny, int row, uint16_t max_rank) {
+  const LiftPair lift{max_rank < (1u << 14)};
+  auto word = [&](int x, int y) -> uint16_t& { return plane[(ptrdiff_t)y * row + (ptrdiff_t)x * col]; };
   const int n = std::min(nx, ny);
-  int p = 1;
-  while (p <= n) p <<= 1;
-  p >>= 1;
-  int p2 = p;
-  p >>= 1;
-  while (p >= 1) {
-    const ptrdiff_t oy1 = (ptrdiff_t)oy * p, oy2 = (ptrdiff_t)oy * p2;
-    const ptrdiff_t ox1 = (ptrdiff_t)ox * p, ox2 = (ptrdiff_t)ox * p2;
-    const ptrdiff_t ey = (ptrdiff_t)oy * (ny - p2);
-    ptrdiff_t py = 0;
-    uint16_t i00, i01, i10, i11;
-    for (; py <= ey; py += oy2) {
-      const ptrdiff_t ex = py + (ptrdiff_t)ox * (nx - p2);
-      ptrdiff_t px = py;
-      for (; px <= ex; px += ox2) {
-        uint16_t* a = in + px;
-        uint16_t* b = in + px + ox1;
-        uint16_t* c = in + px + oy1;
-        uint16_t* e = in + px + oy1 + ox1;
-        dec(*a, *c, i00, i10);
-        dec(*b, *e, i01, i11);
-        dec(i00, i01, *a, *b);
-        dec(i10, i11, *c, *e);
+  int coarse = 1;  // the largest power of two <= n
+  while (coarse * 2 <= n) coarse *= 2;
+  for (int s = coarse / 2; s >= 1; s /= 2) {
+    const int step = 2 * s;
+    int y = 0;
+    for (; y + step <= ny; y += step) {
+      int x = 0;
+      for (; x + step <= nx; x += step) {
+        uint16_t top0, top1, bot0, bot1;
+        lift.undo(word(x, y), word(x, y + s), top0, bot0);          // columns x and x + s
+        lift.undo(word(x + s, y), word(x + s, y + s), top1, bot1);
+        lift.undo(top0, top1, word(x, y), word(x + s, y));            // rows y and y + s
+        lift.undo(bot0, bot1, word(x, y + s), word(x + s, y + s));
       }
-      if (nx & p) {
-        uint16_t* c = in + px + oy1;
-        dec(in[px], *c, i00, *c);
-        in[px] = i00;
+      if (nx & s) {  // the leftover column: its vertical pair
+        uint16_t u, v;
+        lift.undo(word(x, y), word(x, y + s), u, v);
+        word(x, y) = u;
+        word(x, y + s) = v;
       }
     }
-    if (ny & p) {
-      const ptrdiff_t ex = py + (ptrdiff_t)ox * (nx - p2);
-      for (ptrdiff_t px = py; px <= ex; px += ox2) {
-        uint16_t* b = in + px + ox1;
-        dec(in[px], *b, i00, *b);
-        in[px] = i00;
+    if (ny & s) {  // the leftover row: its horizontal pairs
+      for (int x = 0; x + step <= nx; x += step) {
+        uint16_t u, v;
+        lift.undo(word(x, y), word(x + s, y), u, v);
+        word(x, y) = u;
+        word(x + s, y) = v;
       }
     }
-    p2 = p;
-    p >>= 1;
   }
 }
 
@@ -450,7 +448,7 @@ void piz_unpack(const uint8_t* src, size_t n, uint32_t width, uint32_t lines, co
   size_t at = 0;
   for (const Channel& c : chans) {
     const int sz = (int)c.bytes / 2;
-    for (int j = 0; j < sz; ++j) wav2_unpack(tmp.data() + at + j, (int)width, sz, (int)lines, (int)width * sz, max_value);
+    for (int j = 0; j < sz; ++j) piz_unwavelet(tmp.data() + at + j, (int)width, sz, (int)lines, (int)width * sz, max_value);
     at += (size_t)width * lines * sz;
   }
   for (uint16_t& w : tmp) w = lut[w];

@@ -91,56 +91,57 @@ def _rle(data: bytes) -> bytes:
 
 
 # --------------------------------------------------------------------- PIZ
-def _wenc14(a, b):
-    as_ = a - 65536 if a >= 32768 else a
-    bs = b - 65536 if b >= 32768 else b
-    ms = (as_ + bs) >> 1
-    ds = as_ - bs
-    return ms & 0xFFFF, ds & 0xFFFF
+def _lift14(a, b):
+    """Signed 16-bit pair: mean floor((a + b) / 2) and difference a - b."""
+    sa = a - 65536 if a >= 32768 else a
+    sb = b - 65536 if b >= 32768 else b
+    return ((sa + sb) >> 1) & 0xFFFF, (sa - sb) & 0xFFFF
 
 
-def _wenc16(a, b):
+def _lift16(a, b):
+    """Modular pair: a offset by 2^15; the mean moves by 2^15 when the
+    difference is negative."""
     ao = (a + 0x8000) & 0xFFFF
-    m = (ao + b) >> 1
-    d = ao - b
+    m, d = (ao + b) >> 1, ao - b
     if d < 0:
         m = (m + 0x8000) & 0xFFFF
     return m, d & 0xFFFF
 
 
 def wav2_encode(buf, start, nx, ox, ny, oy, mx):
-    """Forward 2D lifting over one plane of `buf` (a list of words), finest
-    level first; an odd column or row at a level takes a 1D step."""
-    enc = _wenc14 if mx < (1 << 14) else _wenc16
-    n = min(nx, ny)
-    p, p2 = 1, 2
-    while p2 <= n:
-        oy1, oy2, ox1, ox2 = oy * p, oy * p2, ox * p, ox * p2
-        ey = start + oy * (ny - p2)
-        py = start
-        while py <= ey:
-            ex = py + ox * (nx - p2)
-            px = py
-            while px <= ex:
-                p01, p10 = px + ox1, px + oy1
-                p11 = p10 + ox1
-                i00, i01 = enc(buf[px], buf[p01])
-                i10, i11 = enc(buf[p10], buf[p11])
-                buf[px], buf[p10] = enc(i00, i10)
-                buf[p01], buf[p11] = enc(i01, i11)
-                px += ox2
-            if nx & p:
-                p10 = px + oy1
-                buf[px], buf[p10] = enc(buf[px], buf[p10])
-            py += oy2
-        if ny & p:
-            ex = py + ox * (nx - p2)
-            px = py
-            while px <= ex:
-                p01 = px + ox1
-                buf[px], buf[p01] = enc(buf[px], buf[p01])
-                px += ox2
-        p, p2 = p2, p2 << 1
+    """The forward 2D transform of one plane of `buf` (a list of words; word
+    (x, y) at start + y * oy + x * ox): pair distances 1, 2, 4, ... while
+    twice the distance fits min(nx, ny); at each level rows, then columns; a
+    leftover column or row (bit s of nx or ny set) takes the 1D pair."""
+    lift = _lift14 if mx < (1 << 14) else _lift16
+
+    def idx(x, y):
+        return start + y * oy + x * ox
+
+    s = 1
+    while 2 * s <= min(nx, ny):
+        step = 2 * s
+        y = 0
+        while y + step <= ny:
+            x = 0
+            while x + step <= nx:
+                q00, q01, q10, q11 = idx(x, y), idx(x + s, y), idx(x, y + s), idx(x + s, y + s)
+                t0, t1 = lift(buf[q00], buf[q01])      # row y
+                u0, u1 = lift(buf[q10], buf[q11])      # row y + s
+                buf[q00], buf[q10] = lift(t0, u0)      # column x
+                buf[q01], buf[q11] = lift(t1, u1)      # column x + s
+                x += step
+            if nx & s:
+                q0, q1 = idx(x, y), idx(x, y + s)
+                buf[q0], buf[q1] = lift(buf[q0], buf[q1])
+            y += step
+        if ny & s:
+            x = 0
+            while x + step <= nx:
+                q0, q1 = idx(x, y), idx(x + s, y)
+                buf[q0], buf[q1] = lift(buf[q0], buf[q1])
+                x += step
+        s *= 2
 
 
 class _Bits:
