@@ -37,6 +37,10 @@ constexpr unsigned kXcds = 8;      // MI355X: workgroups are dealt round-robin o
 #define HPT_CONV_TILE 1
 #endif
 constexpr bool kConvTile = HPT_CONV_TILE;  // forward 3x3 convolutions stage their input tile in LDS
+#ifndef HPT_CONV_FAST
+#define HPT_CONV_FAST 1
+#endif
+constexpr bool kConvFast = HPT_CONV_FAST;  // ... in the kernel specialised on the channel count
 
 // Block b runs on XCD b % 8; this maps it to tile (b % 8) * (grid / 8) + b / 8
 // so that each XCD walks a contiguous range of pixel tiles and a tile's
@@ -294,6 +298,167 @@ bool allow_tile_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
   return ok;
 }
 
+// The tiled forward 3x3 convolution specialised on the input channel count
+// (C8 = cin / 8), for the layer shapes of the network.  Same work split and
+// K order as k_conv3_tile (so the same bits), with:
+//  - the K loop unrolled at compile time: segment s = 4 * step + g of lane
+//    group g is (tap s / C8, channel segment s % C8); the weights' address is
+//    a per-lane base plus an immediate, the tile's a per-lane base plus a
+//    per-step offset computed by constant divisions;
+//  - the tile's pixel stride padded by 8 channels where cin / 2 words would
+//    put the 16 pixels of a read on the same LDS banks (cin 32 and 64);
+//  - the MFMA operands swapped (weights first), so a lane's accumulator
+//    holds 4 consecutive output channels of one pixel: bias, ReLU and the
+//    bf16 rounding then end in one 8-byte store per 4 channels.
+template <int C8>
+struct TileGeom {
+  static constexpr int kCin = 8 * C8;
+  static constexpr int kStride = (C8 % 4 == 0) ? kCin + 8 : kCin;  // halves per tile pixel
+  static constexpr int kSegs = 9 * C8;
+  static constexpr int kSteps = (kSegs + 3) / 4;
+  static constexpr size_t kLds = (size_t)kHalo * kHalo * kStride * sizeof(uint16_t);
+};
+
+template <int C8, int NT, bool UPCAT>
+__global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restrict__ in,
+                                                         const uint16_t* __restrict__ wgt,
+                                                         const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                         float* __restrict__ yf, int n_img, int h, int w, int nstore,
+                                                         int relu, UpCat uc) {
+  using G = TileGeom<C8>;
+  extern __shared__ __align__(16) uint16_t tile[];  // kHalo * kHalo pixels x kStride
+  const int tiles_x = (w + kTile - 1) / kTile, tiles_y = (h + kTile - 1) / kTile;
+  const unsigned b = xcd_block();
+  if (b >= (unsigned)(n_img * tiles_x * tiles_y)) return;  // the whole block, before any barrier
+  const int img = (int)b / (tiles_x * tiles_y);
+  const int rem = (int)b - img * tiles_x * tiles_y;
+  const int ty0 = (rem / tiles_x) * kTile, tx0 = (rem - (rem / tiles_x) * tiles_x) * kTile;
+  const int c8low = UPCAT ? uc.c_low >> 3 : 0;
+  for (int idx = threadIdx.x; idx < kHalo * kHalo * C8; idx += kThreads) {
+    const int pix = idx / C8, c8 = idx - pix * C8;
+    const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + (pix - (pix / kHalo) * kHalo);
+    u32x4 v = u32x4{0u, 0u, 0u, 0u};
+    if ((unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w) {
+      const uint16_t* src;
+      if (!UPCAT)
+        src = in + ((size_t)(img * h + yy) * w + xx) * G::kCin + c8 * 8;
+      else if (c8 < c8low)
+        src = uc.low + ((size_t)(img * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * uc.c_low + c8 * 8;
+      else
+        src = in + ((size_t)(img * h + yy) * w + xx) * (G::kCin - uc.c_low) + (c8 - c8low) * 8;
+      v = *reinterpret_cast<const u32x4*>(src);
+    }
+    *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) = v;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, q = threadIdx.x >> 6;
+  const int n0 = blockIdx.y * (16 * NT);
+  constexpr int K = 9 * G::kCin;
+  const uint16_t* wbase[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) wbase[j] = wgt + (size_t)(n0 + 16 * j + m) * K + g * 8;
+  const uint16_t* abase = tile + (4 * q * kHalo + m) * G::kStride;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int step = 0; step < G::kSteps; ++step) {
+    const int s = 4 * step + g;
+    const bool sv = (4 * step + 3 < G::kSegs) || s < G::kSegs;  // only the last step can be partial
+    const int tap = s / C8, c8 = s - (s / C8) * C8;
+    const int ty = tap / 3, tx = tap - (tap / 3) * 3;
+    const uint16_t* ap = abase + (ty * kHalo + tx) * G::kStride + c8 * 8;
+    bf16x8 a[4], bb[NT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = sv ? load8(ap + i * kHalo * G::kStride) : zero8();
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bb[j] = sv ? load8(wbase[j] + 32 * step) : zero8();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb[j], a[i], acc[i][j], 0, 0, 0);
+  }
+
+  // Lane (m, g) holds D[4g + r][m]: channel n0 + 16j + 4g + r of tile row
+  // 4q + i, column m.
+  const int xx = tx0 + m;
+  if (xx >= w) return;
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = n0 + 16 * j + 4 * g;
+    if (co >= nstore) continue;  // nstore is a multiple of 4 (checked at launch)
+    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + co);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int yy = ty0 + 4 * q + i;
+      if (yy >= h) continue;
+      const size_t o = ((size_t)(img * h + yy) * w + xx) * nstore + co;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        v[r] = acc[i][j][r] + bv[r];
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (y) *reinterpret_cast<uint2*>(y + o) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
+      if (yf) *reinterpret_cast<f32x4*>(yf + o) = f32x4{v[0], v[1], v[2], v[3]};
+    }
+  }
+}
+
+template <int C8, int NT, bool UPCAT>
+bool allow_fast_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_fast<C8, NT, UPCAT>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLdsMax) ==
+                         hipSuccess;
+  return ok;
+}
+
+template <int C8, bool UPCAT>
+int go_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y,
+                  float* yf, int n_img, int h, int wd, int nstore, int relu, UpCat uc) {
+  constexpr size_t lds = TileGeom<C8>::kLds;
+  static_assert(lds <= kTileLdsMax, "tile exceeds the LDS budget");
+  bool ok = true;
+  if (lds > 65536)
+    ok = nt == 1 ? allow_fast_lds<C8, 1, UPCAT>()
+         : nt == 2 ? allow_fast_lds<C8, 2, UPCAT>()
+         : nt == 3 ? allow_fast_lds<C8, 3, UPCAT>()
+                   : allow_fast_lds<C8, 4, UPCAT>();
+  if (!ok) return -1;
+#define HPT_F(NT_)                                                                                                   \
+  hipLaunchKernelGGL((k_conv3_fast<C8, NT_, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, \
+                     nstore, relu, uc)
+  switch (nt) {
+    case 1: HPT_F(1); break;
+    case 2: HPT_F(2); break;
+    case 3: HPT_F(3); break;
+    default: HPT_F(4); break;
+  }
+#undef HPT_F
+  return (int)hipGetLastError();
+}
+
+// The specialised kernel for the channel counts it is built for (16, 32,
+// 48, 64, 80, 112); -1 for others.
+template <bool UPCAT>
+int launch_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const uint16_t* w, const float* b,
+                      uint16_t* y, float* yf, int n_img, int h, int wd, int cin, int nstore, int relu, UpCat uc) {
+  if (nstore % 4) return -1;
+  switch (cin) {
+    case 16: return go_conv3_fast<2, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 32: return go_conv3_fast<4, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 48: return go_conv3_fast<6, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 64: return go_conv3_fast<8, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 80: return go_conv3_fast<10, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 112: return go_conv3_fast<14, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    default: return -1;
+  }
+}
+
 // The tiled forward convolution when the 18 x 18 tile of all input channels
 // fits in kTileLdsMax, the layer has at least 256 tiles (smaller images keep
 // the CUs busier with the pixel-linear kernel and its channel split), and
@@ -314,6 +479,10 @@ int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, con
   }
   const dim3 grid(xcd_grid(blocks), groups);
   const int nt = tiles / groups;
+  if (kConvFast) {
+    const int e = launch_conv3_fast<UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc);
+    if (e >= 0) return e;
+  }
   if (lds > 65536) {
     const bool ok = nt == 1   ? allow_tile_lds<1, UPCAT>()
                     : nt == 2 ? allow_tile_lds<2, UPCAT>()
