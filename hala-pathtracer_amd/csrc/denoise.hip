@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "../../include/hpt_nn.h"
 
@@ -305,15 +306,16 @@ bool allow_tile_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
 //    group g is (tap s / C8, channel segment s % C8); the weights' address is
 //    a per-lane base plus an immediate, the tile's a per-lane base plus a
 //    per-step offset computed by constant divisions;
-//  - the tile's pixel stride padded by 8 channels where cin / 2 words would
-//    put the 16 pixels of a read on the same LDS banks (cin 32 and 64);
+//  - the tile's pixel stride padded by 16 channels where cin is a multiple
+//    of 32 (cin 32 and 64): measured, pixel strides of 8 mod 16 words read
+//    without LDS bank conflicts and strides of 0 or 4 mod 16 words with;
 //  - the MFMA operands swapped (weights first), so a lane's accumulator
 //    holds 4 consecutive output channels of one pixel: bias, ReLU and the
 //    bf16 rounding then end in one 8-byte store per 4 channels.
 template <int C8>
 struct TileGeom {
   static constexpr int kCin = 8 * C8;
-  static constexpr int kStride = (C8 % 4 == 0) ? kCin + 8 : kCin;  // halves per tile pixel
+  static constexpr int kStride = (C8 % 4 == 0) ? kCin + 16 : kCin;  // halves per tile pixel
   static constexpr int kSegs = 9 * C8;
   static constexpr int kSteps = (kSegs + 3) / 4;
   static constexpr size_t kLds = (size_t)kHalo * kHalo * kStride * sizeof(uint16_t);
@@ -365,18 +367,36 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  // Operands of step k: the weights (global, L2 latency) are loaded two
+  // steps ahead and the tile fragments (LDS) one step ahead, so a step's
+  // MFMAs do not wait on the loads issued just before them.
+  auto seg_ok = [&](int step) { return (4 * step + 3 < G::kSegs) || 4 * step + g < G::kSegs; };
+  auto load_b = [&](int step, bf16x8* bb) {
+    const bool sv = seg_ok(step);
 #pragma unroll
-  for (int step = 0; step < G::kSteps; ++step) {
+    for (int j = 0; j < NT; ++j) bb[j] = sv ? load8(wbase[j] + 32 * step) : zero8();
+  };
+  auto load_a = [&](int step, bf16x8* a) {
     const int s = 4 * step + g;
-    const bool sv = (4 * step + 3 < G::kSegs) || s < G::kSegs;  // only the last step can be partial
+    const bool sv = seg_ok(step);
     const int tap = s / C8, c8 = s - (s / C8) * C8;
     const int ty = tap / 3, tx = tap - (tap / 3) * 3;
     const uint16_t* ap = abase + (ty * kHalo + tx) * G::kStride + c8 * 8;
-    bf16x8 a[4], bb[NT];
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = sv ? load8(ap + i * kHalo * G::kStride) : zero8();
+  };
+  bf16x8 bq[3][NT], aq[2][4];
+  load_b(0, bq[0]);
+  if (G::kSteps > 1) load_b(1, bq[1]);
+  load_a(0, aq[0]);
 #pragma unroll
-    for (int j = 0; j < NT; ++j) bb[j] = sv ? load8(wbase[j] + 32 * step) : zero8();
+  for (int step = 0; step < G::kSteps; ++step) {
+    if (step + 2 < G::kSteps) load_b(step + 2, bq[(step + 2) % 3]);
+    if (step + 1 < G::kSteps) load_a(step + 1, aq[(step + 1) % 2]);
+    // Keep the scheduler from sinking those loads to their uses.
+    __builtin_amdgcn_sched_barrier(0);
+    const bf16x8* bb = bq[step % 3];
+    const bf16x8* a = aq[step % 2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -499,6 +519,187 @@ int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, con
   return (int)hipGetLastError();
 }
 
+// The pixel-linear forward 3x3 convolution specialised on the input channel
+// count, for the low-resolution layers (too few 16 x 16 tiles for the tiled
+// kernel to fill the GPU).  There a wave's K loop (up to 63 steps of loads
+// from L2) is a chain of latencies and the waves are few, so the 4 waves of
+// a block share one output tile of 64 pixels x 16*NT channels and split its
+// K loop in four ranges; each range is unrolled at compile time with the
+// operands of step k + 2 loaded while step k multiplies.  The partial sums
+// meet in LDS and are added in wave order (deterministic; a different
+// summation order than k_conv, within the same float32 bound).  MFMA
+// operands are swapped as in k_conv3_fast for 8-byte stores.
+template <int C8, int NT, bool UPCAT>
+__global__ __launch_bounds__(kThreads) void k_conv3_lin(const uint16_t* __restrict__ in,
+                                                        const uint16_t* __restrict__ wgt,
+                                                        const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                        float* __restrict__ yf, int P, int h, int w, int nstore,
+                                                        int relu, UpCat uc) {
+  constexpr int kCin = 8 * C8, kSegs = 9 * C8, kSteps = (kSegs + 3) / 4, K = 9 * kCin;
+  __shared__ f32x4 part[kWavesPerBlock][4][NT][64];
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, q = threadIdx.x >> 6;
+  const long long p0l = (long long)xcd_block() * 64;
+  if (p0l >= P) return;  // the whole block, before the barrier
+  const int p0 = (int)p0l;
+  const int n0 = blockIdx.y * (16 * NT);
+  int px[4], py[4], pimg[4];
+  bool pv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int p = p0 + 16 * i + m;
+    pv[i] = p < P;
+    const int pc = pv[i] ? p : 0;
+    const int row = pc / w;
+    px[i] = pc - row * w;
+    py[i] = row % h;
+    pimg[i] = row / h;
+  }
+  const int c8low = UPCAT ? uc.c_low >> 3 : 0, cskip = UPCAT ? kCin - uc.c_low : 0;
+  const uint16_t* wbase[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) wbase[j] = wgt + (size_t)(n0 + 16 * j + m) * K + g * 8;
+
+  auto seg_ok = [&](int step) { return (4 * step + 3 < kSegs) || 4 * step + g < kSegs; };
+  auto load_b = [&](int step, bf16x8* bb) {
+    const bool sv = seg_ok(step);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bb[j] = sv ? load8(wbase[j] + 32 * step) : zero8();
+  };
+  auto load_a = [&](int step, bf16x8* a) {
+    const int s = 4 * step + g;
+    const bool sv = seg_ok(step);
+    const int tap = s / C8, c8 = s - (s / C8) * C8;
+    const int oy = tap / 3 - 1, ox = tap - (tap / 3) * 3 - 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = sv && pv[i] && (unsigned)(py[i] + oy) < (unsigned)h && (unsigned)(px[i] + ox) < (unsigned)w;
+      if (!ok)
+        a[i] = zero8();
+      else if (!UPCAT)
+        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * kCin + c8 * 8);
+      else if (c8 < c8low)
+        a[i] = load8(uc.low + ((size_t)(pimg[i] * (h >> 1) + ((py[i] + oy) >> 1)) * (w >> 1) + ((px[i] + ox) >> 1)) *
+                                  uc.c_low + c8 * 8);
+      else
+        a[i] = load8(in + (size_t)(p0 + 16 * i + m + oy * w + ox) * cskip + (c8 - c8low) * 8);
+    }
+  };
+
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Wave q takes steps [first, last) of the K loop.
+  auto run = [&](auto first_c, auto last_c) {
+    constexpr int first = decltype(first_c)::value, last = decltype(last_c)::value;
+    bf16x8 bq[3][NT], aq[3][4];
+    if constexpr (first < last) {
+      load_b(first, bq[0]);
+      load_a(first, aq[0]);
+    }
+    if constexpr (first + 1 < last) {
+      load_b(first + 1, bq[1]);
+      load_a(first + 1, aq[1]);
+    }
+#pragma unroll
+    for (int step = first; step < last; ++step) {
+      const int k = step - first;
+      if (step + 2 < last) {
+        load_b(step + 2, bq[(k + 2) % 3]);
+        load_a(step + 2, aq[(k + 2) % 3]);
+      }
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of this step's MFMAs
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[k % 3][j], aq[k % 3][i], acc[i][j], 0, 0, 0);
+    }
+  };
+  constexpr int q1 = kSteps / 4, q2 = kSteps / 2, q3 = (3 * kSteps) / 4;
+  switch (q) {  // wave-uniform
+    case 0: run(std::integral_constant<int, 0>{}, std::integral_constant<int, q1>{}); break;
+    case 1: run(std::integral_constant<int, q1>{}, std::integral_constant<int, q2>{}); break;
+    case 2: run(std::integral_constant<int, q2>{}, std::integral_constant<int, q3>{}); break;
+    default: run(std::integral_constant<int, q3>{}, std::integral_constant<int, kSteps>{}); break;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) part[q][i][j][lane] = acc[i][j];
+  __syncthreads();
+
+  // Wave q finishes pixel row i = q of the tile: the four partial sums in
+  // wave order, then bias, ReLU, bf16.  Lane (m, g) holds D[4g + r][m]:
+  // channel n0 + 16j + 4g + r of pixel p0 + 16q + m.
+  const int i = q;
+  if (!pv[i]) return;
+  const size_t pix = (size_t)(p0 + 16 * i + m);
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = n0 + 16 * j + 4 * g;
+    if (co >= nstore) continue;  // nstore is a multiple of 4 (checked at launch)
+    f32x4 s = part[0][i][j][lane];
+#pragma unroll
+    for (int r = 1; r < kWavesPerBlock; ++r) s += part[r][i][j][lane];
+    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + co);
+    float v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = s[r] + bv[r];
+      if (relu) v[r] = fmaxf(v[r], 0.f);
+    }
+    const size_t o = pix * nstore + co;
+    if (y) *reinterpret_cast<uint2*>(y + o) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
+    if (yf) *reinterpret_cast<f32x4*>(yf + o) = f32x4{v[0], v[1], v[2], v[3]};
+  }
+}
+
+template <int C8, bool UPCAT>
+int go_conv3_lin(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y,
+                 float* yf, int P, int h, int wd, int nstore, int relu, UpCat uc) {
+#define HPT_L(NT_)                                                                                                  \
+  hipLaunchKernelGGL((k_conv3_lin<C8, NT_, UPCAT>), grid, dim3(kThreads), 0, st, in, w, b, y, yf, P, h, wd, nstore, \
+                     relu, uc)
+  switch (nt) {
+    case 1: HPT_L(1); break;
+    case 2: HPT_L(2); break;
+    case 3: HPT_L(3); break;
+    default: HPT_L(4); break;
+  }
+#undef HPT_L
+  return (int)hipGetLastError();
+}
+
+// The specialised pixel-linear kernel for the channel counts it is built for
+// (64, 96, 128, 160, 224): one block per 64-pixel tile and grid row of
+// output channels, the channels split into as few rows (at most 4 tiles of
+// 16 each) as still give 4 blocks per CU; -1 for other channel counts.
+template <bool UPCAT>
+int launch_conv3_lin(hipStream_t st, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y, float* yf,
+                     int P, int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc) {
+  if (nstore % 4) return -1;
+  if (cin != 64 && cin != 96 && cin != 128 && cin != 160 && cin != 224) return -1;
+  const int tiles = nout / 16;
+  const unsigned pixel_blocks = xcd_grid((unsigned)((P + 63) / 64));
+  int groups = (tiles + 3) / 4;
+  while (tiles % groups) ++groups;
+  while (groups < tiles && (long long)pixel_blocks * groups < 1024) {
+    do ++groups;
+    while (tiles % groups);
+  }
+  const int nt = tiles / groups;
+  const dim3 grid(pixel_blocks, groups);
+  switch (cin) {
+    case 64: return go_conv3_lin<8, UPCAT>(st, grid, nt, in, w, b, y, yf, P, h, wd, nstore, relu, uc);
+    case 96: return go_conv3_lin<12, UPCAT>(st, grid, nt, in, w, b, y, yf, P, h, wd, nstore, relu, uc);
+    case 128: return go_conv3_lin<16, UPCAT>(st, grid, nt, in, w, b, y, yf, P, h, wd, nstore, relu, uc);
+    case 160: return go_conv3_lin<20, UPCAT>(st, grid, nt, in, w, b, y, yf, P, h, wd, nstore, relu, uc);
+    default: return go_conv3_lin<28, UPCAT>(st, grid, nt, in, w, b, y, yf, P, h, wd, nstore, relu, uc);
+  }
+}
+
 template <typename InT, int KS, int MT, int NT, bool FORWARD>
 int go_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P, int h,
             int wd, int cin, int nstore, int relu, int groups, UpCat uc) {
@@ -520,6 +721,13 @@ int go_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, ui
 template <typename InT, int KS, bool FORWARD>
 int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
                    int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc) {
+  if constexpr (KS == 3 && FORWARD && sizeof(InT) == 2) {
+    if (kConvFast) {
+      const int e = uc.low ? launch_conv3_lin<true>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc)
+                           : launch_conv3_lin<false>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc);
+      if (e >= 0) return e;
+    }
+  }
   const int tiles = nout / 16;
   int groups = (tiles + 7) / 8;
   while (tiles % groups) ++groups;
