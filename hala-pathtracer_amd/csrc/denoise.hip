@@ -336,21 +336,44 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
   const int rem = (int)b - img * tiles_x * tiles_y;
   const int ty0 = (rem / tiles_x) * kTile, tx0 = (rem - (rem / tiles_x) * tiles_x) * kTile;
   const int c8low = UPCAT ? uc.c_low >> 3 : 0;
-  for (int idx = threadIdx.x; idx < kHalo * kHalo * C8; idx += kThreads) {
-    const int pix = idx / C8, c8 = idx - pix * C8;
+  auto fill_src = [&](int pix, int c8, bool& inside) -> const uint16_t* {
     const int yy = ty0 - 1 + pix / kHalo, xx = tx0 - 1 + (pix - (pix / kHalo) * kHalo);
-    u32x4 v = u32x4{0u, 0u, 0u, 0u};
-    if ((unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w) {
-      const uint16_t* src;
-      if (!UPCAT)
-        src = in + ((size_t)(img * h + yy) * w + xx) * G::kCin + c8 * 8;
-      else if (c8 < c8low)
-        src = uc.low + ((size_t)(img * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * uc.c_low + c8 * 8;
-      else
-        src = in + ((size_t)(img * h + yy) * w + xx) * (G::kCin - uc.c_low) + (c8 - c8low) * 8;
-      v = *reinterpret_cast<const u32x4*>(src);
+    inside = (unsigned)yy < (unsigned)h && (unsigned)xx < (unsigned)w;
+    if (!UPCAT) return in + ((size_t)(img * h + yy) * w + xx) * G::kCin + c8 * 8;
+    if (c8 < c8low) return uc.low + ((size_t)(img * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * uc.c_low + c8 * 8;
+    return in + ((size_t)(img * h + yy) * w + xx) * (G::kCin - uc.c_low) + (c8 - c8low) * 8;
+  };
+  constexpr int kItems = kHalo * kHalo * C8, kPer = (kItems + kThreads - 1) / kThreads;
+  if constexpr (C8 >= 10) {
+    // Wide tiles: every thread issues all of its loads, then stores them,
+    // so the block waits for one round trip to memory instead of one per
+    // item (13-18 items per thread).  Narrow tiles keep the loop below: the
+    // staging registers cost them occupancy (measured slower for cin 32-48).
+    u32x4 fill[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int idx = threadIdx.x + k * kThreads;
+      const int pix = idx / C8, c8 = idx - pix * C8;
+      bool inside;
+      const uint16_t* s = fill_src(pix, c8, inside);
+      fill[k] = (idx < kItems && inside) ? *reinterpret_cast<const u32x4*>(s) : u32x4{0u, 0u, 0u, 0u};
     }
-    *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) = v;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int idx = threadIdx.x + k * kThreads;
+      if (idx < kItems) {
+        const int pix = idx / C8, c8 = idx - pix * C8;
+        *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) = fill[k];
+      }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < kItems; idx += kThreads) {
+      const int pix = idx / C8, c8 = idx - pix * C8;
+      bool inside;
+      const uint16_t* s = fill_src(pix, c8, inside);
+      *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) =
+          inside ? *reinterpret_cast<const u32x4*>(s) : u32x4{0u, 0u, 0u, 0u};
+    }
   }
   __syncthreads();
 
