@@ -321,12 +321,15 @@ struct TileGeom {
   static constexpr size_t kLds = (size_t)kHalo * kHalo * kStride * sizeof(uint16_t);
 };
 
-template <int C8, int NT, bool UPCAT>
+//  - with POOL, the 2 x 2 average of the bf16 output (the next encoder
+//    level's input, as k_pool_forward computes it) written beside it: a
+//    pooling window is two rows of one wave and two neighbouring lanes.
+template <int C8, int NT, bool UPCAT, bool POOL = false>
 __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restrict__ in,
                                                          const uint16_t* __restrict__ wgt,
                                                          const float* __restrict__ bias, uint16_t* __restrict__ y,
                                                          float* __restrict__ yf, int n_img, int h, int w, int nstore,
-                                                         int relu, UpCat uc) {
+                                                         int relu, UpCat uc, uint16_t* __restrict__ ypool) {
   using G = TileGeom<C8>;
   extern __shared__ __align__(16) uint16_t tile[];  // kHalo * kHalo pixels x kStride
   const int tiles_x = (w + kTile - 1) / kTile, tiles_y = (h + kTile - 1) / kTile;
@@ -347,8 +350,8 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
   if constexpr (C8 >= 10) {
     // Wide tiles: every thread issues all of its loads, then stores them,
     // so the block waits for one round trip to memory instead of one per
-    // item (13-18 items per thread).  Narrow tiles keep the loop below: the
-    // staging registers cost them occupancy (measured slower for cin 32-48).
+    // item (13-18 items per thread).  Narrow tiles stage three items at a
+    // time: all of theirs cost occupancy (measured slower for cin 32-48).
     u32x4 fill[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -367,12 +370,27 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
       }
     }
   } else {
-    for (int idx = threadIdx.x; idx < kItems; idx += kThreads) {
-      const int pix = idx / C8, c8 = idx - pix * C8;
-      bool inside;
-      const uint16_t* s = fill_src(pix, c8, inside);
-      *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) =
-          inside ? *reinterpret_cast<const u32x4*>(s) : u32x4{0u, 0u, 0u, 0u};
+    // Three items in flight per round.
+#pragma unroll
+    for (int k0 = 0; k0 < kPer; k0 += 3) {
+      u32x4 fill[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int idx = threadIdx.x + (k0 + k) * kThreads;
+        const int pix = idx / C8, c8 = idx - pix * C8;
+        bool inside;
+        const uint16_t* s = fill_src(pix, c8, inside);
+        fill[k] = (k0 + k < kPer && idx < kItems && inside) ? *reinterpret_cast<const u32x4*>(s)
+                                                            : u32x4{0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int idx = threadIdx.x + (k0 + k) * kThreads;
+        if (k0 + k < kPer && idx < kItems) {
+          const int pix = idx / C8, c8 = idx - pix * C8;
+          *reinterpret_cast<u32x4*>(tile + pix * G::kStride + c8 * 8) = fill[k];
+        }
+      }
     }
   }
   __syncthreads();
@@ -427,18 +445,20 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
   }
 
   // Lane (m, g) holds D[4g + r][m]: channel n0 + 16j + 4g + r of tile row
-  // 4q + i, column m.
+  // 4q + i, column m.  Every lane runs the loops (the pooling shuffles need
+  // them all); stores are guarded.
   const int xx = tx0 + m;
-  if (xx >= w) return;
+  const bool xok = xx < w;
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     const int co = n0 + 16 * j + 4 * g;
-    if (co >= nstore) continue;  // nstore is a multiple of 4 (checked at launch)
-    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + co);
+    const bool cok = co < nstore;  // nstore is a multiple of 4 (checked at launch)
+    const f32x4 bv = cok ? *reinterpret_cast<const f32x4*>(bias + co) : f32x4{0.f, 0.f, 0.f, 0.f};
+    float top[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int yy = ty0 + 4 * q + i;
-      if (yy >= h) continue;
+      const bool ok = xok && cok && yy < h;
       const size_t o = ((size_t)(img * h + yy) * w + xx) * nstore + co;
       float v[4];
 #pragma unroll
@@ -446,35 +466,56 @@ __global__ __launch_bounds__(kThreads) void k_conv3_fast(const uint16_t* __restr
         v[r] = acc[i][j][r] + bv[r];
         if (relu) v[r] = fmaxf(v[r], 0.f);
       }
-      if (y) *reinterpret_cast<uint2*>(y + o) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
-      if (yf) *reinterpret_cast<f32x4*>(yf + o) = f32x4{v[0], v[1], v[2], v[3]};
+      if (ok && y) *reinterpret_cast<uint2*>(y + o) = make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
+      if (ok && yf) *reinterpret_cast<f32x4*>(yf + o) = f32x4{v[0], v[1], v[2], v[3]};
+      if constexpr (POOL) {
+        float cur[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cur[r] = bf16_value(bf16_bits(v[r]));  // the stored value
+        if ((i & 1) == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) top[r] = cur[r];
+        } else {
+          // Window rows yy - 1, yy; columns xx, xx + 1 (lanes m, m + 1).
+          float pv[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float top_r = __shfl_xor(top[r], 1), bot_r = __shfl_xor(cur[r], 1);
+            pv[r] = 0.25f * ((top[r] + top_r) + (cur[r] + bot_r));
+          }
+          if (ok && (m & 1) == 0) {
+            const size_t po = ((size_t)(img * (h >> 1) + (yy >> 1)) * (w >> 1) + (xx >> 1)) * nstore + co;
+            *reinterpret_cast<uint2*>(ypool + po) = make_uint2(pack_bf16(pv[0], pv[1]), pack_bf16(pv[2], pv[3]));
+          }
+        }
+      }
     }
   }
 }
 
-template <int C8, int NT, bool UPCAT>
+template <int C8, int NT, bool UPCAT, bool POOL>
 bool allow_fast_lds() {  // above 64 KB a kernel must ask for its dynamic LDS
-  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_fast<C8, NT, UPCAT>),
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_fast<C8, NT, UPCAT, POOL>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileLdsMax) ==
                          hipSuccess;
   return ok;
 }
 
-template <int C8, bool UPCAT>
+template <int C8, bool UPCAT, bool POOL>
 int go_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y,
-                  float* yf, int n_img, int h, int wd, int nstore, int relu, UpCat uc) {
+                  float* yf, int n_img, int h, int wd, int nstore, int relu, UpCat uc, uint16_t* ypool) {
   constexpr size_t lds = TileGeom<C8>::kLds;
   static_assert(lds <= kTileLdsMax, "tile exceeds the LDS budget");
   bool ok = true;
   if (lds > 65536)
-    ok = nt == 1 ? allow_fast_lds<C8, 1, UPCAT>()
-         : nt == 2 ? allow_fast_lds<C8, 2, UPCAT>()
-         : nt == 3 ? allow_fast_lds<C8, 3, UPCAT>()
-                   : allow_fast_lds<C8, 4, UPCAT>();
+    ok = nt == 1 ? allow_fast_lds<C8, 1, UPCAT, POOL>()
+         : nt == 2 ? allow_fast_lds<C8, 2, UPCAT, POOL>()
+         : nt == 3 ? allow_fast_lds<C8, 3, UPCAT, POOL>()
+                   : allow_fast_lds<C8, 4, UPCAT, POOL>();
   if (!ok) return -1;
 #define HPT_F(NT_)                                                                                                   \
-  hipLaunchKernelGGL((k_conv3_fast<C8, NT_, UPCAT>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, wd, \
-                     nstore, relu, uc)
+  hipLaunchKernelGGL((k_conv3_fast<C8, NT_, UPCAT, POOL>), grid, dim3(kThreads), lds, st, in, w, b, y, yf, n_img, h, \
+                     wd, nstore, relu, uc, ypool)
   switch (nt) {
     case 1: HPT_F(1); break;
     case 2: HPT_F(2); break;
@@ -486,18 +527,20 @@ int go_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const u
 }
 
 // The specialised kernel for the channel counts it is built for (16, 32,
-// 48, 64, 80, 112); -1 for others.
-template <bool UPCAT>
+// 48, 64, 80, 112); -1 for others.  POOL (not with UPCAT): also the 2 x 2
+// average pooling of the output into ypool.
+template <bool UPCAT, bool POOL = false>
 int launch_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, const uint16_t* w, const float* b,
-                      uint16_t* y, float* yf, int n_img, int h, int wd, int cin, int nstore, int relu, UpCat uc) {
+                      uint16_t* y, float* yf, int n_img, int h, int wd, int cin, int nstore, int relu, UpCat uc,
+                      uint16_t* ypool = nullptr) {
   if (nstore % 4) return -1;
   switch (cin) {
-    case 16: return go_conv3_fast<2, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
-    case 32: return go_conv3_fast<4, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
-    case 48: return go_conv3_fast<6, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
-    case 64: return go_conv3_fast<8, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
-    case 80: return go_conv3_fast<10, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
-    case 112: return go_conv3_fast<14, UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc);
+    case 16: return go_conv3_fast<2, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
+    case 32: return go_conv3_fast<4, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
+    case 48: return go_conv3_fast<6, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
+    case 64: return go_conv3_fast<8, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
+    case 80: return go_conv3_fast<10, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
+    case 112: return go_conv3_fast<14, UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, nstore, relu, uc, ypool);
     default: return -1;
   }
 }
@@ -507,9 +550,10 @@ int launch_conv3_fast(hipStream_t st, dim3 grid, int nt, const uint16_t* in, con
 // the CUs busier with the pixel-linear kernel and its channel split), and
 // the output channels split into tiles of at most 4 x 16; returns -1 when
 // it does not apply.
-template <bool UPCAT>
+template <bool UPCAT, bool POOL = false>
 int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, const float* b, uint16_t* y, float* yf,
-                      int n_img, int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc) {
+                      int n_img, int h, int wd, int cin, int nout, int nstore, int relu, UpCat uc,
+                      uint16_t* ypool = nullptr) {
   const size_t lds = (size_t)kHalo * kHalo * cin * sizeof(uint16_t);
   const unsigned blocks = (unsigned)(n_img * ((h + kTile - 1) / kTile) * ((wd + kTile - 1) / kTile));
   if (lds > kTileLdsMax || blocks < 256) return -1;
@@ -523,9 +567,11 @@ int launch_conv3_tile(hipStream_t st, const uint16_t* in, const uint16_t* w, con
   const dim3 grid(xcd_grid(blocks), groups);
   const int nt = tiles / groups;
   if (kConvFast) {
-    const int e = launch_conv3_fast<UPCAT>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc);
+    const int e = launch_conv3_fast<UPCAT, POOL>(st, grid, nt, in, w, b, y, yf, n_img, h, wd, cin, nstore, relu, uc,
+                                                 ypool);
     if (e >= 0) return e;
   }
+  if (POOL) return -1;  // only the specialised kernel pools
   if (lds > 65536) {
     const bool ok = nt == 1   ? allow_tile_lds<1, UPCAT>()
                     : nt == 2 ? allow_tile_lds<2, UPCAT>()
@@ -778,9 +824,53 @@ int launch_conv_ks(hipStream_t st, const InT* in, const uint16_t* w, const float
   }
 }
 
+// A 1x1 forward convolution to at most 4 channels (the network's output
+// layer, 32 -> 3): one thread per pixel reads the pixel's channels once
+// (16-byte loads) and sums each output over them in channel order; the
+// weights are the same for every thread (scalar loads).  As an MFMA GEMM a
+// wave would keep 13 of its 16 output columns idle and store 4 bytes at a
+// time; this way the layer streams its input at close to the HBM rate.
+template <int C8, int NO>
+__global__ __launch_bounds__(kThreads) void k_conv1_narrow(const uint16_t* __restrict__ in,
+                                                           const uint16_t* __restrict__ wgt,
+                                                           const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                                           float* __restrict__ yf, int P, int relu) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= P) return;
+  constexpr int kCin = 8 * C8;
+  float acc[NO];
+#pragma unroll
+  for (int o = 0; o < NO; ++o) acc[o] = 0.f;
+#pragma unroll
+  for (int s = 0; s < C8; ++s) {
+    const u32x4 v = *reinterpret_cast<const u32x4*>(in + (size_t)p * kCin + s * 8);
+    const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = bf16_value((wv[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+#pragma unroll
+      for (int o = 0; o < NO; ++o) acc[o] = fmaf(x, bf16_value(wgt[o * kCin + s * 8 + e]), acc[o]);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    float v = acc[o] + bias[o];
+    if (relu) v = fmaxf(v, 0.f);
+    if (y) y[(size_t)p * NO + o] = (uint16_t)bf16_bits(v);
+    if (yf) yf[(size_t)p * NO + o] = v;
+  }
+}
+
 template <typename InT, bool FORWARD>
 int launch_conv(hipStream_t st, const InT* in, const uint16_t* w, const float* b, uint16_t* y, float* yf, int P,
                 int h, int wd, int cin, int nout, int nstore, int relu, int k, UpCat uc = UpCat{nullptr, 0}) {
+  if constexpr (FORWARD && sizeof(InT) == 2) {
+    if (k == 1 && kConvFast && cin == 32 && nstore == 3) {
+      hipLaunchKernelGGL((k_conv1_narrow<4, 3>), dim3((unsigned)((P + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, in, w, b, y, yf,
+                         P, relu);
+      return (int)hipGetLastError();
+    }
+  }
   if (k == 1) return launch_conv_ks<InT, 1, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc);
   return launch_conv_ks<InT, 3, FORWARD>(st, in, w, b, y, yf, P, h, wd, cin, nout, nstore, relu, uc);
 }
@@ -1176,6 +1266,21 @@ int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16_t* w, cons
   }
   return launch_conv<uint16_t, true>((hipStream_t)stream, x, w, b, y, y_f32, n * h * wd, h, wd, cin, cout_pad, cout,
                                      relu, k);
+}
+
+int hpt_nn_conv_forward_pool(void* stream, const uint16_t* x, const uint16_t* w, const float* b, uint16_t* y,
+                             uint16_t* y_pool, int n, int h, int wd, int cin, int cout, int cout_pad, int relu) {
+  if (!conv_shape_ok(n, h, wd, cin, cout_pad, 3) || cout <= 0 || cout > cout_pad || (h | wd) & 1 || !x || !w || !b ||
+      !y || !y_pool)
+    return bad();
+  if (kConvTile) {
+    const int e = launch_conv3_tile<false, true>((hipStream_t)stream, x, w, b, y, nullptr, n, h, wd, cin, cout_pad,
+                                                 cout, relu, UpCat{nullptr, 0}, y_pool);
+    if (e >= 0) return e;
+  }
+  const int e = hpt_nn_conv_forward(stream, x, w, b, y, nullptr, n, h, wd, cin, cout, cout_pad, 3, relu);
+  if (e) return e;
+  return hpt_nn_pool_forward(stream, y, y_pool, n, h, wd, cout);
 }
 
 int hpt_nn_conv_forward_upcat(void* stream, const uint16_t* low, const uint16_t* skip, const uint16_t* w,

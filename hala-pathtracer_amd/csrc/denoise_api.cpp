@@ -306,14 +306,16 @@ void forward(hpt_denoiser* d, int h, int w) {
   for (int l = 0; l < L; ++l) {
     const int hl = h >> l, wl = w >> l;
     const std::string s = std::to_string(l);
-    if (l > 0) {
-      hip_ok(hpt_nn_pool_forward(d->stream, B(d, "y_enc" + std::to_string(l - 1) + "b"), B(d, "pool" + s), 1, 2 * hl,
-                                 2 * wl, d->levels[l - 1]),
-             "pooling");
-      cur = B(d, "pool" + s);
-    }
+    if (l > 0) cur = B(d, "pool" + s);  // written by the previous level's second convolution
     conv(d, layer(d, "enc" + s + "a"), cur, B(d, "y_enc" + s + "a"), nullptr, hl, wl);
-    conv(d, layer(d, "enc" + s + "b"), B(d, "y_enc" + s + "a"), B(d, "y_enc" + s + "b"), nullptr, hl, wl);
+    const Layer& eb = layer(d, "enc" + s + "b");
+    if (l + 1 < L)
+      hip_ok(hpt_nn_conv_forward_pool(d->stream, B(d, "y_enc" + s + "a"), static_cast<const u16*>(eb.w), bias(d, eb),
+                                      B(d, "y_enc" + s + "b"), B(d, "pool" + std::to_string(l + 1)), 1, hl, wl, eb.cin,
+                                      eb.cout, eb.cout_pad, eb.relu ? 1 : 0),
+             ("convolution " + eb.name).c_str());
+    else
+      conv(d, eb, B(d, "y_enc" + s + "a"), B(d, "y_enc" + s + "b"), nullptr, hl, wl);
     cur = B(d, "y_enc" + s + "b");
   }
   for (int l = L - 2; l >= 0; --l) {

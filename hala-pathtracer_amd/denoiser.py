@@ -44,6 +44,7 @@ _vp, _i, _sz, _f = C.c_void_p, C.c_int, C.c_size_t, C.c_float
 NN_SIGNATURES = {
     "hpt_nn_conv_forward": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_conv_forward_upcat": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i],
+    "hpt_nn_conv_forward_pool": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_conv_backward_data": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_conv_backward_weights": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "hpt_nn_wgrad_scratch_floats": [_i, _i, _i, _i, _i, _i],
@@ -364,6 +365,11 @@ class UNet:
         _call("hpt_nn_conv_forward", stream(), _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_f32), n, h, w,
               L.cin, L.cout, L.cout_pad, L.k, int(L.relu))
 
+    def _conv_pool(self, L, x, y, y_pool, n, h, w):
+        """L (3x3) into y and its 2x2 average pooling into y_pool."""
+        _call("hpt_nn_conv_forward_pool", stream(), _p(x), _p(L.w), _p(self.params.addr(L.b_off)), _p(y), _p(y_pool),
+              n, h, w, L.cin, L.cout, L.cout_pad, int(L.relu))
+
     def _conv_upcat(self, L, low, skip, y, n, h, w, c_low, c_skip):
         """L over concat(upsample2x(low), skip) without materialising it."""
         _call("hpt_nn_conv_forward_upcat", stream(), _p(low), _p(skip), _p(L.w), _p(self.params.addr(L.b_off)), _p(y),
@@ -382,11 +388,12 @@ class UNet:
         for l in range(Lv):
             hl, wl = h >> l, w >> l
             if l > 0:
-                _call("hpt_nn_pool_forward", st, _p(b[f"y_enc{l - 1}b"]), _p(b[f"pool{l}"]), n, 2 * hl, 2 * wl,
-                      self.levels[l - 1])
-                cur = b[f"pool{l}"]
+                cur = b[f"pool{l}"]  # written by the previous level's second convolution
             self._conv(self.layers[f"enc{l}a"], cur, b[f"y_enc{l}a"], None, n, hl, wl)
-            self._conv(self.layers[f"enc{l}b"], b[f"y_enc{l}a"], b[f"y_enc{l}b"], None, n, hl, wl)
+            if l + 1 < Lv:
+                self._conv_pool(self.layers[f"enc{l}b"], b[f"y_enc{l}a"], b[f"y_enc{l}b"], b[f"pool{l + 1}"], n, hl, wl)
+            else:
+                self._conv(self.layers[f"enc{l}b"], b[f"y_enc{l}a"], b[f"y_enc{l}b"], None, n, hl, wl)
             cur = b[f"y_enc{l}b"]
         for l in range(Lv - 2, -1, -1):
             hl, wl = h >> l, w >> l
@@ -520,8 +527,9 @@ class UNet:
 
     def profile_forward(self, n: int, h: int, w: int, reps: int = 5) -> dict:
         """{layer: (average ms, algorithmic FLOP)} of each convolution of the
-        forward pass at n x h x w, timed with HIP events on the stream the
-        kernels run on; the input buffer is used as it is."""
+        forward pass at n x h x w (an encoder's second convolution with the
+        pooling it writes), timed with HIP events on the stream the kernels
+        run on; the input buffer is used as it is."""
         b = self._buffers(n, h, w, False)
         ev = {name: [] for name in self.layers}
 
@@ -535,11 +543,12 @@ class UNet:
             return run
 
         self._conv, self._conv_upcat = timed(self._conv), timed(self._conv_upcat)
+        self._conv_pool = timed(self._conv_pool)
         try:
             for _ in range(reps):
                 self.forward_input(b["x"], n, h, w)
         finally:
-            del self._conv, self._conv_upcat
+            del self._conv, self._conv_upcat, self._conv_pool
         synchronize()
         out = {}
         for name, L in self.layers.items():

@@ -31,6 +31,14 @@ HPT_NN_API int hpt_nn_conv_forward(void* stream, const uint16_t* x, const uint16
                                    uint16_t* y, float* y_f32, int n, int h, int wd, int cin, int cout,
                                    int cout_pad, int k, int relu);
 
+/* hpt_nn_conv_forward (3x3) that also writes the 2 x 2 average pooling of y
+ * (the next encoder level's input, as hpt_nn_pool_forward computes it) into
+ * y_pool (n x h/2 x w/2 x cout bf16); h and w even.  Same bits as the two
+ * calls; the tiled kernel writes both from one pass over the output. */
+HPT_NN_API int hpt_nn_conv_forward_pool(void* stream, const uint16_t* x, const uint16_t* w, const float* b,
+                                        uint16_t* y, uint16_t* y_pool, int n, int h, int wd, int cin, int cout,
+                                        int cout_pad, int relu);
+
 /* hpt_nn_conv_forward with k = 3 on the input concat(upsample2x(low), skip)
  * without materialising it: `low` bf16 [n][h/2][wd/2][c_low] read with
  * nearest upsampling, `skip` bf16 [n][h][wd][c_skip] (h and wd even, channel

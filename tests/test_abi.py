@@ -90,7 +90,7 @@ def test_denoiser_header_symbols_are_exported_and_typed():
     import denoiser
 
     names = sorted(set(re.findall(r"HPT_NN_API\s+[\w\s\*]+?\b(hpt_nn_\w+)\s*\(", open(NN_HEADER).read())))
-    assert len(names) == 30
+    assert len(names) == 31
     assert set(names) == set(denoiser.NN_SIGNATURES)
     lib = denoiser.nn_library()
     for name in names:

@@ -158,6 +158,33 @@ def test_conv_forward_upcat_equals_upsample_concat_conv(dn, shape):
     assert bad != 0  # odd height: no half-resolution tensor to upsample
 
 
+POOL_SHAPES = [  # n, h, w, cin, cout: tiled (cin 32, 48, 64, the encoders' second convolutions), pixel-linear
+    (4, 128, 128, 32, 32), (4, 128, 128, 48, 48), (2, 256, 144, 64, 64), (1, 30, 40, 96, 96)]
+
+
+@pytest.mark.parametrize("shape", POOL_SHAPES)
+def test_conv_forward_pool_equals_conv_then_pool(dn, shape):
+    """The encoder's second convolution with the next level's 2x2 average
+    pooling written from the same pass gives the bits of the two calls."""
+    n, h, wd, cin, cout = shape
+    print(f"conv pool {shape}", flush=True)
+    rng = np.random.default_rng(11)
+    x = _dev_bf16(rng.normal(0, 1, (n, h, wd, cin)).astype(np.float32))
+    _, w, _, cp = _weights(dn, rng, cin, cout, 3)
+    b = _dev(rng.normal(0, 0.1, cout).astype(np.float32))
+    y_ref, y = _zeros(n * h * wd * cout, np.uint16), _zeros(n * h * wd * cout, np.uint16)
+    p_ref, pool = _zeros(n * h * wd * cout // 4, np.uint16), _zeros(n * h * wd * cout // 4, np.uint16)
+    _call(dn, "hpt_nn_conv_forward", dn.stream(), _p(x), _p(w), _p(b), _p(y_ref), None, n, h, wd, cin, cout, cp, 3, 1)
+    _call(dn, "hpt_nn_pool_forward", dn.stream(), _p(y_ref), _p(p_ref), n, h, wd, cout)
+    _call(dn, "hpt_nn_conv_forward_pool", dn.stream(), _p(x), _p(w), _p(b), _p(y), _p(pool), n, h, wd, cin, cout, cp, 1)
+    dn.synchronize()
+    assert np.array_equal(y.numpy(), y_ref.numpy())
+    assert np.array_equal(pool.numpy(), p_ref.numpy())
+    bad = dn.nn_library().hpt_nn_conv_forward_pool(dn.stream(), _p(x), _p(w), _p(b), _p(y), _p(pool), n, h - 1, wd,
+                                                     cin, cout, cp, 1)
+    assert bad != 0  # odd height: no 2x2 pooling
+
+
 @pytest.mark.parametrize("shape", CONV_SHAPES)
 def test_conv_backward_data_matches_oracle(dn, shape):
     n, h, wd, cin, cout, k = shape
