@@ -91,20 +91,23 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s
             "sample": f"{what} ({paths} paths, {dt:.1f} s)"}
 
 
-def profiled_traffic(kernel, workload, batch_frames, tail_from):
+def profiled_traffic(kernel, workload, batch_frames, tail_from, block):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     of the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
     the reading of a 1 GiB copy), when it was taken at the same batch size
-    (frames traced together) and tail bounce; else None."""
+    (frames traced together), tail bounce and traversal block size; else
+    None."""
     d = os.path.join(ROOT, "profiles")
     tail = "none" if tail_from == hpt.TAIL_FROM_NONE else str(tail_from)
+    pat = r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+), tail_from (\w+), block (\d+)\) = (\d+)"
     for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
         if not name.endswith(f"_{workload}.md"):
             continue
         for line in open(os.path.join(d, name)):
-            m = re.match(r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+), tail_from (\w+)\) = (\d+)", line)
-            if m and m.group(1) == kernel and int(m.group(2)) == batch_frames and m.group(3) == tail:
-                return int(m.group(4))
+            m = re.match(pat, line)
+            if m and m.group(1) == kernel and int(m.group(2)) == batch_frames and m.group(3) == tail and \
+                    int(m.group(4)) == block:
+                return int(m.group(5))
     return None
 
 
@@ -240,7 +243,7 @@ def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
             "curve_spp_render_s_denoise_ms_plain_denoised": curve}
 
 
-def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort, tails):
+def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort, tails, blocks):
     """The same target through the serving path: one job per variant
     submitted to serve.Server -- plain at the plain tracer's threshold sample
     count, denoised (the server denoises finished jobs on the device) at the
@@ -255,7 +258,8 @@ def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_den
     try:
         for label, spp, den in (("plain", spp_plain, None), ("denoised", spp_denoised, net)):
             tuning = autotune.TuningCache()
-            tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort, tails[label]))
+            tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort, tails[label],
+                                                         blocks[label]))
             srv = serve.Server({"scene": rs}, 1 << 38, max_frames=32, keep_images=True, tuning=tuning, denoiser=den)
             srv.run([serve.JobSpec(0, "scene", w, h, min(spp, 8), depth, rr, 0.0)])  # warm-up: allocations, capture
             res = srv.run([serve.JobSpec(0, "scene", w, h, spp, depth, rr, 0.0)])[0]
@@ -488,6 +492,8 @@ def main():
                          "choice among off and bounces 0-2 at this batch size, measured before the timed region)")
     ap.add_argument("--launch", choices=["graph", "eager"], default="graph",
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
+    ap.add_argument("--traversal-block", default="auto",
+                    help="threads per block of the traversal kernels: auto (autotuned with the sort), 64, 128 or 256")
     ap.add_argument("--tail-from", default="auto",
                     help="first bounce run by the tail kernel: auto (autotuned), none, or a bounce index")
     ap.add_argument("--vmis-phases", choices=["on", "off"], default="on",
@@ -544,11 +550,14 @@ def main():
         rr.set_shade_sort(shade_sort)
         rr.set_vmis_phases(args.vmis_phases == "on")
         rr.set_tail_from(tails["denoised"] if frames_in_flight else tails["plain"])
+        rr.set_traversal_block(blocks["denoised"] if frames_in_flight else blocks["plain"])
         return rr
 
     shade_sort = hpt.SHADE_SORT_OFF if args.shade_sort in ("off", "auto") else int(args.shade_sort)
     fixed_tail = hpt.TAIL_FROM_NONE if args.tail_from in ("none", "auto") else int(args.tail_from)
     tails = {"plain": fixed_tail, "denoised": fixed_tail}  # big batches / 2-frame batches
+    fixed_block = 128 if args.traversal_block == "auto" else int(args.traversal_block)
+    blocks = {"plain": fixed_block, "denoised": fixed_block}  # threads per block of the traversal kernels
     t0 = time.perf_counter()
     r = make_renderer()
     log(f"create + commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
@@ -568,6 +577,9 @@ def main():
         if args.tail_from == "auto":
             tails["plain"] = tuned.tail_from
             r.set_tail_from(tuned.tail_from)
+        if args.traversal_block == "auto":
+            blocks["plain"] = tuned.trav_block
+            r.set_traversal_block(tuned.trav_block)
         log(f"shade sort and tail autotuned in {time.perf_counter() - t0:.1f} s: "
             f"{'off' if shade_sort == hpt.SHADE_SORT_OFF else f'from bounce {shade_sort}'}, tail from "
             f"{'none' if tails['plain'] == hpt.TAIL_FROM_NONE else tails['plain']} ({sort_table})")
@@ -637,13 +649,14 @@ def main():
     # the extension's rate per closest-hit query); the walk kernel (that rate
     # per query) when shading is phased.
     roof_ext = stage_roofline("k_extend", ext_t, NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
-                              traffic=profiled_traffic("k_extend", args.workload, batch_frames, tails["plain"]),
+                              traffic=profiled_traffic("k_extend", args.workload, batch_frames, tails["plain"],
+                                                       blocks["plain"]),
                               extra={"nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)})
     shade_bytes = SHADE_EVENT_BYTES + (0.0 if phased else walk_queries * walk_bytes_per_query)
     shade_name = "k_vmis_a/b/c" if phased else "k_shade"
     roof_shade = stage_roofline(shade_name, shade_t, shade_bytes,
                                 traffic=profiled_traffic("k_vmis" if phased else "k_shade", args.workload,
-                                                         batch_frames, tails["plain"]),
+                                                         batch_frames, tails["plain"], blocks["plain"]),
                                 extra={"bytes_per_event": round(shade_bytes, 1),
                                        "walk_queries_per_event": round(walk_queries, 4)})
     rooflines = {"extend": roof_ext, "shade": roof_shade}
@@ -652,7 +665,8 @@ def main():
         # run's shading events times the warm-up's queries per event
         walk_t = dict(walk_t, rays=int(round(shade_t["rays"] * walk_queries)))
         rooflines["walk"] = stage_roofline("k_walk", walk_t, walk_bytes_per_query,
-                                           traffic=profiled_traffic("k_walk", args.workload, batch_frames, tails["plain"]),
+                                           traffic=profiled_traffic("k_walk", args.workload, batch_frames,
+                                                                    tails["plain"], blocks["plain"]),
                                            extra={"bytes_per_query": round(walk_bytes_per_query, 1)})
     stage_of = {"extend": ext_t, "shade": shade_t, "shadow": shadow_t, "walk": walk_t}
     dominant = max((k for k in rooflines), key=lambda k: stage_of[k]["total_ms"])
@@ -677,6 +691,8 @@ def main():
                                    modes=(hpt.LAUNCH_GRAPH,), sorts=())
                 tmpl.close()
                 tails["denoised"] = t2.tail_from
+                if args.traversal_block == "auto":
+                    blocks["denoised"] = t2.trav_block
                 log(f"tail for 2-frame batches: {'none' if t2.tail_from == hpt.TAIL_FROM_NONE else t2.tail_from} "
                     f"({ {k: round(v, 1) for k, v in t2.table.items()} })")
             den = time_to_relmse_denoised(make_renderer, ref, denoiser.UNet.load(path), args.threshold,
@@ -685,7 +701,7 @@ def main():
             if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
                 den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
                 conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, denoiser.UNet.load(path),
-                                               conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort, tails)
+                                               conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort, tails, blocks)
             conv["denoised"] = den
     serving = None
     if args.serving:
@@ -698,6 +714,7 @@ def main():
               "shade_sort": "off" if shade_sort == hpt.SHADE_SORT_OFF else shade_sort,
               "shade_sort_tuning": sort_table,
               "tail_from": {k: (None if v == hpt.TAIL_FROM_NONE else v) for k, v in tails.items()},
+              "traversal_block": blocks,
               "vmis_phases": args.vmis_phases if FEATURES & hpt.FEATURE_VOL_MIS else None,
               "eager_profiled_wall_s": round(profiled_wall, 4),
               "stage_ms": {n: round(stats[s]["total_ms"], 3) for s, n in enumerate(names)},

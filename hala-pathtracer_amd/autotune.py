@@ -6,7 +6,8 @@ queue size, `hpt_set_frames_in_flight`), the launch mode (graph replay or
 eager launches), the shading sort threshold (the first bounce whose paths
 are grouped by material before shading, `hpt_set_shade_sort`, or never) and
 the bounce from which the live paths run to their end in the tail kernel
-(`hpt_set_tail_from`, or none) for one (scene, resolution, depth) class.  Larger batches
+(`hpt_set_tail_from`, or none) and the threads per block of the traversal
+kernels (`hpt_set_traversal_block`) for one (scene, resolution, depth) class.  Larger batches
 fill the GPU with more paths per launch and amortise the short tail bounces;
 they cost k * W * H * 288 bytes of device memory, which the serving loop
 spends from its admission budget.  The tuner measures, it does not model:
@@ -14,7 +15,8 @@ each candidate renders a warm-up batch and then a timed run on the job's own
 scene, and the smallest batch within `tolerance` of the best throughput
 wins (less memory and a finer progress granularity for the same speed),
 then graph replay, then no sort (it costs 5 bytes per path); the tail
-bounce, which costs no memory, is the fastest at that choice.
+bounce and the traversal block size, which cost no memory, are the fastest
+at that choice.
 
 Results are kept in a TuningCache keyed by (scene name, width, height,
 max_depth) and can be saved as JSON.
@@ -33,6 +35,10 @@ SORT_BYTES = 5  # key byte and sorted entry per queue slot while shading sort is
 CANDIDATES = (2, 4, 8, 16, 32, 64)
 SORT_CANDIDATES = (0, 1, 2)
 TAIL_CANDIDATES = (3, 4, 5, 6, 8)
+BLOCK_CANDIDATES = (64, 256)  # besides the default 128
+# A tail bounce or block size replaces the current choice only when it is this
+# much faster: near-equal candidates then do not flip from run to run.
+MARGIN = 0.015
 
 
 @dataclass
@@ -44,6 +50,7 @@ class Tuning:
     seconds: float = 0.0                       # time the tuning took
     shade_sort: int = hpt.SHADE_SORT_OFF       # first sorted bounce
     tail_from: int = hpt.TAIL_FROM_NONE        # first bounce run by the tail kernel
+    trav_block: int = 128                      # threads per block of the traversal kernels
 
 
 def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: int | None = None,
@@ -63,7 +70,7 @@ def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: 
 def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, rr_depth: int = 3, features: int = 0,
          samples: float = 32e6, max_frames: int = 64, memory_bytes: int | None = None, tolerance: float = 0.03,
          modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), sorts=SORT_CANDIDATES, candidates=CANDIDATES,
-         tails=TAIL_CANDIDATES, device: int = 0) -> Tuning:
+         tails=TAIL_CANDIDATES, blocks=BLOCK_CANDIDATES, device: int = 0) -> Tuning:
     """Measures every candidate batch size (then launch mode, then shading
     sort threshold, then tail bounce) on a handle that shares `template`'s
     committed scene, each over about `samples` paths (whole batches); returns
@@ -74,11 +81,12 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
     try:
         r.share_scene(template)
 
-        def measure(k, mode, sort=hpt.SHADE_SORT_OFF, tail=hpt.TAIL_FROM_NONE):
+        def measure(k, mode, sort=hpt.SHADE_SORT_OFF, tail=hpt.TAIL_FROM_NONE, block=128):
             r.frames_in_flight = k
             r.set_launch_mode(mode)
             r.set_shade_sort(sort)
             r.set_tail_from(tail)
+            r.set_traversal_block(block)
             n = max(1, -(-int(samples) // (width * height * k))) * k
             r.reset_accumulation()
             r.render(k)  # warm-up: allocation and graph capture
@@ -91,6 +99,8 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
                 key += f"/s{sort}"
             if tail != hpt.TAIL_FROM_NONE:
                 key += f"/t{tail}"
+            if block != 128:
+                key += f"/b{block}"
             table[key] = width * height * n / (time.perf_counter() - t0) / 1e6
 
         # Batch sizes with the first mode, then the other modes at the best size.
@@ -106,7 +116,7 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
         ok = []
         for key, v in table.items():
             parts = key.split("/")
-            if v >= (1 - tolerance) * best and len(parts) <= 3:
+            if v >= (1 - tolerance) * best and len(parts) <= 3 and not parts[-1].startswith(("t", "b")):
                 sort = int(parts[2][1:]) if len(parts) > 2 else hpt.SHADE_SORT_OFF
                 ok.append((int(parts[0]), parts[1], sort, v))
         k, mode, sort, v = min(ok, key=lambda t: (t[0], t[1] != "graph", t[2] != hpt.SHADE_SORT_OFF, -t[3]))
@@ -119,11 +129,19 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
             before = set(table)
             measure(k, mode, sort, tf)
             (key,) = set(table) - before
-            if table[key] > v:
+            if table[key] > v * (1 + MARGIN):
                 tail, v = tf, table[key]
+        # The traversal block size at that choice.
+        block = 128
+        for bs in blocks:
+            before = set(table)
+            measure(k, mode, sort, tail, bs)
+            (key,) = set(table) - before
+            if table[key] > v * (1 + MARGIN):
+                block, v = bs, table[key]
     finally:
         r.close()
-    return Tuning(k, mode, v, table, time.perf_counter() - t_start, sort, tail)
+    return Tuning(k, mode, v, table, time.perf_counter() - t_start, sort, tail, block)
 
 
 class TuningCache:

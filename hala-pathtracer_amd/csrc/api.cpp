@@ -138,6 +138,8 @@ struct hpt_renderer {
   // Bounces from tail_from on run in the tail kernel (HPT_TAIL_FROM_NONE:
   // only the paths that outlive max_depth + 1 bounces with TRANSPARENT).
   uint32_t tail_from = HPT_TAIL_FROM_NONE;
+  // Threads per block of the traversal kernels (extend, walk, shadow).
+  uint32_t trav_block = kDefaultTraversalBlock;
   DevBuf d_accum, d_albedo, d_normal, d_final;
   uint32_t npix = 0;          // width * height
   uint32_t batch_cap = 0;     // frames the path buffers hold
@@ -177,6 +179,7 @@ struct hpt_group {
   uint32_t launch_mode = HPT_LAUNCH_GRAPH;
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
   uint32_t tail_from = HPT_TAIL_FROM_NONE;
+  uint32_t trav_block = kDefaultTraversalBlock;
   size_t cap = 0;  // paths the path buffers hold
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
@@ -432,7 +435,7 @@ constexpr uint32_t kMaxBounces = 4096;
 // launches, or null.
 void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, const FrameParams& shade_fp,
                      const WaveBuffers& wb, const std::vector<WaveJob>& jobs, uint32_t n, uint32_t fixed_bounces,
-                     uint32_t sort_from, uint32_t tail_from, hpt_renderer* timing) {
+                     uint32_t sort_from, uint32_t tail_from, uint32_t trav_block, hpt_renderer* timing) {
   const SegGeom g = seg_geom(n);
   uint32_t* counts = wb.counts;
   // Segment counters: path queues A and B (ping-pong), the shadow queue, the
@@ -482,7 +485,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     const Queue& qout = qbuf[out];
     const bool sort = b >= sort_from && ss.keys && ss.items;
     begin(HPT_STAGE_EXTEND);
-    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr, phase_counts);
+    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr, phase_counts, trav_block);
     end();
     Queue qshade = qin;
     if (sort) {
@@ -496,14 +499,14 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
       launch_vmis_a(s, transparent, sv, shade_fp, ps, qshade, n, qout, qnee, sq, sc_shade);
       end();
       begin(HPT_STAGE_WALK);
-      launch_walk(s, transparent, sv, shade_fp, ps, sq, g.seg_cap, sp, sc_walk);
+      launch_walk(s, transparent, sv, shade_fp, ps, sq, g.seg_cap, sp, sc_walk, trav_block);
       end();
       if (shade_fp.env_type != HPT_ENV_SKY) {
         begin(HPT_STAGE_SHADE);
         launch_vmis_b(s, transparent, sv, shade_fp, ps, qnee, wq1);
         end();
         begin(HPT_STAGE_WALK);
-        launch_walk(s, transparent, sv, shade_fp, ps, wq1, g.seg_cap, sp, sc_walk);
+        launch_walk(s, transparent, sv, shade_fp, ps, wq1, g.seg_cap, sp, sc_walk, trav_block);
         end();
       }
       begin(HPT_STAGE_SHADE);
@@ -516,7 +519,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     }
     if (!vmis) {
       begin(HPT_STAGE_SHADOW);
-      launch_shadow(s, sv, ps, sq, n, sp, sc_shadow);
+      launch_shadow(s, sv, ps, sq, n, sp, sc_shadow, trav_block);
       end();
     }
     check(hipGetLastError(), "bounce launch");
@@ -570,7 +573,7 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   wb.qc = r->vmis_phases ? r->d_qc.as<uint32_t>() : nullptr;
   const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
   trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
-                  r->shade_sort_from, r->tail_from, r);
+                  r->shade_sort_from, r->tail_from, r->trav_block, r);
 }
 
 template <typename T>
@@ -598,6 +601,7 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, r->info.features);
   key_append(k, r->shade_sort_from);
   key_append(k, r->tail_from);
+  key_append(k, r->trav_block);
   return k;
 }
 
@@ -1336,6 +1340,28 @@ hpt_status hpt_group_set_tail_from(hpt_group* g, uint32_t first_bounce) {
   });
 }
 
+static bool traversal_block_ok(uint32_t threads) {
+  for (uint32_t b : kTraversalBlocks)
+    if (b == threads) return true;
+  return false;
+}
+
+hpt_status hpt_set_traversal_block(hpt_renderer* r, uint32_t threads) {
+  return guarded([&] {
+    need(r, "null renderer");
+    need(traversal_block_ok(threads), "the traversal kernels are built for 64, 128 or 256 threads per block");
+    r->trav_block = threads;
+  });
+}
+
+hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads) {
+  return guarded([&] {
+    need(g, "null group");
+    need(traversal_block_ok(threads), "the traversal kernels are built for 64, 128 or 256 threads per block");
+    g->trav_block = threads;
+  });
+}
+
 hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on) {
   return guarded([&] {
     need(r, "null renderer");
@@ -1954,6 +1980,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     key_append(key, r0->info.features);
     key_append(key, g->shade_sort_from);
     key_append(key, g->tail_from);
+    key_append(key, g->trav_block);
     for (const WaveJob& j : wj) key_append(key, j.base);
     const uint32_t njobs = (uint32_t)wj.size();
     const uint32_t fixed_bounces = max_depth + 1;
@@ -1981,7 +2008,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
         check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
           launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
-          trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, cj, n, fixed_bounces, g->shade_sort_from, g->tail_from,
+          trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, cj, n, fixed_bounces, g->shade_sort_from, g->tail_from, g->trav_block,
                           nullptr);
           for (size_t i = 0; i < cj.size(); ++i)
             launch_frame_counter(g->stream, live[i]->d_frame.as<uint32_t>(), cj[i].fp.batch_frames, true);
@@ -2003,7 +2030,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     } else {
       launch_job_limits(g->stream, g->d_limits.as<uint2>(), lim, njobs);
       check(hipGetLastError(), "job limits launch");
-      trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, wj, n, fixed_bounces, g->shade_sort_from, g->tail_from,
+      trace_wavefront(g->stream, sv, r0->info.features, shade_fp, wb, wj, n, fixed_bounces, g->shade_sort_from, g->tail_from, g->trav_block,
                       nullptr);
       for (hpt_renderer* r : live) r->frame_synced = false;
     }

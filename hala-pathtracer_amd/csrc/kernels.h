@@ -57,6 +57,12 @@ constexpr uint32_t PATH_IN_MEDIUM = 2u;       // is_in_medium
 constexpr uint32_t PATH_SURFACE_SCATTER = 4u; // is_surface_scatter (MEDIUM without VOL_MIS)
 
 constexpr int kBlock = 256;
+// Block sizes the traversal kernels (extend, walk, shadow) are built for:
+// hpt_set_traversal_block picks one; queue segments are multiples of
+// kBlock entries, so each divides them.  128 measured fastest on both 1080p
+// benchmark scenes (+3-4 % over 256).
+constexpr uint32_t kTraversalBlocks[] = {64, 128, 256};
+constexpr uint32_t kDefaultTraversalBlock = 128;
 
 // Compacted queues are split into kSeg segments, each with its own append
 // counter kSegStride words (256 B) from the next.  Block b of a launch reads
@@ -154,9 +160,11 @@ struct StageCounters {
 // keys: null, or the sort keys of qin's slots to write (see ShadeSort).
 // phase_counts: null, or two counter blocks (kSeg * kSegStride words each)
 // of the phased volume-MIS shading's queues, zeroed with qnext and sq.
+// bs: threads per block of the traversal kernels (extend, walk, shadow):
+// one of kTraversalBlocks.
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
-                   uint32_t* phase_counts);
+                   uint32_t* phase_counts, uint32_t bs);
 // Groups qin's entries by the keys extend wrote: ss.items and ss.counts form
 // the segmented queue shading then reads (qin may be the identity queue).
 void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss);
@@ -170,7 +178,7 @@ constexpr uint32_t kTailChunks = 16;
 void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                  const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
-                   Spill sp, StageCounters sc);
+                   Spill sp, StageCounters sc, uint32_t bs);
 // Volume-MIS shading in phases (MEDIUM|VOL_MIS): A over qin (finished and
 // passing paths to qout, scattering events to qnee, their first walk to wq),
 // the walks of a walk queue, B over qnee (environment maps: the second walk
@@ -179,7 +187,7 @@ void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameP
                    const Queue& qin, uint32_t n_max, const Queue& qout, const Queue& qnee, const ShadowQueue& wq,
                    StageCounters sc);
 void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc);
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs);
 void launch_vmis_b(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                    const Queue& qnee, const ShadowQueue& wq);
 void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,

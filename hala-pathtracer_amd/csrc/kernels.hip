@@ -58,10 +58,11 @@ __device__ __forceinline__ uint32_t seg_reserve(bool want, const ShadowQueue& sq
 struct SegSlot {
   uint32_t seg, j, n;
 };
+template <uint32_t BS = kBlock>
 __device__ __forceinline__ SegSlot seg_slot(const uint32_t* counts) {
   SegSlot t;
   t.seg = blockIdx.x % kSeg;
-  t.j = (blockIdx.x / kSeg) * kBlock + threadIdx.x;
+  t.j = (blockIdx.x / kSeg) * BS + threadIdx.x;
   t.n = counts[t.seg * kSegStride];
   return t;
 }
@@ -150,10 +151,11 @@ struct LaneRange {
   uint32_t n;     // entries in the segment (or n_max)
   uint32_t base;  // offset of the segment in the entry arrays
 };
+template <uint32_t BS>
 __device__ __forceinline__ LaneRange lane_range(const uint32_t* counts, uint32_t seg_cap, bool identity,
                                                 uint32_t n_max, unsigned long long* stats) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  constexpr uint32_t kWaveSpan = 64u * kRaysPerLane, kBlockSpan = kBlock * kRaysPerLane;
+  constexpr uint32_t kWaveSpan = 64u * kRaysPerLane, kBlockSpan = BS * kRaysPerLane;
   LaneRange r;
   if (identity) {
     if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n_max);
@@ -177,12 +179,12 @@ __device__ __forceinline__ uint8_t shade_key(const SceneView& sv, uint32_t id) {
   return sv.tri_key[id];
 }
 
-template <bool COUNT, bool SORT>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
+template <bool COUNT, bool SORT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_extend(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext,
                                                    ShadowQueue sq, Spill sp, unsigned long long* stats,
                                                    uint8_t* keys, uint32_t* phase_counts) {
-  __shared__ uint2 lds_stack[kShortStack * kBlock];
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
   if (gid < kSeg) {  // the shade launch after this one appends to these
     qnext.counts[gid * kSegStride] = 0;
     sq.counts[gid * kSegStride] = 0;
@@ -191,10 +193,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
       phase_counts[(kSeg + gid) * kSegStride] = 0;
     }
   }
-  const LaneRange lr = lane_range(qin.counts, qin.seg_cap, qin.items == nullptr, n_max, stats);
+  const LaneRange lr = lane_range<BS>(qin.counts, qin.seg_cap, qin.items == nullptr, n_max, stats);
   const uint32_t* items = qin.items ? qin.items + lr.base : nullptr;
   auto entry = [&](uint32_t j) { return items ? items[j] : j; };
-  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
   uint32_t cv = 0, ct = 0;
   auto store = [&](uint32_t p, uint32_t j, const TraceResult& r) {
     ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
@@ -1133,12 +1135,12 @@ done:
 
 // The queued transmittance walks (inc/lighting.glsl:7-61), one lane per walk:
 // the path's RNG stream advances for partly transparent surfaces, as inline.
-template <bool TRANSP, bool JOBS, bool COUNT>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
+template <bool TRANSP, bool JOBS, bool COUNT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
     k_walk(SceneView sv, FrameParams fp, PathState ps, ShadowQueue wq, Spill sp, unsigned long long* stats) {
-  __shared__ uint2 lds_stack[kShortStack * kBlock];
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  const SegSlot t = seg_slot(wq.counts);
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  const SegSlot t = seg_slot<BS>(wq.counts);
   if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
   WalkCounts wc;
   if (t.j < t.n) {
@@ -1146,7 +1148,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
     const float4 o4 = wq.o[k], d4 = wq.d[k];
     const uint32_t p = __float_as_uint(d4.w);
     const uint32_t max_depth = JOBS ? fp.job_limits[ps.job[p]].x : fp.max_depth;
-    LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+    LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
     uint4 rng = ps.rng[p];
     const f3 tr = eval_transmittance<TRANSP, COUNT>(sv, max_depth, rng, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
                                                      st, wc);
@@ -1287,13 +1289,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT
 // path's environment (slot 0) or light (slot 1) NEE sum.  Each path has at
 // most one ray per slot per bounce, so the sum can be read when the ray
 // starts and written when it ends.
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, Spill sp,
+template <bool COUNT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU))) k_shadow(SceneView sv, PathState ps, ShadowQueue sq, Spill sp,
                                                    unsigned long long* stats) {
-  __shared__ uint2 lds_stack[kShortStack * kBlock];
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  const LaneRange lr = lane_range(sq.counts, sq.seg_cap, false, 0, stats);
-  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  const LaneRange lr = lane_range<BS>(sq.counts, sq.seg_cap, false, 0, stats);
+  LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
   uint32_t cv = 0, ct = 0;
   // Current ray: tmax, tag, contribution, the NEE sum it adds to.
   float tmax = 0.0f;
@@ -1526,27 +1528,43 @@ static inline uint32_t queue_grid(const Queue& qin, uint32_t n_max) {
   return qin.items ? kSeg * seg_geom(n_max).chunks : grid_for(n_max);
 }
 
-// Blocks of a traversal launch (kRaysPerLane entries per lane).
-static inline uint32_t walk_blocks(uint32_t entries) {
-  constexpr uint32_t span = kBlock * kRaysPerLane;
+// Blocks of a traversal launch of bs-thread blocks (kRaysPerLane entries per lane).
+static inline uint32_t walk_blocks(uint32_t entries, uint32_t bs) {
+  const uint32_t span = bs * kRaysPerLane;
   return (entries + span - 1) / span;
 }
 
+// The block sizes the traversal kernels are built for (kTraversalBlocks).
+#define HPT_BY_BS(bs, CALL) \
+  switch (bs) {             \
+    case 64: CALL(64); break;   \
+    case 256: CALL(256); break; \
+    default: CALL(128); break;  \
+  }
+
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
-                   uint32_t* phase_counts) {
-  const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap) : walk_blocks(n_max));
+                   uint32_t* phase_counts, uint32_t bs) {
+  const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap, bs) : walk_blocks(n_max, bs));
   const bool count = sc.count && sc.stats;
-#define HPT_EXTEND(C, S)                                                                                       \
-  hipLaunchKernelGGL((k_extend<C, S>), grid, dim3(kBlock), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys, \
+#define HPT_EXTEND(C, S, BS_)                                                                                   \
+  hipLaunchKernelGGL((k_extend<C, S, BS_>), grid, dim3(BS_), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys, \
                      phase_counts)
+#define HPT_E_TT(BS_) HPT_EXTEND(true, true, BS_)
+#define HPT_E_FT(BS_) HPT_EXTEND(false, true, BS_)
+#define HPT_E_TF(BS_) HPT_EXTEND(true, false, BS_)
+#define HPT_E_FF(BS_) HPT_EXTEND(false, false, BS_)
   if (keys) {
-    if (count) HPT_EXTEND(true, true);
-    else HPT_EXTEND(false, true);
+    if (count) HPT_BY_BS(bs, HPT_E_TT)
+    else HPT_BY_BS(bs, HPT_E_FT)
   } else {
-    if (count) HPT_EXTEND(true, false);
-    else HPT_EXTEND(false, false);
+    if (count) HPT_BY_BS(bs, HPT_E_TF)
+    else HPT_BY_BS(bs, HPT_E_FF)
   }
+#undef HPT_E_TT
+#undef HPT_E_FT
+#undef HPT_E_TF
+#undef HPT_E_FF
 #undef HPT_EXTEND
 }
 
@@ -1640,20 +1658,33 @@ void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameP
 }
 
 void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc) {
-  const dim3 grid(kSeg * ((seg_cap + kBlock - 1) / kBlock));
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs) {
+  const dim3 grid(kSeg * ((seg_cap + bs - 1) / bs));
   const bool count = sc.count && sc.stats;
-#define HPT_W(T, J, C) hipLaunchKernelGGL((k_walk<T, J, C>), grid, dim3(kBlock), 0, s, sv, fp, ps, wq, sp, sc.stats)
+#define HPT_W(T, J, C, BS_) \
+  hipLaunchKernelGGL((k_walk<T, J, C, BS_>), grid, dim3(BS_), 0, s, sv, fp, ps, wq, sp, sc.stats)
+#define HPT_W_TTF(BS_) HPT_W(true, true, false, BS_)
+#define HPT_W_FTF(BS_) HPT_W(false, true, false, BS_)
+#define HPT_W_TFT(BS_) HPT_W(true, false, true, BS_)
+#define HPT_W_FFT(BS_) HPT_W(false, false, true, BS_)
+#define HPT_W_TFF(BS_) HPT_W(true, false, false, BS_)
+#define HPT_W_FFF(BS_) HPT_W(false, false, false, BS_)
   if (fp.job_limits) {
-    if (transp) HPT_W(true, true, false);
-    else HPT_W(false, true, false);
+    if (transp) HPT_BY_BS(bs, HPT_W_TTF)
+    else HPT_BY_BS(bs, HPT_W_FTF)
   } else if (count) {
-    if (transp) HPT_W(true, false, true);
-    else HPT_W(false, false, true);
+    if (transp) HPT_BY_BS(bs, HPT_W_TFT)
+    else HPT_BY_BS(bs, HPT_W_FFT)
   } else {
-    if (transp) HPT_W(true, false, false);
-    else HPT_W(false, false, false);
+    if (transp) HPT_BY_BS(bs, HPT_W_TFF)
+    else HPT_BY_BS(bs, HPT_W_FFF)
   }
+#undef HPT_W_TTF
+#undef HPT_W_FTF
+#undef HPT_W_TFT
+#undef HPT_W_FFT
+#undef HPT_W_TFF
+#undef HPT_W_FFF
 #undef HPT_W
 }
 
@@ -1686,12 +1717,14 @@ void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameP
 }
 
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
-                   Spill sp, StageCounters sc) {
-  const dim3 grid(kSeg * walk_blocks(sq.seg_cap));
-  if (sc.count && sc.stats)
-    hipLaunchKernelGGL(k_shadow<true>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
-  else
-    hipLaunchKernelGGL(k_shadow<false>, grid, dim3(kBlock), 0, s, sv, ps, sq, sp, sc.stats);
+                   Spill sp, StageCounters sc, uint32_t bs) {
+  const dim3 grid(kSeg * walk_blocks(sq.seg_cap, bs));
+#define HPT_S_T(BS_) hipLaunchKernelGGL((k_shadow<true, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats)
+#define HPT_S_F(BS_) hipLaunchKernelGGL((k_shadow<false, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats)
+  if (sc.count && sc.stats) HPT_BY_BS(bs, HPT_S_T)
+  else HPT_BY_BS(bs, HPT_S_F)
+#undef HPT_S_T
+#undef HPT_S_F
 }
 
 void launch_env_texels(hipStream_t s, const float4* rgb, const float4* table, uint32_t w, uint32_t h, EnvTexel* out) {

@@ -36,6 +36,7 @@ HIT_NONE, HIT_TRIANGLE, HIT_LIGHT = range(3)
 STAGE_CAMERA, STAGE_EXTEND, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_SORT, STAGE_WALK, STAGE_TAIL = range(8)
 PROFILE_TIMING, PROFILE_COUNT_VISITS = 1, 2
 TAIL_FROM_NONE = 0xFFFFFFFF
+TRAVERSAL_BLOCKS = (64, 128, 256)  # threads per block of the traversal kernels; 128 is the default
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 SHADE_SORT_OFF = 0xFFFFFFFF
 BVH_HOST, BVH_GPU = 0, 1
@@ -198,6 +199,8 @@ SIGNATURES = {
     "hpt_get_launch_stats": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
     "hpt_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_set_tail_from": [C.c_void_p, C.c_uint32],
+    "hpt_set_traversal_block": [C.c_void_p, C.c_uint32],
+    "hpt_group_set_traversal_block": [C.c_void_p, C.c_uint32],
     "hpt_group_set_tail_from": [C.c_void_p, C.c_uint32],
     "hpt_group_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_present_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
@@ -544,6 +547,11 @@ class HalaRenderer:
         max_depth + 1 in the wavefront); same images."""
         _check(self._lib.hpt_set_tail_from(self._h, first_bounce))
 
+    def set_traversal_block(self, threads: int = 128) -> None:
+        """Threads per block of the traversal kernels (extension, shadow,
+        transmittance walks): 64, 128 or 256; same images."""
+        _check(self._lib.hpt_set_traversal_block(self._h, threads))
+
     def truncated_paths(self) -> int:
         """Paths stopped at the 4096-bounce limit since the handle was made
         (TRANSPARENT pass-throughs do not count toward max_depth)."""
@@ -624,6 +632,9 @@ class HalaGroup:
 
     def set_tail_from(self, first_bounce: int = TAIL_FROM_NONE) -> None:
         _check(self._lib.hpt_group_set_tail_from(self._h, first_bounce))
+
+    def set_traversal_block(self, threads: int = 128) -> None:
+        _check(self._lib.hpt_group_set_traversal_block(self._h, threads))
 
     def truncated_paths(self) -> int:
         """Paths of this group's steps stopped at the 4096-bounce limit."""

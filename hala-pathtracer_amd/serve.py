@@ -113,6 +113,14 @@ def tail_from(spec: JobSpec, tuning=None) -> int:
     return hpt.TAIL_FROM_NONE
 
 
+def traversal_block(spec: JobSpec, tuning=None) -> int:
+    if tuning is not None:
+        t = tuning.get(spec.scene, spec.width, spec.height, 0)
+        if t is not None:
+            return t.trav_block
+    return 128
+
+
 def footprint(spec: JobSpec, max_frames: int = 32, tuning=None) -> int:
     px = spec.width * spec.height
     per_path = PATH_BYTES + (SORT_BYTES if shade_sort(spec, tuning) != hpt.SHADE_SORT_OFF else 0)
@@ -180,6 +188,7 @@ class Server:
         r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
         r.set_shade_sort(shade_sort(spec, self.tuning))
         r.set_tail_from(tail_from(spec, self.tuning))
+        r.set_traversal_block(traversal_block(spec, self.tuning))
         return r
 
     def _run_job(self, spec: JobSpec, t0: float, admitted: float, results: list, fp: int):
@@ -242,6 +251,10 @@ class Server:
                  if key.startswith(scene + "|")]
         if tails:
             g.set_tail_from(max(set(tails), key=tails.count))
+        blocks = [t.trav_block for key, t in (self.tuning.entries.items() if self.tuning else [])
+                  if key.startswith(scene + "|")]
+        if blocks:
+            g.set_traversal_block(max(set(blocks), key=blocks.count))
         return g
 
     def _finish(self, a: _Active, t0: float, results: list, error: str | None = None):

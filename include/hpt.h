@@ -402,6 +402,13 @@ HPT_API void hpt_group_destroy(hpt_group* g);
 HPT_API hpt_status hpt_set_tail_from(hpt_renderer* r, uint32_t first_bounce);
 HPT_API hpt_status hpt_group_set_tail_from(hpt_group* g, uint32_t first_bounce);
 
+/* Threads per block of the traversal kernels (closest-hit extension, shadow
+ * rays, the volume-MIS transmittance walks): 64, 128 (the default) or 256
+ * (no counterpart in the reference).  Same images; the autotuner measures
+ * it. */
+HPT_API hpt_status hpt_set_traversal_block(hpt_renderer* r, uint32_t threads);
+HPT_API hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads);
+
 /* Batches replayed as graphs and graphs captured since the handle was made. */
 HPT_API hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures);
 

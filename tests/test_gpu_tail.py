@@ -169,3 +169,31 @@ def test_group_with_an_earlier_tail_equals_solo():
     assert g.truncated_paths() == 0
     g.close()
     t.close()
+
+
+@pytest.mark.parametrize("case", ["disney", "icosphere_medium"])
+def test_traversal_block_sizes_give_the_same_images(case):
+    """hpt_set_traversal_block: the extension, shadow and walk kernels at 64,
+    128 and 256 threads per block give the same images, eager and graph."""
+    if case == "disney":
+        sc = hpt_scenes.disney_grid(seed=4, width=96, height=64, env_width=64, env_height=32, segments=16, rings=8)
+        features, w, h = 0, 96, 64
+    else:
+        sc = hpt_scenes.displaced_icosphere_lights_medium(level=3, width=96, height=54)
+        features, w, h = hpt_scenes.MEDIUM_SCENE_FEATURES, 96, 54
+    want = None
+    for bs in hpt.TRAVERSAL_BLOCKS:
+        for mode in (hpt.LAUNCH_EAGER, hpt.LAUNCH_GRAPH):
+            r = _renderer(sc, w, h, 8, 3, features, mode=mode, frames_in_flight=2)
+            r.set_traversal_block(bs)
+            r.render(4)
+            got = [r.read_image(im) for im in (hpt.IMAGE_ACCUM, hpt.IMAGE_ALBEDO, hpt.IMAGE_FINAL_RGBA8)]
+            r.close()
+            if want is None:
+                want = got
+            for a, b in zip(want, got):
+                assert np.array_equal(a, b), (case, bs, mode)
+    r = _renderer(sc, w, h, 8, 3, features)
+    with pytest.raises(hpt.HptError):
+        r.set_traversal_block(96)
+    r.close()
