@@ -9,10 +9,11 @@ camera path per pixel through the full bounce loop plus accumulation; the
 default --steps is the configuration's 128 samples per pixel.  The line
 carries
   * value: Msamples/s (pixels x frames / wall seconds, inputs resident in HBM);
-  * roofline: the dominant kernel's (the shading kernel with its volume-MIS
-    transmittance walks), with the extension walk's as roofline_extend,
-    each from HIP events recorded on the renderer's stream around every
-    launch and an algorithmic byte model per unit;
+  * roofline: the dominant stage's kernel (of extension, shading and the
+    queued transmittance walks of the phased volume-MIS shading: on this
+    scene the walks), each stage's also as roofline_extend, roofline_shade
+    and roofline_walk, from HIP events recorded on the renderer's stream
+    around every launch and an algorithmic byte model per unit;
   * time_to_relmse: the plain tracer, the tracer + learned denoiser and the
     served (submission to image in host memory) times to relMSE <= 1e-3
     against a 4096-spp render of the same scene;
@@ -21,7 +22,9 @@ carries
   * cpu_baseline: the CPU oracle on a bounded sample of the same workload.
 --quick skips the relMSE and serving runs.  --workload selects the Disney
 grid (1080p, 256 spp, depth 10), the Cornell box (512x512, 64 spp, depth 8),
-the serving run alone, or the denoiser alone.
+the serving run alone, the denoiser alone, or `merge`: jobs of one scene
+merged into one wavefront per step against the same jobs on their own
+streams.
 """
 from __future__ import annotations
 
@@ -445,6 +448,79 @@ def serving_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def merge_bench(args):
+    """J jobs of one resident scene (512x512, 16-frame chunks, depth 10),
+    traced either on their own streams (the serving default, one thread per
+    job) or merged into one wavefront per step (hpt_group_render), for J in
+    1..16 on three of the 8 resident scenes.  Merging jobs of one scene is the
+    most a wavefront shared across scenes could gain: the same BVH in every
+    wave and no per-path scene lookups."""
+    import threading
+    import serve
+
+    W, H, F, DEPTH = 512, 512, 16, 10
+    scenes = {n: (sc, f) for n, sc, f in hpt_scenes.batched_job_scenes()}
+    names = [n for n in scenes if n in ("cornell", "disney_grid")] + [max((n for n in scenes if n.startswith("soup_")),
+                                                                        key=lambda n: scenes[n][0].triangle_count)]
+    rows = {}
+    for name in names:
+        sc, feat = scenes[name]
+        rs = serve.ResidentScene(name, sc, feat)
+        log(f"{name}: {sc.triangle_count} triangles")
+
+        def job(frames):
+            r = hpt.HalaRenderer(W, H, max_depth=DEPTH, rr_depth=3, features=feat)
+            r.frames_in_flight = frames
+            r.share_scene(rs.template)
+            r.set_launch_mode(hpt.LAUNCH_GRAPH)
+            return r
+
+        def chunks(r, n):
+            for _ in range(n):
+                r.render(F)
+                r.wait_idle()
+
+        for J in (1, 2, 4, 8, 16):
+            R = max(2, 16 // J)
+            samples = J * R * F * W * H
+            jobs = [job(F) for _ in range(J)]
+            for r in jobs:
+                chunks(r, 1)  # allocation and graph capture
+            threads = [threading.Thread(target=chunks, args=(r, R)) for r in jobs]
+            t0 = time.perf_counter()
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+            streams = samples / (time.perf_counter() - t0) / 1e6
+            for r in jobs:
+                r.close()
+            jobs = [job(1) for _ in range(J)]
+            g = hpt.HalaGroup()
+            g.set_launch_mode(hpt.LAUNCH_GRAPH)
+            g.render(jobs, [F] * J)
+            g.wait_idle()
+            t0 = time.perf_counter()
+            for _ in range(R):
+                g.render(jobs, [F] * J)
+            g.wait_idle()
+            merged = samples / (time.perf_counter() - t0) / 1e6
+            g.close()
+            for r in jobs:
+                r.close()
+            log(f"  {J:2d} jobs: streams {streams:7.1f}, merged {merged:7.1f} Msamples/s ({merged / streams:.3f})")
+            rows[f"{name}/{J}"] = {"streams": round(streams, 1), "merged": round(merged, 1)}
+        rs.close()
+    best = max(rows, key=lambda k: rows[k]["merged"])
+    out = {"metric": "Msamples/s", "value": rows[best]["merged"], "unit": "Msamples/s", "n_gpus": 1,
+           "steps": len(rows), "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "merge", "width": W, "height": H, "frames_per_chunk": F, "max_depth": DEPTH,
+                      "scenes": names},
+           "merge": rows, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def stage_roofline(kernel, stats_timed, per_unit_bytes, units_key="rays", traffic=None, extra=None):
     """Roofline of one stage's kernel: algorithmic bytes per launch (units
     per launch in the timed run x bytes per unit) over its mean launch time
@@ -465,7 +541,7 @@ def stage_roofline(kernel, stats_timed, per_unit_bytes, units_key="rays", traffi
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "denoise"], default=HEADLINE)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "denoise", "merge"], default=HEADLINE)
     ap.add_argument("--denoise", action="store_true",
                     help="with --relmse: also the time to the threshold of the denoised estimate (and served)")
     ap.add_argument("--weights", default="", help="denoiser weights (default: the committed network)")
@@ -527,6 +603,9 @@ def main():
         return
     if args.workload == "denoise":
         denoise_bench(args)
+        return
+    if args.workload == "merge":
+        merge_bench(args)
         return
     if args.workload == HEADLINE and not args.quick:
         args.relmse = args.denoise = args.serving = True
