@@ -128,6 +128,18 @@ def profiled_conv_traffic(layer):
     return None
 
 
+def device_memory() -> tuple:
+    """(free, total) device bytes of the current device, from the HIP runtime
+    the library runs on."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return 0, 0
+    return free.value, total.value
+
+
 def copy_calibration(mib=1024, reps=4):
     """Plain device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -407,7 +419,8 @@ def run_serving(args, jobs_seed=0):
         log(f"autotuned {len(tuning.entries)} (scene, resolution) classes in {t_tune:.1f} s")
     srv = serve.Server(resident, int(args.budget_gib * 2**30), max_frames=args.job_frames, tuning=tuning,
                        launch=hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER,
-                       wavefront=args.wavefront, step_paths=int(args.step_mpaths * 2**20))
+                       wavefront=args.wavefront, step_paths=int(args.step_mpaths * 2**20),
+                       max_running=args.max_running)
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
@@ -416,10 +429,13 @@ def run_serving(args, jobs_seed=0):
     summ["arrival_span_s"] = round(jobs[-1].arrival, 3)
     summ["arrival_rate_per_s"] = args.rate
     summ["mode"] = "wavefront per scene" if args.wavefront else "stream per job"
-    if args.wavefront and srv.step_sizes:
-        summ["wavefront_steps"] = len(srv.step_sizes)
-        summ["mean_jobs_per_step"] = round(float(np.mean(srv.step_sizes)), 2)
-        summ["max_jobs_per_step"] = int(max(srv.step_sizes))
+    summ["loop"] = "native (hpt_server)"
+    summ["max_running"] = args.max_running
+    if args.wavefront and srv.stats.get("wavefront_steps"):
+        st = srv.stats
+        summ["wavefront_steps"] = st["wavefront_steps"]
+        summ["mean_jobs_per_step"] = round(st["jobs_in_steps"] / st["wavefront_steps"], 2)
+        summ["max_jobs_per_step"] = st["max_jobs_per_step"]
     log(f"served: {summ}")
     sizes = {k: v.scene.triangle_count for k, v in resident.items()}
     for rs in resident.values():
@@ -553,6 +569,8 @@ def main():
     ap.add_argument("--job-frames", type=int, default=16, help="serving: frames in flight per job (at most)")
     ap.add_argument("--wavefront", action="store_true",
                     help="serving: the jobs of each scene share one wavefront per step (hpt_group_render)")
+    ap.add_argument("--max-running", type=int, default=8,
+                    help="serving: jobs rendering at once in stream-per-job mode (0: as many as the budget admits)")
     ap.add_argument("--step-mpaths", type=float, default=8.0,
                     help="serving --wavefront: paths (in 2^20) per job and step")
     ap.add_argument("--tune", action="store_true",
@@ -641,6 +659,9 @@ def main():
     r = make_renderer()
     log(f"create + commit (BVH build + upload) {time.perf_counter() - t0:.2f} s, bvh {r.bvh_info()}, "
         f"{r.frames_in_flight} frames in flight")
+    free_b, total_b = device_memory()
+    log(f"device memory: {free_b / 2**30:.1f} GiB free of {total_b / 2**30:.1f} GiB; this handle holds "
+        f"{sum(r.memory_usage()) / 2**30:.1f} GiB")
     fif = r.frames_in_flight
     sort_table = None
     if args.shade_sort == "auto":
@@ -649,7 +670,7 @@ def main():
         t0 = time.perf_counter()
         tuned = autotune.tune(r, W, H, DEPTH, RR, FEATURES, samples=4 * fif * W * H, candidates=(fif,),
                               max_frames=fif, modes=(hpt.LAUNCH_GRAPH if args.launch == "graph" else hpt.LAUNCH_EAGER,),
-                              tolerance=0.05)
+                              tolerance=0.05, in_place=True)
         shade_sort = tuned.shade_sort
         sort_table = {k: round(v, 1) for k, v in tuned.table.items()}
         r.set_shade_sort(shade_sort)
@@ -765,9 +786,9 @@ def main():
             if args.tail_from == "auto":  # the tail bounce for 2-frame batches
                 import autotune
 
-                tmpl = make_renderer()
+                tmpl = make_renderer(frames_in_flight=2)
                 t2 = autotune.tune(tmpl, W, H, DEPTH, RR, FEATURES, samples=8 * W * H, candidates=(2,),
-                                   modes=(hpt.LAUNCH_GRAPH,), sorts=())
+                                   modes=(hpt.LAUNCH_GRAPH,), sorts=(), in_place=True)
                 tmpl.close()
                 tails["denoised"] = t2.tail_from
                 if args.traversal_block == "auto":

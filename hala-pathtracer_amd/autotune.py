@@ -70,16 +70,24 @@ def candidates_for(width: int, height: int, max_frames: int = 64, memory_bytes: 
 def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, rr_depth: int = 3, features: int = 0,
          samples: float = 32e6, max_frames: int = 64, memory_bytes: int | None = None, tolerance: float = 0.03,
          modes=(hpt.LAUNCH_GRAPH, hpt.LAUNCH_EAGER), sorts=SORT_CANDIDATES, candidates=CANDIDATES,
-         tails=TAIL_CANDIDATES, blocks=BLOCK_CANDIDATES, device: int = 0) -> Tuning:
+         tails=TAIL_CANDIDATES, blocks=BLOCK_CANDIDATES, device: int = 0, in_place: bool = False) -> Tuning:
     """Measures every candidate batch size (then launch mode, then shading
     sort threshold, then tail bounce) on a handle that shares `template`'s
     committed scene, each over about `samples` paths (whole batches); returns
-    the choice."""
+    the choice.  `in_place`: measure on `template` itself (it must have the
+    given size, depths and features; its accumulation is reset, its batch
+    size restored, and the caller sets the other settings it wants)."""
     t_start = time.perf_counter()
-    r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=rr_depth, features=features, device=device)
+    if in_place:
+        r, frames0 = template, template.frames_in_flight
+    else:
+        r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=rr_depth, features=features, device=device)
     table = {}
     try:
-        r.share_scene(template)
+        if not in_place:
+            # Sized for the first candidate from the start (not the default batch).
+            r.frames_in_flight = candidates_for(width, height, max_frames, memory_bytes, candidates)[0]
+            r.share_scene(template)
 
         def measure(k, mode, sort=hpt.SHADE_SORT_OFF, tail=hpt.TAIL_FROM_NONE, block=128):
             r.frames_in_flight = k
@@ -140,7 +148,11 @@ def tune(template: "hpt.HalaRenderer", width: int, height: int, max_depth: int, 
             if table[key] > v * (1 + MARGIN):
                 block, v = bs, table[key]
     finally:
-        r.close()
+        if in_place:
+            r.frames_in_flight = frames0
+            r.reset_accumulation()
+        else:
+            r.close()
     return Tuning(k, mode, v, table, time.perf_counter() - t_start, sort, tail, block)
 
 

@@ -61,7 +61,20 @@ struct DevBuf {
     if (n == bytes && p) return;
     release();
     if (n == 0) return;
-    check(hipMalloc(&p, n), "hipMalloc");
+    // A request the device cannot hold fails here with the sizes; a failure
+    // inside the runtime's own host bookkeeping is reported as such.
+    size_t free_b = 0, total_b = 0;
+    check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (n > free_b)
+      throw Fail(HPT_ERROR_OUT_OF_MEMORY, "device memory: " + std::to_string(n) + " B requested, " +
+                                              std::to_string(free_b) + " B free of " + std::to_string(total_b));
+    try {
+      check(hipMalloc(&p, n), "hipMalloc");
+    } catch (const std::bad_alloc&) {
+      p = nullptr;
+      throw Fail(HPT_ERROR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(n) + " B: host allocation inside the "
+                                          "HIP runtime failed (" + std::to_string(free_b) + " B of device memory free)");
+    }
     bytes = n;
   }
   template <typename T>
@@ -95,7 +108,15 @@ struct StageTimer {
 
 }  // namespace
 
+// Each handle struct starts with a tag of its kind: an entry point given a
+// handle of another kind (or of a destroyed one) fails instead of writing
+// into memory that is not the struct it expects.
+constexpr uint32_t kRendererTag = 0x52545048u;   // "HPTR"
+constexpr uint32_t kGroupTag = 0x47545048u;      // "HPTG"
+constexpr uint32_t kPresenterTag = 0x50545048u;  // "HPTP"
+
 struct hpt_renderer {
+  uint32_t tag = kRendererTag;
   hpt_create_info info{};
   hipStream_t stream = nullptr;
   // Scene, host side.
@@ -169,6 +190,7 @@ struct hpt_renderer {
 // A wavefront shared by several jobs of one scene (hpt_group_render): its own
 // path state, queues and counters, sized for the largest step so far.
 struct hpt_group {
+  uint32_t tag = kGroupTag;
   int32_t device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -180,6 +202,7 @@ struct hpt_group {
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
   uint32_t tail_from = HPT_TAIL_FROM_NONE;
   uint32_t trav_block = kDefaultTraversalBlock;
+  uint32_t vmis_phases = 1;  // volume-MIS shading in phases (hpt_group_set_vmis_phases)
   size_t cap = 0;  // paths the path buffers hold
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
   DevBuf d_qa, d_qb, d_sh_o, d_sh_d, d_sh_c, d_counts, d_spill, d_sort_keys, d_sort_items, d_limits;
@@ -222,6 +245,9 @@ hpt_status guarded(F&& fn) {
 void need(bool c, const std::string& m, hpt_status code = HPT_ERROR_INVALID_ARGUMENT) {
   if (!c) throw Fail(code, m);
 }
+
+bool ok_renderer(const hpt_renderer* r) { return r && r->tag == kRendererTag; }
+bool ok_group(const hpt_group* g) { return g && g->tag == kGroupTag; }
 
 void set_device(hpt_renderer* r) { check(hipSetDevice(r->info.device), "hipSetDevice"); }
 
@@ -873,7 +899,7 @@ hpt_status hpt_create(const hpt_create_info* info, hpt_renderer** out) {
 
 hpt_status hpt_set_scene(hpt_renderer* r, const hpt_scene_desc* d) {
   return guarded([&] {
-    need(r && d, "null argument");
+    need(ok_renderer(r) && d, "null argument");
     need(d->camera_count >= 1 && d->camera_count <= HPT_MAX_CAMERAS && d->cameras, "1..8 cameras required");
     need(d->light_count <= HPT_MAX_LIGHTS, "at most 16 lights");
     need(d->light_count == 0 || d->lights, "null lights");
@@ -992,7 +1018,7 @@ hpt_status hpt_exr_decode_rgb(const void* data, size_t size, float* rgb, uint32_
 
 hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t w, uint32_t h, float rotation) {
   return guarded([&] {
-    need(r && rgb && w > 0 && h > 0, "invalid environment map");
+    need(ok_renderer(r) && rgb && w > 0 && h > 0, "invalid environment map");
     need((uint64_t)w * h < (1ull << 31), "environment map too large");
     const size_t n = (size_t)w * h;
     std::vector<float4> texels(n);
@@ -1033,7 +1059,7 @@ hpt_status hpt_set_env_map(hpt_renderer* r, const float* rgb, uint32_t w, uint32
 
 hpt_status hpt_set_ground_color(hpt_renderer* r, const float rgba[4]) {
   return guarded([&] {
-    need(r && rgba, "null argument");
+    need(ok_renderer(r) && rgba, "null argument");
     for (int i = 0; i < 3; ++i) r->ground[i] = rgba[i];
     r->env_type = HPT_ENV_SKY;
   });
@@ -1041,7 +1067,7 @@ hpt_status hpt_set_ground_color(hpt_renderer* r, const float rgba[4]) {
 
 hpt_status hpt_set_sky_color(hpt_renderer* r, const float rgba[4]) {
   return guarded([&] {
-    need(r && rgba, "null argument");
+    need(ok_renderer(r) && rgba, "null argument");
     for (int i = 0; i < 3; ++i) r->sky[i] = rgba[i];
     r->env_type = HPT_ENV_SKY;
   });
@@ -1049,21 +1075,21 @@ hpt_status hpt_set_sky_color(hpt_renderer* r, const float rgba[4]) {
 
 hpt_status hpt_set_env_intensity(hpt_renderer* r, float v) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     r->env_intensity = v;
   });
 }
 
 hpt_status hpt_set_exposure_value(hpt_renderer* r, float ev) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     r->info.exposure_value = ev;
   });
 }
 
 hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t w, uint32_t h) {
   return guarded([&] {
-    need(r && rgba8, "null argument");
+    need(ok_renderer(r) && rgba8, "null argument");
     need(w == 1024 && h == 1024, "the blue-noise texture must be 1024x1024 (rand4blue wraps at 1024)");
     need(!(r->committed && r->sd.use_count() > 1),
          "the blue-noise texture belongs to a scene shared with other handles; set it before sharing",
@@ -1079,7 +1105,7 @@ hpt_status hpt_set_blue_noise(hpt_renderer* r, const uint8_t* rgba8, uint32_t w,
 
 hpt_status hpt_set_textures(hpt_renderer* r, const hpt_texture_desc* textures, uint32_t count) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(count == 0 || textures, "null textures");
     std::vector<uchar4> texels;
     std::vector<TexInfo> info;
@@ -1107,7 +1133,7 @@ hpt_status hpt_set_textures(hpt_renderer* r, const hpt_texture_desc* textures, u
 
 hpt_status hpt_commit(hpt_renderer* r) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(r->scene_set, "hpt_set_scene has not been called", HPT_ERROR_STATE);
     set_device(r);
     hipStream_t s = r->stream;
@@ -1184,19 +1210,27 @@ hpt_status hpt_commit(hpt_renderer* r) {
 
 hpt_status hpt_share_scene(hpt_renderer* dst, const hpt_renderer* src) {
   return guarded([&] {
-    need(dst && src, "null renderer");
+    need(ok_renderer(dst) && ok_renderer(src), "null or invalid renderer handle");
     need(src->committed, "the source renderer has no committed scene", HPT_ERROR_STATE);
     need(dst != src, "a renderer cannot share its own scene");
     need(dst->info.device == src->info.device, "renderers on different devices cannot share a scene");
     set_device(dst);
     // Host-side copies the frame parameters and the ray-query decoding read.
-    dst->cameras = src->cameras;
+    try {
+      dst->cameras = src->cameras;
+      dst->lights = src->lights;
+      dst->materials = src->materials;
+      dst->tri_prim = src->tri_prim;
+      dst->prims = src->prims;
+    } catch (const std::bad_alloc&) {
+      throw Fail(HPT_ERROR_OUT_OF_MEMORY,
+                 "share scene: host copy of the scene arrays failed (cameras " + std::to_string(src->cameras.size()) +
+                     ", lights " + std::to_string(src->lights.size()) + ", materials " +
+                     std::to_string(src->materials.size()) + ", triangles " + std::to_string(src->tri_prim.size()) +
+                     ", primitives " + std::to_string(src->prims.size()) + ")");
+    }
     dst->camera_count = src->camera_count;
-    dst->lights = src->lights;
     dst->num_lights = src->num_lights;
-    dst->materials = src->materials;
-    dst->tri_prim = src->tri_prim;
-    dst->prims = src->prims;
     dst->env_type = src->env_type;
     std::memcpy(dst->ground, src->ground, sizeof dst->ground);
     std::memcpy(dst->sky, src->sky, sizeof dst->sky);
@@ -1218,7 +1252,7 @@ hpt_status hpt_share_scene(hpt_renderer* dst, const hpt_renderer* src) {
 
 hpt_status hpt_query_idle(hpt_renderer* r, int* idle) {
   return guarded([&] {
-    need(r && idle, "null argument");
+    need(ok_renderer(r) && idle, "null argument");
     set_device(r);
     const hipError_t e = hipStreamQuery(r->stream);
     if (e == hipErrorNotReady) {
@@ -1232,7 +1266,7 @@ hpt_status hpt_query_idle(hpt_renderer* r, int* idle) {
 
 hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, uint64_t* scene_bytes) {
   return guarded([&] {
-    need(r && frame_bytes && scene_bytes, "null argument");
+    need(ok_renderer(r) && frame_bytes && scene_bytes, "null argument");
     uint64_t f = 0, sc = 0;
     for (const DevBuf* b : {&r->d_rng, &r->d_org, &r->d_dir, &r->d_thr, &r->d_rad, &r->d_alb, &r->d_nrm, &r->d_med,
                             &r->d_med2, &r->d_nee_e, &r->d_nee_l, &r->d_hit, &r->d_qa, &r->d_qb, &r->d_sh_o,
@@ -1253,7 +1287,7 @@ hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, uint64_t
 
 hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     require_committed(r);
     need(camera_index < r->camera_count, "camera index out of range");
     set_device(r);
@@ -1284,7 +1318,7 @@ hpt_status hpt_render(hpt_renderer* r, uint32_t frames, uint32_t camera_index) {
 
 hpt_status hpt_get_truncated_paths(hpt_renderer* r, uint64_t* paths) {
   return guarded([&] {
-    need(r && paths, "null argument");
+    need(ok_renderer(r) && paths, "null argument");
     *paths = 0;
     if (!r->d_tail_cut.p) return;
     set_device(r);
@@ -1297,7 +1331,7 @@ hpt_status hpt_get_truncated_paths(hpt_renderer* r, uint64_t* paths) {
 
 hpt_status hpt_set_bvh_builder(hpt_renderer* r, uint32_t builder) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(builder == HPT_BVH_HOST || builder == HPT_BVH_GPU, "unknown BVH builder");
     r->bvh_builder = builder;
   });
@@ -1305,7 +1339,7 @@ hpt_status hpt_set_bvh_builder(hpt_renderer* r, uint32_t builder) {
 
 hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, double* build_ms) {
   return guarded([&] {
-    need(r && builder_used && build_ms, "null argument");
+    need(ok_renderer(r) && builder_used && build_ms, "null argument");
     require_committed(r);
     *builder_used = r->bvh_builder_used;
     *build_ms = r->bvh_build_ms;
@@ -1314,7 +1348,7 @@ hpt_status hpt_get_bvh_build_stats(hpt_renderer* r, uint32_t* builder_used, doub
 
 hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     r->shade_sort_from = first_bounce;
     if (r->committed && r->batch_cap) {
       set_device(r);
@@ -1326,7 +1360,7 @@ hpt_status hpt_set_shade_sort(hpt_renderer* r, uint32_t first_bounce) {
 
 hpt_status hpt_set_tail_from(hpt_renderer* r, uint32_t first_bounce) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(first_bounce >= 1, "the first bounce always runs as a wavefront");
     r->tail_from = first_bounce;
   });
@@ -1334,7 +1368,7 @@ hpt_status hpt_set_tail_from(hpt_renderer* r, uint32_t first_bounce) {
 
 hpt_status hpt_group_set_tail_from(hpt_group* g, uint32_t first_bounce) {
   return guarded([&] {
-    need(g, "null group");
+    need(ok_group(g), "null or invalid group handle");
     need(first_bounce >= 1, "the first bounce always runs as a wavefront");
     g->tail_from = first_bounce;
   });
@@ -1348,7 +1382,7 @@ static bool traversal_block_ok(uint32_t threads) {
 
 hpt_status hpt_set_traversal_block(hpt_renderer* r, uint32_t threads) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(traversal_block_ok(threads), "the traversal kernels are built for 64, 128 or 256 threads per block");
     r->trav_block = threads;
   });
@@ -1356,7 +1390,7 @@ hpt_status hpt_set_traversal_block(hpt_renderer* r, uint32_t threads) {
 
 hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads) {
   return guarded([&] {
-    need(g, "null group");
+    need(ok_group(g), "null or invalid group handle");
     need(traversal_block_ok(threads), "the traversal kernels are built for 64, 128 or 256 threads per block");
     g->trav_block = threads;
   });
@@ -1364,7 +1398,7 @@ hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads) {
 
 hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(on <= 1, "on must be 0 or 1");
     r->vmis_phases = on;
   });
@@ -1372,7 +1406,7 @@ hpt_status hpt_set_vmis_phases(hpt_renderer* r, uint32_t on) {
 
 hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(mode == HPT_LAUNCH_EAGER || mode == HPT_LAUNCH_GRAPH, "unknown launch mode");
     r->launch_mode = mode;
   });
@@ -1380,7 +1414,7 @@ hpt_status hpt_set_launch_mode(hpt_renderer* r, uint32_t mode) {
 
 hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint64_t* graph_captures) {
   return guarded([&] {
-    need(r && graph_launches && graph_captures, "null argument");
+    need(ok_renderer(r) && graph_launches && graph_captures, "null argument");
     *graph_launches = r->graph_launches;
     *graph_captures = r->graph_captures;
   });
@@ -1388,7 +1422,7 @@ hpt_status hpt_get_launch_stats(hpt_renderer* r, uint64_t* graph_launches, uint6
 
 hpt_status hpt_reset_accumulation(hpt_renderer* r) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     r->frame_index = 0;
     r->frame_synced = false;
     if (r->committed) {
@@ -1403,7 +1437,7 @@ hpt_status hpt_reset_accumulation(hpt_renderer* r) {
 
 hpt_status hpt_set_frame_index(hpt_renderer* r, uint64_t frame_index) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(frame_index <= 0xFFFFFFFFull, "frame index must fit the 32-bit UBO field");
     r->frame_index = frame_index;
     r->frame_synced = false;
@@ -1412,14 +1446,14 @@ hpt_status hpt_set_frame_index(hpt_renderer* r, uint64_t frame_index) {
 
 hpt_status hpt_get_frame_index(hpt_renderer* r, uint64_t* fi) {
   return guarded([&] {
-    need(r && fi, "null argument");
+    need(ok_renderer(r) && fi, "null argument");
     *fi = r->frame_index;
   });
 }
 
 hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t bytes) {
   return guarded([&] {
-    need(r && dst, "null argument");
+    need(ok_renderer(r) && dst, "null argument");
     require_committed(r);
     set_device(r);
     const DevBuf* b = nullptr;
@@ -1438,7 +1472,7 @@ hpt_status hpt_read_image(hpt_renderer* r, hpt_image which, void* dst, size_t by
 
 hpt_status hpt_get_image_pointer(hpt_renderer* r, hpt_image which, void** ptr, size_t* bytes) {
   return guarded([&] {
-    need(r && ptr && bytes, "null argument");
+    need(ok_renderer(r) && ptr && bytes, "null argument");
     require_committed(r);
     const DevBuf* b = nullptr;
     switch (which) {
@@ -1484,7 +1518,7 @@ static void trace_api(hpt_renderer* r, const hpt_ray* rays, uint32_t n, int any,
 hpt_status hpt_trace_counts(hpt_renderer* r, const hpt_ray* rays, uint32_t n, int any_hit, uint32_t* node_visits,
                             uint32_t* tri_tests) {
   return guarded([&] {
-    need(r && (n == 0 || (rays && node_visits && tri_tests)), "null argument");
+    need(ok_renderer(r) && (n == 0 || (rays && node_visits && tri_tests)), "null argument");
     std::vector<float4> out;
     trace_api(r, rays, n, any_hit ? 1 : 0, out, node_visits, tri_tests);
   });
@@ -1492,7 +1526,7 @@ hpt_status hpt_trace_counts(hpt_renderer* r, const hpt_ray* rays, uint32_t n, in
 
 hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint32_t n, hpt_hit* hits) {
   return guarded([&] {
-    need(r && (n == 0 || (rays && hits)), "null argument");
+    need(ok_renderer(r) && (n == 0 || (rays && hits)), "null argument");
     std::vector<float4> out;
     trace_api(r, rays, n, 0, out);
     for (uint32_t i = 0; i < n; ++i) {
@@ -1513,7 +1547,7 @@ hpt_status hpt_trace_closest(hpt_renderer* r, const hpt_ray* rays, uint32_t n, h
 
 hpt_status hpt_trace_any(hpt_renderer* r, const hpt_ray* rays, uint32_t n, uint32_t* occluded) {
   return guarded([&] {
-    need(r && (n == 0 || (rays && occluded)), "null argument");
+    need(ok_renderer(r) && (n == 0 || (rays && occluded)), "null argument");
     std::vector<float4> out;
     trace_api(r, rays, n, 1, out);
     for (uint32_t i = 0; i < n; ++i) {
@@ -1557,7 +1591,7 @@ static uint2* scratch_on(int device) {
 
 hpt_status hpt_trace_closest_device(hpt_renderer* r, const hpt_ray* rays, uint32_t n, hpt_hit* hits) {
   return guarded([&] {
-    need(r && (n == 0 || (rays && hits)), "null argument");
+    need(ok_renderer(r) && (n == 0 || (rays && hits)), "null argument");
     set_device(r);
     require_committed(r);
     if (n == 0) return;
@@ -1579,14 +1613,15 @@ hpt_status hpt_trace_closest_device(hpt_renderer* r, const hpt_ray* rays, uint32
 
 hpt_status hpt_wait_idle(hpt_renderer* r) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     set_device(r);
     check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
   });
 }
 
 void hpt_destroy(hpt_renderer* r) {
-  if (!r) return;
+  if (!ok_renderer(r)) return;  // null, another kind of handle, or destroyed
+  r->tag = 0;
   (void)hipSetDevice(r->info.device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
@@ -1605,6 +1640,7 @@ void hpt_destroy(hpt_renderer* r) {
 // one without waiting for the tracer (the role of the swapchain hand-off in
 // the reference's update()/render() loop, hala-pathtracer.rs:236-289).
 struct hpt_presenter {
+  uint32_t tag = kPresenterTag;
   hpt_renderer* r = nullptr;
   int32_t device = 0;
   hipStream_t copy_stream = nullptr;
@@ -1615,10 +1651,13 @@ struct hpt_presenter {
   std::vector<uint64_t> frame;
   // Slot states: free, copy in flight, ready, held by the caller.
   enum { kFree = 0, kInFlight = 1, kReady = 2, kHeld = 3 };
+
   std::vector<int> state;
   uint32_t next = 0;
   uint64_t presented = 0, dropped = 0;
 };
+
+static bool ok_presenter(const hpt_presenter* p) { return p && p->tag == kPresenterTag; }
 
 namespace {
 
@@ -1649,7 +1688,7 @@ static void present_free(hpt_presenter* p) {
 hpt_status hpt_present_create(hpt_renderer* r, uint32_t slots, hpt_presenter** out) {
   hpt_presenter* p = nullptr;
   const hpt_status st = guarded([&] {
-    need(r && out, "null argument");
+    need(ok_renderer(r) && out, "null argument");
     *out = nullptr;
     need(slots >= 2 && slots <= 16, "slots must be 2..16");
     require_committed(r);
@@ -1678,7 +1717,7 @@ hpt_status hpt_present_create(hpt_renderer* r, uint32_t slots, hpt_presenter** o
 
 hpt_status hpt_present_frame(hpt_presenter* p) {
   return guarded([&] {
-    need(p, "null presenter");
+    need(ok_presenter(p), "null or invalid presenter handle");
     hpt_renderer* r = p->r;
     set_device(r);
     need(r->d_final.bytes == p->bytes, "the renderer's image size changed");
@@ -1716,7 +1755,7 @@ hpt_status hpt_present_frame(hpt_presenter* p) {
 
 hpt_status hpt_present_acquire(hpt_presenter* p, const void** pixels, uint64_t* frame_index) {
   return guarded([&] {
-    need(p && pixels && frame_index, "null argument");
+    need(ok_presenter(p) && pixels && frame_index, "null argument");
     *pixels = nullptr;
     *frame_index = 0;
     set_device(p->r);
@@ -1742,21 +1781,23 @@ hpt_status hpt_present_acquire(hpt_presenter* p, const void** pixels, uint64_t* 
 
 hpt_status hpt_present_get_stats(hpt_presenter* p, uint64_t* presented, uint64_t* dropped) {
   return guarded([&] {
-    need(p && presented && dropped, "null argument");
+    need(ok_presenter(p) && presented && dropped, "null argument");
     *presented = p->presented;
     *dropped = p->dropped;
   });
 }
 
 void hpt_present_destroy(hpt_presenter* p) {
-  if (p) present_free(p);
+  if (!ok_presenter(p)) return;
+  p->tag = 0;
+  present_free(p);
 }
 
 const char* hpt_last_error(void) { return g_last_error.c_str(); }
 
 hpt_status hpt_get_image_size(hpt_renderer* r, uint32_t* width, uint32_t* height) {
   return guarded([&] {
-    need(r && width && height, "null argument");
+    need(ok_renderer(r) && width && height, "null argument");
     *width = r->info.width;
     *height = r->info.height;
   });
@@ -1764,7 +1805,7 @@ hpt_status hpt_get_image_size(hpt_renderer* r, uint32_t* width, uint32_t* height
 
 hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info) {
   return guarded([&] {
-    need(r && info, "null argument");
+    need(ok_renderer(r) && info, "null argument");
     require_committed(r);
     info->node_count = r->sd->node_count;
     info->triangle_count = r->sd->leaf_tri_count;
@@ -1775,7 +1816,7 @@ hpt_status hpt_get_bvh_info(hpt_renderer* r, hpt_bvh_info* info) {
 
 hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes, void* tris, size_t tri_bytes) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     require_committed(r);
     need(node_bytes == (size_t)r->sd->node_count * sizeof(Node8), "node buffer size mismatch");
     need(tri_bytes == (size_t)r->sd->leaf_tri_count * sizeof(LeafTri), "triangle buffer size mismatch");
@@ -1787,7 +1828,7 @@ hpt_status hpt_export_bvh(hpt_renderer* r, void* nodes, size_t node_bytes, void*
 
 hpt_status hpt_set_frames_in_flight(hpt_renderer* r, uint32_t frames) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need(frames <= kMaxBatchFrames, "at most 64 frames in flight");
     const uint64_t px = (uint64_t)r->info.width * r->info.height;
     need((uint64_t)frames * px <= (1ull << 31), "frames * width * height must not exceed 2^31");
@@ -1797,14 +1838,14 @@ hpt_status hpt_set_frames_in_flight(hpt_renderer* r, uint32_t frames) {
 
 hpt_status hpt_get_frames_in_flight(hpt_renderer* r, uint32_t* frames) {
   return guarded([&] {
-    need(r && frames, "null argument");
+    need(ok_renderer(r) && frames, "null argument");
     *frames = r->npix ? batch_frames_for(r) : r->frames_in_flight;
   });
 }
 
 hpt_status hpt_set_profiling(hpt_renderer* r, int enable) {
   return guarded([&] {
-    need(r, "null renderer");
+    need(ok_renderer(r), "null or invalid renderer handle");
     need((enable & ~3) == 0, "unknown profiling bits");
     set_device(r);
     if (r->timers_used) collect_timers(r);
@@ -1821,7 +1862,7 @@ hpt_status hpt_set_profiling(hpt_renderer* r, int enable) {
 
 hpt_status hpt_get_stage_stats(hpt_renderer* r, int stage, hpt_stage_stats* out) {
   return guarded([&] {
-    need(r && out, "null argument");
+    need(ok_renderer(r) && out, "null argument");
     need(stage >= 0 && stage < HPT_STAGE_COUNT, "unknown stage");
     set_device(r);
     if (r->timers_used) collect_timers(r);
@@ -1863,15 +1904,23 @@ hpt_status hpt_group_create(int32_t device, hpt_group** out) {
 
 hpt_status hpt_group_set_launch_mode(hpt_group* g, uint32_t mode) {
   return guarded([&] {
-    need(g, "null group");
+    need(ok_group(g), "null or invalid group handle");
     need(mode == HPT_LAUNCH_EAGER || mode == HPT_LAUNCH_GRAPH, "unknown launch mode");
     g->launch_mode = mode;
   });
 }
 
+hpt_status hpt_group_set_vmis_phases(hpt_group* g, uint32_t on) {
+  return guarded([&] {
+    need(ok_group(g), "null or invalid group handle");
+    need(on <= 1, "on must be 0 or 1");
+    g->vmis_phases = on;
+  });
+}
+
 hpt_status hpt_group_set_shade_sort(hpt_group* g, uint32_t first_bounce) {
   return guarded([&] {
-    need(g, "null group");
+    need(ok_group(g), "null or invalid group handle");
     g->shade_sort_from = first_bounce;
   });
 }
@@ -1879,13 +1928,13 @@ hpt_status hpt_group_set_shade_sort(hpt_group* g, uint32_t first_bounce) {
 hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint32_t* frames,
                             const uint32_t* camera_indices, uint32_t count) {
   return guarded([&] {
-    need(g && jobs && frames, "null argument");
+    need(ok_group(g) && jobs && frames, "null argument");
     need(count >= 1 && count <= kMaxWaveJobs, "a group step takes 1 to 256 jobs");
     hpt_renderer* r0 = jobs[0];
-    need(r0 != nullptr, "null job");
+    need(ok_renderer(r0), "null or invalid job handle");
     for (uint32_t i = 0; i < count; ++i) {
       hpt_renderer* r = jobs[i];
-      need(r != nullptr, "null job");
+      need(ok_renderer(r), "null or invalid job handle");
       require_committed(r);
       need(r->info.device == g->device, "job on another device than the group");
       need(r->sd == r0->sd, "the jobs of a group step must render one committed scene (hpt_share_scene)");
@@ -1966,7 +2015,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     wb.sort_keys = g->d_sort_keys.as<uint8_t>();
     wb.sort_items = g->d_sort_items.as<uint32_t>();
     wb.tail_cut = g->d_tail_cut.as<unsigned long long>();
-    wb.qc = g->d_qc.as<uint32_t>();
+    wb.qc = g->vmis_phases ? g->d_qc.as<uint32_t>() : nullptr;
     key_append(key, sv);
     key_append(key, shade_fp);
     key_append(key, wb.ps);
@@ -1981,6 +2030,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     key_append(key, g->shade_sort_from);
     key_append(key, g->tail_from);
     key_append(key, g->trav_block);
+    key_append(key, g->vmis_phases);
     for (const WaveJob& j : wj) key_append(key, j.base);
     const uint32_t njobs = (uint32_t)wj.size();
     const uint32_t fixed_bounces = max_depth + 1;
@@ -2045,7 +2095,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
 
 hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_launches, uint64_t* device_bytes) {
   return guarded([&] {
-    need(g && steps && graph_launches && device_bytes, "null argument");
+    need(ok_group(g) && steps && graph_launches && device_bytes, "null argument");
     uint64_t b = 0;
     for (const DevBuf* d : {&g->d_rng, &g->d_org, &g->d_dir, &g->d_thr, &g->d_rad, &g->d_alb, &g->d_nrm, &g->d_med,
                             &g->d_med2, &g->d_nee_e, &g->d_nee_l, &g->d_hit, &g->d_job, &g->d_qa, &g->d_qb,
@@ -2061,7 +2111,7 @@ hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_la
 
 hpt_status hpt_group_get_truncated_paths(hpt_group* g, uint64_t* paths) {
   return guarded([&] {
-    need(g && paths, "null argument");
+    need(ok_group(g) && paths, "null argument");
     *paths = 0;
     if (!g->d_tail_cut.p) return;
     check(hipSetDevice(g->device), "hipSetDevice");
@@ -2074,14 +2124,15 @@ hpt_status hpt_group_get_truncated_paths(hpt_group* g, uint64_t* paths) {
 
 hpt_status hpt_group_wait_idle(hpt_group* g) {
   return guarded([&] {
-    need(g, "null group");
+    need(ok_group(g), "null or invalid group handle");
     check(hipSetDevice(g->device), "hipSetDevice");
     check(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
   });
 }
 
 void hpt_group_destroy(hpt_group* g) {
-  if (!g) return;
+  if (!ok_group(g)) return;
+  g->tag = 0;
   (void)hipSetDevice(g->device);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);

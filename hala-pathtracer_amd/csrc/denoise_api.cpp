@@ -181,7 +181,12 @@ int pad16(int c) { return (c + 15) / 16 * 16; }
 
 }  // namespace
 
+// The handle starts with a tag of its kind, so another kind of handle (or a
+// destroyed one) is refused instead of written into.
+constexpr uint32_t kDenoiserTag = 0x44545048u;  // "HPTD"
+
 struct hpt_denoiser {
+  uint32_t tag = kDenoiserTag;
   int32_t device = 0;
   hipStream_t stream = nullptr;
   std::vector<int> levels;
@@ -391,7 +396,7 @@ hpt_status hpt_denoiser_create(const void* npz, size_t size, int32_t device, hpt
 
 hpt_status hpt_denoise(hpt_denoiser* d, hpt_renderer* r, float* dst, size_t bytes, int dst_on_device) {
   return guarded([&] {
-    need(d && r && dst, "null argument");
+    need(d && d->tag == kDenoiserTag && r && dst, "null argument or invalid denoiser handle");
     hip_ok(hipSetDevice(d->device), "hipSetDevice");
     uint32_t width = 0, height = 0;
     if (hpt_get_image_size(r, &width, &height) != HPT_OK) throw Fail(HPT_ERROR_STATE, hpt_last_error());
@@ -416,10 +421,13 @@ hpt_status hpt_denoise(hpt_denoiser* d, hpt_renderer* r, float* dst, size_t byte
     forward(d, h, w);
     // The denoised colour as float4 pixels (alpha 1): into the caller's
     // device image, or through a device buffer of the handle into host memory.
+    // That buffer holds the padded h*w pixels, so every image size that
+    // shares these activation buffers fits it.
     float* target = dst;
     if (!dst_on_device) {
       auto it = d->buf.find("result");
-      target = static_cast<float*>(it != d->buf.end() ? it->second : (d->buf["result"] = dev_alloc(d, bytes)));
+      target = static_cast<float*>(it != d->buf.end() ? it->second
+                                                       : (d->buf["result"] = dev_alloc(d, (size_t)h * w * 16)));
     }
     hip_ok(hpt_nn_write_output(d->stream, static_cast<const float*>(d->buf.at("out")), target, (int)width,
                                (int)height, 0, 0, h, w),
@@ -429,6 +437,10 @@ hpt_status hpt_denoise(hpt_denoiser* d, hpt_renderer* r, float* dst, size_t byte
   });
 }
 
-void hpt_denoiser_destroy(hpt_denoiser* d) { delete d; }
+void hpt_denoiser_destroy(hpt_denoiser* d) {
+  if (!d || d->tag != kDenoiserTag) return;
+  d->tag = 0;
+  delete d;
+}
 
 }  // extern "C"

@@ -233,6 +233,119 @@ pub struct hpt_denoiser {
     _private: [u8; 0],
 }
 
+// ------------------------------------------------------------ serving loop
+// hpt_server_* of include/hpt.h: admission by device footprint, jobs on
+// streams of their own (or one wavefront per scene), results by id.
+
+/// Opaque server handle (hpt_server_*).
+#[repr(C)]
+pub struct hpt_server {
+    _private: [u8; 0],
+}
+
+pub const HPT_SERVE_STREAMS: u32 = 0;
+pub const HPT_SERVE_WAVEFRONT: u32 = 1;
+pub const HPT_SERVE_DEFAULT: u32 = 0xFFFF_FFFF;
+
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
+pub struct hpt_server_config {
+    pub device: i32,
+    pub mode: u32,
+    pub budget_bytes: u64,
+    pub max_frames: u32,
+    pub keep_images: u32,
+    pub step_paths: u64,
+    pub launch_mode: u32,
+    pub max_running: u32,
+    pub dry_run_ns_per_path: u64,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy)]
+pub struct hpt_serve_scene_desc {
+    pub scene: *const hpt_renderer,
+    pub features: u32,
+    pub step_launch_mode: u32,
+    pub step_shade_sort: u32,
+    pub step_tail_from: u32,
+    pub step_traversal_block: u32,
+    pub dry_run_fail: u32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
+pub struct hpt_job_desc {
+    pub scene: u32,
+    pub width: u32,
+    pub height: u32,
+    pub spp: u32,
+    pub max_depth: u32,
+    pub rr_depth: u32,
+    pub camera_index: u32,
+    pub enable_tonemap: u32,
+    pub enable_aces: u32,
+    pub use_simple_aces: u32,
+    pub exposure_value: f32,
+    pub frames_in_flight: u32,
+    pub launch_mode: u32,
+    pub shade_sort: u32,
+    pub tail_from: u32,
+    pub traversal_block: u32,
+    pub denoise: u32,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy)]
+pub struct hpt_job_result {
+    pub job_id: u64,
+    pub status: hpt_status,
+    pub width: u32,
+    pub height: u32,
+    pub frames: u32,
+    pub has_image: u32,
+    pub _pad: u32,
+    pub footprint_bytes: u64,
+    pub submitted_s: f64,
+    pub admitted_s: f64,
+    pub finished_s: f64,
+    pub error: [c_char; 256],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
+pub struct hpt_server_stats {
+    pub submitted: u64,
+    pub finished: u64,
+    pub failed: u64,
+    pub bytes_in_use: u64,
+    pub peak_bytes: u64,
+    pub wavefront_steps: u64,
+    pub max_jobs_per_step: u64,
+    pub jobs_in_steps: u64,
+}
+
+extern "C" {
+    pub fn hpt_group_set_vmis_phases(g: *mut hpt_group, on: u32) -> hpt_status;
+    pub fn hpt_server_config_init(c: *mut hpt_server_config);
+    pub fn hpt_job_desc_init(d: *mut hpt_job_desc);
+    pub fn hpt_server_create(config: *const hpt_server_config, out: *mut *mut hpt_server) -> hpt_status;
+    pub fn hpt_server_add_scene(s: *mut hpt_server, scene: *const hpt_serve_scene_desc, scene_id: *mut u32)
+        -> hpt_status;
+    pub fn hpt_server_set_denoiser(s: *mut hpt_server, d: *mut hpt_denoiser) -> hpt_status;
+    pub fn hpt_server_plan_job(s: *mut hpt_server, job: *const hpt_job_desc, footprint_bytes: *mut u64,
+        chunk_frames: *mut u32) -> hpt_status;
+    pub fn hpt_server_submit(s: *mut hpt_server, job: *const hpt_job_desc, job_id: *mut u64) -> hpt_status;
+    pub fn hpt_server_wait(s: *mut hpt_server, job_id: u64, out: *mut hpt_job_result) -> hpt_status;
+    pub fn hpt_server_next_result(s: *mut hpt_server, timeout_s: f64, out: *mut hpt_job_result, got: *mut c_int)
+        -> hpt_status;
+    pub fn hpt_server_read_image(s: *mut hpt_server, job_id: u64, dst: *mut f32, bytes: usize) -> hpt_status;
+    pub fn hpt_server_drain(s: *mut hpt_server) -> hpt_status;
+    pub fn hpt_server_clock(s: *mut hpt_server, seconds: *mut f64) -> hpt_status;
+    pub fn hpt_server_get_stats(s: *mut hpt_server, out: *mut hpt_server_stats) -> hpt_status;
+    pub fn hpt_server_destroy(s: *mut hpt_server);
+}
+
 /// The learned denoiser in place of denoise.py's oidnDenoise step
 /// (denoise.py:26-30), run on the renderer's resident colour, albedo and
 /// normal images.

@@ -146,6 +146,40 @@ class _StageStats(C.Structure):
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("queries", C.c_uint64)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("mode", C.c_uint32), ("budget_bytes", C.c_uint64),
+                ("max_frames", C.c_uint32), ("keep_images", C.c_uint32), ("step_paths", C.c_uint64),
+                ("launch_mode", C.c_uint32), ("max_running", C.c_uint32), ("dry_run_ns_per_path", C.c_uint64)]
+
+
+class _ServeSceneDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("features", C.c_uint32), ("step_launch_mode", C.c_uint32),
+                ("step_shade_sort", C.c_uint32), ("step_tail_from", C.c_uint32),
+                ("step_traversal_block", C.c_uint32), ("dry_run_fail", C.c_uint32)]
+
+
+class _JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("camera_index", C.c_uint32),
+                ("enable_tonemap", C.c_uint32), ("enable_aces", C.c_uint32), ("use_simple_aces", C.c_uint32),
+                ("exposure_value", C.c_float), ("frames_in_flight", C.c_uint32), ("launch_mode", C.c_uint32),
+                ("shade_sort", C.c_uint32), ("tail_from", C.c_uint32), ("traversal_block", C.c_uint32),
+                ("denoise", C.c_uint32)]
+
+
+class _JobResult(C.Structure):
+    _fields_ = [("job_id", C.c_uint64), ("status", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("frames", C.c_uint32), ("has_image", C.c_uint32), ("_pad", C.c_uint32),
+                ("footprint_bytes", C.c_uint64), ("submitted_s", C.c_double), ("admitted_s", C.c_double),
+                ("finished_s", C.c_double), ("error", C.c_char * 256)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("finished", C.c_uint64), ("failed", C.c_uint64),
+                ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("wavefront_steps", C.c_uint64),
+                ("max_jobs_per_step", C.c_uint64), ("jobs_in_steps", C.c_uint64)]
+
+
 # Every exported entry point with its argument types (the declarations of
 # include/hpt.h).
 SIGNATURES = {
@@ -212,6 +246,21 @@ SIGNATURES = {
     "hpt_denoiser_create": [C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(C.c_void_p)],
     "hpt_denoise": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int],
     "hpt_denoiser_destroy": [C.c_void_p],
+    "hpt_group_set_vmis_phases": [C.c_void_p, C.c_uint32],
+    "hpt_server_config_init": [C.POINTER(_ServerConfig)],
+    "hpt_job_desc_init": [C.POINTER(_JobDesc)],
+    "hpt_server_create": [C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)],
+    "hpt_server_add_scene": [C.c_void_p, C.POINTER(_ServeSceneDesc), C.POINTER(C.c_uint32)],
+    "hpt_server_set_denoiser": [C.c_void_p, C.c_void_p],
+    "hpt_server_plan_job": [C.c_void_p, C.POINTER(_JobDesc), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "hpt_server_submit": [C.c_void_p, C.POINTER(_JobDesc), C.POINTER(C.c_uint64)],
+    "hpt_server_wait": [C.c_void_p, C.c_uint64, C.POINTER(_JobResult)],
+    "hpt_server_next_result": [C.c_void_p, C.c_double, C.POINTER(_JobResult), C.POINTER(C.c_int)],
+    "hpt_server_read_image": [C.c_void_p, C.c_uint64, C.c_void_p, C.c_size_t],
+    "hpt_server_drain": [C.c_void_p],
+    "hpt_server_clock": [C.c_void_p, C.POINTER(C.c_double)],
+    "hpt_server_get_stats": [C.c_void_p, C.POINTER(_ServerStats)],
+    "hpt_server_destroy": [C.c_void_p],
 }
 
 _lib = None
@@ -228,7 +277,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        if name in ("hpt_destroy", "hpt_group_destroy", "hpt_present_destroy", "hpt_denoiser_destroy"):
+        if name in ("hpt_destroy", "hpt_group_destroy", "hpt_present_destroy", "hpt_denoiser_destroy",
+                    "hpt_server_destroy", "hpt_server_config_init", "hpt_job_desc_init"):
             fn.restype = None
         elif name == "hpt_last_error":
             fn.restype = C.c_char_p
@@ -620,7 +670,7 @@ class HalaGroup:
 
     def set_vmis_phases(self, on: bool = True) -> None:
         """Phased (default) or single-kernel volume-MIS shading; same images."""
-        _check(self._lib.hpt_set_vmis_phases(self._h, int(bool(on))))
+        _check(self._lib.hpt_group_set_vmis_phases(self._h, int(bool(on))))
 
     def set_shade_sort(self, first_bounce: int = SHADE_SORT_OFF) -> None:
         _check(self._lib.hpt_group_set_shade_sort(self._h, first_bounce))
@@ -684,6 +734,115 @@ class Denoiser:
         if getattr(self, "_h", None):
             self._lib.hpt_denoiser_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SERVE_STREAMS = 0
+SERVE_WAVEFRONT = 1
+SERVE_DEFAULT = 0xFFFFFFFF
+
+
+class RenderServer:
+    """hpt_server_*: the serving loop behind the C ABI.  Jobs over resident
+    scenes are admitted in submission order while their device footprints fit
+    `budget_bytes`, run on streams of their own (SERVE_STREAMS) or in one
+    wavefront per scene (SERVE_WAVEFRONT), and end with their image in host
+    memory.  `dry_run_ns_per_path` > 0 schedules without a device (each batch
+    sleeps frames * pixels * that many ns; images hold the job id)."""
+
+    def __init__(self, budget_bytes: int, mode: int = SERVE_STREAMS, max_frames: int = 32, keep_images: bool = False,
+                 step_paths: int = 8 << 20, launch: int = LAUNCH_GRAPH, device: int = 0, dry_run_ns_per_path: int = 0,
+                 max_running: int = 8):
+        self._lib = load_library()
+        cfg = _ServerConfig()
+        self._lib.hpt_server_config_init(C.byref(cfg))
+        cfg.device, cfg.mode, cfg.budget_bytes, cfg.max_frames = device, mode, budget_bytes, max_frames
+        cfg.keep_images, cfg.step_paths, cfg.launch_mode = int(keep_images), step_paths, launch
+        cfg.dry_run_ns_per_path = dry_run_ns_per_path
+        cfg.max_running = max_running
+        h = C.c_void_p()
+        _check(self._lib.hpt_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._keep = []  # scenes and denoiser the server borrows
+
+    def add_scene(self, template: "HalaRenderer | None", features: int, launch: int = SERVE_DEFAULT,
+                  shade_sort: int = SERVE_DEFAULT, tail_from: int = SERVE_DEFAULT, traversal_block: int = SERVE_DEFAULT,
+                  dry_run_fail: bool = False) -> int:
+        d = _ServeSceneDesc(template._h if template is not None else None, features, launch, shade_sort, tail_from,
+                            traversal_block, int(dry_run_fail))
+        sid = C.c_uint32()
+        _check(self._lib.hpt_server_add_scene(self._h, C.byref(d), C.byref(sid)))
+        self._keep.append(template)
+        return sid.value
+
+    def set_denoiser(self, d: "Denoiser | None") -> None:
+        _check(self._lib.hpt_server_set_denoiser(self._h, d._h if d is not None else None))
+        self._keep.append(d)
+
+    def job_desc(self, scene: int, width: int, height: int, spp: int, max_depth: int, rr_depth: int = 3,
+                 **fields) -> _JobDesc:
+        d = _JobDesc()
+        self._lib.hpt_job_desc_init(C.byref(d))
+        d.scene, d.width, d.height, d.spp, d.max_depth, d.rr_depth = scene, width, height, spp, max_depth, rr_depth
+        for k, v in fields.items():
+            setattr(d, k, v)
+        return d
+
+    def plan(self, desc: _JobDesc) -> tuple:
+        """(footprint bytes, frames per batch or step) of a job."""
+        fp, k = C.c_uint64(), C.c_uint32()
+        _check(self._lib.hpt_server_plan_job(self._h, C.byref(desc), C.byref(fp), C.byref(k)))
+        return fp.value, k.value
+
+    def submit(self, desc: _JobDesc) -> int:
+        jid = C.c_uint64()
+        _check(self._lib.hpt_server_submit(self._h, C.byref(desc), C.byref(jid)))
+        return jid.value
+
+    @staticmethod
+    def _result(r: _JobResult) -> dict:
+        return {"job_id": r.job_id, "status": r.status, "width": r.width, "height": r.height, "frames": r.frames,
+                "has_image": bool(r.has_image), "footprint_bytes": r.footprint_bytes, "submitted": r.submitted_s,
+                "admitted": r.admitted_s, "finished": r.finished_s, "error": r.error.decode() or None}
+
+    def wait(self, job_id: int) -> dict:
+        r = _JobResult()
+        _check(self._lib.hpt_server_wait(self._h, job_id, C.byref(r)))
+        return self._result(r)
+
+    def next_result(self, timeout: float = -1.0) -> dict | None:
+        r, got = _JobResult(), C.c_int()
+        _check(self._lib.hpt_server_next_result(self._h, timeout, C.byref(r), C.byref(got)))
+        return self._result(r) if got.value else None
+
+    def read_image(self, job_id: int, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), np.float32)
+        _check(self._lib.hpt_server_read_image(self._h, job_id, _ptr(out), out.nbytes))
+        return out
+
+    def drain(self) -> None:
+        _check(self._lib.hpt_server_drain(self._h))
+
+    def clock(self) -> float:
+        t = C.c_double()
+        _check(self._lib.hpt_server_clock(self._h, C.byref(t)))
+        return t.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(self._lib.hpt_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.hpt_server_destroy(self._h)
+            self._h = None
+            self._keep = []
 
     def __del__(self):
         try:

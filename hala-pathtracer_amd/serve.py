@@ -1,6 +1,12 @@
 """Render serving on one GPU: progressive jobs of mixed scene, resolution,
 samples and depth, admitted by device-memory footprint and run concurrently.
 
+The loop itself is native (hpt_server_* in include/hpt.h, csrc/serve.cpp):
+admission, the jobs' host threads, wavefront steps and read-back run in C++,
+so a host without Python serves the same way.  This module holds the job
+specs, the per-class tuning it passes into each job, the Poisson job draw,
+the resident-scene wrapper and the latency summary.
+
 Design (no counterpart in the reference, which renders one scene per
 process):
 
@@ -13,11 +19,13 @@ process):
   with k frames in flight.  Jobs are admitted in arrival order while the sum
   of admitted footprints fits the budget; a job that does not fit waits (and
   blocks the ones behind it, so large jobs are not starved).
-* Execution.  An admitted job runs on its own HIP stream in chunks of k
-  frames from a worker thread; the device interleaves the kernels of the
-  active jobs (the hardware queues run streams concurrently).  When its last
-  chunk ends, the job's image is read back (the latency end point), its
-  handle is closed and its memory returns to the budget.
+* Execution.  An admitted job runs on its own HIP stream: a host thread of
+  the server renders its batches of k frames one at a time (the streams share
+  the process's few in-order hardware queues, so one batch in flight per job
+  keeps a long job from holding the others); the device interleaves the
+  kernels of the active jobs.  When its last batch ends, the job's image is read back (the
+  latency end point), its handle is closed and its memory returns to the
+  budget.
 * Wavefront mode (`wavefront=True`).  The jobs of one resident scene share
   one wavefront (`hpt_group_render`): a worker per scene renders, step after
   step, the next chunk of every active job of that scene together -- one
@@ -37,8 +45,8 @@ process):
 """
 from __future__ import annotations
 
+import io
 import math
-import threading
 import time
 from dataclasses import dataclass, field
 
@@ -146,242 +154,136 @@ class ResidentScene:
         self.template.close()
 
 
-MAX_STEP_JOBS = 256  # jobs per wavefront step (hpt_group_render)
-
-
-class _Active:
-    """A job in a scene's wavefront."""
-
-    def __init__(self, spec, handle, result, fp, k):
-        self.spec, self.handle, self.result, self.fp, self.k = spec, handle, result, fp, k
-        self.done = 0
+@dataclass
+class DryRunScene:
+    """A resident scene for dry runs (hpt_server_config.dry_run_ns_per_path):
+    no device copy; `fail` makes each of its jobs fail when it starts."""
+    features: int = 0
+    fail: bool = False
+    template = None
 
 
 class Server:
+    """Serves JobSpecs through the native serving loop (hpt_server_*,
+    csrc/serve.cpp): admission in arrival order by device footprint, one
+    stream per job or one wavefront per scene (`wavefront`), each job's
+    image read back (denoised on the device when a denoiser is given).
+
+    `scenes` maps names to ResidentScene (or DryRunScene with
+    `dry_run_ns_per_path` > 0); `tuning` (an autotune.TuningCache keyed by
+    (scene, width, height, 0)) gives each job its batch size, launch mode,
+    shading sort, tail bounce and traversal block; in wavefront mode each
+    scene's steps take the choice most of its classes were tuned to.  run()
+    submits every spec at its arrival time (seconds after the call) and
+    returns the results in completion order; times are seconds after the
+    call.  `stats` holds the native server's counters of the last run."""
+
     def __init__(self, scenes: dict, budget_bytes: int, max_frames: int = 32, keep_images: bool = False,
                  device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH, wavefront: bool = False,
-                 step_paths: int = 8 << 20, denoiser=None):
-        self.scenes = scenes                  # name -> ResidentScene
+                 step_paths: int = 8 << 20, denoiser=None, dry_run_ns_per_path: int = 0, max_running: int = 8):
+        self.scenes = scenes
         self.budget = budget_bytes
         self.max_frames = max_frames
-        self.tuning = tuning                  # autotune.TuningCache keyed by (scene, width, height, 0)
-        self.launch = launch                  # launch mode of jobs the tuning does not cover
+        self.tuning = tuning
+        self.launch = launch
         self.keep_images = keep_images
         self.device = device
-        self.wavefront = wavefront            # jobs of a scene share one wavefront per step
-        self.step_paths = step_paths          # paths per job and wavefront step
-        self.denoiser = denoiser              # denoiser.UNet: a finished job's image is denoised before read-back
-        self._net_lock = threading.Lock()     # one network, one set of activation buffers
-        self._lock = threading.Condition()
-        self._used = 0
-        self._pending = {}                    # scene -> jobs admitted but not yet in its wavefront
-        self._closed = False
-        self.step_sizes = []                  # jobs per wavefront step, in step order
+        self.wavefront = wavefront
+        self.step_paths = step_paths
+        self.denoiser = denoiser          # denoiser.UNet or hpt.Denoiser
+        self.dry_run_ns_per_path = dry_run_ns_per_path
+        self.max_running = max_running    # streams mode: jobs running at once (0: as the budget allows)
+        self.stats = {}
+        self._net = None                  # the hpt.Denoiser the native server uses
 
-    def _make_job(self, spec: JobSpec):
-        rs = self.scenes[spec.scene]
-        r = hpt.HalaRenderer(spec.width, spec.height, max_depth=spec.max_depth, rr_depth=spec.rr_depth,
-                             enable_tonemap=True, features=rs.features, device=self.device)
-        # A wavefront job traces in its scene's group; its own batch stays one frame.
-        r.frames_in_flight = 1 if self.wavefront else frames_in_flight(spec, self.max_frames, self.tuning)
-        r.share_scene(rs.template)
-        r.set_launch_mode(launch_mode(spec, self.tuning, self.launch))
-        r.set_shade_sort(shade_sort(spec, self.tuning))
-        r.set_tail_from(tail_from(spec, self.tuning))
-        r.set_traversal_block(traversal_block(spec, self.tuning))
-        return r
+    def _native(self, budget=None):
+        """A native server with this server's settings and scenes, and the
+        scene ids by name."""
+        srv = hpt.RenderServer(self.budget if budget is None else budget,
+                               hpt.SERVE_WAVEFRONT if self.wavefront else hpt.SERVE_STREAMS, self.max_frames,
+                               self.keep_images, self.step_paths, self.launch, self.device,
+                               self.dry_run_ns_per_path, self.max_running)
+        ids = {}
+        for name, rs in self.scenes.items():
+            ids[name] = srv.add_scene(rs.template, rs.features, *self._step_settings(name),
+                                      dry_run_fail=bool(getattr(rs, "fail", False)))
+        return srv, ids
 
-    def _run_job(self, spec: JobSpec, t0: float, admitted: float, results: list, fp: int):
-        res = JobResult(spec.job_id, spec, admitted, 0.0)
+    def _step_settings(self, scene: str) -> tuple:
+        """Launch mode, shading sort, tail bounce and traversal block of a
+        scene's wavefront steps: the choice most of its tuned classes made."""
+        entries = [t for key, t in (self.tuning.entries.items() if self.tuning else []) if key.startswith(scene + "|")]
+
+        def most(attr):
+            vals = [getattr(t, attr) for t in entries]
+            return max(set(vals), key=vals.count) if vals else hpt.SERVE_DEFAULT
+
+        return self.launch, most("shade_sort"), most("tail_from"), most("trav_block")
+
+    def _desc(self, srv, scene_id: int, spec: JobSpec):
+        return srv.job_desc(scene_id, spec.width, spec.height, spec.spp, spec.max_depth, spec.rr_depth,
+                            frames_in_flight=frames_in_flight(spec, self.max_frames, self.tuning),
+                            launch_mode=launch_mode(spec, self.tuning, self.launch),
+                            shade_sort=shade_sort(spec, self.tuning), tail_from=tail_from(spec, self.tuning),
+                            traversal_block=traversal_block(spec, self.tuning),
+                            denoise=int(self.denoiser is not None))
+
+    def _plan(self, spec: JobSpec) -> tuple:
+        srv, ids = self._native(budget=1 << 62)
         try:
-            r = self._make_job(spec)
-            k = frames_in_flight(spec, self.max_frames, self.tuning)
-            done = 0
-            while done < spec.spp:
-                n = min(k, spec.spp - done)
-                r.render(n)
-                r.wait_idle()
-                done += n
-            img = self._result_image(r)
-            res.finished = time.perf_counter() - t0
-            if self.keep_images:
-                res.image = img
-            r.close()
-        except Exception as e:  # a failing job does not take the others down
-            res.finished = time.perf_counter() - t0
-            res.error = f"{type(e).__name__}: {e}"
-        with self._lock:
-            self._used -= fp
-            results.append(res)
-            self._lock.notify_all()
+            return srv.plan(self._desc(srv, ids[spec.scene], spec))
+        finally:
+            srv.close()
 
     def step_frames(self, spec: JobSpec) -> int:
-        """Frames of `spec` per wavefront step: about step_paths paths."""
-        return max(1, min(spec.spp, self.max_frames * 4, round(self.step_paths / (spec.width * spec.height))))
+        """Frames of `spec` per batch (streams) or per wavefront step (about
+        step_paths paths)."""
+        return self._plan(spec)[1]
 
     def job_footprint(self, spec: JobSpec) -> int:
-        """Device bytes admitting `spec` takes: its frame state, or in wavefront
-        mode its share of the scene's wavefront plus its one-frame handle."""
-        if not self.wavefront:
-            return footprint(spec, self.max_frames, self.tuning)
-        px = spec.width * spec.height
-        return px * ((self.step_frames(spec) + 1) * PATH_BYTES + IMAGE_BYTES)
+        """Device bytes admitting `spec` takes."""
+        return self._plan(spec)[0]
 
-    def _result_image(self, r) -> np.ndarray:
-        """A finished job's image in host memory: its accumulated radiance, or
-        that image denoised on the device from the job's resident colour,
-        albedo and normal images and its sample count."""
-        if self.denoiser is None:
-            return r.read_image(hpt.IMAGE_ACCUM)
-        import denoiser
-
-        with self._net_lock:
-            out = self.denoiser.denoise_renderer(r)
-            denoiser.synchronize()
-            return out.numpy()
-
-    def _new_group(self, scene: str):
-        g = hpt.HalaGroup(self.device)
-        g.set_launch_mode(self.launch)
-        sorts = [t.shade_sort for key, t in (self.tuning.entries.items() if self.tuning else [])
-                 if key.startswith(scene + "|")]
-        if sorts:  # the choice most classes of this scene were tuned to
-            g.set_shade_sort(max(set(sorts), key=sorts.count))
-        tails = [t.tail_from for key, t in (self.tuning.entries.items() if self.tuning else [])
-                 if key.startswith(scene + "|")]
-        if tails:
-            g.set_tail_from(max(set(tails), key=tails.count))
-        blocks = [t.trav_block for key, t in (self.tuning.entries.items() if self.tuning else [])
-                  if key.startswith(scene + "|")]
-        if blocks:
-            g.set_traversal_block(max(set(blocks), key=blocks.count))
-        return g
-
-    def _finish(self, a: _Active, t0: float, results: list, error: str | None = None):
-        res = a.result
-        try:
-            if error is None:
-                img = self._result_image(a.handle)
-                if self.keep_images:
-                    res.image = img
-        except Exception as e:
-            error = f"{type(e).__name__}: {e}"
-        res.finished = time.perf_counter() - t0
-        res.error = error
-        try:
-            if a.handle is not None:
-                a.handle.close()
-        finally:
-            with self._lock:
-                self._used -= a.fp
-                results.append(res)
-                self._lock.notify_all()
-
-    def _scene_worker(self, scene: str, t0: float, results: list):
-        """The wavefront of one scene: step after step, the next chunk of every
-        active job together; finished jobs leave, newly admitted ones join."""
-        g = None
-        active, finishing = [], []
-        try:
-            while True:
-                with self._lock:
-                    while not self._pending[scene] and not active and not finishing and not self._closed:
-                        self._lock.wait()
-                    active.extend(self._pending[scene])
-                    self._pending[scene].clear()
-                    if not active and not finishing:
-                        return
-                ended = []
-                if active:
-                    if g is None:
-                        g = self._new_group(scene)
-                    step = active[:MAX_STEP_JOBS]
-                    frames = [min(a.k, a.spec.spp - a.done) for a in step]
-                    try:
-                        g.render([a.handle for a in step], frames)  # returns with at most two steps queued
-                    except Exception as e:  # the step's jobs fail; the server goes on
-                        for a in step:
-                            active.remove(a)
-                            self._finish(a, t0, results, f"{type(e).__name__}: {e}")
-                        continue
-                    with self._lock:
-                        self.step_sizes.append(len(step))
-                    for a, n in zip(step, frames):
-                        a.done += n
-                        if a.done >= a.spec.spp:
-                            active.remove(a)
-                            ended.append(a)
-                # Jobs whose last step was queued before this one: their read-back
-                # waits for that step while the device runs the next.
-                for a in finishing:
-                    self._finish(a, t0, results)
-                finishing = ended
-        finally:
-            if g is not None:
-                g.close()
-
-    def _run_wavefront(self, specs: list) -> list:
-        results, workers = [], {}
-        t0 = time.perf_counter()
-        self._closed = False
-        for spec in specs:
-            wait = spec.arrival - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            fp = self.job_footprint(spec)
-            with self._lock:
-                while self._used + fp > self.budget:
-                    self._lock.wait()
-                self._used += fp
-            admitted = time.perf_counter() - t0
-            res = JobResult(spec.job_id, spec, admitted, 0.0)
-            k = self.step_frames(spec)
-            try:
-                a = _Active(spec, self._make_job(spec), res, fp, k)
-            except Exception as e:
-                self._finish(_Active(spec, None, res, fp, k), t0, results, f"{type(e).__name__}: {e}")
-                continue
-            with self._lock:
-                self._pending.setdefault(spec.scene, []).append(a)
-                if spec.scene not in workers:
-                    th = threading.Thread(target=self._scene_worker, args=(spec.scene, t0, results), daemon=True)
-                    workers[spec.scene] = th
-                    th.start()
-                self._lock.notify_all()
-        with self._lock:
-            self._closed = True
-            self._lock.notify_all()
-        for th in workers.values():
-            th.join()
-        return results
+    def _native_denoiser(self):
+        if self.denoiser is None or isinstance(self.denoiser, hpt.Denoiser):
+            return self.denoiser
+        if self._net is None:  # a denoiser.UNet: its weights behind the C ABI
+            buf = io.BytesIO()
+            self.denoiser.save(buf)
+            self._net = hpt.Denoiser(buf.getvalue(), self.device)
+        return self._net
 
     def run(self, specs: list) -> list:
         """Serves `specs` (arrival times relative to now); returns the results
         in completion order."""
         specs = sorted(specs, key=lambda s: s.arrival)
-        for s in specs:
-            if self.job_footprint(s) > self.budget:
-                raise ValueError(f"job {s.job_id} needs {self.job_footprint(s)} B, above the budget")
-        if self.wavefront:
-            return self._run_wavefront(specs)
-        results, threads = [], []
-        t0 = time.perf_counter()
-        for spec in specs:
-            wait = spec.arrival - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            fp = footprint(spec, self.max_frames, self.tuning)
-            with self._lock:
-                while self._used + fp > self.budget:
-                    self._lock.wait()
-                self._used += fp
-            admitted = time.perf_counter() - t0
-            th = threading.Thread(target=self._run_job, args=(spec, t0, admitted, results, fp), daemon=True)
-            th.start()
-            threads.append(th)
-        for th in threads:
-            th.join()
-        return results
+        srv, ids = self._native()
+        try:
+            descs = [(s, self._desc(srv, ids[s.scene], s)) for s in specs]
+            for s, d in descs:
+                fp, _ = srv.plan(d)
+                if fp > self.budget:
+                    raise ValueError(f"job {s.job_id} needs {fp} B, above the budget")
+            net = self._native_denoiser()
+            if net is not None:
+                srv.set_denoiser(net)
+            t0 = srv.clock()
+            by_id = {}
+            for s, d in descs:
+                wait = s.arrival - (srv.clock() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                by_id[srv.submit(d)] = s
+            srv.drain()
+            results = []
+            while (r := srv.next_result(0.0)) is not None:
+                s = by_id[r["job_id"]]
+                img = srv.read_image(r["job_id"], s.width, s.height) if r["has_image"] else None
+                results.append(JobResult(s.job_id, s, r["admitted"] - t0, r["finished"] - t0, img, r["error"]))
+            self.stats = srv.stats()
+            return results
+        finally:
+            srv.close()
 
 
 def tune_classes(scenes: dict, specs: list, max_frames: int = 16, tolerance: float = 0.05, log=None):

@@ -381,6 +381,8 @@ HPT_API hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, con
                                     const uint32_t* camera_indices, uint32_t count);
 HPT_API hpt_status hpt_group_set_launch_mode(hpt_group* g, uint32_t mode);
 HPT_API hpt_status hpt_group_set_shade_sort(hpt_group* g, uint32_t first_bounce);
+/* hpt_set_vmis_phases for a group's steps (phased, 1, by default). */
+HPT_API hpt_status hpt_group_set_vmis_phases(hpt_group* g, uint32_t on);
 /* Steps rendered, steps replayed as graphs, device bytes the group holds. */
 HPT_API hpt_status hpt_group_get_stats(hpt_group* g, uint64_t* steps, uint64_t* graph_launches,
                                        uint64_t* device_bytes);
@@ -620,6 +622,132 @@ HPT_API hpt_status hpt_get_memory_usage(hpt_renderer* r, uint64_t* frame_bytes, 
  * exceeds 2^31).  Takes effect at the next hpt_render. */
 HPT_API hpt_status hpt_set_frames_in_flight(hpt_renderer* r, uint32_t frames);
 HPT_API hpt_status hpt_get_frames_in_flight(hpt_renderer* r, uint32_t* frames);
+
+/* The serving loop (no counterpart in the reference).  A server takes
+ * progressive render jobs over scenes committed once (resident scenes), admits
+ * them in submission order while the sum of their device footprints fits a
+ * budget (a job that does not fit waits, and holds back the jobs behind it so
+ * large jobs are not starved), runs them concurrently and hands back each
+ * job's final image in host memory, denoised when the job asks for it.
+ *
+ *  - HPT_SERVE_STREAMS: an admitted job is a handle of its own (sharing its
+ *    scene's device copy, hpt_share_scene) on its own stream, rendered in
+ *    batches of its frames in flight from a host thread of the server; the
+ *    device interleaves the kernels of the jobs in flight.
+ *  - HPT_SERVE_WAVEFRONT: the jobs of one resident scene share one wavefront
+ *    per step (hpt_group_render): a host thread per scene renders the next
+ *    chunk of about step_paths paths of every active job of that scene
+ *    together; finished jobs leave after their step and jobs admitted
+ *    meanwhile join the next one.
+ *
+ * A job's image is a function of its scene, size, depths and frame range
+ * (the RNG is seeded by pixel and frame), so it is the same alone and among
+ * any other jobs, in either mode, bit for bit.  A job's footprint is
+ * pixels * (frames * 288 B of path state and queues [+ 5 B with shading sort]
+ * + 52 B of images); in wavefront mode its share of its scene's wavefront
+ * (chunk frames + 1) * 288 B per pixel plus its images.  Timestamps are
+ * seconds on the server's clock (0 at hpt_server_create).  A failing job ends
+ * with its status and message and leaves the others running.  The server
+ * borrows its scenes' template handles and its denoiser: they must outlive
+ * it.  Calls on one server may come from any host thread. */
+typedef struct hpt_server hpt_server;
+enum { HPT_SERVE_STREAMS = 0, HPT_SERVE_WAVEFRONT = 1 };
+#define HPT_SERVE_DEFAULT 0xFFFFFFFFu /* a job field that takes the server's setting */
+
+typedef struct hpt_server_config {
+  int32_t device;
+  uint32_t mode;               /* HPT_SERVE_STREAMS (default) or HPT_SERVE_WAVEFRONT */
+  uint64_t budget_bytes;       /* device bytes the admitted jobs' footprints may sum to */
+  uint32_t max_frames;         /* frames in flight of a job that does not set them (32) */
+  uint32_t keep_images;        /* 1: finished jobs keep their image for hpt_server_read_image */
+  uint64_t step_paths;         /* wavefront mode: paths per job and step (8 Mi) */
+  uint32_t launch_mode;        /* HPT_LAUNCH_* of jobs that do not set it (HPT_LAUNCH_GRAPH) */
+  uint32_t max_running;        /* streams mode: jobs admitted to run at once, 0 = as the budget allows;
+                                  8 by default (C4 serving on MI355X: 1133 Msamples/s and p99 6.25 s at 8,
+                                  about 1050-1090 and 6.4-6.7 s without a cap) */
+  /* Dry run: 0 (the default) renders on the device.  Non-zero runs the
+   * admission and scheduling without touching a device: a batch of n frames
+   * of a job takes n * pixels * dry_run_ns_per_path nanoseconds of host time
+   * and its image is filled with the job id -- for planning a budget on a
+   * host without a GPU, and for testing the scheduling. */
+  uint64_t dry_run_ns_per_path;
+} hpt_server_config;
+HPT_API void hpt_server_config_init(hpt_server_config* c);
+
+typedef struct hpt_serve_scene_desc {
+  const hpt_renderer* scene;   /* a committed handle whose scene the jobs render */
+  uint32_t features;           /* HPT_FEATURE_* of the jobs (the shader permutation) */
+  /* Settings of this scene's wavefront steps (HPT_SERVE_DEFAULT: the
+   * library's defaults).  Streams-mode jobs take their own (hpt_job_desc). */
+  uint32_t step_launch_mode, step_shade_sort, step_tail_from, step_traversal_block;
+  /* Dry runs only: 1 makes every job of the scene fail when it starts (its
+   * handle cannot be made), for testing failure isolation. */
+  uint32_t dry_run_fail;
+} hpt_serve_scene_desc;
+
+typedef struct hpt_job_desc {
+  uint32_t scene;              /* from hpt_server_add_scene */
+  uint32_t width, height;
+  uint32_t spp;                /* frames of one sample per pixel */
+  uint32_t max_depth, rr_depth;
+  uint32_t camera_index;
+  uint32_t enable_tonemap, enable_aces, use_simple_aces;
+  float exposure_value;
+  /* Per-job tuning (autotune.py measures these per scene and resolution);
+   * HPT_SERVE_DEFAULT takes the server's setting or the library default. */
+  uint32_t frames_in_flight;   /* streams mode: frames per batch (capped at spp) */
+  uint32_t launch_mode, shade_sort, tail_from, traversal_block;
+  uint32_t denoise;            /* 1: the result is the denoised colour (hpt_server_set_denoiser) */
+} hpt_job_desc;
+HPT_API void hpt_job_desc_init(hpt_job_desc* d);
+
+typedef struct hpt_job_result {
+  uint64_t job_id;
+  hpt_status status;           /* HPT_OK, or why the job failed */
+  uint32_t width, height;
+  uint32_t frames;             /* frames rendered */
+  uint32_t has_image;          /* the image is held for hpt_server_read_image */
+  uint32_t _pad;
+  uint64_t footprint_bytes;
+  double submitted_s, admitted_s, finished_s;  /* finished = image in host memory */
+  char error[256];
+} hpt_job_result;
+
+typedef struct hpt_server_stats {
+  uint64_t submitted, finished, failed;
+  uint64_t bytes_in_use, peak_bytes;
+  uint64_t wavefront_steps, max_jobs_per_step;
+  uint64_t jobs_in_steps;      /* sum over wavefront steps of the jobs each held */
+} hpt_server_stats;
+
+HPT_API hpt_status hpt_server_create(const hpt_server_config* config, hpt_server** out);
+/* Adds a resident scene; *scene_id is the index jobs name it by. */
+HPT_API hpt_status hpt_server_add_scene(hpt_server* s, const hpt_serve_scene_desc* scene, uint32_t* scene_id);
+/* The network that denoises jobs with `denoise` set (borrowed; NULL clears). */
+HPT_API hpt_status hpt_server_set_denoiser(hpt_server* s, hpt_denoiser* d);
+/* What admitting `job` would charge against the budget and its frames per
+ * batch (streams) or per step (wavefront). */
+HPT_API hpt_status hpt_server_plan_job(hpt_server* s, const hpt_job_desc* job, uint64_t* footprint_bytes,
+                                       uint32_t* chunk_frames);
+/* Queues a job and returns at once; HPT_ERROR_INVALID_ARGUMENT when its
+ * footprint exceeds the whole budget. */
+HPT_API hpt_status hpt_server_submit(hpt_server* s, const hpt_job_desc* job, uint64_t* job_id);
+/* Waits until `job_id` has finished and describes it.  Its result stays
+ * listed for hpt_server_next_result. */
+HPT_API hpt_status hpt_server_wait(hpt_server* s, uint64_t job_id, hpt_job_result* out);
+/* The next finished job in completion order, waiting up to timeout_s
+ * seconds (negative: no limit); *got = 0 when none finished in time. */
+HPT_API hpt_status hpt_server_next_result(hpt_server* s, double timeout_s, hpt_job_result* out, int* got);
+/* Copies a finished job's image (width*height float4: accumulated radiance,
+ * or the denoised colour with alpha 1) and releases it. */
+HPT_API hpt_status hpt_server_read_image(hpt_server* s, uint64_t job_id, float* dst, size_t bytes);
+/* Waits until every submitted job has finished. */
+HPT_API hpt_status hpt_server_drain(hpt_server* s);
+/* Seconds on the server's clock. */
+HPT_API hpt_status hpt_server_clock(hpt_server* s, double* seconds);
+HPT_API hpt_status hpt_server_get_stats(hpt_server* s, hpt_server_stats* out);
+/* Drains, then stops the server's threads and frees what it holds. */
+HPT_API void hpt_server_destroy(hpt_server* s);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -102,9 +102,10 @@ def test_c3_batched_uneven_jobs_equal_solo_renders(resident, wavefront):
     res = {r.job_id: r for r in srv.run(jobs)}
     assert all(r.error is None for r in res.values()), [r.error for r in res.values() if r.error]
     if wavefront:
-        print(f"[gpu] C3: {len(srv.step_sizes)} wavefront steps, up to {max(srv.step_sizes)} jobs per step",
+        st = srv.stats
+        print(f"[gpu] C3: {st['wavefront_steps']} wavefront steps, up to {st['max_jobs_per_step']} jobs per step",
               flush=True)
-        assert max(srv.step_sizes) > 1  # jobs of one scene did share steps
+        assert st["max_jobs_per_step"] > 1  # jobs of one scene did share steps
     for j in jobs:
         r = _solo(resident[j.scene], j)
         assert np.array_equal(r.read_image(hpt.IMAGE_ACCUM), res[j.job_id].image), j

@@ -52,3 +52,21 @@ def test_c_abi_denoiser_rejects_wrong_sizes(nets):
     assert lib.hpt_denoise(c_net._h, r._h, out.ctypes.data, out.nbytes, 0) == 1  # HPT_ERROR_INVALID_ARGUMENT
     assert b"size" in lib.hpt_last_error()
     r.close()
+
+
+def test_c_abi_denoiser_two_widths_with_one_padded_size(nets):
+    """Widths 97 and 100 pad to the same network size (112), so the second
+    call reuses the first call's buffers: its host copy must still hold the
+    larger image (the buffer is sized by the padded size)."""
+    c_net, py_net = nets
+    for w in (97, 100):
+        print(f"[gpu] C-ABI denoiser {w}x60 after a smaller image", flush=True)
+        r = hpt.HalaRenderer(w, 60, max_depth=6, rr_depth=3)
+        r.set_scene(hpt_scenes.cornell_box())
+        r.commit()
+        r.render(4)
+        got = c_net.denoise(r)
+        want = py_net.denoise_renderer(r).numpy()
+        denoiser.synchronize()
+        assert np.array_equal(got, want)
+        r.close()

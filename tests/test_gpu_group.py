@@ -103,3 +103,26 @@ def test_group_refuses_mixed_scenes_and_duplicates():
     g.close()
     for r in (ja, jb, a, b):
         r.close()
+
+
+def test_handles_of_another_kind_are_refused_on_the_device():
+    """A renderer handle given to a group's entry point, or a group handle to
+    a renderer's, fails with HPT_ERROR_INVALID_ARGUMENT and leaves both
+    handles working (each handle struct starts with a tag of its kind)."""
+    lib = hpt.load_library()
+    scene, _ = _scene("disney")
+    r = hpt.HalaRenderer(32, 24, max_depth=4, rr_depth=2)
+    r.set_scene(scene)
+    r.commit()
+    g = hpt.HalaGroup()
+    print("[gpu] wrong-kind handles", flush=True)
+    assert lib.hpt_group_set_vmis_phases(r._h, 0) == 1 and b"invalid group" in lib.hpt_last_error()
+    assert lib.hpt_set_vmis_phases(g._h, 0) == 1 and b"invalid renderer" in lib.hpt_last_error()
+    assert lib.hpt_set_frames_in_flight(g._h, 4) == 1
+    g.set_vmis_phases(True)
+    r.set_vmis_phases(True)
+    r.render(2)
+    solo = r.read_image(hpt.IMAGE_ACCUM)
+    assert np.isfinite(solo).all() and r.frame_index == 2
+    g.close()
+    r.close()

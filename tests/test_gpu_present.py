@@ -51,7 +51,17 @@ def test_newest_frame_wins_and_full_rings_drop():
         p.present()
         r.wait_idle()
         finals[r.frame_index] = r.read_image(hpt.IMAGE_FINAL_RGBA8)
-    f, img = p.acquire()
+    # r.wait_idle() waits for the renderer's stream only: the last host copy
+    # may still run on the presenter's stream, and acquire does not wait
+    f = img = None
+    for _ in range(1000):
+        got = p.acquire()
+        if got is None:  # both slots' copies still in flight
+            continue
+        assert got[0] in (7, 8)  # with two slots, older copies were overwritten
+        f, img = got
+        if f == 8:
+            break
     assert f == 8 and np.array_equal(img, finals[8])
     # with one slot held by the caller, a present finds at most one slot
     # free: presents beyond it are dropped, never waited for

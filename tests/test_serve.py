@@ -1,8 +1,7 @@
-"""The serving loop: admission by device-memory footprint, continuous
-batching of jobs over shared resident scenes, and results that do not
-depend on the mix.  Host logic runs on the CPU against fake handles; the
-GPU tests run real jobs through the C ABI."""
-import threading
+"""The serving loop (hpt_server_*): admission by device-memory footprint,
+continuous batching of jobs over shared resident scenes, and results that do
+not depend on the mix.  The scheduling runs on the CPU as the native loop's
+dry run (no device); the GPU tests run real jobs through the C ABI."""
 import time
 
 import numpy as np
@@ -13,72 +12,18 @@ import hpt_scenes
 import serve
 
 
-class _FakeRenderer:
-    """Stands in for a job handle: renders by sleeping."""
-
-    live = 0
-    peak = 0
-    lock = threading.Lock()
-
-    def __init__(self, spec):
-        self.spec = spec
-        with _FakeRenderer.lock:
-            _FakeRenderer.live += 1
-            _FakeRenderer.peak = max(_FakeRenderer.peak, _FakeRenderer.live)
-
-    def render(self, n):
-        time.sleep(0.002 * n)
-
-    def wait_idle(self):
-        pass
-
-    def read_image(self, which):
-        return np.full((self.spec.height, self.spec.width, 4), self.spec.job_id, np.float32)
-
-    def close(self):
-        with _FakeRenderer.lock:
-            _FakeRenderer.live -= 1
+NS_PER_PATH = 100  # dry-run cost: a 128x96 frame takes 1.2 ms of host time
 
 
-class _FakeGroup:
-    """Stands in for a scene's wavefront: renders its jobs' chunks in turn and
-    checks that every step holds jobs of one scene."""
-
-    def __init__(self, scene):
-        self.scene = scene
-
-    def render(self, jobs, frames):
-        assert all(j.spec.scene == self.scene for j in jobs)
-        for j, n in zip(jobs, frames):
-            assert 1 <= n <= j.spec.spp
-            j.render(n)
-
-    def wait_idle(self):
-        pass
-
-    def close(self):
-        pass
-
-
-class _FakeServer(serve.Server):
-    def _make_job(self, spec):
-        if spec.scene == "broken":
-            raise RuntimeError("device lost")
-        return _FakeRenderer(spec)
-
-    def _new_group(self, scene):
-        return _FakeGroup(scene)
-
-
-def _fake_scenes():
-    return {"a": None, "b": None, "broken": None}
+def _dry_scenes():
+    return {"a": serve.DryRunScene(), "b": serve.DryRunScene(), "broken": serve.DryRunScene(fail=True)}
 
 
 def test_admission_keeps_the_footprint_within_budget():
     jobs = serve.poisson_jobs(40, rate=400.0, scene_names=["a", "b"], seed=1,
                               resolutions=[(64, 64), (128, 96), (256, 128)], spp_range=(4, 40))
     budget = 3 * serve.footprint(serve.JobSpec(0, "a", 256, 128, 40, 8))
-    srv = _FakeServer(_fake_scenes(), budget, keep_images=True)
+    srv = serve.Server(_dry_scenes(), budget, keep_images=True, dry_run_ns_per_path=NS_PER_PATH)
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
@@ -90,15 +35,17 @@ def test_admission_keeps_the_footprint_within_budget():
     # admitted in arrival order (first come, first served)
     order = sorted(res, key=lambda q: q.admitted)
     assert [q.spec.job_id for q in order] == sorted(q.spec.job_id for q in res)
-    for r in res:
-        assert r.image[0, 0, 0] == r.spec.job_id and r.latency >= 0
+    for r in res:  # a dry run's image holds the server's job id (1, 2, ... in submission order)
+        assert r.image[0, 0, 0] == r.spec.job_id + 1 and r.latency >= 0
+        assert r.image.shape == (r.spec.height, r.spec.width, 4)
+    assert srv.stats["peak_bytes"] <= budget and srv.stats["finished"] == 40
     s = serve.summarize(res, makespan)
     assert s["jobs"] == 40 and s["failed"] == 0 and s["p99_latency_s"] >= s["p50_latency_s"]
 
 
 def test_a_failing_job_is_isolated():
     jobs = [serve.JobSpec(i, "broken" if i == 2 else "a", 32, 32, 8, 8) for i in range(5)]
-    res = _FakeServer(_fake_scenes(), 1 << 40).run(jobs)
+    res = serve.Server(_dry_scenes(), 1 << 40, dry_run_ns_per_path=NS_PER_PATH).run(jobs)
     bad = [r for r in res if r.error]
     assert len(res) == 5 and len(bad) == 1 and bad[0].job_id == 2 and "device lost" in bad[0].error
 
@@ -109,41 +56,100 @@ def test_wavefront_mode_steps_jobs_of_a_scene_together():
     every job gets exactly its samples."""
     jobs = serve.poisson_jobs(40, rate=400.0, scene_names=["a", "b"], seed=2,
                               resolutions=[(64, 64), (128, 96)], spp_range=(4, 64))
-    srv = _FakeServer(_fake_scenes(), 0, max_frames=8, keep_images=True, wavefront=True, step_paths=64 * 64 * 8)
+    srv = serve.Server(_dry_scenes(), 0, max_frames=8, keep_images=True, wavefront=True, step_paths=64 * 64 * 8,
+                       dry_run_ns_per_path=NS_PER_PATH)
     budget = srv.budget = 6 * srv.job_footprint(serve.JobSpec(0, "a", 128, 96, 64, 8))
+    fp = {j.job_id: srv.job_footprint(j) for j in jobs}
     t0 = time.perf_counter()
     res = srv.run(jobs)
     assert len(res) == 40 and all(r.error is None for r in res)
     for r in res:
         live = [q for q in res if q.admitted <= r.admitted < q.finished]
-        assert sum(srv.job_footprint(q.spec) for q in live) <= budget
-        assert r.image[0, 0, 0] == r.spec.job_id
-    assert max(srv.step_sizes) > 1  # steps shared by several jobs
+        assert sum(fp[q.job_id] for q in live) <= budget
+        assert r.image[0, 0, 0] == r.spec.job_id + 1
+    assert srv.stats["max_jobs_per_step"] > 1  # steps shared by several jobs
     # chunks of about step_paths paths: 8 frames at 64x64, 3 at 128x96
     assert srv.step_frames(serve.JobSpec(0, "a", 64, 64, 64, 8)) == 8
     assert srv.step_frames(serve.JobSpec(0, "a", 128, 96, 64, 8)) == 3
     assert srv.step_frames(serve.JobSpec(0, "a", 128, 96, 2, 8)) == 2
-    assert len(srv.step_sizes) >= max(-(-j.spp // srv.step_frames(j)) for j in jobs)
+    assert srv.stats["wavefront_steps"] >= max(-(-j.spp // srv.step_frames(j)) for j in jobs)
+    # every job's chunks sum to its samples: each step holds a job at most once
+    assert srv.stats["jobs_in_steps"] == sum(-(-j.spp // srv.step_frames(j)) for j in jobs)
     assert serve.summarize(res, time.perf_counter() - t0)["failed"] == 0
-    bad = _FakeServer(_fake_scenes(), 1 << 40, wavefront=True).run(
+    bad = serve.Server(_dry_scenes(), 1 << 40, wavefront=True, dry_run_ns_per_path=NS_PER_PATH).run(
         [serve.JobSpec(i, "broken" if i == 1 else "a", 32, 32, 8, 8) for i in range(3)])
     assert sorted(r.job_id for r in bad if r.error) == [1]
 
 
 def test_job_larger_than_the_budget_is_refused():
     with pytest.raises(ValueError, match="above the budget"):
-        _FakeServer(_fake_scenes(), 1000).run([serve.JobSpec(0, "a", 64, 64, 8, 8)])
+        serve.Server(_dry_scenes(), 1000, dry_run_ns_per_path=NS_PER_PATH).run([serve.JobSpec(0, "a", 64, 64, 8, 8)])
+    srv = hpt.RenderServer(1000, dry_run_ns_per_path=NS_PER_PATH)
+    sid = srv.add_scene(None, 0)
+    with pytest.raises(hpt.HptError, match="above the budget"):
+        srv.submit(srv.job_desc(sid, 64, 64, 8, 8))
+    srv.close()
 
 
-def test_poisson_job_distribution():
-    jobs = serve.poisson_jobs(4000, rate=20.0, scene_names=list("abcdefgh"), seed=3)
-    gaps = np.diff([0.0] + [j.arrival for j in jobs])
-    assert abs(gaps.mean() - 1 / 20) < 0.005 and abs(gaps.std() - 1 / 20) < 0.006  # exponential
-    spp = np.log([j.spp for j in jobs])
-    assert spp.min() >= np.log(16) - 0.05 and spp.max() <= np.log(512) + 0.05
-    assert abs(np.median(spp) - 0.5 * (np.log(16) + np.log(512))) < 0.1  # log-uniform
-    assert {j.max_depth for j in jobs} == set(range(8, 13))
-    assert {(j.width, j.height) for j in jobs} == {(256, 256), (512, 512), (1024, 768), (1920, 1080)}
+def test_native_footprint_equals_the_model():
+    """The native planner charges what serve.footprint models: per path in
+    flight 288 B (+5 B with shading sort), per pixel 52 B; the batch is the
+    tuned or default frames in flight, never more than the job's samples."""
+    import autotune
+
+    tuning = autotune.TuningCache()
+    tuning.put("a", 64, 64, 0, autotune.Tuning(8, hpt.LAUNCH_GRAPH, 1.0, {}, 0.1, 1))
+    for max_frames in (4, 32):
+        for t in (None, tuning):
+            srv = serve.Server(_dry_scenes(), 1 << 40, max_frames=max_frames, tuning=t, dry_run_ns_per_path=1)
+            for spec in (serve.JobSpec(0, "a", 64, 64, 100, 8), serve.JobSpec(1, "a", 64, 64, 3, 8),
+                         serve.JobSpec(2, "b", 1920, 1080, 128, 12), serve.JobSpec(3, "b", 7, 5, 1, 1)):
+                assert srv.job_footprint(spec) == serve.footprint(spec, max_frames, t)
+                assert srv.step_frames(spec) == serve.frames_in_flight(spec, max_frames, t)
+
+
+def test_native_server_results_and_validation():
+    """Results by id and in completion order; an image is read once; bad
+    job descriptions are refused with the reason."""
+    srv = hpt.RenderServer(1 << 40, keep_images=True, dry_run_ns_per_path=NS_PER_PATH)
+    a = srv.add_scene(None, 0)
+    ids = [srv.submit(srv.job_desc(a, 16 + i, 8, 2, 4)) for i in range(4)]
+    r = srv.wait(ids[2])
+    assert r["status"] == 0 and r["frames"] == 2 and r["width"] == 18 and r["has_image"]
+    assert r["submitted"] <= r["admitted"] <= r["finished"]
+    srv.drain()
+    got = []
+    while (x := srv.next_result(0.0)) is not None:
+        got.append(x["job_id"])
+    assert sorted(got) == ids and srv.next_result(0.01) is None
+    img = srv.read_image(ids[0], 16, 8)
+    assert img.shape == (8, 16, 4) and (img == ids[0]).all()
+    with pytest.raises(hpt.HptError, match="unknown or released"):
+        srv.read_image(ids[0], 16, 8)
+    with pytest.raises(hpt.HptError, match="16"):
+        srv.read_image(ids[1], 8, 8)  # wrong size
+    for bad, msg in ((dict(scene=7), "scene"), (dict(spp=0), "spp"), (dict(traversal_block=96), "traversal_block"),
+                     (dict(frames_in_flight=65), "frames_in_flight"), (dict(width=0), "size")):
+        d = srv.job_desc(a, 16, 8, 2, 4)
+        for k, v in bad.items():
+            setattr(d, k, v)
+        with pytest.raises(hpt.HptError, match=msg):
+            srv.submit(d)
+    dev = hpt.RenderServer(1 << 40)  # a server on the device takes scenes only as committed handles
+    with pytest.raises(hpt.HptError, match="null scene handle"):
+        dev.add_scene(None, 0)
+    dev.close()
+    st = srv.stats()
+    assert st["submitted"] == 4 and st["finished"] == 4 and st["failed"] == 0 and st["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_server_close_waits_for_running_jobs():
+    srv = hpt.RenderServer(1 << 40, dry_run_ns_per_path=NS_PER_PATH)
+    a = srv.add_scene(None, 0)
+    for i in range(6):
+        srv.submit(srv.job_desc(a, 64, 64, 4, 4))
+    srv.close()  # returns after the six jobs (about 10 ms of dry-run work)
 
 
 @pytest.mark.gpu
@@ -257,3 +263,33 @@ def test_server_denoises_finished_jobs():
         r.close()
         assert res.image.shape == (48, 64, 4) and np.array_equal(res.image, want)
     rs.close()
+
+
+def test_handles_of_another_kind_are_refused():
+    """Every handle starts with a tag of its kind: an entry point given a
+    handle of another kind fails with HPT_ERROR_INVALID_ARGUMENT instead of
+    writing into a struct it does not own (here a server handle passed to a
+    renderer's, a group's and a denoiser's setters)."""
+    lib = hpt.load_library()
+    srv = hpt.RenderServer(1 << 30, dry_run_ns_per_path=1)
+    assert lib.hpt_set_vmis_phases(srv._h, 1) == 1 and b"invalid renderer" in lib.hpt_last_error()
+    assert lib.hpt_group_set_vmis_phases(srv._h, 1) == 1 and b"invalid group" in lib.hpt_last_error()
+    assert lib.hpt_set_frames_in_flight(srv._h, 4) == 1
+    out = np.zeros(4, np.float32)
+    assert lib.hpt_denoise(srv._h, srv._h, out.ctypes.data, 16, 0) == 1
+    lib.hpt_destroy(srv._h)  # ignored: not a renderer
+    assert srv.stats()["submitted"] == 0  # the server is intact
+    srv.close()
+
+
+def test_running_jobs_capped_in_stream_mode():
+    """max_running bounds the jobs rendering at once (streams mode) on top of
+    the memory budget; admission stays first come, first served."""
+    jobs = serve.poisson_jobs(30, rate=1000.0, scene_names=["a"], seed=4, resolutions=[(64, 64)], spp_range=(4, 16))
+    srv = serve.Server(_dry_scenes(), 1 << 40, dry_run_ns_per_path=NS_PER_PATH, max_running=3)
+    res = srv.run(jobs)
+    assert len(res) == 30 and all(r.error is None for r in res)
+    for r in res:
+        assert sum(1 for q in res if q.admitted <= r.admitted < q.finished) <= 3
+    order = sorted(res, key=lambda q: q.admitted)
+    assert [q.spec.job_id for q in order] == sorted(q.spec.job_id for q in res)
