@@ -218,23 +218,23 @@ def render_reference(make_renderer, ref_spp):
     return ref, t_ref
 
 
-def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
-    """The denoised estimate at 2, 4, 8, ... spp (progressive accumulation
-    from frame 0, 2-frame batches): time = render time up to that sample
-    count + one denoiser pass (on the renderer's resident images, same GPU);
-    the first sample count whose denoised image meets `threshold` (relMSE
-    evaluation and the passes of earlier sample counts excluded)."""
+def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp, batch=2):
+    """The denoised estimate at k, 2k, 4k, ... spp for batches of k frames
+    (progressive accumulation from frame 0): time = render time up to that
+    sample count + one denoiser pass (on the renderer's resident images, same
+    GPU); the first sample count whose denoised image meets `threshold`
+    (relMSE evaluation and the passes of earlier sample counts excluded)."""
     import denoiser
 
-    r = make_renderer(2)
-    r.render(2)  # warm-up: graph capture
+    r = make_renderer(batch)
+    r.render(batch)  # warm-up: graph capture
     curve, t_render, spp, hit = [], 0.0, 0, None
     out = denoiser.DeviceArray(r.height * r.width * 4, np.float32, (r.height, r.width, 4))
     net.denoise_renderer(r, out)  # allocate the network's buffers outside the timed region
     denoiser.synchronize()
     r.reset_accumulation()
     while spp < max_spp:
-        step = 2 if spp == 0 else spp
+        step = batch if spp == 0 else spp
         r.wait_idle()
         t0 = time.perf_counter()
         r.render(step)
@@ -252,8 +252,9 @@ def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp):
             hit = (spp, t_render + t_den, t_den)
             break
     r.close()
-    log("denoised relMSE: " + ", ".join(f"{c[0]} spp {c[4]:.2e} (plain {c[3]:.2e})" for c in curve))
-    return {"threshold": threshold, "time_to_threshold_s": round(hit[1], 4) if hit else None,
+    log(f"denoised relMSE ({batch}-frame batches): " +
+        ", ".join(f"{c[0]} spp {c[4]:.2e} (plain {c[3]:.2e})" for c in curve))
+    return {"threshold": threshold, "batch_frames": batch, "time_to_threshold_s": round(hit[1], 4) if hit else None,
             "spp_at_threshold": hit[0] if hit else None, "denoise_ms": round(hit[2] * 1e3, 3) if hit else None,
             "curve_spp_render_s_denoise_ms_plain_denoised": curve}
 
@@ -783,24 +784,39 @@ def main():
             import denoiser
 
             path = args.weights or denoiser.weights_path()
-            if args.tail_from == "auto":  # the tail bounce for 2-frame batches
-                import autotune
+            # The denoised estimate in batches of 2 and of 4 frames (checked after
+            # every doubling of the sample count), each batch size with its own
+            # tuned tail bounce and traversal block; the faster is reported.
+            net = denoiser.UNet.load(path)
+            by_batch, tuned_small = {}, {}
+            for k in (2, 4):
+                if args.tail_from == "auto":
+                    import autotune
 
-                tmpl = make_renderer(frames_in_flight=2)
-                t2 = autotune.tune(tmpl, W, H, DEPTH, RR, FEATURES, samples=8 * W * H, candidates=(2,),
-                                   modes=(hpt.LAUNCH_GRAPH,), sorts=(), in_place=True)
-                tmpl.close()
-                tails["denoised"] = t2.tail_from
-                if args.traversal_block == "auto":
-                    blocks["denoised"] = t2.trav_block
-                log(f"tail for 2-frame batches: {'none' if t2.tail_from == hpt.TAIL_FROM_NONE else t2.tail_from} "
-                    f"({ {k: round(v, 1) for k, v in t2.table.items()} })")
-            den = time_to_relmse_denoised(make_renderer, ref, denoiser.UNet.load(path), args.threshold,
-                                          args.ref_spp // 2)
+                    tmpl = make_renderer(frames_in_flight=k)
+                    tk = autotune.tune(tmpl, W, H, DEPTH, RR, FEATURES, samples=4 * k * W * H, candidates=(k,),
+                                       modes=(hpt.LAUNCH_GRAPH,), sorts=(), in_place=True)
+                    tmpl.close()
+                    tails["denoised"] = tk.tail_from
+                    if args.traversal_block == "auto":
+                        blocks["denoised"] = tk.trav_block
+                    log(f"tail for {k}-frame batches: "
+                        f"{'none' if tk.tail_from == hpt.TAIL_FROM_NONE else tk.tail_from} "
+                        f"({ {key: round(v, 1) for key, v in tk.table.items()} })")
+                tuned_small[k] = (tails["denoised"], blocks["denoised"])
+                by_batch[k] = time_to_relmse_denoised(make_renderer, ref, net, args.threshold, args.ref_spp // 2, k)
+            hits = [k for k in by_batch if by_batch[k]["time_to_threshold_s"]]
+            best = min(hits, key=lambda k: by_batch[k]["time_to_threshold_s"]) if hits else 2
+            tails["denoised"], blocks["denoised"] = tuned_small[best]
+            den = dict(by_batch[best])
+            den["by_batch_frames"] = {k: {"time_to_threshold_s": v["time_to_threshold_s"],
+                                          "spp_at_threshold": v["spp_at_threshold"],
+                                          "tail_from": tuned_small[k][0] if tuned_small[k][0] != hpt.TAIL_FROM_NONE
+                                          else None} for k, v in by_batch.items()}
             den["weights"] = os.path.relpath(path, ROOT)
             if den["time_to_threshold_s"] and conv["time_to_threshold_s"]:
                 den["speedup_vs_plain"] = round(conv["time_to_threshold_s"] / den["time_to_threshold_s"], 2)
-                conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, denoiser.UNet.load(path),
+                conv["served"] = served_relmse(scene, FEATURES, W, H, DEPTH, RR, ref, net,
                                                conv["spp_at_threshold"], den["spp_at_threshold"], shade_sort, tails, blocks)
             conv["denoised"] = den
     serving = None

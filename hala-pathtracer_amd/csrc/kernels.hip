@@ -897,6 +897,16 @@ __device__ __forceinline__ void walk_request(bool want, const ShadowQueue& wq, f
   }
 }
 
+// Direction class of a walk: its octant and the axis of its largest
+// component (24 classes; kWalkKeys - 1 marks an empty slot).
+constexpr uint32_t kWalkKeys = 32;
+__device__ __forceinline__ uint32_t walk_key(float4 d) {
+  const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  const uint32_t major = ax >= ay ? (ax >= az ? 0u : 2u) : (ay >= az ? 1u : 2u);
+  return oct * 3u + major;
+}
+
 // The light pick and sample of inc/lighting.glsl:154-175 at hit_pos into
 // nee_dir / nee_li; returns whether the sample needs its walk.
 __device__ __forceinline__ bool light_sample_request(const SceneView& sv, const FrameParams& fp, uint4& rng,
@@ -1139,10 +1149,39 @@ template <bool TRANSP, bool JOBS, bool COUNT, uint32_t BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
     k_walk(SceneView sv, FrameParams fp, PathState ps, ShadowQueue wq, Spill sp, unsigned long long* stats) {
   __shared__ uint2 lds_stack[kShortStack * BS];
+  __shared__ uint32_t s_bin[kWalkKeys];
+  __shared__ uint16_t s_order[BS];
   const uint32_t gid = blockIdx.x * BS + threadIdx.x;
-  const SegSlot t = seg_slot<BS>(wq.counts);
+  SegSlot t = seg_slot<BS>(wq.counts);
   if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + t.seg * kStatStride, (unsigned long long)t.n);
   WalkCounts wc;
+  // The block's walks grouped by direction (octant and major axis) before
+  // the traversal: a block holds the walks of neighbouring paths, and the
+  // light each picked is random, so unsorted a wave mixes walks toward every
+  // light.  Grouped, a wave's lanes mostly share their origin region and
+  // target, and with them their node fetches.  Each walk writes only its own
+  // path's transmittance and RNG state, so the order changes no result.
+  const uint32_t jb = t.j - threadIdx.x;  // the block's first entry
+  if (jb < t.n) {
+    const uint32_t key = t.j < t.n ? walk_key(wq.d[t.seg * wq.seg_cap + t.j]) : kWalkKeys - 1;
+    if (threadIdx.x < kWalkKeys) s_bin[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t rank = atomicAdd(&s_bin[key], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the bins by the first wave
+      const uint32_t c = threadIdx.x < kWalkKeys ? s_bin[threadIdx.x] : 0u;
+      uint32_t incl = c;
+      for (uint32_t off = 1; off < kWalkKeys; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane_id() >= off) incl += v;
+      }
+      if (threadIdx.x < kWalkKeys) s_bin[threadIdx.x] = incl - c;
+    }
+    __syncthreads();
+    s_order[s_bin[key] + rank] = (uint16_t)threadIdx.x;
+    __syncthreads();
+    t.j = jb + s_order[threadIdx.x];
+  }
   if (t.j < t.n) {
     const uint32_t k = t.seg * wq.seg_cap + t.j;
     const float4 o4 = wq.o[k], d4 = wq.d[k];

@@ -58,8 +58,8 @@ def render_reference(r: "hpt.HalaRenderer", spp: int) -> np.ndarray:
 
 
 def render_sample(seed: int, width: int, height: int, spp_levels=SPP_LEVELS, ref_spp: int = 2048,
-                  max_depth: int = 8, device: int = 0) -> Sample:
-    scene, features = hpt_scenes.random_scene(seed, width, height)
+                  max_depth: int = 8, device: int = 0, layouts=hpt_scenes.RANDOM_LAYOUTS) -> Sample:
+    scene, features = hpt_scenes.random_scene(seed, width, height, layouts)
     r = hpt.HalaRenderer(width, height, max_depth=max_depth, rr_depth=3, features=features, device=device)
     try:
         r.set_scene(scene)
@@ -76,10 +76,10 @@ def render_sample(seed: int, width: int, height: int, spp_levels=SPP_LEVELS, ref
 
 
 def render_dataset(seeds, width: int, height: int, spp_levels=SPP_LEVELS, ref_spp: int = 2048, log=print,
-                   device: int = 0) -> list:
+                   device: int = 0, layouts=hpt_scenes.RANDOM_LAYOUTS) -> list:
     out, t0 = [], time.perf_counter()
     for i, s in enumerate(seeds):
-        out.append(render_sample(int(s), width, height, spp_levels, ref_spp, device=device))
+        out.append(render_sample(int(s), width, height, spp_levels, ref_spp, device=device, layouts=layouts))
         if log and (i + 1) % 16 == 0:
             log(f"rendered {i + 1}/{len(seeds)} scenes in {time.perf_counter() - t0:.1f} s")
     return out
@@ -149,14 +149,18 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default="", help="continue from these weights")
+    ap.add_argument("--layouts", default="0,1,2",
+                    help="random_scene layouts to train on (0 room, 1 outdoor, 2 displaced icosphere); "
+                         "0,1 keeps the large-mesh benchmark scene's family out of training")
     ap.add_argument("--out", default=denoiser.weights_path())
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
     # Scene seeds start at 1000: seed 0 is kept out of training.
     train_seeds = range(1000 + a.seed * 100000, 1000 + a.seed * 100000 + a.scenes)
     hold_seeds = range(900, 900 + a.holdout)
-    data = render_dataset(train_seeds, a.size, a.size, ref_spp=a.ref_spp, log=log)
-    hold = render_dataset(hold_seeds, a.size, a.size, ref_spp=a.ref_spp, log=None)
+    layouts = tuple(int(k) for k in a.layouts.split(","))
+    data = render_dataset(train_seeds, a.size, a.size, ref_spp=a.ref_spp, log=log, layouts=layouts)
+    hold = render_dataset(hold_seeds, a.size, a.size, ref_spp=a.ref_spp, log=None, layouts=layouts)
     net = denoiser.UNet.load(a.init) if a.init else denoiser.UNet(seed=a.seed)
     t0 = time.perf_counter()
     train(net, data, a.steps, a.batch, a.crop, a.lr, a.seed, log, 500)

@@ -642,16 +642,27 @@ def _random_lights(rng, center, extent, count):
     return lights
 
 
-def random_scene(seed, width=256, height=256):
+RANDOM_LAYOUTS = (0, 1, 2)  # room, outdoor, displaced icosphere with mixed lights
+
+
+def random_scene(seed, width=256, height=256, layouts=RANDOM_LAYOUTS):
     """A seeded random scene for training the denoiser: (scene, features).
     One of three layouts -- a walled room with objects and ceiling lights, an
     outdoor ground plane under a sky map with a sun, or a displaced
     icosphere with mixed lights -- filled with random spheres, boxes and
     materials (Lambert, Oren-Nayar, Disney with metal, clearcoat, sheen and
     glass lobes, textured or bump-mapped), sometimes a participating medium
-    and a thin-lens camera."""
+    and a thin-lens camera.  `layouts` restricts the draw (their weights
+    renormalised): without layout 2 no displaced icosphere appears, which
+    keeps the large-mesh benchmark scene's family out of a training set."""
     rng = np.random.default_rng(seed)
-    layout = int(rng.choice(3, p=[0.35, 0.35, 0.3]))
+    weights = [0.35, 0.35, 0.3]
+    if set(int(k) for k in layouts) != {0, 1, 2}:
+        keep = [w if k in layouts else 0.0 for k, w in enumerate(weights)]
+        if not any(keep):
+            raise ValueError("random_scene: no layout to draw from")
+        weights = [w / sum(keep) for w in keep]
+    layout = int(rng.choice(3, p=weights))
     textures = [checker_texture(int(rng.integers(4, 16)), 64, tuple(int(c) for c in rng.integers(120, 255, 3)),
                                 tuple(int(c) for c in rng.integers(0, 120, 3))),
                 bump_normal_map(64, int(rng.integers(2, 8)), float(rng.uniform(0.2, 0.8))),
