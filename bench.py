@@ -784,12 +784,12 @@ def main():
             import denoiser
 
             path = args.weights or denoiser.weights_path()
-            # The denoised estimate in batches of 2 and of 4 frames (checked after
+            # The denoised estimate in batches of 2, 4 and 8 frames (checked after
             # every doubling of the sample count), each batch size with its own
-            # tuned tail bounce and traversal block; the faster is reported.
+            # tuned tail bounce and traversal block; the fastest is reported.
             net = denoiser.UNet.load(path)
             by_batch, tuned_small = {}, {}
-            for k in (2, 4):
+            for k in (2, 4, 8):
                 if args.tail_from == "auto":
                     import autotune
 

@@ -419,8 +419,10 @@ def test_training_lowers_the_loss(dn):
 
 def test_trained_network_beats_the_plain_estimate(dn):
     """The committed network (trained on the GPU by the kernels above, on
-    random scenes with seeds 1000+) against the plain 16-spp estimate on
-    held-out scenes: relMSE to a 2048-spp reference from disjoint frames."""
+    3000 random room and outdoor scenes: no displaced icosphere, the layout of
+    the large-mesh benchmark scene) against the plain 16-spp estimate on
+    held-out scenes of all three layouts: relMSE to a 2048-spp reference from
+    disjoint frames."""
     import denoise_train
 
     path = dn.weights_path()
