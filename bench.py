@@ -260,12 +260,15 @@ def time_to_relmse_denoised(make_renderer, ref, net, threshold, max_spp, batch=2
 
 
 def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_denoised, shade_sort, tails, blocks):
-    """The same target through the serving path: one job per variant
-    submitted to serve.Server -- plain at the plain tracer's threshold sample
-    count, denoised (the server denoises finished jobs on the device) at the
+    """The same target through the serving path: jobs submitted to
+    serve.Server -- plain at the plain tracer's threshold sample count,
+    denoised (the server denoises finished jobs on the device) at the
     denoised threshold count -- timed from submission to the image in host
     memory (admission, tracing, denoising, read-back), then scored against
-    the reference."""
+    the reference.  Each variant runs twice on one server: the first job of
+    its class makes its handle and captures its batch graph (cold_latency_s);
+    the second renders on the handle the first left (latency_s, the steady
+    state of a server that has served that class before)."""
     import autotune
     import serve
 
@@ -277,11 +280,17 @@ def served_relmse(scene, features, w, h, depth, rr, ref, net, spp_plain, spp_den
             tuning.put("scene", w, h, 0, autotune.Tuning(32, hpt.LAUNCH_GRAPH, 0.0, {}, 0.0, shade_sort, tails[label],
                                                          blocks[label]))
             srv = serve.Server({"scene": rs}, 1 << 38, max_frames=32, keep_images=True, tuning=tuning, denoiser=den)
-            srv.run([serve.JobSpec(0, "scene", w, h, min(spp, 8), depth, rr, 0.0)])  # warm-up: allocations, capture
-            res = srv.run([serve.JobSpec(0, "scene", w, h, spp, depth, rr, 0.0)])[0]
-            if res.error:
-                raise RuntimeError(res.error)
-            out[label] = {"spp": spp, "latency_s": round(res.latency, 4), "relmse": relmse(res.image, ref)}
+            try:
+                cold = srv.run([serve.JobSpec(0, "scene", w, h, spp, depth, rr, 0.0)])[0]
+                res = srv.run([serve.JobSpec(1, "scene", w, h, spp, depth, rr, 0.0)])[0]
+                reused = srv.stats.get("handles_reused", 0)
+            finally:
+                srv.close()
+            if cold.error or res.error:
+                raise RuntimeError(cold.error or res.error)
+            assert np.array_equal(cold.image, res.image)  # same job: same image on a kept handle
+            out[label] = {"spp": spp, "latency_s": round(res.latency, 4), "cold_latency_s": round(cold.latency, 4),
+                          "handle_reused": bool(reused), "relmse": relmse(res.image, ref)}
     finally:
         rs.close()
     out["speedup"] = round(out["plain"]["latency_s"] / out["denoised"]["latency_s"], 2)
@@ -425,6 +434,7 @@ def run_serving(args, jobs_seed=0):
     t0 = time.perf_counter()
     res = srv.run(jobs)
     makespan = time.perf_counter() - t0
+    srv.close()
     summ = serve.summarize(res, makespan)
     summ["offered_gsamples"] = round(offered / 1e9, 3)
     summ["arrival_span_s"] = round(jobs[-1].arrival, 3)
@@ -432,6 +442,7 @@ def run_serving(args, jobs_seed=0):
     summ["mode"] = "wavefront per scene" if args.wavefront else "stream per job"
     summ["loop"] = "native (hpt_server)"
     summ["max_running"] = args.max_running
+    summ["handles_reused"] = srv.stats.get("handles_reused", 0)
     if args.wavefront and srv.stats.get("wavefront_steps"):
         st = srv.stats
         summ["wavefront_steps"] = st["wavefront_steps"]

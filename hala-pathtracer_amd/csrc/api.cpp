@@ -888,7 +888,6 @@ hpt_status hpt_create(const hpt_create_info* info, hpt_renderer** out) {
     try {
       set_device(r);
       check(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking), "hipStreamCreate");
-      default_blue_noise(r->blue_noise);
     } catch (...) {
       delete r;
       throw;
@@ -1200,6 +1199,9 @@ hpt_status hpt_commit(hpt_renderer* r) {
     std::vector<float> lut(256);
     for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear(i);
     upload(r->sd->d_srgb_lut, lut, s);
+    // The procedural stand-in is made here, when first needed: handles that
+    // render a shared scene never upload one.
+    if (r->blue_noise.empty()) default_blue_noise(r->blue_noise);
     upload(r->sd->d_blue_noise, r->blue_noise, s);
     alloc_frame_state(r);
     check(hipStreamSynchronize(s), "commit");

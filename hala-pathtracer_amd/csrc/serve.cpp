@@ -18,8 +18,10 @@
 //    the device runs the next step.
 //
 // A job's result image reaches host memory before its finish time is taken
-// (the latency end point); then its handle is destroyed and its footprint
-// returns to the budget.
+// (the latency end point).  Then its handle is kept for a later job of the
+// same class (reuse_handles, the default: no handle to make, no frame state
+// to allocate, no batch graph to capture), its footprint still charged to
+// the budget until a job needs the room, or destroyed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -27,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -104,6 +107,21 @@ struct Job {
 };
 using JobPtr = std::shared_ptr<Job>;
 
+// What a job handle is made with: a finished job's handle renders a later
+// job with the same key without being made again (zero-filled before use, so
+// padding compares equal).
+struct HandleKey {
+  uint32_t scene, width, height, max_depth, rr_depth, tonemap, aces, simple_aces;
+  float exposure;
+  uint32_t frames, launch, sort, tail, block;
+};
+
+struct CachedHandle {
+  HandleKey key;
+  hpt_renderer* h;
+  uint64_t fp;
+};
+
 struct SceneSlot {
   hpt_serve_scene_desc desc{};
   // wavefront mode
@@ -130,6 +148,8 @@ struct hpt_server {
   std::vector<uint64_t> reap;          // streams-mode jobs whose threads have ended
   uint64_t next_id = 1, used = 0, peak = 0;
   uint32_t running = 0;                // admitted, unfinished jobs
+  std::list<CachedHandle> cache;       // idle handles of finished jobs, oldest first
+  uint64_t cached_bytes = 0, reused = 0;
   uint64_t n_submitted = 0, n_finished = 0, n_failed = 0;
   uint64_t steps = 0, max_step_jobs = 0, step_jobs = 0;
   bool closing = false;
@@ -147,6 +167,26 @@ namespace {
 uint32_t resolve(uint32_t v, uint32_t dflt) { return v == HPT_SERVE_DEFAULT ? dflt : v; }
 
 bool ok_server(const hpt_server* s) { return s && s->tag == kServerTag; }
+
+HandleKey handle_key(const hpt_server* s, const Job& j) {
+  HandleKey k;
+  std::memset(&k, 0, sizeof k);
+  k.scene = j.d.scene;
+  k.width = j.d.width;
+  k.height = j.d.height;
+  k.max_depth = j.d.max_depth;
+  k.rr_depth = j.d.rr_depth;
+  k.tonemap = j.d.enable_tonemap;
+  k.aces = j.d.enable_aces;
+  k.simple_aces = j.d.use_simple_aces;
+  k.exposure = j.d.exposure_value;
+  k.frames = s->cfg.mode == HPT_SERVE_WAVEFRONT ? 1 : j.chunk;
+  k.launch = resolve(j.d.launch_mode, s->cfg.launch_mode);
+  k.sort = resolve(j.d.shade_sort, HPT_SHADE_SORT_OFF);
+  k.tail = resolve(j.d.tail_from, HPT_TAIL_FROM_NONE);
+  k.block = resolve(j.d.traversal_block, 128);
+  return k;
+}
 
 // A resident scene's slot (slots are added under the lock and never removed).
 SceneSlot& slot(hpt_server* s, uint32_t id) {
@@ -267,11 +307,19 @@ void finish(hpt_server* s, Job& j, hpt_status st, const std::string& err) {
     j.error = err;
   }
   const double t = s->now();
-  if (j.h) {
+  // A handle that ended well is kept for the next job of its class (its
+  // footprint stays charged to the budget while it waits); any other is freed.
+  const bool keep = j.h && st == HPT_OK && s->cfg.reuse_handles;
+  if (j.h && !keep) {
     hpt_destroy(j.h);
     j.h = nullptr;
   }
   std::lock_guard<std::mutex> g(s->mu);
+  if (keep) {
+    s->cache.push_back(CachedHandle{handle_key(s, j), j.h, j.fp});
+    s->cached_bytes += j.fp;
+    j.h = nullptr;
+  }
   j.finished = t;
   j.status = st;
   if (s->cfg.keep_images && st == HPT_OK) j.image = std::move(img);
@@ -289,7 +337,10 @@ void run_job(hpt_server* s, JobPtr j) {
   hpt_status st = HPT_OK;
   std::string err;
   try {
-    j->h = make_handle(s, *j);
+    if (j->h)  // a handle of the same class, kept from a finished job
+      ok(hpt_reset_accumulation(j->h), "hpt_reset_accumulation");
+    else
+      j->h = make_handle(s, *j);
     // One batch in flight per job: the streams of the jobs share the
     // process's few hardware queues, each in order, so a job that queued all
     // of its batches at once would hold the jobs behind it on its queue.
@@ -415,6 +466,35 @@ void scene_worker(hpt_server* s, uint32_t scene) {
 
 void maybe_release(hpt_server* s, uint64_t id);
 
+// Admission of the queue's head, under the lock: a kept handle of its class
+// if there is one (its footprint is already charged), else room in the
+// budget, made by evicting kept handles oldest first when needed (returned
+// in `evicted` for the caller to free before the job allocates).
+bool try_admit(hpt_server* s, Job& j, uint32_t cap, std::vector<hpt_renderer*>& evicted) {
+  if (cap != 0 && s->running >= cap) return false;
+  if (s->cfg.reuse_handles && !s->cache.empty()) {
+    const HandleKey k = handle_key(s, j);
+    for (auto it = s->cache.end(); it != s->cache.begin();) {
+      --it;  // the most recently kept first
+      if (std::memcmp(&it->key, &k, sizeof k) != 0) continue;
+      j.h = it->h;
+      s->cached_bytes -= it->fp;
+      s->used += j.fp;
+      s->reused += 1;
+      s->cache.erase(it);
+      return true;
+    }
+  }
+  while (s->used + s->cached_bytes + j.fp > s->cfg.budget_bytes && !s->cache.empty()) {
+    evicted.push_back(s->cache.front().h);
+    s->cached_bytes -= s->cache.front().fp;
+    s->cache.pop_front();
+  }
+  if (s->used + s->cached_bytes + j.fp > s->cfg.budget_bytes) return false;
+  s->used += j.fp;
+  return true;
+}
+
 // Admits jobs; in streams mode it also owns the jobs' threads and joins each
 // one after it ends (and all of them before it returns).
 void dispatcher(hpt_server* s) {
@@ -430,6 +510,7 @@ void dispatcher(hpt_server* s) {
   for (;;) {
     JobPtr j;
     std::vector<uint64_t> reaped;
+    std::vector<hpt_renderer*> evicted;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv.wait(lk, [&] { return !s->queue.empty() || s->closing; });
@@ -440,15 +521,16 @@ void dispatcher(hpt_server* s) {
       }
       JobPtr head = s->queue.front();
       const uint32_t cap = s->cfg.mode == HPT_SERVE_STREAMS ? s->cfg.max_running : 0;
-      s->cv.wait(lk, [&] { return s->used + head->fp <= s->cfg.budget_bytes && (cap == 0 || s->running < cap); });
+      while (!try_admit(s, *head, cap, evicted)) s->cv.wait(lk);
       s->queue.pop_front();
-      s->used += head->fp;
       s->running += 1;
-      s->peak = std::max(s->peak, s->used);
+      s->peak = std::max(s->peak, s->used + s->cached_bytes);
       head->admitted = s->now();
       j = head;
       reaped.swap(s->reap);
     }
+    // Idle handles evicted to make room are freed before the job allocates.
+    for (hpt_renderer* h : evicted) hpt_destroy(h);
     join(reaped);
     if (s->cfg.mode == HPT_SERVE_STREAMS) {
       workers.emplace(j->id, std::thread(run_job, s, j));
@@ -456,7 +538,10 @@ void dispatcher(hpt_server* s) {
     }
     // Wavefront mode: the job's one-frame handle, then its scene's wavefront.
     try {
-      j->h = make_handle(s, *j);
+      if (j->h)
+        ok(hpt_reset_accumulation(j->h), "hpt_reset_accumulation");
+      else
+        j->h = make_handle(s, *j);
     } catch (const Fail& f) {
       finish(s, *j, f.code, f.what());
       continue;
@@ -515,6 +600,7 @@ void hpt_server_config_init(hpt_server_config* c) {
   c->launch_mode = HPT_LAUNCH_GRAPH;
   c->max_running = 8;
   c->dry_run_ns_per_path = 0;
+  c->reuse_handles = 1;
 }
 
 void hpt_job_desc_init(hpt_job_desc* d) {
@@ -540,6 +626,7 @@ hpt_status hpt_server_create(const hpt_server_config* config, hpt_server** out) 
     need(config->mode != HPT_SERVE_WAVEFRONT || config->step_paths >= 1, "step_paths must be at least 1");
     need(config->launch_mode == HPT_LAUNCH_EAGER || config->launch_mode == HPT_LAUNCH_GRAPH, "unknown launch mode");
     need(config->keep_images <= 1, "keep_images must be 0 or 1");
+    need(config->reuse_handles <= 1, "reuse_handles must be 0 or 1");
     auto s = std::make_unique<hpt_server>();
     s->cfg = *config;
     s->t0 = std::chrono::steady_clock::now();
@@ -677,6 +764,8 @@ hpt_status hpt_server_get_stats(hpt_server* s, hpt_server_stats* out) {
     out->wavefront_steps = s->steps;
     out->max_jobs_per_step = s->max_step_jobs;
     out->jobs_in_steps = s->step_jobs;
+    out->handles_reused = s->reused;
+    out->cached_bytes = s->cached_bytes;
   });
 }
 
@@ -691,6 +780,8 @@ void hpt_server_destroy(hpt_server* s) {
   if (s->dispatcher_started) s->dispatcher.join();
   for (auto& sc : s->scenes)
     if (sc->worker_started) sc->worker.join();
+  for (const CachedHandle& c : s->cache) hpt_destroy(c.h);
+  s->cache.clear();
   s->tag = 0;
   delete s;
 }

@@ -259,6 +259,8 @@ pub struct hpt_server_config {
     pub launch_mode: u32,
     pub max_running: u32,
     pub dry_run_ns_per_path: u64,
+    pub reuse_handles: u32,
+    pub _reserved: u32,
 }
 
 #[repr(C)]
@@ -323,6 +325,8 @@ pub struct hpt_server_stats {
     pub wavefront_steps: u64,
     pub max_jobs_per_step: u64,
     pub jobs_in_steps: u64,
+    pub handles_reused: u64,
+    pub cached_bytes: u64,
 }
 
 extern "C" {

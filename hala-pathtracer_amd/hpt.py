@@ -149,7 +149,8 @@ class _StageStats(C.Structure):
 class _ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("mode", C.c_uint32), ("budget_bytes", C.c_uint64),
                 ("max_frames", C.c_uint32), ("keep_images", C.c_uint32), ("step_paths", C.c_uint64),
-                ("launch_mode", C.c_uint32), ("max_running", C.c_uint32), ("dry_run_ns_per_path", C.c_uint64)]
+                ("launch_mode", C.c_uint32), ("max_running", C.c_uint32), ("dry_run_ns_per_path", C.c_uint64),
+                ("reuse_handles", C.c_uint32), ("_reserved", C.c_uint32)]
 
 
 class _ServeSceneDesc(C.Structure):
@@ -177,7 +178,8 @@ class _JobResult(C.Structure):
 class _ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("finished", C.c_uint64), ("failed", C.c_uint64),
                 ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("wavefront_steps", C.c_uint64),
-                ("max_jobs_per_step", C.c_uint64), ("jobs_in_steps", C.c_uint64)]
+                ("max_jobs_per_step", C.c_uint64), ("jobs_in_steps", C.c_uint64), ("handles_reused", C.c_uint64),
+                ("cached_bytes", C.c_uint64)]
 
 
 # Every exported entry point with its argument types (the declarations of
@@ -757,7 +759,7 @@ class RenderServer:
 
     def __init__(self, budget_bytes: int, mode: int = SERVE_STREAMS, max_frames: int = 32, keep_images: bool = False,
                  step_paths: int = 8 << 20, launch: int = LAUNCH_GRAPH, device: int = 0, dry_run_ns_per_path: int = 0,
-                 max_running: int = 8):
+                 max_running: int = 8, reuse_handles: bool = True):
         self._lib = load_library()
         cfg = _ServerConfig()
         self._lib.hpt_server_config_init(C.byref(cfg))
@@ -765,6 +767,7 @@ class RenderServer:
         cfg.keep_images, cfg.step_paths, cfg.launch_mode = int(keep_images), step_paths, launch
         cfg.dry_run_ns_per_path = dry_run_ns_per_path
         cfg.max_running = max_running
+        cfg.reuse_handles = int(reuse_handles)
         h = C.c_void_p()
         _check(self._lib.hpt_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

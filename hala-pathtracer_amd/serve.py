@@ -180,7 +180,8 @@ class Server:
 
     def __init__(self, scenes: dict, budget_bytes: int, max_frames: int = 32, keep_images: bool = False,
                  device: int = 0, tuning=None, launch: int = hpt.LAUNCH_GRAPH, wavefront: bool = False,
-                 step_paths: int = 8 << 20, denoiser=None, dry_run_ns_per_path: int = 0, max_running: int = 8):
+                 step_paths: int = 8 << 20, denoiser=None, dry_run_ns_per_path: int = 0, max_running: int = 8,
+                 reuse_handles: bool = True):
         self.scenes = scenes
         self.budget = budget_bytes
         self.max_frames = max_frames
@@ -193,8 +194,10 @@ class Server:
         self.denoiser = denoiser          # denoiser.UNet or hpt.Denoiser
         self.dry_run_ns_per_path = dry_run_ns_per_path
         self.max_running = max_running    # streams mode: jobs running at once (0: as the budget allows)
+        self.reuse_handles = reuse_handles  # a finished job's handle renders the next job of its class
         self.stats = {}
         self._net = None                  # the hpt.Denoiser the native server uses
+        self._srv, self._ids = None, {}   # the native server, made at the first run
 
     def _native(self, budget=None):
         """A native server with this server's settings and scenes, and the
@@ -202,7 +205,7 @@ class Server:
         srv = hpt.RenderServer(self.budget if budget is None else budget,
                                hpt.SERVE_WAVEFRONT if self.wavefront else hpt.SERVE_STREAMS, self.max_frames,
                                self.keep_images, self.step_paths, self.launch, self.device,
-                               self.dry_run_ns_per_path, self.max_running)
+                               self.dry_run_ns_per_path, self.max_running, self.reuse_handles)
         ids = {}
         for name, rs in self.scenes.items():
             ids[name] = srv.add_scene(rs.template, rs.features, *self._step_settings(name),
@@ -255,35 +258,48 @@ class Server:
 
     def run(self, specs: list) -> list:
         """Serves `specs` (arrival times relative to now); returns the results
-        in completion order."""
+        in completion order.  Successive runs share one native server, so a
+        job can render on the kept handle of an earlier job of its class;
+        close() frees it."""
         specs = sorted(specs, key=lambda s: s.arrival)
-        srv, ids = self._native()
-        try:
-            descs = [(s, self._desc(srv, ids[s.scene], s)) for s in specs]
-            for s, d in descs:
-                fp, _ = srv.plan(d)
-                if fp > self.budget:
-                    raise ValueError(f"job {s.job_id} needs {fp} B, above the budget")
+        if self._srv is None:
+            self._srv, self._ids = self._native()
             net = self._native_denoiser()
             if net is not None:
-                srv.set_denoiser(net)
-            t0 = srv.clock()
-            by_id = {}
-            for s, d in descs:
-                wait = s.arrival - (srv.clock() - t0)
-                if wait > 0:
-                    time.sleep(wait)
-                by_id[srv.submit(d)] = s
-            srv.drain()
-            results = []
-            while (r := srv.next_result(0.0)) is not None:
-                s = by_id[r["job_id"]]
-                img = srv.read_image(r["job_id"], s.width, s.height) if r["has_image"] else None
-                results.append(JobResult(s.job_id, s, r["admitted"] - t0, r["finished"] - t0, img, r["error"]))
-            self.stats = srv.stats()
-            return results
-        finally:
-            srv.close()
+                self._srv.set_denoiser(net)
+        srv, ids = self._srv, self._ids
+        descs = [(s, self._desc(srv, ids[s.scene], s)) for s in specs]
+        for s, d in descs:
+            fp, _ = srv.plan(d)
+            if fp > self.budget:
+                raise ValueError(f"job {s.job_id} needs {fp} B, above the budget")
+        t0 = srv.clock()
+        by_id = {}
+        for s, d in descs:
+            wait = s.arrival - (srv.clock() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            by_id[srv.submit(d)] = s
+        srv.drain()
+        results = []
+        while (r := srv.next_result(0.0)) is not None:
+            s = by_id[r["job_id"]]
+            img = srv.read_image(r["job_id"], s.width, s.height) if r["has_image"] else None
+            results.append(JobResult(s.job_id, s, r["admitted"] - t0, r["finished"] - t0, img, r["error"]))
+        self.stats = srv.stats()
+        return results
+
+    def close(self) -> None:
+        """Stops the native server (waiting for its jobs) and frees its kept handles."""
+        if getattr(self, "_srv", None) is not None:
+            self._srv.close()
+            self._srv = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def tune_classes(scenes: dict, specs: list, max_frames: int = 16, tolerance: float = 0.05, log=None):

@@ -671,6 +671,14 @@ typedef struct hpt_server_config {
    * and its image is filled with the job id -- for planning a budget on a
    * host without a GPU, and for testing the scheduling. */
   uint64_t dry_run_ns_per_path;
+  /* 1 (the default): a finished job's handle is kept for a later job of the
+   * same class (scene, size, depths, tonemap, exposure, batch and tuning),
+   * which then renders without making a handle, allocating its frame state
+   * or capturing its batch graph again.  Kept handles hold their footprint
+   * within the budget and are freed, least recently kept first, when a job
+   * needs the room.  0 frees every handle when its job ends. */
+  uint32_t reuse_handles;
+  uint32_t _reserved;
 } hpt_server_config;
 HPT_API void hpt_server_config_init(hpt_server_config* c);
 
@@ -718,6 +726,8 @@ typedef struct hpt_server_stats {
   uint64_t bytes_in_use, peak_bytes;
   uint64_t wavefront_steps, max_jobs_per_step;
   uint64_t jobs_in_steps;      /* sum over wavefront steps of the jobs each held */
+  uint64_t handles_reused;     /* jobs that rendered on a kept handle */
+  uint64_t cached_bytes;       /* footprint of the handles kept now */
 } hpt_server_stats;
 
 HPT_API hpt_status hpt_server_create(const hpt_server_config* config, hpt_server** out);
@@ -746,7 +756,8 @@ HPT_API hpt_status hpt_server_drain(hpt_server* s);
 /* Seconds on the server's clock. */
 HPT_API hpt_status hpt_server_clock(hpt_server* s, double* seconds);
 HPT_API hpt_status hpt_server_get_stats(hpt_server* s, hpt_server_stats* out);
-/* Drains, then stops the server's threads and frees what it holds. */
+/* Drains, then stops the server's threads and frees what it holds (kept
+ * handles included). */
 HPT_API void hpt_server_destroy(hpt_server* s);
 
 #ifdef __cplusplus

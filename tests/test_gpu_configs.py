@@ -100,6 +100,7 @@ def test_c3_batched_uneven_jobs_equal_solo_renders(resident, wavefront):
     srv = serve.Server(resident, 64 << 30, max_frames=16, keep_images=True, wavefront=wavefront,
                        step_paths=4 << 20)
     res = {r.job_id: r for r in srv.run(jobs)}
+    srv.close()
     assert all(r.error is None for r in res.values()), [r.error for r in res.values() if r.error]
     if wavefront:
         st = srv.stats
@@ -123,6 +124,7 @@ def test_c4_poisson_serving_with_the_denoiser(resident):
     srv = serve.Server(resident, 64 << 30, max_frames=16, keep_images=True, denoiser=net)
     t = time.perf_counter()
     results = srv.run(jobs)
+    srv.close()
     summ = serve.summarize(results, time.perf_counter() - t)
     print(f"[gpu] C4: {summ}", flush=True)
     res = {r.job_id: r for r in results}

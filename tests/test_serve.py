@@ -168,6 +168,7 @@ def test_results_identical_alone_and_in_the_mix(wavefront):
     print(f"[gpu] serving 6 jobs together (wavefront {wavefront})", flush=True)
     srv = serve.Server(scenes, 1 << 34, max_frames=8, keep_images=True, wavefront=wavefront)
     mixed = {r.job_id: r for r in srv.run(specs)}
+    srv.close()
     assert all(r.error is None for r in mixed.values()), [r.error for r in mixed.values()]
     for s in specs:
         print(f"[gpu] job {s.job_id} alone", flush=True)
@@ -293,3 +294,36 @@ def test_running_jobs_capped_in_stream_mode():
         assert sum(1 for q in res if q.admitted <= r.admitted < q.finished) <= 3
     order = sorted(res, key=lambda q: q.admitted)
     assert [q.spec.job_id for q in order] == sorted(q.spec.job_id for q in res)
+
+
+@pytest.mark.gpu
+def test_kept_handles_render_later_jobs_of_their_class():
+    """A finished job's handle renders the next job of its class (same scene,
+    size, depths, batch and tuning) with its images cleared: every image still
+    equals the job's solo render; a job of another class that needs the room
+    evicts kept handles, and without reuse nothing is kept."""
+    rs = serve.ResidentScene("cornell", hpt_scenes.cornell_box(), 0)
+    specs = [serve.JobSpec(i, "cornell", 64, 48, 8 + 4 * (i % 2), 6, 3, 0.05 * i) for i in range(4)]
+    specs.append(serve.JobSpec(4, "cornell", 96, 96, 8, 6, 3, 0.25))  # another class
+    print("[gpu] kept handles: 5 jobs, 2 classes", flush=True)
+    one = serve.footprint(specs[0], 32)
+    # room for one 64x48 job and the kept handle of another, not for the 96x96 job beside them
+    budget = max(2 * one + serve.footprint(specs[4], 32) // 2, serve.footprint(specs[4], 32))
+    srv = serve.Server({"cornell": rs}, budget, keep_images=True, max_running=1)
+    res = {r.job_id: r for r in srv.run(specs)}
+    assert all(r.error is None for r in res.values()), [r.error for r in res.values()]
+    assert srv.stats["handles_reused"] >= 2, srv.stats  # jobs 2 and 3 reuse the handles of 0 and 1
+    assert srv.stats["peak_bytes"] <= budget
+    for s in specs:
+        r = hpt.HalaRenderer(s.width, s.height, max_depth=s.max_depth, rr_depth=s.rr_depth, enable_tonemap=True)
+        r.share_scene(rs.template)
+        r.render(s.spp)
+        assert np.array_equal(r.read_image(hpt.IMAGE_ACCUM), res[s.job_id].image), s
+        r.close()
+    fresh = serve.Server({"cornell": rs}, 1 << 34, keep_images=True, reuse_handles=False)
+    again = {r.job_id: r for r in fresh.run(specs[:2])}
+    assert fresh.stats["handles_reused"] == 0 and fresh.stats["cached_bytes"] == 0
+    assert all(np.array_equal(again[i].image, res[i].image) for i in range(2))
+    srv.close()
+    fresh.close()
+    rs.close()
