@@ -156,6 +156,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default="", help="continue from these weights")
+    ap.add_argument("--spp-levels", default=",".join(str(k) for k in SPP_LEVELS),
+                    help="sample counts of the noisy training inputs (the low ones weigh more when fewer are listed)")
     ap.add_argument("--layouts", default="0,1,2",
                     help="random_scene layouts to train on (0 room, 1 outdoor, 2 displaced icosphere); "
                          "0,1 keeps the large-mesh benchmark scene's family out of training")
@@ -166,8 +168,9 @@ def main(argv=None) -> int:
     train_seeds = range(1000 + a.seed * 100000, 1000 + a.seed * 100000 + a.scenes)
     hold_seeds = range(900, 900 + a.holdout)
     layouts = tuple(int(k) for k in a.layouts.split(","))
-    data = render_dataset(train_seeds, a.size, a.size, ref_spp=a.ref_spp, log=log, layouts=layouts)
-    hold = render_dataset(hold_seeds, a.size, a.size, ref_spp=a.ref_spp, log=None, layouts=layouts)
+    levels = tuple(int(k) for k in a.spp_levels.split(","))
+    data = render_dataset(train_seeds, a.size, a.size, levels, ref_spp=a.ref_spp, log=log, layouts=layouts)
+    hold = render_dataset(hold_seeds, a.size, a.size, SPP_LEVELS, ref_spp=a.ref_spp, log=None, layouts=layouts)
     net = denoiser.UNet.load(a.init) if a.init else denoiser.UNet(seed=a.seed)
     t0 = time.perf_counter()
     train(net, data, a.steps, a.batch, a.crop, a.lr, a.seed, log, 500)
