@@ -5,14 +5,17 @@ and a high-sample reference from a disjoint frame range.  Batches are random
 crops, mirrored at random, cut from the device images by the library's
 kernels; the network, loss and optimizer are those of denoiser.UNet.
 
-The committed network (weights/unet_denoiser.npz) was made in two runs on
-3000 scenes of the room and outdoor layouts only (no displaced icosphere, so
-the large-mesh benchmark scene's family is never seen in training):
+The committed network (weights/unet_denoiser.npz) was made in three runs on
+4200 scenes of the room and outdoor layouts only (no displaced icosphere, so
+the large-mesh benchmark scene's family is never seen in training), the last
+one on low-sample inputs only:
 
     python hala-pathtracer_amd/denoise_train.py --scenes 2000 --steps 40000 \
         --ref-spp 4096 --layouts 0,1 --seed 3 --out first.npz
     python hala-pathtracer_amd/denoise_train.py --scenes 1000 --steps 40000 \
-        --ref-spp 4096 --layouts 0,1 --seed 4 --init first.npz --lr 5e-4 \
+        --ref-spp 4096 --layouts 0,1 --seed 4 --init first.npz --lr 5e-4 --out second.npz
+    python hala-pathtracer_amd/denoise_train.py --scenes 1200 --steps 40000 \
+        --ref-spp 4096 --layouts 0,1 --seed 5 --spp-levels 4,8,16 --init second.npz --lr 3e-4 \
         --out hala-pathtracer_amd/weights/unet_denoiser.npz
 """
 from __future__ import annotations

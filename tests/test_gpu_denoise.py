@@ -419,7 +419,7 @@ def test_training_lowers_the_loss(dn):
 
 def test_trained_network_beats_the_plain_estimate(dn):
     """The committed network (trained on the GPU by the kernels above, on
-    3000 random room and outdoor scenes: no displaced icosphere, the layout of
+    4200 random room and outdoor scenes: no displaced icosphere, the layout of
     the large-mesh benchmark scene) against the plain 16-spp estimate on
     held-out scenes of all three layouts: relMSE to a 2048-spp reference from
     disjoint frames."""
