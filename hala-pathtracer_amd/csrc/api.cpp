@@ -161,6 +161,7 @@ struct hpt_renderer {
   uint32_t tail_from = HPT_TAIL_FROM_NONE;
   // Threads per block of the traversal kernels (extend, walk, shadow).
   uint32_t trav_block = kDefaultTraversalBlock;
+  bool refill = true;  // lane refill in the traversal kernels (hpt_set_traversal_refill)
   DevBuf d_accum, d_albedo, d_normal, d_final;
   uint32_t npix = 0;          // width * height
   uint32_t batch_cap = 0;     // frames the path buffers hold
@@ -202,6 +203,7 @@ struct hpt_group {
   uint32_t shade_sort_from = HPT_SHADE_SORT_OFF;
   uint32_t tail_from = HPT_TAIL_FROM_NONE;
   uint32_t trav_block = kDefaultTraversalBlock;
+  bool refill = true;  // lane refill in the traversal kernels (hpt_set_traversal_refill)
   uint32_t vmis_phases = 1;  // volume-MIS shading in phases (hpt_group_set_vmis_phases)
   size_t cap = 0;  // paths the path buffers hold
   DevBuf d_rng, d_org, d_dir, d_thr, d_rad, d_alb, d_nrm, d_med, d_med2, d_nee_e, d_nee_l, d_hit, d_job;
@@ -449,6 +451,7 @@ struct WaveBuffers {
   uint8_t* sort_keys;
   uint32_t* sort_items;
   unsigned long long* tail_cut;  // paths the tail cut at kMaxBounces (one device word)
+  bool refill = true;  // traversal lanes refill from their queues (k_extend_refill, k_walk_refill)
 };
 
 // Bounces (closest-hit queries) a path may take; the tail cuts a path that
@@ -511,7 +514,8 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     const Queue& qout = qbuf[out];
     const bool sort = b >= sort_from && ss.keys && ss.items;
     begin(HPT_STAGE_EXTEND);
-    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr, phase_counts, trav_block);
+    launch_extend(s, sv, ps, qin, n, qout, sq, sp, sc_extend, sort ? ss.keys : nullptr, phase_counts, trav_block,
+                  wb.refill ? counts + kCursorOffset : nullptr);
     end();
     Queue qshade = qin;
     if (sort) {
@@ -525,14 +529,16 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
       launch_vmis_a(s, transparent, sv, shade_fp, ps, qshade, n, qout, qnee, sq, sc_shade);
       end();
       begin(HPT_STAGE_WALK);
-      launch_walk(s, transparent, sv, shade_fp, ps, sq, g.seg_cap, sp, sc_walk, trav_block);
+      launch_walk(s, transparent, sv, shade_fp, ps, sq, g.seg_cap, sp, sc_walk, trav_block,
+                  wb.refill ? counts + kCursorOffset : nullptr);
       end();
       if (shade_fp.env_type != HPT_ENV_SKY) {
         begin(HPT_STAGE_SHADE);
         launch_vmis_b(s, transparent, sv, shade_fp, ps, qnee, wq1);
         end();
         begin(HPT_STAGE_WALK);
-        launch_walk(s, transparent, sv, shade_fp, ps, wq1, g.seg_cap, sp, sc_walk, trav_block);
+        launch_walk(s, transparent, sv, shade_fp, ps, wq1, g.seg_cap, sp, sc_walk, trav_block,
+                    wb.refill ? counts + kCursorOffset : nullptr);
         end();
       }
       begin(HPT_STAGE_SHADE);
@@ -597,6 +603,7 @@ void render_batch(hpt_renderer* r, uint32_t camera_index, uint32_t frames, const
   wb.sort_items = r->d_sort_items.as<uint32_t>();
   wb.tail_cut = r->d_tail_cut.as<unsigned long long>();
   wb.qc = r->vmis_phases ? r->d_qc.as<uint32_t>() : nullptr;
+  wb.refill = r->refill;
   const std::vector<WaveJob> jobs{WaveJob{fp, images_of(r), 0, n}};
   trace_wavefront(r->stream, scene_view(r), r->info.features, fp, wb, jobs, n, r->info.max_depth + 1,
                   r->shade_sort_from, r->tail_from, r->trav_block, r);
@@ -628,6 +635,7 @@ std::vector<uint8_t> graph_key(hpt_renderer* r, uint32_t camera_index, uint32_t 
   key_append(k, r->shade_sort_from);
   key_append(k, r->tail_from);
   key_append(k, r->trav_block);
+  key_append(k, r->refill);
   return k;
 }
 
@@ -762,7 +770,7 @@ void alloc_frame_state(hpt_renderer* r) {
   r->npix = n;
   alloc_paths(r, batch_frames_for(r));
   const size_t f4 = (size_t)n * 16;
-  r->d_counts.alloc((6 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
+  r->d_counts.alloc(kCountWords * sizeof(uint32_t));
   check(hipMemsetAsync(r->d_counts.p, 0, r->d_counts.bytes, s), "hipMemsetAsync");
   r->d_accum.alloc(f4);
   r->d_albedo.alloc(f4);
@@ -845,7 +853,7 @@ void group_alloc(hpt_group* g, uint32_t n, uint32_t bvh_depth, bool vmis) {
     if (g->d_spill.bytes < entries * sizeof(uint2)) g->d_spill.alloc(entries * sizeof(uint2));
   }
   if (!g->d_counts.p) {
-    g->d_counts.alloc((6 * kSeg * kSegStride + 2 * kSeg * kSortKeys) * sizeof(uint32_t));
+    g->d_counts.alloc(kCountWords * sizeof(uint32_t));
     check(hipMemsetAsync(g->d_counts.p, 0, g->d_counts.bytes, g->stream), "hipMemsetAsync");
   }
   if (!g->d_limits.p) g->d_limits.alloc(kMaxWaveJobs * sizeof(uint2));
@@ -1395,6 +1403,20 @@ hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads) {
     need(ok_group(g), "null or invalid group handle");
     need(traversal_block_ok(threads), "the traversal kernels are built for 64, 128 or 256 threads per block");
     g->trav_block = threads;
+  });
+}
+
+hpt_status hpt_set_traversal_refill(hpt_renderer* r, uint32_t on) {
+  return guarded([&] {
+    need(ok_renderer(r), "null or invalid renderer handle");
+    r->refill = on != 0;
+  });
+}
+
+hpt_status hpt_group_set_traversal_refill(hpt_group* g, uint32_t on) {
+  return guarded([&] {
+    need(ok_group(g), "null or invalid group handle");
+    g->refill = on != 0;
   });
 }
 
@@ -2018,6 +2040,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     wb.sort_items = g->d_sort_items.as<uint32_t>();
     wb.tail_cut = g->d_tail_cut.as<unsigned long long>();
     wb.qc = g->vmis_phases ? g->d_qc.as<uint32_t>() : nullptr;
+    wb.refill = g->refill;
     key_append(key, sv);
     key_append(key, shade_fp);
     key_append(key, wb.ps);
@@ -2032,6 +2055,7 @@ hpt_status hpt_group_render(hpt_group* g, hpt_renderer* const* jobs, const uint3
     key_append(key, g->shade_sort_from);
     key_append(key, g->tail_from);
     key_append(key, g->trav_block);
+    key_append(key, g->refill);
     key_append(key, g->vmis_phases);
     for (const WaveJob& j : wj) key_append(key, j.base);
     const uint32_t njobs = (uint32_t)wj.size();

@@ -115,6 +115,11 @@ struct ShadowQueue {
 // depend on its queue position: sorted and unsorted launches give the same
 // images bit for bit.
 constexpr uint32_t kSortKeys = 64;
+// Words of a handle's segment-counter buffer: path queues A and B, the shadow
+// queue, the sorted shading queue, the sort's key counts and cursors, the
+// phased shading's two queues, then the refill cursors (kCursorOffset).
+constexpr size_t kCursorOffset = 6 * kSeg * kSegStride + 2 * kSeg * kSortKeys;
+constexpr size_t kCountWords = kCursorOffset + 2 * kSeg * kSegStride;
 struct ShadeSort {
   uint8_t* keys;     // one per queue slot, segment s at keys + s * seg_cap
   uint32_t* items;   // sorted path ids, same segment layout
@@ -162,9 +167,13 @@ struct StageCounters {
 // of the phased volume-MIS shading's queues, zeroed with qnext and sq.
 // bs: threads per block of the traversal kernels (extend, walk, shadow):
 // one of kTraversalBlocks.
+// cursors: null, or kSeg + 1 refill counters (kSegStride words apart, zero
+// between launches: the launch's last block clears them).  With cursors a
+// launch over a compacted queue runs resident blocks whose lanes take new
+// entries when enough of a wave's rays have ended (k_extend_refill).
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
-                   uint32_t* phase_counts, uint32_t bs);
+                   uint32_t* phase_counts, uint32_t bs, uint32_t* cursors = nullptr);
 // Groups qin's entries by the keys extend wrote: ss.items and ss.counts form
 // the segmented queue shading then reads (qin may be the identity queue).
 void launch_shade_sort(hipStream_t s, const Queue& qin, uint32_t n_max, const ShadeSort& ss);
@@ -187,7 +196,8 @@ void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameP
                    const Queue& qin, uint32_t n_max, const Queue& qout, const Queue& qnee, const ShadowQueue& wq,
                    StageCounters sc);
 void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs);
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs,
+                 uint32_t* cursors = nullptr);
 void launch_vmis_b(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                    const Queue& qnee, const ShadowQueue& wq);
 void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,

@@ -9,6 +9,8 @@
 // answers the any-hit queries and adds the unoccluded NEE terms.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dev_shading.h"
 #include "dev_traverse.h"
 #include "kernels.h"
@@ -260,6 +262,118 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
     }
   }
   if (COUNT) add_visits(stats, cv, ct);
+}
+
+// Idle lanes a wave collects before it takes new queue entries.
+#ifndef HPT_REFILL_IDLE
+#define HPT_REFILL_IDLE 32
+#endif
+
+// Extension over a compacted queue by resident blocks that refill their
+// lanes.  One ray per lane ties a wave to its longest ray (bounce-1 rays on
+// the 1080p icosphere: 7.5 node visits on average, 16 at the 90th percentile,
+// up to 108), so most lanes of a wave idle for most of its steps.  Here a wave
+// keeps stepping and, once HPT_REFILL_IDLE of its lanes have ended, those
+// lanes store their hits and take the next entries of the block's segment in
+// one atomic, so every node step carries more live lanes.  Block b starts on
+// segment b % kSeg and moves on to the other segments of its XCD (b % 8)
+// when that one is drained.  Each entry is traced exactly as by k_extend and
+// its results go to the same places.  cursors[s * kSegStride] counts the
+// entries of segment s handed out, cursors[kSeg * kSegStride] the finished
+// blocks; the last block clears them all for the next launch.
+template <bool COUNT, bool SORT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
+    k_extend_refill(SceneView sv, PathState ps, Queue qin, Queue qnext, ShadowQueue sq, Spill sp,
+                    unsigned long long* stats, uint8_t* keys, uint32_t* phase_counts, uint32_t* cursors) {
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  if (gid < kSeg) {  // the shade launch after this one appends to these
+    qnext.counts[gid * kSegStride] = 0;
+    sq.counts[gid * kSegStride] = 0;
+    if (phase_counts) {
+      phase_counts[gid * kSegStride] = 0;
+      phase_counts[(kSeg + gid) * kSegStride] = 0;
+    }
+  }
+  uint32_t seg = blockIdx.x % kSeg;
+  uint32_t segn = qin.counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)segn);
+  LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  const uint32_t lane = lane_id();
+  uint32_t cv = 0, ct = 0, hops = 0;
+  uint32_t p = 0, k = 0;
+  bool more = true, active = false, ended = false;
+  Walker<false> w;
+  auto finish = [&] {
+    TraceResult r{0.0f, 0.0f, 0.0f, MISS_ID};
+    if (w.found) r = TraceResult{w.best_t, w.bu, w.bv, w.best_id};
+    // The direction is read again rather than kept through the walk, which
+    // needs only its reciprocal and the triangle test's shear.
+    const float4 d4 = ps.dir[p];
+    closest_lights(sv, w.r.org, mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, w.found, r);
+    ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+    if (SORT) keys[k] = shade_key(sv, r.id);
+    if (COUNT) {
+      cv += w.nv;
+      ct += w.nt;
+    }
+  };
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (more && __popcll(idle) >= HPT_REFILL_IDLE) {
+      if (ended) {
+        finish();
+        ended = false;
+      }
+      unsigned long long need = idle;
+      while (need != 0ull && more) {
+        const uint32_t cnt = (uint32_t)__popcll(need);
+        const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+        uint32_t got = 0;
+        if (lane == leader) got = atomicAdd(cursors + seg * kSegStride, cnt);
+        got = (uint32_t)__builtin_amdgcn_readlane((int)got, (int)leader);
+        const uint32_t avail = got < segn ? min(cnt, segn - got) : 0u;
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        const bool take = ((need >> lane) & 1ull) != 0ull && rank < avail;
+        if (take) {
+          k = seg * qin.seg_cap + got + rank;
+          p = qin.items[k];
+          const float4 o4 = ps.org[p], d4 = ps.dir[p];
+          w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, WORLD_SIZE, st);
+          active = true;
+        }
+        need &= ~__ballot(take);
+        if (avail < cnt) {  // segment drained: the next one of this XCD
+          if (++hops == kSeg / 8) {
+            more = false;
+          } else {
+            seg = (seg + 8) % kSeg;
+            segn = qin.counts[seg * kSegStride];
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0ull) {
+      if (!more) break;
+      continue;
+    }
+    if (active && w.step(sv, st)) {
+      active = false;
+      ended = true;
+    }
+  }
+  if (ended) finish();
+  if (COUNT) add_visits(stats, cv, ct);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(cursors + kSeg * kSegStride, 1u) == gridDim.x - 1) {
+      for (uint32_t s = 0; s < kSeg; ++s) cursors[s * kSegStride] = 0;
+      cursors[kSeg * kSegStride] = 0;
+      __threadfence();
+    }
+  }
 }
 
 // ------------------------------------------------------------ shading sort
@@ -1197,6 +1311,141 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
   if (COUNT) add_walks(stats, wc);
 }
 
+// The queued walks by resident blocks that refill their lanes, as
+// k_extend_refill does for extension rays: a lane traces one closest-hit
+// segment of its walk at a time; once HPT_REFILL_IDLE lanes of a wave are
+// idle, the lanes whose segment ended take its end together (the surface,
+// its material, the RNG draw of a partly transparent one, Beer-Lambert on an
+// exit face) and either start the walk's next segment or store its
+// transmittance, and the lanes still idle take the next walks of the
+// block's segment.  Every walk runs the operations of eval_transmittance in
+// its order, its RNG draws included, so the results equal k_walk's.
+#ifndef HPT_WALK_REFILL_WAVES_PER_EU
+#define HPT_WALK_REFILL_WAVES_PER_EU 7
+#endif
+template <bool TRANSP, bool JOBS, bool COUNT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_REFILL_WAVES_PER_EU)))
+    k_walk_refill(SceneView sv, FrameParams fp, PathState ps, ShadowQueue wq, Spill sp, unsigned long long* stats,
+                  uint32_t* cursors) {
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  uint32_t seg = blockIdx.x % kSeg;
+  uint32_t segn = wq.counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)segn);
+  LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  WalkCounts wc;
+  const uint32_t lane = lane_id();
+  uint32_t hops = 0, k = 0, depth = 0;
+  f3 tr = mk3(1.0f);
+  bool more = true, active = false, ended = false;
+  Walker<false> w;
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (__popcll(idle) >= HPT_REFILL_IDLE || idle == ~0ull) {
+      if (ended) {  // the end of a segment (eval_transmittance's loop body)
+        ended = false;
+        bool done = true;
+        TraceResult r{0.0f, 0.0f, 0.0f, MISS_ID};
+        if (w.found) r = TraceResult{w.best_t, w.bu, w.bv, w.best_id};
+        const float4 d4 = wq.d[k];
+        const uint32_t p = __float_as_uint(d4.w);
+        const f3 o = w.r.org, d = mk3(d4.x, d4.y, d4.z);
+        closest_lights(sv, o, d, 0.0f, WORLD_SIZE, w.found, r);
+        if (COUNT) {
+          wc.q++;
+          wc.nv += w.nv;
+          wc.nt += w.nt;
+        }
+        if (!(r.id == MISS_ID || (r.id & LIGHT_BIT))) {
+          const HitState h = triangle_state(sv, r.id, r.t, r.u, r.v, o, d);
+          if (h.material == INVALID) {
+            tr = mk3(0.0f);
+          } else {
+            const Mat m = load_material(sv.materials + h.material);
+            bool is_transparent = false;
+            if (TRANSP) {
+              if (m.opacity == 0.0f) {
+                is_transparent = true;
+              } else {
+                uint4 rng = ps.rng[p];
+                is_transparent = rnd(rng) > m.opacity;
+                ps.rng[p] = rng;
+              }
+            }
+            const bool is_refractive = (1.0f - m.metallic) * m.specular_transmission > 0.0f;
+            if (!(is_transparent || is_refractive)) {
+              tr = mk3(0.0f);
+            } else {
+              if (dot(d, h.normal) > 0.0f && m.medium_type != HPT_MEDIUM_NONE) {
+                const f3 color = m.medium_type == HPT_MEDIUM_ABSORB ? mk3(1.0f) - m.medium_color : mk3(1.0f);
+                const f3 e = -color * m.medium_density * r.t;
+                tr *= mk3(expf(e.x), expf(e.y), expf(e.z));
+              }
+              const uint32_t max_depth = JOBS ? fp.job_limits[ps.job[p]].x : fp.max_depth;
+              if (++depth < max_depth) {
+                w.begin(prepare_ray(h.pos + d * EPS, d), 0.0f, WORLD_SIZE, st);
+                active = true;
+                done = false;
+              }
+            }
+          }
+        }
+        if (done) ps.nee_tr[p] = make_float4(tr.x, tr.y, tr.z, 0.0f);
+      }
+      unsigned long long need = __ballot(!active);
+      while (need != 0ull && more) {
+        const uint32_t cnt = (uint32_t)__popcll(need);
+        const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+        uint32_t got = 0;
+        if (lane == leader) got = atomicAdd(cursors + seg * kSegStride, cnt);
+        got = (uint32_t)__builtin_amdgcn_readlane((int)got, (int)leader);
+        const uint32_t avail = got < segn ? min(cnt, segn - got) : 0u;
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        const bool take = ((need >> lane) & 1ull) != 0ull && rank < avail;
+        if (take) {
+          k = seg * wq.seg_cap + got + rank;
+          const float4 o4 = wq.o[k], d4 = wq.d[k];
+          const uint32_t p = __float_as_uint(d4.w);
+          const uint32_t max_depth = JOBS ? fp.job_limits[ps.job[p]].x : fp.max_depth;
+          tr = mk3(1.0f);
+          depth = 0;
+          if (max_depth > 0) {
+            w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, WORLD_SIZE, st);
+            active = true;
+          } else {
+            ps.nee_tr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+          }
+        }
+        need &= ~__ballot(take);
+        if (avail < cnt) {  // segment drained: the next one of this XCD
+          if (++hops == kSeg / 8) {
+            more = false;
+          } else {
+            seg = (seg + 8) % kSeg;
+            segn = wq.counts[seg * kSegStride];
+          }
+        }
+      }
+      if (__ballot(active) == 0ull) break;
+    }
+    if (active && w.step(sv, st)) {
+      active = false;
+      ended = true;
+    }
+  }
+  if (COUNT) add_walks(stats, wc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(cursors + kSeg * kSegStride, 1u) == gridDim.x - 1) {
+      for (uint32_t s = 0; s < kSeg; ++s) cursors[s * kSegStride] = 0;
+      cursors[kSeg * kSegStride] = 0;
+      __threadfence();
+    }
+  }
+}
+
 // A scattering event as A left it: the hit state of the surface that ended
 // the segment (its position replaced by the path's origin, the scattering
 // point, for a medium event) and, for a surface event, its material and
@@ -1581,11 +1830,48 @@ static inline uint32_t walk_blocks(uint32_t entries, uint32_t bs) {
     default: CALL(128); break;  \
   }
 
+// Blocks of bs threads per segment that fit on the device at once at the
+// traversal kernels' waves per SIMD (4 SIMDs per CU).
+static uint32_t resident_blocks_per_segment(uint32_t bs, uint32_t waves_per_simd = HPT_WALK_WAVES_PER_EU) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  const uint32_t waves = (uint32_t)cus * 4u * waves_per_simd;
+  return std::max<uint32_t>(1u, waves / (bs / 64u) / kSeg);
+}
+
 void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, const Queue& qin, uint32_t n_max,
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
-                   uint32_t* phase_counts, uint32_t bs) {
-  const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap, bs) : walk_blocks(n_max, bs));
+                   uint32_t* phase_counts, uint32_t bs, uint32_t* cursors) {
   const bool count = sc.count && sc.stats;
+  if (cursors && qin.items && sv.node_count > 0) {
+    const dim3 grid(kSeg * std::min(walk_blocks(qin.seg_cap, bs), resident_blocks_per_segment(bs)));
+#define HPT_EXTEND_R(C, S, BS_)                                                                                  \
+  hipLaunchKernelGGL((k_extend_refill<C, S, BS_>), grid, dim3(BS_), 0, s, sv, ps, qin, qnext, sq, sp, sc.stats, keys, \
+                     phase_counts, cursors)
+#define HPT_R_TT(BS_) HPT_EXTEND_R(true, true, BS_)
+#define HPT_R_FT(BS_) HPT_EXTEND_R(false, true, BS_)
+#define HPT_R_TF(BS_) HPT_EXTEND_R(true, false, BS_)
+#define HPT_R_FF(BS_) HPT_EXTEND_R(false, false, BS_)
+    if (keys) {
+      if (count) HPT_BY_BS(bs, HPT_R_TT)
+      else HPT_BY_BS(bs, HPT_R_FT)
+    } else {
+      if (count) HPT_BY_BS(bs, HPT_R_TF)
+      else HPT_BY_BS(bs, HPT_R_FF)
+    }
+#undef HPT_R_TT
+#undef HPT_R_FT
+#undef HPT_R_TF
+#undef HPT_R_FF
+#undef HPT_EXTEND_R
+    return;
+  }
+  const dim3 grid(qin.items ? kSeg * walk_blocks(qin.seg_cap, bs) : walk_blocks(n_max, bs));
 #define HPT_EXTEND(C, S, BS_)                                                                                   \
   hipLaunchKernelGGL((k_extend<C, S, BS_>), grid, dim3(BS_), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, keys, \
                      phase_counts)
@@ -1697,11 +1983,17 @@ void launch_vmis_a(hipStream_t s, bool transp, const SceneView& sv, const FrameP
 }
 
 void launch_walk(hipStream_t s, bool transp, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs) {
-  const dim3 grid(kSeg * ((seg_cap + bs - 1) / bs));
+                 const ShadowQueue& wq, uint32_t seg_cap, Spill sp, StageCounters sc, uint32_t bs, uint32_t* cursors) {
+  const bool refill = cursors && sv.node_count > 0;
+  const uint32_t per_seg = (seg_cap + bs - 1) / bs;
+  const dim3 grid(kSeg * (refill ? std::min(per_seg, resident_blocks_per_segment(bs, HPT_WALK_REFILL_WAVES_PER_EU))
+                                  : per_seg));
   const bool count = sc.count && sc.stats;
-#define HPT_W(T, J, C, BS_) \
-  hipLaunchKernelGGL((k_walk<T, J, C, BS_>), grid, dim3(BS_), 0, s, sv, fp, ps, wq, sp, sc.stats)
+#define HPT_W(T, J, C, BS_)                                                                                   \
+  if (refill)                                                                                                 \
+    hipLaunchKernelGGL((k_walk_refill<T, J, C, BS_>), grid, dim3(BS_), 0, s, sv, fp, ps, wq, sp, sc.stats, cursors); \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_walk<T, J, C, BS_>), grid, dim3(BS_), 0, s, sv, fp, ps, wq, sp, sc.stats)
 #define HPT_W_TTF(BS_) HPT_W(true, true, false, BS_)
 #define HPT_W_FTF(BS_) HPT_W(false, true, false, BS_)
 #define HPT_W_TFT(BS_) HPT_W(true, false, true, BS_)

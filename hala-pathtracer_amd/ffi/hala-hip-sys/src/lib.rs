@@ -194,6 +194,8 @@ extern "C" {
     pub fn hpt_group_set_tail_from(g: *mut hpt_group, first_bounce: u32) -> hpt_status;
     pub fn hpt_set_traversal_block(r: *mut hpt_renderer, threads: u32) -> hpt_status;
     pub fn hpt_group_set_traversal_block(g: *mut hpt_group, threads: u32) -> hpt_status;
+    pub fn hpt_set_traversal_refill(r: *mut hpt_renderer, on: u32) -> hpt_status;
+    pub fn hpt_group_set_traversal_refill(g: *mut hpt_group, on: u32) -> hpt_status;
     pub fn hpt_group_create(device: i32, out: *mut *mut hpt_group) -> hpt_status;
     pub fn hpt_group_render(g: *mut hpt_group, jobs: *const *mut hpt_renderer, frames: *const u32,
         camera_indices: *const u32, count: u32) -> hpt_status;

@@ -237,6 +237,8 @@ SIGNATURES = {
     "hpt_set_tail_from": [C.c_void_p, C.c_uint32],
     "hpt_set_traversal_block": [C.c_void_p, C.c_uint32],
     "hpt_group_set_traversal_block": [C.c_void_p, C.c_uint32],
+    "hpt_set_traversal_refill": [C.c_void_p, C.c_uint32],
+    "hpt_group_set_traversal_refill": [C.c_void_p, C.c_uint32],
     "hpt_group_set_tail_from": [C.c_void_p, C.c_uint32],
     "hpt_group_get_truncated_paths": [C.c_void_p, C.POINTER(C.c_uint64)],
     "hpt_present_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)],
@@ -604,6 +606,12 @@ class HalaRenderer:
         transmittance walks): 64, 128 or 256; same images."""
         _check(self._lib.hpt_set_traversal_block(self._h, threads))
 
+    def set_traversal_refill(self, on: bool = True) -> None:
+        """Lane refill in the traversal kernels (on by default): resident
+        blocks whose waves take new queue entries for lanes whose rays have
+        ended; same images."""
+        _check(self._lib.hpt_set_traversal_refill(self._h, int(on)))
+
     def truncated_paths(self) -> int:
         """Paths stopped at the 4096-bounce limit since the handle was made
         (TRANSPARENT pass-throughs do not count toward max_depth)."""
@@ -687,6 +695,9 @@ class HalaGroup:
 
     def set_traversal_block(self, threads: int = 128) -> None:
         _check(self._lib.hpt_group_set_traversal_block(self._h, threads))
+
+    def set_traversal_refill(self, on: bool = True) -> None:
+        _check(self._lib.hpt_group_set_traversal_refill(self._h, int(on)))
 
     def truncated_paths(self) -> int:
         """Paths of this group's steps stopped at the 4096-bounce limit."""
