@@ -551,7 +551,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     }
     if (!vmis) {
       begin(HPT_STAGE_SHADOW);
-      launch_shadow(s, sv, ps, sq, n, sp, sc_shadow, trav_block);
+      launch_shadow(s, sv, ps, sq, n, sp, sc_shadow, trav_block, wb.refill ? counts + kCursorOffset : nullptr);
       end();
     }
     check(hipGetLastError(), "bounce launch");

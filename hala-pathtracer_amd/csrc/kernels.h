@@ -187,7 +187,7 @@ constexpr uint32_t kTailChunks = 16;
 void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
                  const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
-                   Spill sp, StageCounters sc, uint32_t bs);
+                   Spill sp, StageCounters sc, uint32_t bs, uint32_t* cursors = nullptr);
 // Volume-MIS shading in phases (MEDIUM|VOL_MIS): A over qin (finished and
 // passing paths to qout, scattering events to qnee, their first walk to wq),
 // the walks of a walk queue, B over qnee (environment maps: the second walk

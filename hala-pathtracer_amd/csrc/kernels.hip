@@ -1658,6 +1658,94 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
   if (COUNT) add_visits(stats, cv, ct);
 }
 
+// Shadow rays by resident blocks that refill their lanes, as k_extend_refill
+// does for extension rays.  A ray's origin, direction, tmax, tag and
+// contribution are read again from its queue slot when it ends, and its NEE
+// sum is read then rather than when it starts: no other ray of the launch
+// adds to that sum.
+template <bool COUNT, uint32_t BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
+    k_shadow_refill(SceneView sv, PathState ps, ShadowQueue sq, Spill sp, unsigned long long* stats,
+                    uint32_t* cursors) {
+  __shared__ uint2 lds_stack[kShortStack * BS];
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  uint32_t seg = blockIdx.x % kSeg;
+  uint32_t segn = sq.counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)segn);
+  LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  const uint32_t lane = lane_id();
+  uint32_t cv = 0, ct = 0, hops = 0, k = 0;
+  bool more = true, active = false, ended = false;
+  Walker<true> w;
+  auto finish = [&] {
+    const float4 o4 = sq.o[k], d4 = sq.d[k];
+    if (w.found || any_light(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, o4.w)) return;
+    const uint32_t tag = __float_as_uint(d4.w);
+    float4* sum = ((tag & 1u) ? ps.nee_light : ps.nee_env) + (tag >> 1);
+    const float4 c4 = sq.c[k];
+    float4 acc = *sum;
+    acc.x += c4.x;
+    acc.y += c4.y;
+    acc.z += c4.z;
+    *sum = acc;
+  };
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (__popcll(idle) >= HPT_REFILL_IDLE || idle == ~0ull) {
+      if (ended) {
+        finish();
+        if (COUNT) {
+          cv += w.nv;
+          ct += w.nt;
+        }
+        ended = false;
+      }
+      unsigned long long need = idle;
+      while (need != 0ull && more) {
+        const uint32_t cnt = (uint32_t)__popcll(need);
+        const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+        uint32_t got = 0;
+        if (lane == leader) got = atomicAdd(cursors + seg * kSegStride, cnt);
+        got = (uint32_t)__builtin_amdgcn_readlane((int)got, (int)leader);
+        const uint32_t avail = got < segn ? min(cnt, segn - got) : 0u;
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        const bool take = ((need >> lane) & 1ull) != 0ull && rank < avail;
+        if (take) {
+          k = seg * sq.seg_cap + got + rank;
+          const float4 o4 = sq.o[k], d4 = sq.d[k];
+          w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, o4.w, st);
+          active = true;
+        }
+        need &= ~__ballot(take);
+        if (avail < cnt) {  // segment drained: the next one of this XCD
+          if (++hops == kSeg / 8) {
+            more = false;
+          } else {
+            seg = (seg + 8) % kSeg;
+            segn = sq.counts[seg * kSegStride];
+          }
+        }
+      }
+      if (__ballot(active) == 0ull) break;
+    }
+    if (active && w.step(sv, st)) {
+      active = false;
+      ended = true;
+    }
+  }
+  if (COUNT) add_visits(stats, cv, ct);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(cursors + kSeg * kSegStride, 1u) == gridDim.x - 1) {
+      for (uint32_t s = 0; s < kSeg; ++s) cursors[s * kSegStride] = 0;
+      cursors[kSeg * kSegStride] = 0;
+      __threadfence();
+    }
+  }
+}
+
 // -------------------------------------------------------------- accumulate
 __device__ __forceinline__ float srgb(float x) {  // common/common.glsl:26-32
   if (x <= 0.0031308f) return 12.92f * x;
@@ -2048,7 +2136,19 @@ void launch_vmis_c(hipStream_t s, bool transp, const SceneView& sv, const FrameP
 }
 
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
-                   Spill sp, StageCounters sc, uint32_t bs) {
+                   Spill sp, StageCounters sc, uint32_t bs, uint32_t* cursors) {
+  if (cursors && sv.node_count > 0) {
+    const dim3 grid(kSeg * std::min(walk_blocks(sq.seg_cap, bs), resident_blocks_per_segment(bs)));
+#define HPT_S_T(BS_) \
+  hipLaunchKernelGGL((k_shadow_refill<true, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats, cursors)
+#define HPT_S_F(BS_) \
+  hipLaunchKernelGGL((k_shadow_refill<false, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats, cursors)
+    if (sc.count && sc.stats) HPT_BY_BS(bs, HPT_S_T)
+    else HPT_BY_BS(bs, HPT_S_F)
+#undef HPT_S_T
+#undef HPT_S_F
+    return;
+  }
   const dim3 grid(kSeg * walk_blocks(sq.seg_cap, bs));
 #define HPT_S_T(BS_) hipLaunchKernelGGL((k_shadow<true, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats)
 #define HPT_S_F(BS_) hipLaunchKernelGGL((k_shadow<false, BS_>), grid, dim3(BS_), 0, s, sv, ps, sq, sp, sc.stats)

@@ -10,7 +10,7 @@ the resident lanes of its blocks: waves refill many times) and a 96x64 one
 (segments drain early and blocks move on to the other segments of their
 XCD); the medium scene's transmittance walks through a transparent and a
 glass boundary, absorbing and scattering media, an environment map (two
-walks per scattering event); the Disney grid (extension rays only); eager
+walks per scattering event); the Disney grid (extension and shadow rays); eager
 and graph launches, sorted shading, the three traversal block sizes; group
 steps of several jobs."""
 import numpy as np
@@ -55,7 +55,7 @@ def _render(scene, w, h, features, refill, mode=hpt.LAUNCH_EAGER, sort=hpt.SHADE
     out = [r.read_image(im) for im in IMAGES]
     stats = None
     if count:
-        stats = {s: r.stage_stats(s) for s in (hpt.STAGE_EXTEND, hpt.STAGE_WALK)}
+        stats = {s: r.stage_stats(s) for s in (hpt.STAGE_EXTEND, hpt.STAGE_WALK, hpt.STAGE_SHADOW)}
     r.close()
     return out, stats
 
@@ -76,6 +76,8 @@ def test_refill_gives_the_same_images_and_counts(case, size):
     assert base_stats[hpt.STAGE_EXTEND]["node_visits"] > 0
     if features:
         assert base_stats[hpt.STAGE_WALK]["queries"] > 0
+    else:
+        assert base_stats[hpt.STAGE_SHADOW]["node_visits"] > 0
     # the plain kernels (no counting) under graph replay, sorted shading and
     # each block size
     plain, _ = _render(scene, w, h, features, False)
