@@ -94,23 +94,24 @@ def cpu_baseline(scene, width, height, max_depth, rr_depth, features=0, budget_s
             "sample": f"{what} ({paths} paths, {dt:.1f} s)"}
 
 
-def profiled_traffic(kernel, workload, batch_frames, tail_from, block):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    of the workload (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
-    the reading of a 1 GiB copy), when it was taken at the same batch size
-    (frames traced together), tail bounce and traversal block size; else
-    None."""
+def profiled_traffic(kernel, workload, batch_frames, tail_from, block, refill=True):
+    """HBM bytes per launch of the stage `kernel` (its kernel family) from the
+    committed rocprofv3 summary of the workload (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the reading of a 1 GiB copy), when it was taken
+    at the same batch size (frames traced together), tail bounce, traversal
+    block size and lane refill setting; else None."""
     d = os.path.join(ROOT, "profiles")
     tail = "none" if tail_from == hpt.TAIL_FROM_NONE else str(tail_from)
-    pat = r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+), tail_from (\w+), block (\d+)\) = (\d+)"
+    pat = (r"traffic_bytes_per_launch (\w+) \(frames_in_flight (\d+), tail_from (\w+), block (\d+)"
+           r"(?:, refill (on|off))?\) = (\d+)")
     for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
         if not name.endswith(f"_{workload}.md"):
             continue
         for line in open(os.path.join(d, name)):
             m = re.match(pat, line)
             if m and m.group(1) == kernel and int(m.group(2)) == batch_frames and m.group(3) == tail and \
-                    int(m.group(4)) == block:
-                return int(m.group(5))
+                    int(m.group(4)) == block and (m.group(5) or "off") == ("on" if refill else "off"):
+                return int(m.group(6))
     return None
 
 
@@ -600,6 +601,8 @@ def main():
                     help="full batches replayed as captured HIP graphs (default) or every kernel launched")
     ap.add_argument("--traversal-block", default="auto",
                     help="threads per block of the traversal kernels: auto (autotuned with the sort), 64, 128 or 256")
+    ap.add_argument("--traversal-refill", choices=["on", "off"], default="on",
+                    help="traversal lanes refill from their queues (resident blocks; on) or trace one entry each")
     ap.add_argument("--tail-from", default="auto",
                     help="first bounce run by the tail kernel: auto (autotuned), none, or a bounce index")
     ap.add_argument("--vmis-phases", choices=["on", "off"], default="on",
@@ -660,6 +663,7 @@ def main():
         rr.set_vmis_phases(args.vmis_phases == "on")
         rr.set_tail_from(tails["denoised"] if frames_in_flight else tails["plain"])
         rr.set_traversal_block(blocks["denoised"] if frames_in_flight else blocks["plain"])
+        rr.set_traversal_refill(args.traversal_refill == "on")
         return rr
 
     shade_sort = hpt.SHADE_SORT_OFF if args.shade_sort in ("off", "auto") else int(args.shade_sort)
@@ -760,15 +764,17 @@ def main():
     # and written -- plus, when the walks run inline, its volume-MIS walks at
     # the extension's rate per closest-hit query); the walk kernel (that rate
     # per query) when shading is phased.
-    roof_ext = stage_roofline("k_extend", ext_t, NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
+    refill = args.traversal_refill == "on"
+    roof_ext = stage_roofline("k_extend+k_extend_refill" if refill else "k_extend", ext_t,
+                              NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
                               traffic=profiled_traffic("k_extend", args.workload, batch_frames, tails["plain"],
-                                                       blocks["plain"]),
+                                                       blocks["plain"], refill),
                               extra={"nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)})
     shade_bytes = SHADE_EVENT_BYTES + (0.0 if phased else walk_queries * walk_bytes_per_query)
     shade_name = "k_vmis_a/b/c" if phased else "k_shade"
     roof_shade = stage_roofline(shade_name, shade_t, shade_bytes,
                                 traffic=profiled_traffic("k_vmis" if phased else "k_shade", args.workload,
-                                                         batch_frames, tails["plain"], blocks["plain"]),
+                                                         batch_frames, tails["plain"], blocks["plain"], refill),
                                 extra={"bytes_per_event": round(shade_bytes, 1),
                                        "walk_queries_per_event": round(walk_queries, 4)})
     rooflines = {"extend": roof_ext, "shade": roof_shade}
@@ -776,9 +782,9 @@ def main():
         # the walk kernel's units are its closest-hit queries: the timed
         # run's shading events times the warm-up's queries per event
         walk_t = dict(walk_t, rays=int(round(shade_t["rays"] * walk_queries)))
-        rooflines["walk"] = stage_roofline("k_walk", walk_t, walk_bytes_per_query,
+        rooflines["walk"] = stage_roofline("k_walk_refill" if refill else "k_walk", walk_t, walk_bytes_per_query,
                                            traffic=profiled_traffic("k_walk", args.workload, batch_frames,
-                                                                    tails["plain"], blocks["plain"]),
+                                                                    tails["plain"], blocks["plain"], refill),
                                            extra={"bytes_per_query": round(walk_bytes_per_query, 1)})
     stage_of = {"extend": ext_t, "shade": shade_t, "shadow": shadow_t, "walk": walk_t}
     dominant = max((k for k in rooflines), key=lambda k: stage_of[k]["total_ms"])
@@ -842,6 +848,7 @@ def main():
               "shade_sort_tuning": sort_table,
               "tail_from": {k: (None if v == hpt.TAIL_FROM_NONE else v) for k, v in tails.items()},
               "traversal_block": blocks,
+              "traversal_refill": args.traversal_refill,
               "vmis_phases": args.vmis_phases if FEATURES & hpt.FEATURE_VOL_MIS else None,
               "eager_profiled_wall_s": round(profiled_wall, 4),
               "stage_ms": {n: round(stats[s]["total_ms"], 3) for s, n in enumerate(names)},

@@ -180,20 +180,34 @@ struct TraceResult {
 // are `lds_stride` uint2 apart; entries past kShortStack go to `spill`
 // (stride `spill_stride`).  Needed depth <= tree levels.
 
+// The LDS part is held as an LDS-space pointer: with a generic pointer the
+// compiler merges the LDS and spill accesses into one flat access through a
+// select of two 64-bit addresses (flat instructions take the vector memory
+// path and the addresses two registers each); with distinct address spaces
+// the LDS entries use ds_read/ds_write and the spill accesses stay in their
+// own, rarely taken, branch.
+// Entries are node groups (two words) kept as one 64-bit word in LDS.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 struct LaneStack {
-  uint2* lds;
+  lds_u64* lds;
   int lds_stride;
   uint2* spill;
   int spill_stride;
   int sp;
+  __device__ __forceinline__ LaneStack(uint2* lds_slot, int lds_stride_, uint2* spill_slot, int spill_stride_, int sp_)
+      : lds((lds_u64*)lds_slot), lds_stride(lds_stride_), spill(spill_slot), spill_stride(spill_stride_), sp(sp_) {}
   __device__ __forceinline__ void push(uint2 e) {
-    if (sp < kShortStack) lds[sp * lds_stride] = e;
+    if (sp < kShortStack) lds[sp * lds_stride] = (unsigned long long)e.y << 32 | e.x;
     else spill[(sp - kShortStack) * spill_stride] = e;
     sp++;
   }
   __device__ __forceinline__ uint2 pop() {
     sp--;
-    return sp < kShortStack ? lds[sp * lds_stride] : spill[(sp - kShortStack) * spill_stride];
+    if (sp < kShortStack) {
+      const unsigned long long v = lds[sp * lds_stride];
+      return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+    }
+    return spill[(sp - kShortStack) * spill_stride];
   }
 };
 

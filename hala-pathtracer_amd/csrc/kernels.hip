@@ -1320,8 +1320,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
 // transmittance, and the lanes still idle take the next walks of the
 // block's segment.  Every walk runs the operations of eval_transmittance in
 // its order, its RNG draws included, so the results equal k_walk's.
+// 6 waves per SIMD (80 registers) measured ahead of 7 (spills) and 5 on the
+// 1080p medium scene: 10.1 against 10.8 and 10.5 ms of walks per 20-frame batch.
 #ifndef HPT_WALK_REFILL_WAVES_PER_EU
-#define HPT_WALK_REFILL_WAVES_PER_EU 7
+#define HPT_WALK_REFILL_WAVES_PER_EU 6
 #endif
 template <bool TRANSP, bool JOBS, bool COUNT, uint32_t BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_REFILL_WAVES_PER_EU)))
@@ -1335,8 +1337,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
   LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
   WalkCounts wc;
   const uint32_t lane = lane_id();
-  uint32_t hops = 0, k = 0, depth = 0;
-  f3 tr = mk3(1.0f);
+  // A walk's transmittance so far and its segment count live in its
+  // path's nee_tr (xyz, w as the count) between segments: registers are
+  // what limits these kernels' waves per SIMD.
+  uint32_t hops = 0, k = 0;
   bool more = true, active = false, ended = false;
   Walker<false> w;
   while (true) {
@@ -1356,6 +1360,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
           wc.nv += w.nv;
           wc.nt += w.nt;
         }
+        const float4 t4 = ps.nee_tr[p];
+        f3 tr = mk3(t4.x, t4.y, t4.z);
+        uint32_t depth = __float_as_uint(t4.w);
         if (!(r.id == MISS_ID || (r.id & LIGHT_BIT))) {
           const HitState h = triangle_state(sv, r.id, r.t, r.u, r.v, o, d);
           if (h.material == INVALID) {
@@ -1386,6 +1393,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
                 w.begin(prepare_ray(h.pos + d * EPS, d), 0.0f, WORLD_SIZE, st);
                 active = true;
                 done = false;
+                ps.nee_tr[p] = make_float4(tr.x, tr.y, tr.z, __uint_as_float(depth));
               }
             }
           }
@@ -1408,13 +1416,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
           const float4 o4 = wq.o[k], d4 = wq.d[k];
           const uint32_t p = __float_as_uint(d4.w);
           const uint32_t max_depth = JOBS ? fp.job_limits[ps.job[p]].x : fp.max_depth;
-          tr = mk3(1.0f);
-          depth = 0;
+          ps.nee_tr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);  // no segment traced yet
           if (max_depth > 0) {
             w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, WORLD_SIZE, st);
             active = true;
-          } else {
-            ps.nee_tr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
           }
         }
         need &= ~__ballot(take);
@@ -1936,6 +1941,9 @@ void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, cons
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
                    uint32_t* phase_counts, uint32_t bs, uint32_t* cursors) {
   const bool count = sc.count && sc.stats;
+  // The first bounce's camera rays (the identity queue) keep one ray per
+  // lane: they are coherent, and refilling them measured slower on the 1080p
+  // Disney grid (extension 17.0 -> 18.7 ms per 32-frame batch).
   if (cursors && qin.items && sv.node_count > 0) {
     const dim3 grid(kSeg * std::min(walk_blocks(qin.seg_cap, bs), resident_blocks_per_segment(bs)));
 #define HPT_EXTEND_R(C, S, BS_)                                                                                  \
