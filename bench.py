@@ -765,7 +765,7 @@ def main():
     # the extension's rate per closest-hit query); the walk kernel (that rate
     # per query) when shading is phased.
     refill = args.traversal_refill == "on"
-    roof_ext = stage_roofline("k_extend+k_extend_refill" if refill else "k_extend", ext_t,
+    roof_ext = stage_roofline("k_extend_refill" if refill else "k_extend", ext_t,
                               NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray + RAY_BYTES,
                               traffic=profiled_traffic("k_extend", args.workload, batch_frames, tails["plain"],
                                                        blocks["plain"], refill),

@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
 // blocks; the last block clears them all for the next launch.
 template <bool COUNT, bool SORT, uint32_t BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WALK_WAVES_PER_EU)))
-    k_extend_refill(SceneView sv, PathState ps, Queue qin, Queue qnext, ShadowQueue sq, Spill sp,
+    k_extend_refill(SceneView sv, PathState ps, Queue qin, uint32_t n_max, Queue qnext, ShadowQueue sq, Spill sp,
                     unsigned long long* stats, uint8_t* keys, uint32_t* phase_counts, uint32_t* cursors) {
   __shared__ uint2 lds_stack[kShortStack * BS];
   const uint32_t gid = blockIdx.x * BS + threadIdx.x;
@@ -295,8 +295,20 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
       phase_counts[(kSeg + gid) * kSegStride] = 0;
     }
   }
+  // The first bounce's identity queue over [0, n_max): segment s holds the
+  // kBlock-path chunks c with c % kSeg == s, in order -- the chunks the
+  // one-ray-per-lane launch gives block c, so each XCD keeps the same
+  // interleaved share of the image.
+  const uint32_t nchunks = (n_max + kBlock - 1) / kBlock;
+  auto seg_count = [&](uint32_t sg) -> uint32_t {
+    if (qin.items) return qin.counts[sg * kSegStride];
+    if (sg >= nchunks) return 0u;
+    uint32_t c = ((nchunks - 1u - sg) / kSeg + 1u) * kBlock;
+    if ((nchunks - 1u) % kSeg == sg) c -= nchunks * kBlock - n_max;
+    return c;
+  };
   uint32_t seg = blockIdx.x % kSeg;
-  uint32_t segn = qin.counts[seg * kSegStride];
+  uint32_t segn = seg_count(seg);
   if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)segn);
   LaneStack st{lds_stack + threadIdx.x, BS, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
   const uint32_t lane = lane_id();
@@ -337,8 +349,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
             __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
         const bool take = ((need >> lane) & 1ull) != 0ull && rank < avail;
         if (take) {
-          k = seg * qin.seg_cap + got + rank;
-          p = qin.items[k];
+          const uint32_t e = got + rank;
+          k = qin.items ? seg * qin.seg_cap + e : ((e / kBlock) * kSeg + seg) * kBlock + e % kBlock;
+          p = qin.items ? qin.items[k] : k;
           const float4 o4 = ps.org[p], d4 = ps.dir[p];
           w.begin(prepare_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), 0.0f, WORLD_SIZE, st);
           active = true;
@@ -349,7 +362,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
             more = false;
           } else {
             seg = (seg + 8) % kSeg;
-            segn = qin.counts[seg * kSegStride];
+            segn = seg_count(seg);
           }
         }
       }
@@ -1941,14 +1954,11 @@ void launch_extend(hipStream_t s, const SceneView& sv, const PathState& ps, cons
                    const Queue& qnext, const ShadowQueue& sq, Spill sp, StageCounters sc, uint8_t* keys,
                    uint32_t* phase_counts, uint32_t bs, uint32_t* cursors) {
   const bool count = sc.count && sc.stats;
-  // The first bounce's camera rays (the identity queue) keep one ray per
-  // lane: they are coherent, and refilling them measured slower on the 1080p
-  // Disney grid (extension 17.0 -> 18.7 ms per 32-frame batch).
-  if (cursors && qin.items && sv.node_count > 0) {
+  if (cursors && sv.node_count > 0) {
     const dim3 grid(kSeg * std::min(walk_blocks(qin.seg_cap, bs), resident_blocks_per_segment(bs)));
-#define HPT_EXTEND_R(C, S, BS_)                                                                                  \
-  hipLaunchKernelGGL((k_extend_refill<C, S, BS_>), grid, dim3(BS_), 0, s, sv, ps, qin, qnext, sq, sp, sc.stats, keys, \
-                     phase_counts, cursors)
+#define HPT_EXTEND_R(C, S, BS_)                                                                                   \
+  hipLaunchKernelGGL((k_extend_refill<C, S, BS_>), grid, dim3(BS_), 0, s, sv, ps, qin, n_max, qnext, sq, sp, sc.stats, \
+                     keys, phase_counts, cursors)
 #define HPT_R_TT(BS_) HPT_EXTEND_R(true, true, BS_)
 #define HPT_R_FT(BS_) HPT_EXTEND_R(false, true, BS_)
 #define HPT_R_TF(BS_) HPT_EXTEND_R(true, false, BS_)
