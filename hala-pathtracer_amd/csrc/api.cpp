@@ -565,7 +565,7 @@ void trace_wavefront(hipStream_t s, const SceneView& sv, uint32_t features, cons
     // (pass-throughs do not count toward depth, pathtracer.rgen.glsl:196-200).
     begin(HPT_STAGE_TAIL);
     launch_tail(s, features, sv, shade_fp, ps, qin, sp, wave_bounces < kMaxBounces ? kMaxBounces - wave_bounces : 0u,
-                wb.tail_cut, sc_tail);
+                wb.tail_cut, sc_tail, wb.refill ? counts + kCursorOffset : nullptr);
     end();
     check(hipGetLastError(), "tail launch");
   }

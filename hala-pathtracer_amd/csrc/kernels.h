@@ -184,8 +184,11 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
 // end by one lane (k_tail): at most kTailChunks blocks per segment, a path cut
 // after max_steps bounces and counted in *cut.
 constexpr uint32_t kTailChunks = 16;
+// cursors (see launch_extend): non-null runs k_tail_refill, lanes that take
+// new paths as others end.
 void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc);
+                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc,
+                 uint32_t* cursors = nullptr);
 void launch_shadow(hipStream_t s, const SceneView& sv, const PathState& ps, const ShadowQueue& sq, uint32_t n_max,
                    Spill sp, StageCounters sc, uint32_t bs, uint32_t* cursors = nullptr);
 // Volume-MIS shading in phases (MEDIUM|VOL_MIS): A over qin (finished and

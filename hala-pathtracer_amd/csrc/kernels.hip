@@ -938,6 +938,43 @@ __global__ void __launch_bounds__(kBlock)
 // takes the next at once, measured slower: 18.7 against 16.5 ms for two
 // 2-frame 1080p batches.)  A path is cut after max_steps tail bounces; cut
 // paths keep the radiance gathered so far and are counted in *cut.
+// One bounce of a tail path: closest hit, shade_path, and without VOL_MIS
+// the two shadow rays; returns whether the path continues.
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
+__device__ __forceinline__ bool tail_bounce(const SceneView& sv, const FrameParams& fp, const PathState& ps,
+                                            uint32_t p, LaneStack& st, WalkCounts& wc) {
+  const float4 o4 = ps.org[p], d4 = ps.dir[p];
+  const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st);
+  ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
+  wc.q++;
+  ShadowOut s_env, s_light;
+  const bool cont = shade_path<MED, VMIS, TRANSP, JOBS, false>(sv, fp, ps, p, st, wc, s_env, s_light);
+  if (!VMIS) {
+    if (s_env.valid && !trace_any(sv, s_env.o, s_env.d, 0.0f, s_env.tmax, st)) {
+      float4 a = ps.nee_env[p];
+      a.x += s_env.c.x;
+      a.y += s_env.c.y;
+      a.z += s_env.c.z;
+      ps.nee_env[p] = a;
+    }
+    if (s_light.valid && !trace_any(sv, s_light.o, s_light.d, 0.0f, s_light.tmax, st)) {
+      float4 a = ps.nee_light[p];
+      a.x += s_light.c.x;
+      a.y += s_light.c.y;
+      a.z += s_light.c.z;
+      ps.nee_light[p] = a;
+    }
+  }
+  return cont;
+}
+
+// The tail's bounces traced (its stage's query count), one atomic per wave.
+__device__ __forceinline__ void add_tail_queries(unsigned long long* stats, const WalkCounts& wc) {
+  unsigned long long q = wc.q;
+  for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
+  if (lane_id() == 0) atomicAdd(stats + (3 * kSeg + blockIdx.x % kSeg) * kStatStride, q);
+}
+
 template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
 __global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, PathState ps, Queue qin, Spill sp,
                                                  uint32_t max_steps, unsigned long long* cut,
@@ -956,35 +993,87 @@ __global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, P
         atomicAdd(cut, 1ull);
         break;
       }
-      const float4 o4 = ps.org[p], d4 = ps.dir[p];
-      const TraceResult r = trace_closest(sv, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), 0.0f, WORLD_SIZE, st);
-      ps.hit[p] = make_float4(r.t, r.u, r.v, __uint_as_float(r.id));
-      wc.q++;
-      ShadowOut s_env, s_light;
-      const bool cont = shade_path<MED, VMIS, TRANSP, JOBS, false>(sv, fp, ps, p, st, wc, s_env, s_light);
-      if (!VMIS) {
-        if (s_env.valid && !trace_any(sv, s_env.o, s_env.d, 0.0f, s_env.tmax, st)) {
-          float4 a = ps.nee_env[p];
-          a.x += s_env.c.x;
-          a.y += s_env.c.y;
-          a.z += s_env.c.z;
-          ps.nee_env[p] = a;
-        }
-        if (s_light.valid && !trace_any(sv, s_light.o, s_light.d, 0.0f, s_light.tmax, st)) {
-          float4 a = ps.nee_light[p];
-          a.x += s_light.c.x;
-          a.y += s_light.c.y;
-          a.z += s_light.c.z;
-          ps.nee_light[p] = a;
-        }
-      }
-      if (!cont) break;
+      if (!tail_bounce<MED, VMIS, TRANSP, JOBS>(sv, fp, ps, p, st, wc)) break;
     }
   }
-  if (stats) {  // bounces traced: the stage's query count
-    unsigned long long q = wc.q;
-    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
-    if (lane_id() == 0) atomicAdd(stats + (3 * kSeg + seg) * kStatStride, q);
+  if (stats) add_tail_queries(stats, wc);
+}
+
+// Waves per SIMD the tail kernels reach (their register count allows two).
+#ifndef HPT_TAIL_WAVES_PER_EU
+#define HPT_TAIL_WAVES_PER_EU 2
+#endif
+
+// The tail with lane refill (as k_extend_refill): resident blocks step
+// their lanes' paths a bounce at a time, and once HPT_REFILL_IDLE lanes of a
+// wave have no path those lanes take the next paths of the block's segment
+// (then the other segments of its XCD), so a wave is not held by the lane
+// whose paths happen to be longest.  Each path runs the same bounces as in
+// k_tail.
+template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
+__global__ void __launch_bounds__(kBlock) k_tail_refill(SceneView sv, FrameParams fp, PathState ps, Queue qin,
+                                                        Spill sp, uint32_t max_steps, unsigned long long* cut,
+                                                        unsigned long long* stats, uint32_t* cursors) {
+  __shared__ uint2 lds_stack[kShortStack * kBlock];
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t seg = blockIdx.x % kSeg;
+  uint32_t segn = qin.counts[seg * kSegStride];
+  if (stats && threadIdx.x == 0 && blockIdx.x < kSeg) atomicAdd(stats + seg * kStatStride, (unsigned long long)segn);
+  LaneStack st{lds_stack + threadIdx.x, kBlock, sp.buf ? sp.buf + gid : nullptr, (int)sp.stride, 0};
+  WalkCounts wc;
+  const uint32_t lane = lane_id();
+  uint32_t hops = 0, p = 0, step = 0;
+  bool more = true, has = false;
+  while (true) {
+    const unsigned long long idle = __ballot(!has);
+    if (__popcll(idle) >= HPT_REFILL_IDLE || idle == ~0ull) {
+      unsigned long long need = idle;
+      while (need != 0ull && more) {
+        const uint32_t cnt = (uint32_t)__popcll(need);
+        const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+        uint32_t got = 0;
+        if (lane == leader) got = atomicAdd(cursors + seg * kSegStride, cnt);
+        got = (uint32_t)__builtin_amdgcn_readlane((int)got, (int)leader);
+        const uint32_t avail = got < segn ? min(cnt, segn - got) : 0u;
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+        const bool take = ((need >> lane) & 1ull) != 0ull && rank < avail;
+        if (take) {
+          p = qin.items[seg * qin.seg_cap + got + rank];
+          step = 0;
+          has = true;
+        }
+        need &= ~__ballot(take);
+        if (avail < cnt) {  // segment drained: the next one of this XCD
+          if (++hops == kSeg / 8) {
+            more = false;
+          } else {
+            seg = (seg + 8) % kSeg;
+            segn = qin.counts[seg * kSegStride];
+          }
+        }
+      }
+      if (__ballot(has) == 0ull) break;
+    }
+    if (has) {
+      if (step == max_steps) {
+        atomicAdd(cut, 1ull);
+        has = false;
+      } else {
+        ++step;
+        has = tail_bounce<MED, VMIS, TRANSP, JOBS>(sv, fp, ps, p, st, wc);
+      }
+    }
+  }
+  if (stats) add_tail_queries(stats, wc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(cursors + kSeg * kSegStride, 1u) == gridDim.x - 1) {
+      for (uint32_t s = 0; s < kSeg; ++s) cursors[s * kSegStride] = 0;
+      cursors[kSeg * kSegStride] = 0;
+      __threadfence();
+    }
   }
 }
 
@@ -2047,14 +2136,22 @@ void launch_shade(hipStream_t s, uint32_t features, const SceneView& sv, const F
 }
 
 void launch_tail(hipStream_t s, uint32_t features, const SceneView& sv, const FrameParams& fp, const PathState& ps,
-                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc) {
+                 const Queue& qin, Spill sp, uint32_t max_steps, unsigned long long* cut, StageCounters sc,
+                 uint32_t* cursors) {
   const uint32_t chunks = (qin.seg_cap + kBlock - 1) / kBlock;
-  const dim3 grid(kSeg * (chunks < kTailChunks ? chunks : kTailChunks));
+  const bool refill = cursors != nullptr;
+  const uint32_t per_seg = refill ? std::min(chunks, resident_blocks_per_segment(kBlock, HPT_TAIL_WAVES_PER_EU))
+                                  : std::min(chunks, kTailChunks);
+  const dim3 grid(kSeg * per_seg);
   const bool med = features & HPT_FEATURE_MEDIUM;
   const bool vmis = med && (features & HPT_FEATURE_VOL_MIS);
   const bool tr = features & HPT_FEATURE_TRANSPARENT;
-#define HPT_T(M, V, T, J) \
-  hipLaunchKernelGGL((k_tail<M, V, T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, sp, max_steps, cut, sc.stats)
+#define HPT_T(M, V, T, J)                                                                                      \
+  if (refill)                                                                                                  \
+    hipLaunchKernelGGL((k_tail_refill<M, V, T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, sp, max_steps, cut, \
+                       sc.stats, cursors);                                                                     \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_tail<M, V, T, J>), grid, dim3(kBlock), 0, s, sv, fp, ps, qin, sp, max_steps, cut, sc.stats)
 #define HPT_TJ(M, V, T)            \
   if (fp.job_limits) HPT_T(M, V, T, true); \
   else HPT_T(M, V, T, false)

@@ -11,8 +11,8 @@ the resident lanes of its blocks: waves refill many times) and a 96x64 one
 XCD); the medium scene's transmittance walks through a transparent and a
 glass boundary, absorbing and scattering media, an environment map (two
 walks per scattering event); the Disney grid (extension and shadow rays); eager
-and graph launches, sorted shading, the three traversal block sizes; group
-steps of several jobs."""
+and graph launches, sorted shading, the three traversal block sizes, the
+tail kernel from bounce 2; group steps of several jobs."""
 import numpy as np
 import pytest
 
@@ -40,7 +40,7 @@ def _scene(case, w, h):
 
 
 def _render(scene, w, h, features, refill, mode=hpt.LAUNCH_EAGER, sort=hpt.SHADE_SORT_OFF, block=128,
-            count=False):
+            count=False, tail=hpt.TAIL_FROM_NONE):
     r = hpt.HalaRenderer(w, h, max_depth=8, rr_depth=3, features=features)
     r.set_scene(scene)
     r.frames_in_flight = 4
@@ -49,13 +49,14 @@ def _render(scene, w, h, features, refill, mode=hpt.LAUNCH_EAGER, sort=hpt.SHADE
     r.set_shade_sort(sort)
     r.set_traversal_block(block)
     r.set_traversal_refill(refill)
+    r.set_tail_from(tail)
     if count:
         r.set_profiling(hpt.PROFILE_TIMING | hpt.PROFILE_COUNT_VISITS)
     r.render(4)
     out = [r.read_image(im) for im in IMAGES]
     stats = None
     if count:
-        stats = {s: r.stage_stats(s) for s in (hpt.STAGE_EXTEND, hpt.STAGE_WALK, hpt.STAGE_SHADOW)}
+        stats = {s: r.stage_stats(s) for s in (hpt.STAGE_EXTEND, hpt.STAGE_WALK, hpt.STAGE_SHADOW, hpt.STAGE_TAIL)}
     r.close()
     return out, stats
 
@@ -86,6 +87,18 @@ def test_refill_gives_the_same_images_and_counts(case, size):
         out, _ = _render(scene, w, h, features, True, mode, sort, block)
         for k, (a, b) in enumerate(zip(plain, out)):
             assert np.array_equal(a, b), (case, mode, sort, block, k)
+    # the tail kernel from bounce 2 (k_tail against k_tail_refill), whose
+    # counts must match too
+    t_plain, t_stats = _render(scene, w, h, features, False, count=True, tail=2)
+    t_got, t_got_stats = _render(scene, w, h, features, True, count=True, tail=2)
+    for k, (a, b) in enumerate(zip(plain, t_plain)):
+        assert np.array_equal(a, b), (case, "tail", k)
+    for k, (a, b) in enumerate(zip(t_plain, t_got)):
+        assert np.array_equal(a, b), (case, "tail refill", k)
+    for stage in t_stats:
+        for key in ("rays", "node_visits", "tri_tests", "queries"):
+            assert t_stats[stage][key] == t_got_stats[stage][key], (stage, key, t_stats[stage], t_got_stats[stage])
+    assert t_stats[hpt.STAGE_TAIL]["queries"] > 0
 
 
 def test_refill_in_group_steps():
