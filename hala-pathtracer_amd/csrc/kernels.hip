@@ -999,9 +999,12 @@ __global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, P
   if (stats) add_tail_queries(stats, wc);
 }
 
-// Waves per SIMD the tail kernels reach (their register count allows two).
+// Waves per SIMD of the refilling tail kernel.  Three (168 registers, some
+// spilled) measured ahead of two on the 1080p medium scene's 4-frame batches
+// (tail 3.1 -> 2.6 ms) and level with it on 20-frame batches; four matched
+// three.
 #ifndef HPT_TAIL_WAVES_PER_EU
-#define HPT_TAIL_WAVES_PER_EU 2
+#define HPT_TAIL_WAVES_PER_EU 3
 #endif
 
 // The tail with lane refill (as k_extend_refill): resident blocks step
@@ -1011,7 +1014,8 @@ __global__ void __launch_bounds__(kBlock) k_tail(SceneView sv, FrameParams fp, P
 // whose paths happen to be longest.  Each path runs the same bounces as in
 // k_tail.
 template <bool MED, bool VMIS, bool TRANSP, bool JOBS>
-__global__ void __launch_bounds__(kBlock) k_tail_refill(SceneView sv, FrameParams fp, PathState ps, Queue qin,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HPT_TAIL_WAVES_PER_EU)))
+k_tail_refill(SceneView sv, FrameParams fp, PathState ps, Queue qin,
                                                         Spill sp, uint32_t max_steps, unsigned long long* cut,
                                                         unsigned long long* stats, uint32_t* cursors) {
   __shared__ uint2 lds_stack[kShortStack * kBlock];
@@ -1424,6 +1428,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
 // its order, its RNG draws included, so the results equal k_walk's.
 // 6 waves per SIMD (80 registers) measured ahead of 7 (spills) and 5 on the
 // 1080p medium scene: 10.1 against 10.8 and 10.5 ms of walks per 20-frame batch.
+#ifndef HPT_WALK_REFILL_IDLE
+#define HPT_WALK_REFILL_IDLE HPT_REFILL_IDLE
+#endif
 #ifndef HPT_WALK_REFILL_WAVES_PER_EU
 #define HPT_WALK_REFILL_WAVES_PER_EU 6
 #endif
@@ -1447,7 +1454,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(HPT_WAL
   Walker<false> w;
   while (true) {
     const unsigned long long idle = __ballot(!active);
-    if (__popcll(idle) >= HPT_REFILL_IDLE || idle == ~0ull) {
+    if (__popcll(idle) >= HPT_WALK_REFILL_IDLE || idle == ~0ull) {
       if (ended) {  // the end of a segment (eval_transmittance's loop body)
         ended = false;
         bool done = true;
