@@ -118,6 +118,8 @@ def test_handles_of_another_kind_are_refused_on_the_device():
     print("[gpu] wrong-kind handles", flush=True)
     assert lib.hpt_group_set_vmis_phases(r._h, 0) == 1 and b"invalid group" in lib.hpt_last_error()
     assert lib.hpt_set_vmis_phases(g._h, 0) == 1 and b"invalid renderer" in lib.hpt_last_error()
+    assert lib.hpt_group_set_traversal_refill(r._h, 0) == 1 and b"invalid group" in lib.hpt_last_error()
+    assert lib.hpt_set_traversal_refill(g._h, 0) == 1 and b"invalid renderer" in lib.hpt_last_error()
     assert lib.hpt_set_frames_in_flight(g._h, 4) == 1
     g.set_vmis_phases(True)
     r.set_vmis_phases(True)

@@ -275,6 +275,8 @@ def test_handles_of_another_kind_are_refused():
     srv = hpt.RenderServer(1 << 30, dry_run_ns_per_path=1)
     assert lib.hpt_set_vmis_phases(srv._h, 1) == 1 and b"invalid renderer" in lib.hpt_last_error()
     assert lib.hpt_group_set_vmis_phases(srv._h, 1) == 1 and b"invalid group" in lib.hpt_last_error()
+    assert lib.hpt_set_traversal_refill(srv._h, 0) == 1 and b"invalid renderer" in lib.hpt_last_error()
+    assert lib.hpt_group_set_traversal_refill(srv._h, 0) == 1 and b"invalid group" in lib.hpt_last_error()
     assert lib.hpt_set_frames_in_flight(srv._h, 4) == 1
     out = np.zeros(4, np.float32)
     assert lib.hpt_denoise(srv._h, srv._h, out.ctypes.data, 16, 0) == 1
