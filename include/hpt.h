@@ -412,11 +412,12 @@ HPT_API hpt_status hpt_set_traversal_block(hpt_renderer* r, uint32_t threads);
 HPT_API hpt_status hpt_group_set_traversal_block(hpt_group* g, uint32_t threads);
 
 /* Lane refill in the traversal kernels (no counterpart in the reference):
- * with it on (the default), the extension and shadow rays and the
- * transmittance walks run in blocks that stay resident, and a wave whose
- * lanes' rays have ended takes new queue entries for them instead of
- * stepping on with most lanes idle until its longest ray ends.  Off: one
- * queue entry per lane.  Same images either way. */
+ * with it on (the default), the extension and shadow rays, the
+ * transmittance walks and the tail's paths run in blocks that stay
+ * resident, and a wave whose lanes' rays have ended takes new queue
+ * entries for them instead of stepping on with most lanes idle until its
+ * longest ray ends.  Off: one queue entry per lane.  Same images either
+ * way. */
 HPT_API hpt_status hpt_set_traversal_refill(hpt_renderer* r, uint32_t on);
 HPT_API hpt_status hpt_group_set_traversal_refill(hpt_group* g, uint32_t on);
 
